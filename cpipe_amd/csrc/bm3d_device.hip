@@ -1,0 +1,1117 @@
+// BM3D denoise engine for MI355X (gfx950): kernels and the C ABI of include/cpipe_bm3d.h.
+//
+// Data layout in HBM (per plan, dense FP32 planes, row pitch = width):
+//   z[c]      noisy planes in the opponent colour space (c < channels)
+//   bz[c]     basic estimate (step-1 output), opponent space
+//   pos/cnt   match lists: per reference position 16 packed (y<<16 | x) + raw count
+//   partial   per (channel, tile) num/den planes of the tile footprint (<= 91x91)
+// Reference positions: x in {0, 3, 6, ...} < W-8, plus W-8 (same in y).
+// A tile is 16x16 reference positions; its footprint is every pixel a member
+// patch of one of its groups can touch (refs +- 19 px search, + 8 px patch).
+//
+// Arithmetic is FP32 with a fixed operation order (explicit fmaf, -ffp-contract=off):
+//   SSD      = sum_i rowsum_i, rowsum_i = fma chain over j of (ref - cand)^2
+//   DCT      = row pass then column pass, each an fma chain in index order
+//   Haar     = orthonormal multi-level, (a+b)*s / (a-b)*s
+//   Wiener   = W = b^2/(b^2+s^2); weight 1/sum W^2 (lane tree 32,16,..,1)
+//   HT       = keep |c| > 2.7 sigma_c; weight 1/N_nonzero
+//   tile sum = groups in ref raster order, members in match order, num += wk*v
+//   merge    = tiles in (ty, tx) ascending order, out = num / den
+// so the result is reproducible run to run and equal to a CPU FP32 restatement
+// that follows the same order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cpipe_bm3d.h"
+
+namespace {
+
+constexpr int K = 8;          // patch edge
+constexpr int R = 19;         // search radius (39x39 window)
+constexpr int STRIDE = 3;     // reference stride
+constexpr int NMAX = 16;      // max group size
+constexpr int TILE = 16;      // refs per tile edge
+constexpr int FOOT = (TILE - 1) * STRIDE + 2 * R + K;  // 91: max tile footprint edge
+constexpr int FOOT2 = FOOT * FOOT;
+constexpr int BM_LW = FOOT + 1;  // block-matching LDS row: +1 column for the dx=20 overrun
+constexpr int BM_LH = FOOT;
+
+enum Status { OK = 0, FAILED = 1, OOM = 3, BAD_INDEX = 5, INTERNAL = 7, UNSUPPORTED = 8 };
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? OOM : INTERNAL,                      \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));              \
+        }                                                                                \
+    } while (0)
+
+__constant__ float c_dct[64];     // C[u][i], row u = frequency
+__constant__ float c_kaiser2[64]; // Kaiser(8, beta=2) outer product
+
+__host__ __device__ inline int ref_pos(int i, int n) {
+    const int last = n - K;
+    const int p = i * STRIDE;
+    return p < last ? p : last;
+}
+
+inline int ref_count(int n) { return n < K ? 0 : (n - K + STRIDE - 1) / STRIDE + 1; }
+
+// Tau on the un-normalised 64-pixel SSD (tau * k^2), Lebrun's 2500 and 400 on 0..255.
+inline float tau_ssd(int step) {
+    const double t = step == 1 ? 2500.0 / (255.0 * 255.0) : 400.0 / (255.0 * 255.0);
+    return (float)(64.0 * t);
+}
+
+// ---------------------------------------------------------------------------
+// Colour
+// ---------------------------------------------------------------------------
+
+__global__ void k_opp_fwd(const float* __restrict__ r, const float* __restrict__ g,
+                          const float* __restrict__ b, size_t pitch, float* __restrict__ z0,
+                          float* __restrict__ z1, float* __restrict__ z2, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    const size_t i = (size_t)y * pitch + x;
+    const size_t o = (size_t)y * w + x;
+    const float rv = r[i], gv = g[i], bv = b[i];
+    const float c13 = (float)(1.0 / 3.0);
+    z0[o] = ((rv + gv) + bv) * c13;
+    z1[o] = (rv - bv) * 0.5f;
+    z2[o] = ((rv + bv) - (gv + gv)) * 0.25f;
+}
+
+__global__ void k_copy_plane(const float* __restrict__ src, size_t pitch, float* __restrict__ dst,
+                             size_t dpitch, int w) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    dst[(size_t)y * dpitch + x] = src[(size_t)y * pitch + x];
+}
+
+__global__ void k_opp_inv(const float* __restrict__ z0, const float* __restrict__ z1,
+                          const float* __restrict__ z2, float* __restrict__ r,
+                          float* __restrict__ g, float* __restrict__ b, size_t pitch, int w) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    const size_t i = (size_t)y * w + x;
+    const size_t o = (size_t)y * pitch + x;
+    const float c23 = (float)(2.0 / 3.0), c43 = (float)(4.0 / 3.0);
+    const float yv = z0[i], u = z1[i], v = z2[i];
+    r[o] = (yv + u) + v * c23;
+    g[o] = yv - v * c43;
+    b[o] = (yv - u) + v * c23;
+}
+
+__device__ inline float h2f(uint16_t bits) { return (float)__builtin_bit_cast(_Float16, bits); }
+__device__ inline uint16_t f2h(float v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+
+__global__ void k_unpack_rgba16f(const uint16_t* __restrict__ in, size_t isp, float* __restrict__ r,
+                                 float* __restrict__ g, float* __restrict__ b, int w) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    const uint16_t* p = in + ((size_t)y * isp + x) * 4;
+    const size_t o = (size_t)y * w + x;
+    r[o] = h2f(p[0]);
+    g[o] = h2f(p[1]);
+    b[o] = h2f(p[2]);
+}
+
+__global__ void k_pack_rgba16f(const float* __restrict__ r, const float* __restrict__ g,
+                               const float* __restrict__ b, const uint16_t* __restrict__ in,
+                               size_t isp, uint16_t* __restrict__ out, size_t osp, int w) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    const size_t i = (size_t)y * w + x;
+    uint16_t* q = out + ((size_t)y * osp + x) * 4;
+    q[0] = f2h(r[i]);
+    q[1] = f2h(g[i]);
+    q[2] = f2h(b[i]);
+    q[3] = in[((size_t)y * isp + x) * 4 + 3];
+}
+
+// ---------------------------------------------------------------------------
+// Reference-compatible filter: bm3d_step1_generator.cpp:9-16,42-56 (2x2 box
+// mean blended by s = clamp(sigma/0.03, 0, 1), RGB clamped >= 0, alpha kept;
+// repeat_edge at the borders) and bm3d_step2_generator.cpp:33-42 (identity).
+// ---------------------------------------------------------------------------
+
+__global__ void k_compat_rgba16f(const uint16_t* __restrict__ in, size_t isp,
+                                 uint16_t* __restrict__ out, size_t osp, int w, int h, float sigma,
+                                 int stage) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    const float strength = fminf(1.0f, fmaxf(0.0f, sigma / 0.03f));
+    const int bx = (x / 2) * 2, by = (y / 2) * 2;
+    const int x0 = bx, x1 = min(bx + 1, w - 1);
+    const int y0 = by, y1 = min(by + 1, h - 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float v = h2f(in[((size_t)y * isp + x) * 4 + c]);
+        float r;
+        if (stage == 2) {
+            r = v + sigma * 0.0f;
+        } else if (c == 3) {
+            r = v;
+        } else {
+            const float a00 = h2f(in[((size_t)y0 * isp + x0) * 4 + c]);
+            const float a10 = h2f(in[((size_t)y0 * isp + x1) * 4 + c]);
+            const float a01 = h2f(in[((size_t)y1 * isp + x0) * 4 + c]);
+            const float a11 = h2f(in[((size_t)y1 * isp + x1) * 4 + c]);
+            const float basic = (((a00 + a10) + a01) + a11) * 0.25f;
+            const float pa = v * (1.0f - strength);
+            const float pb = basic * strength;
+            r = fmaxf(0.0f, pa + pb);
+        }
+        out[((size_t)y * osp + x) * 4 + c] = f2h(r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block matching, exact FP32.
+// One workgroup = one tile of 16x16 reference positions, one lane per ref.
+// The tile's search region (<= 91x91 px) is staged in LDS once; every lane
+// scans its clipped 39x39 window in (dy, dx) raster order, 4 candidates at a
+// time (11 LDS reads per patch row serve 4 SSDs), and keeps a sorted top-16
+// in registers (reference pinned first, strict insertion => ties keep raster
+// order). Only SSD < tau enters.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void insert16(float (&bd)[NMAX], uint32_t (&bp)[NMAX], float s,
+                                         uint32_t p) {
+    // precondition: bd sorted ascending, bd[0] = 0 <= s < bd[15]
+#pragma unroll
+    for (int k = NMAX - 1; k >= 1; --k) {
+        const bool sh = s < bd[k - 1];
+        const bool here = s < bd[k];
+        bd[k] = sh ? bd[k - 1] : (here ? s : bd[k]);
+        bp[k] = sh ? bp[k - 1] : (here ? p : bp[k]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ plane, int w, int h,
+                                                     int nx, int ny, float tau,
+                                                     uint32_t* __restrict__ pos,
+                                                     uint8_t* __restrict__ cnt) {
+    __shared__ float tile[BM_LW * BM_LH];
+    const int bx0 = blockIdx.x * TILE, by0 = blockIdx.y * TILE;
+    const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
+    const int Y0 = ref_pos(by0, h) - R;
+    for (int idx = threadIdx.x; idx < BM_LW * BM_LH; idx += blockDim.x) {
+        const int yy = idx / BM_LW, xx = idx - yy * BM_LW;
+        const int gx = X0 + xx, gy = Y0 + yy;
+        float v = 0.0f;
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = plane[(size_t)gy * w + gx];
+        tile[idx] = v;
+    }
+    __syncthreads();
+
+    const int bx = bx0 + (threadIdx.x & 15), by = by0 + (threadIdx.x >> 4);
+    if (bx >= nx || by >= ny) return;  // no barrier follows
+    const int rx = ref_pos(bx, w), ry = ref_pos(by, h);
+    const int lrx = rx - X0, lry = ry - Y0;
+
+    float ref[64];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ref[i * 8 + j] = tile[(lry + i) * BM_LW + lrx + j];
+
+    const float inf = __builtin_inff();
+    float bd[NMAX];
+    uint32_t bp[NMAX];
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+        bd[k] = inf;
+        bp[k] = 0u;
+    }
+    bd[0] = 0.0f;
+    bp[0] = ((uint32_t)ry << 16) | (uint32_t)rx;
+
+    for (int dy = -R; dy <= R; ++dy) {
+        const int qy = ry + dy;
+        if (qy < 0 || qy > h - K) continue;
+        const float* rowp = tile + (lry + dy) * BM_LW + lrx;
+        for (int dx0 = -R; dx0 <= R; dx0 += 4) {
+            float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                float c[11];
+#pragma unroll
+                for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    float e = ref[i * 8] - c[k];
+                    float r = e * e;
+#pragma unroll
+                    for (int j = 1; j < 8; ++j) {
+                        e = ref[i * 8 + j] - c[k + j];
+                        r = __builtin_fmaf(e, e, r);
+                    }
+                    s[k] = s[k] + r;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int dx = dx0 + k;
+                const int qx = rx + dx;
+                const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
+                if (valid && s[k] < tau && s[k] < bd[NMAX - 1])
+                    insert16(bd, bp, s[k], ((uint32_t)qy << 16) | (uint32_t)qx);
+            }
+        }
+    }
+    int n = 1;
+#pragma unroll
+    for (int k = 1; k < NMAX; ++k) n += bd[k] < inf ? 1 : 0;
+    const size_t r = (size_t)by * nx + bx;
+    uint32_t* o = pos + r * NMAX;
+#pragma unroll
+    for (int k = 0; k < NMAX; k += 4) {
+        uint4 q;
+        q.x = k + 0 < n ? bp[k + 0] : 0u;
+        q.y = k + 1 < n ? bp[k + 1] : 0u;
+        q.z = k + 2 < n ? bp[k + 2] : 0u;
+        q.w = k + 3 < n ? bp[k + 3] : 0u;
+        *reinterpret_cast<uint4*>(o + k) = q;
+    }
+    cnt[r] = (uint8_t)n;
+}
+
+// ---------------------------------------------------------------------------
+// Group transform + shrinkage + tile aggregation (fused), exact FP32.
+// One workgroup (4 waves) = one (tile, channel). Lane p = (i, j) holds pixel
+// (i, j) of every member patch of its wave's group in registers; the 2D DCT
+// passes gather the 8 row/column values with ds_bpermute; the Haar transform
+// along the stack is in-register. Groups are taken in ref raster order, four
+// at a time (one per wave), and added to the LDS num/den footprint in wave
+// order, one wave per barrier phase, so the accumulation order is fixed.
+// ---------------------------------------------------------------------------
+
+struct GroupParams {
+    const float* noisy[3];
+    const float* basic[3];
+    float sig[3];
+    int w, h, nx, ny, ntx, nty;
+    const uint32_t* pos;
+    const uint8_t* cnt;
+    const int* fx0;
+    const int* fx1;
+    const int* fy0;
+    const int* fy1;
+    float* partial;
+};
+
+__device__ __forceinline__ float dct_fwd(float x, int li, int lj, const float (&Cr)[8],
+                                         const float (&Cc)[8]) {
+    float y = Cr[0] * __shfl(x, li * 8);
+#pragma unroll
+    for (int q = 1; q < 8; ++q) y = __builtin_fmaf(Cr[q], __shfl(x, li * 8 + q), y);
+    float t = Cc[0] * __shfl(y, lj);
+#pragma unroll
+    for (int q = 1; q < 8; ++q) t = __builtin_fmaf(Cc[q], __shfl(y, q * 8 + lj), t);
+    return t;
+}
+
+__device__ __forceinline__ float dct_inv(float t, int li, int lj, const float (&Ci1)[8],
+                                         const float (&Ci2)[8]) {
+    float y = Ci1[0] * __shfl(t, lj);
+#pragma unroll
+    for (int q = 1; q < 8; ++q) y = __builtin_fmaf(Ci1[q], __shfl(t, q * 8 + lj), y);
+    float x = Ci2[0] * __shfl(y, li * 8);
+#pragma unroll
+    for (int q = 1; q < 8; ++q) x = __builtin_fmaf(Ci2[q], __shfl(y, li * 8 + q), x);
+    return x;
+}
+
+template <int N>
+__device__ __forceinline__ void haar_fwd_n(float (&v)[NMAX], float s) {
+#pragma unroll
+    for (int len = N; len > 1; len >>= 1) {
+        float tmp[NMAX];
+        const int hh = len >> 1;
+#pragma unroll
+        for (int k = 0; k < hh; ++k) {
+            const float a = v[2 * k], b = v[2 * k + 1];
+            tmp[k] = (a + b) * s;
+            tmp[hh + k] = (a - b) * s;
+        }
+#pragma unroll
+        for (int k = 0; k < len; ++k) v[k] = tmp[k];
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void haar_inv_n(float (&v)[NMAX], float s) {
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+        float tmp[NMAX];
+        const int hh = len >> 1;
+#pragma unroll
+        for (int k = 0; k < hh; ++k) {
+            const float a = v[k], d = v[hh + k];
+            tmp[2 * k] = (a + d) * s;
+            tmp[2 * k + 1] = (a - d) * s;
+        }
+#pragma unroll
+        for (int k = 0; k < len; ++k) v[k] = tmp[k];
+    }
+}
+
+__device__ __forceinline__ void haar_fwd(float (&v)[NMAX], int n, float s) {
+    switch (n) {
+        case 16: haar_fwd_n<16>(v, s); break;
+        case 8: haar_fwd_n<8>(v, s); break;
+        case 4: haar_fwd_n<4>(v, s); break;
+        case 2: haar_fwd_n<2>(v, s); break;
+        default: break;
+    }
+}
+
+__device__ __forceinline__ void haar_inv(float (&v)[NMAX], int n, float s) {
+    switch (n) {
+        case 16: haar_inv_n<16>(v, s); break;
+        case 8: haar_inv_n<8>(v, s); break;
+        case 4: haar_inv_n<4>(v, s); break;
+        case 2: haar_inv_n<2>(v, s); break;
+        default: break;
+    }
+}
+
+__device__ __forceinline__ int floor_pow2(int c) {
+    return c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+}
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_group(GroupParams p) {
+    __shared__ float acc[2 * FOOT2];
+    const int tile = blockIdx.x;
+    const int c = blockIdx.y;
+    const int tx = tile % p.ntx, ty = tile / p.ntx;
+    const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
+    const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
+    for (int idx = threadIdx.x; idx < 2 * FOOT2; idx += blockDim.x) acc[idx] = 0.0f;
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane >> 3, lj = lane & 7;
+    float Cr[8], Cc[8], Ci1[8], Ci2[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        Cr[q] = c_dct[lj * 8 + q];
+        Cc[q] = c_dct[li * 8 + q];
+        Ci1[q] = c_dct[q * 8 + li];
+        Ci2[q] = c_dct[q * 8 + lj];
+    }
+    const float kw = c_kaiser2[lane];
+    const float hs = (float)M_SQRT1_2;
+    const float* __restrict__ src = p.noisy[c];
+    const float* __restrict__ bsrc = p.basic[c];
+    const float sc = p.sig[c];
+    const float thr = 2.7f * sc;
+    const float s2 = sc * sc;
+    const int w = p.w;
+    const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TILE, p.ny - ty * TILE);
+    const int G = nbx * nby;
+
+    for (int rnd = 0; rnd < (G + 3) / 4; ++rnd) {
+        const int g = rnd * 4 + wave;
+        const bool has = g < G;
+        int n = 0;
+        float wgt = 0.0f;
+        float v[NMAX];
+        uint32_t mp[NMAX];
+        if (has) {
+            const int bx = tx * TILE + g % nbx, by = ty * TILE + g / nbx;
+            const size_t ref = (size_t)by * p.nx + bx;
+            n = floor_pow2(p.cnt[ref]);
+            const uint4* pp = reinterpret_cast<const uint4*>(p.pos + ref * NMAX);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint4 t = pp[q];
+                mp[4 * q + 0] = t.x;
+                mp[4 * q + 1] = t.y;
+                mp[4 * q + 2] = t.z;
+                mp[4 * q + 3] = t.w;
+            }
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m) {
+                v[m] = 0.0f;
+                if (m < n) {
+                    const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
+                    v[m] = src[(size_t)(qy + li) * w + qx + lj];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m)
+                if (m < n) v[m] = dct_fwd(v[m], li, lj, Cr, Cc);
+            haar_fwd(v, n, hs);
+            if (STEP == 1) {
+                int nnz = 0;
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) {
+                    if (m < n) {
+                        const bool keep = fabsf(v[m]) > thr;
+                        nnz += keep ? 1 : 0;
+                        v[m] = keep ? v[m] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
+                wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
+            } else {
+                float b[NMAX];
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) {
+                    b[m] = 0.0f;
+                    if (m < n) {
+                        const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
+                        b[m] = dct_fwd(bsrc[(size_t)(qy + li) * w + qx + lj], li, lj, Cr, Cc);
+                    }
+                }
+                haar_fwd(b, n, hs);
+                float aw = 0.0f;
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) {
+                    if (m < n) {
+                        const float bb = b[m] * b[m];
+                        const float d = bb + s2;
+                        const float W = d > 0.0f ? bb / d : 1.0f;
+                        v[m] = W * v[m];
+                        aw = (m == 0) ? W * W : __builtin_fmaf(W, W, aw);
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
+                wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
+            }
+            haar_inv(v, n, hs);
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m)
+                if (m < n) v[m] = dct_inv(v[m], li, lj, Ci1, Ci2);
+        }
+        for (int k = 0; k < 4; ++k) {
+            __syncthreads();
+            if (wave == k && has) {
+                const float wk = wgt * kw;
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) {
+                    if (m < n) {
+                        const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
+                        const int o = (qy + li - fy0) * fw + (qx + lj - fx0);
+                        __hip_atomic_fetch_add(&acc[o], wk * v[m], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&acc[FOOT2 + o], wk, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * (2 * FOOT2);
+    for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
+        dst[idx] = acc[idx];
+        dst[FOOT2 + idx] = acc[FOOT2 + idx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile merge + normalisation. Every pixel sums the partial num/den of the
+// tiles whose footprint contains it, in (ty, tx) ascending order.
+// Step 1 writes the basic estimate (opponent space); step 2 also applies the
+// inverse colour transform and writes the caller's output planes.
+// ---------------------------------------------------------------------------
+
+struct MergeParams {
+    const float* partial;
+    int w, h, ntx, nty, channels;
+    const int* txlo;
+    const int* txhi;
+    const int* tylo;
+    const int* tyhi;
+    const int* fx0;
+    const int* fx1;
+    const int* fy0;
+    float* out[3];
+    size_t out_pitch;
+    int inverse_colour;
+};
+
+__global__ void k_merge(MergeParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= p.w) return;
+    const int txl = p.txlo[x], txh = p.txhi[x], tyl = p.tylo[y], tyh = p.tyhi[y];
+    float o[3] = {0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < p.channels; ++c) {
+        float num = 0.0f, den = 0.0f;
+        for (int ty = tyl; ty <= tyh; ++ty) {
+            const int ry = y - p.fy0[ty];
+            for (int tx = txl; tx <= txh; ++tx) {
+                const int fw = p.fx1[tx] - p.fx0[tx];
+                const float* pn =
+                    p.partial + ((size_t)c * p.ntx * p.nty + (size_t)ty * p.ntx + tx) * (2 * FOOT2);
+                const int off = ry * fw + (x - p.fx0[tx]);
+                num = num + pn[off];
+                den = den + pn[FOOT2 + off];
+            }
+        }
+        o[c] = num / den;
+    }
+    const size_t oi = (size_t)y * p.out_pitch + x;
+    if (p.inverse_colour) {
+        const float c23 = (float)(2.0 / 3.0), c43 = (float)(4.0 / 3.0);
+        p.out[0][oi] = (o[0] + o[1]) + o[2] * c23;
+        p.out[1][oi] = o[0] - o[2] * c43;
+        p.out[2][oi] = (o[0] - o[1]) + o[2] * c23;
+    } else {
+        for (int c = 0; c < p.channels; ++c) p.out[c][oi] = o[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+std::mutex g_tables_mu;
+bool g_tables_ready[64] = {};
+
+int ensure_tables(int device) {
+    std::lock_guard<std::mutex> lk(g_tables_mu);
+    if (device < 0 || device >= 64) return fail(BAD_INDEX, "device ordinal out of range");
+    if (g_tables_ready[device]) return OK;
+    double c[64], k1[8];
+    for (int u = 0; u < 8; ++u) {
+        const double a = (u == 0) ? std::sqrt(1.0 / 8.0) : std::sqrt(2.0 / 8.0);
+        for (int i = 0; i < 8; ++i) c[u * 8 + i] = a * std::cos((2.0 * i + 1.0) * u * M_PI / 16.0);
+    }
+    auto i0 = [](double x) {
+        double sum = 1.0, term = 1.0;
+        for (int k = 1; k < 200; ++k) {
+            term *= (x / (2.0 * k));
+            const double t2 = term * term;
+            sum += t2;
+            if (t2 < 1e-18 * sum) break;
+        }
+        return sum;
+    };
+    for (int n = 0; n < 8; ++n) {
+        const double r = 2.0 * n / 7.0 - 1.0;
+        k1[n] = i0(2.0 * std::sqrt(1.0 - r * r)) / i0(2.0);
+    }
+    float cf[64], kf[64];
+    for (int i = 0; i < 64; ++i) cf[i] = (float)c[i];
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) kf[i * 8 + j] = (float)(k1[i] * k1[j]);
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dct), cf, sizeof(cf)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_kaiser2), kf, sizeof(kf)));
+    g_tables_ready[device] = true;
+    return OK;
+}
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+        if (prev_ != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+
+private:
+    int prev_ = -1;
+};
+
+}  // namespace
+
+struct cpipe_bm3d_plan_s {
+    int w = 0, h = 0, c = 0, precision = 0, device = 0;
+    int nx = 0, ny = 0, ntx = 0, nty = 0;
+    size_t npx = 0;
+    float* z = nullptr;        // channels planes
+    float* bz = nullptr;       // channels planes
+    float* rgb = nullptr;      // 6 planes for the RGBA16F path (in 3 + out 3)
+    uint32_t* pos[2] = {nullptr, nullptr};
+    uint8_t* cnt[2] = {nullptr, nullptr};
+    float* partial = nullptr;
+    int* geo = nullptr;  // txlo[w] txhi[w] tylo[h] tyhi[h] fx0[ntx] fx1[ntx] fy0[nty] fy1[nty]
+    bool timing = false;
+    hipEvent_t ev[CPIPE_BM3D_T_COUNT + 1] = {};
+    float ms[CPIPE_BM3D_T_COUNT] = {};
+    int runs = 0;
+    bool ran = false;
+};
+
+namespace {
+
+void tile_box_host(int t, int nref, int n, int* f0, int* f1) {
+    const int first = ref_pos(t * TILE, n);
+    const int lastref = std::min(t * TILE + TILE - 1, nref - 1);
+    const int last = ref_pos(lastref, n);
+    *f0 = std::max(0, first - R);
+    *f1 = std::min(n, last + R + K);
+}
+
+void free_plan(cpipe_bm3d_plan* p) {
+    if (!p) return;
+    DeviceGuard g(p->device);
+    (void)hipFree(p->z);
+    (void)hipFree(p->bz);
+    (void)hipFree(p->rgb);
+    for (int s = 0; s < 2; ++s) {
+        (void)hipFree(p->pos[s]);
+        (void)hipFree(p->cnt[s]);
+    }
+    (void)hipFree(p->partial);
+    (void)hipFree(p->geo);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete p;
+}
+
+int enqueue(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float* const* out,
+            size_t out_pitch, float sigma, hipStream_t st) {
+    const int w = p->w, h = p->h, C = p->c;
+    const dim3 rowgrid((w + 255) / 256, h);
+    if (w < K || h < K) {
+        for (int c = 0; c < C; ++c)
+            HIP_CHECK(hipMemcpy2DAsync(out[c], out_pitch * sizeof(float), in[c],
+                                       in_pitch * sizeof(float), (size_t)w * sizeof(float), h,
+                                       hipMemcpyDeviceToDevice, st));
+        return OK;
+    }
+    const size_t npx = p->npx;
+    float* z[3] = {p->z, p->z + npx, p->z + 2 * npx};
+    float* bz[3] = {p->bz, p->bz + npx, p->bz + 2 * npx};
+    auto mark = [&](int i) -> int {
+        if (p->timing) HIP_CHECK(hipEventRecord(p->ev[i], st));
+        return OK;
+    };
+    int rc;
+    if ((rc = mark(0))) return rc;
+    if (C == 3) {
+        k_opp_fwd<<<rowgrid, 256, 0, st>>>(in[0], in[1], in[2], in_pitch, z[0], z[1], z[2], w, h);
+    } else {
+        k_copy_plane<<<rowgrid, 256, 0, st>>>(in[0], in_pitch, z[0], (size_t)w, w);
+    }
+    HIP_CHECK(hipGetLastError());
+    if ((rc = mark(1))) return rc;
+
+    const int* txlo = p->geo;
+    const int* txhi = txlo + w;
+    const int* tylo = txhi + w;
+    const int* tyhi = tylo + h;
+    const int* fx0 = tyhi + h;
+    const int* fx1 = fx0 + p->ntx;
+    const int* fy0 = fx1 + p->ntx;
+    const int* fy1 = fy0 + p->nty;
+
+    float sig[3];
+    if (C == 1) {
+        sig[0] = sig[1] = sig[2] = sigma;
+    } else {
+        sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
+        sig[1] = sigma * (float)std::sqrt(0.5);
+        sig[2] = sigma * (float)std::sqrt(0.375);
+    }
+    for (int step = 1; step <= 2; ++step) {
+        const float* bmplane = step == 1 ? z[0] : bz[0];
+        k_block_match<<<dim3(p->ntx, p->nty), 256, 0, st>>>(bmplane, w, h, p->nx, p->ny,
+                                                            tau_ssd(step), p->pos[step - 1],
+                                                            p->cnt[step - 1]);
+        HIP_CHECK(hipGetLastError());
+        if ((rc = mark(step == 1 ? 2 : 5))) return rc;
+        GroupParams gp{};
+        for (int c = 0; c < 3; ++c) {
+            gp.noisy[c] = c < C ? z[c] : z[0];
+            gp.basic[c] = c < C ? bz[c] : bz[0];
+            gp.sig[c] = sig[c];
+        }
+        gp.w = w;
+        gp.h = h;
+        gp.nx = p->nx;
+        gp.ny = p->ny;
+        gp.ntx = p->ntx;
+        gp.nty = p->nty;
+        gp.pos = p->pos[step - 1];
+        gp.cnt = p->cnt[step - 1];
+        gp.fx0 = fx0;
+        gp.fx1 = fx1;
+        gp.fy0 = fy0;
+        gp.fy1 = fy1;
+        gp.partial = p->partial;
+        const dim3 ggrid(p->ntx * p->nty, C);
+        if (step == 1) {
+            k_group<1><<<ggrid, 256, 0, st>>>(gp);
+        } else {
+            k_group<2><<<ggrid, 256, 0, st>>>(gp);
+        }
+        HIP_CHECK(hipGetLastError());
+        if ((rc = mark(step == 1 ? 3 : 6))) return rc;
+        MergeParams mp{};
+        mp.partial = p->partial;
+        mp.w = w;
+        mp.h = h;
+        mp.ntx = p->ntx;
+        mp.nty = p->nty;
+        mp.channels = C;
+        mp.txlo = txlo;
+        mp.txhi = txhi;
+        mp.tylo = tylo;
+        mp.tyhi = tyhi;
+        mp.fx0 = fx0;
+        mp.fx1 = fx1;
+        mp.fy0 = fy0;
+        if (step == 1) {
+            for (int c = 0; c < 3; ++c) mp.out[c] = bz[c < C ? c : 0];
+            mp.out_pitch = (size_t)w;
+            mp.inverse_colour = 0;
+        } else {
+            for (int c = 0; c < 3; ++c) mp.out[c] = out[c < C ? c : 0];
+            mp.out_pitch = out_pitch;
+            mp.inverse_colour = C == 3 ? 1 : 0;
+        }
+        k_merge<<<rowgrid, 256, 0, st>>>(mp);
+        HIP_CHECK(hipGetLastError());
+        if ((rc = mark(step == 1 ? 4 : 7))) return rc;
+    }
+    return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cpipe_bm3d_last_error(void) { return g_err.c_str(); }
+
+int cpipe_bm3d_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_plan) {
+    if (!d || !out_plan) return fail(BAD_INDEX, "null argument");
+    *out_plan = nullptr;
+    if (d->struct_size < sizeof(cpipe_bm3d_plan_desc))
+        return fail(BAD_INDEX, "cpipe_bm3d_plan_desc.struct_size too small");
+    if (d->width <= 0 || d->height <= 0 || d->width > 65535 || d->height > 65535)
+        return fail(BAD_INDEX, "width/height out of range (1..65535)");
+    if (d->channels != 1 && d->channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
+    if (d->precision != CPIPE_BM3D_PRECISION_FP32)
+        return fail(UNSUPPORTED, "only CPIPE_BM3D_PRECISION_FP32 is built");
+    const int ndev = cpipe_bm3d_device_count();
+    if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
+    if (d->device < 0 || d->device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
+    DeviceGuard guard(d->device);
+    int rc = ensure_tables(d->device);
+    if (rc) return rc;
+
+    auto* p = new cpipe_bm3d_plan_s();
+    p->w = d->width;
+    p->h = d->height;
+    p->c = d->channels;
+    p->precision = d->precision;
+    p->device = d->device;
+    p->npx = (size_t)p->w * p->h;
+    if (p->w >= K && p->h >= K) {
+        p->nx = ref_count(p->w);
+        p->ny = ref_count(p->h);
+        p->ntx = (p->nx + TILE - 1) / TILE;
+        p->nty = (p->ny + TILE - 1) / TILE;
+        const size_t nref = (size_t)p->nx * p->ny;
+        auto alloc = [&](void** ptr, size_t bytes) -> int {
+            hipError_t e = hipMalloc(ptr, bytes);
+            if (e != hipSuccess) {
+                return fail(e == hipErrorOutOfMemory ? OOM : INTERNAL,
+                            std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
+            return OK;
+        };
+        const size_t ntiles = (size_t)p->ntx * p->nty;
+        const size_t ngeo = 2 * (size_t)p->w + 2 * (size_t)p->h + 2 * (size_t)p->ntx + 2 * (size_t)p->nty;
+        if ((rc = alloc((void**)&p->z, sizeof(float) * p->npx * p->c)) ||
+            (rc = alloc((void**)&p->bz, sizeof(float) * p->npx * p->c)) ||
+            (rc = alloc((void**)&p->pos[0], sizeof(uint32_t) * nref * NMAX)) ||
+            (rc = alloc((void**)&p->pos[1], sizeof(uint32_t) * nref * NMAX)) ||
+            (rc = alloc((void**)&p->cnt[0], nref)) || (rc = alloc((void**)&p->cnt[1], nref)) ||
+            (rc = alloc((void**)&p->partial, sizeof(float) * 2 * FOOT2 * ntiles * p->c)) ||
+            (rc = alloc((void**)&p->geo, sizeof(int) * ngeo))) {
+            free_plan(p);
+            return rc;
+        }
+        std::vector<int> geo(ngeo);
+        int* txlo = geo.data();
+        int* txhi = txlo + p->w;
+        int* tylo = txhi + p->w;
+        int* tyhi = tylo + p->h;
+        int* fx0 = tyhi + p->h;
+        int* fx1 = fx0 + p->ntx;
+        int* fy0 = fx1 + p->ntx;
+        int* fy1 = fy0 + p->nty;
+        for (int t = 0; t < p->ntx; ++t) tile_box_host(t, p->nx, p->w, &fx0[t], &fx1[t]);
+        for (int t = 0; t < p->nty; ++t) tile_box_host(t, p->ny, p->h, &fy0[t], &fy1[t]);
+        for (int x = 0; x < p->w; ++x) {
+            txlo[x] = -1;
+            txhi[x] = -2;
+        }
+        for (int t = 0; t < p->ntx; ++t)
+            for (int x = fx0[t]; x < fx1[t]; ++x) {
+                if (txlo[x] < 0) txlo[x] = t;
+                txhi[x] = t;
+            }
+        for (int y = 0; y < p->h; ++y) {
+            tylo[y] = -1;
+            tyhi[y] = -2;
+        }
+        for (int t = 0; t < p->nty; ++t)
+            for (int y = fy0[t]; y < fy1[t]; ++y) {
+                if (tylo[y] < 0) tylo[y] = t;
+                tyhi[y] = t;
+            }
+        for (int x = 0; x < p->w; ++x)
+            if (txlo[x] < 0) {
+                free_plan(p);
+                return fail(INTERNAL, "tile cover has a gap in x");
+            }
+        for (int y = 0; y < p->h; ++y)
+            if (tylo[y] < 0) {
+                free_plan(p);
+                return fail(INTERNAL, "tile cover has a gap in y");
+            }
+        hipError_t e = hipMemcpy(p->geo, geo.data(), sizeof(int) * ngeo, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            free_plan(p);
+            return fail(INTERNAL, std::string("hipMemcpy geo: ") + hipGetErrorString(e));
+        }
+    }
+    if (p->c == 3) {
+        hipError_t e = hipMalloc((void**)&p->rgb, sizeof(float) * p->npx * 6);
+        if (e != hipSuccess) {
+            free_plan(p);
+            return fail(OOM, std::string("hipMalloc rgb: ") + hipGetErrorString(e));
+        }
+    }
+    *out_plan = p;
+    return OK;
+}
+
+void cpipe_bm3d_plan_destroy(cpipe_bm3d_plan* plan) { free_plan(plan); }
+
+int cpipe_bm3d_plan_run(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch,
+                        float* const* out, size_t out_pitch, float sigma, void* stream) {
+    if (!p || !in || !out) return fail(BAD_INDEX, "null argument");
+    for (int c = 0; c < p->c; ++c)
+        if (!in[c] || !out[c]) return fail(BAD_INDEX, "null plane pointer");
+    if (in_pitch < (size_t)p->w || out_pitch < (size_t)p->w)
+        return fail(BAD_INDEX, "pitch smaller than width");
+    if (!(sigma >= 0.0f) || !std::isfinite(sigma)) return fail(BAD_INDEX, "sigma must be finite and >= 0");
+    DeviceGuard guard(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = enqueue(p, in, in_pitch, out, out_pitch, sigma, st);
+    if (rc) return rc;
+    p->ran = true;
+    if (p->timing && p->w >= K && p->h >= K) {
+        HIP_CHECK(hipEventSynchronize(p->ev[CPIPE_BM3D_T_COUNT]));
+        for (int i = 0; i < CPIPE_BM3D_T_COUNT; ++i) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+            p->ms[i] += ms;
+        }
+        p->runs++;
+    }
+    return OK;
+}
+
+int cpipe_bm3d_plan_run_rgba16f(cpipe_bm3d_plan* p, const uint16_t* in, size_t isp, uint16_t* out,
+                                size_t osp, float sigma, void* stream) {
+    if (!p || !in || !out) return fail(BAD_INDEX, "null argument");
+    if (p->c != 3) return fail(BAD_INDEX, "RGBA16F needs a 3-channel plan");
+    if (isp < (size_t)p->w || osp < (size_t)p->w) return fail(BAD_INDEX, "stride smaller than width");
+    DeviceGuard guard(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t npx = p->npx;
+    float* rin[3] = {p->rgb, p->rgb + npx, p->rgb + 2 * npx};
+    float* rout[3] = {p->rgb + 3 * npx, p->rgb + 4 * npx, p->rgb + 5 * npx};
+    const dim3 rowgrid((p->w + 255) / 256, p->h);
+    k_unpack_rgba16f<<<rowgrid, 256, 0, st>>>(in, isp, rin[0], rin[1], rin[2], p->w);
+    HIP_CHECK(hipGetLastError());
+    const int rc = cpipe_bm3d_plan_run(p, rin, (size_t)p->w, rout, (size_t)p->w, sigma, stream);
+    if (rc) return rc;
+    k_pack_rgba16f<<<rowgrid, 256, 0, st>>>(rout[0], rout[1], rout[2], in, isp, out, osp, p->w);
+    HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+int cpipe_bm3d_plan_basic(cpipe_bm3d_plan* p, float* const* out, size_t out_pitch, void* stream) {
+    if (!p || !out) return fail(BAD_INDEX, "null argument");
+    if (!p->ran) return fail(BAD_INDEX, "plan has not run yet");
+    if (p->w < K || p->h < K) return fail(UNSUPPORTED, "image smaller than one patch has no basic estimate");
+    DeviceGuard guard(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t npx = p->npx;
+    const dim3 rowgrid((p->w + 255) / 256, p->h);
+    if (p->c == 3) {
+        k_opp_inv<<<rowgrid, 256, 0, st>>>(p->bz, p->bz + npx, p->bz + 2 * npx, out[0], out[1],
+                                           out[2], out_pitch, p->w);
+    } else {
+        k_copy_plane<<<rowgrid, 256, 0, st>>>(p->bz, (size_t)p->w, out[0], out_pitch, p->w);
+    }
+    HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+int cpipe_bm3d_plan_matches(cpipe_bm3d_plan* p, int step, uint32_t* pos, uint8_t* count,
+                            void* stream) {
+    if (!p || !pos || !count || (step != 1 && step != 2)) return fail(BAD_INDEX, "bad argument");
+    if (!p->ran || p->nx == 0) return fail(BAD_INDEX, "no match lists");
+    DeviceGuard guard(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nref = (size_t)p->nx * p->ny;
+    HIP_CHECK(hipMemcpyAsync(pos, p->pos[step - 1], nref * NMAX * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(count, p->cnt[step - 1], nref, hipMemcpyDeviceToDevice, st));
+    return OK;
+}
+
+int cpipe_bm3d_plan_ref_grid(const cpipe_bm3d_plan* p, int* nx, int* ny) {
+    if (!p || !nx || !ny) return fail(BAD_INDEX, "null argument");
+    *nx = p->nx;
+    *ny = p->ny;
+    return OK;
+}
+
+int cpipe_bm3d_plan_set_timing(cpipe_bm3d_plan* p, int enable) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    DeviceGuard guard(p->device);
+    if (enable && !p->ev[0]) {
+        for (auto& e : p->ev) HIP_CHECK(hipEventCreate(&e));
+    }
+    p->timing = enable != 0;
+    for (auto& m : p->ms) m = 0.0f;
+    p->runs = 0;
+    return OK;
+}
+
+int cpipe_bm3d_plan_timing(cpipe_bm3d_plan* p, float* ms, int n, int* runs) {
+    if (!p || !ms) return fail(BAD_INDEX, "null argument");
+    for (int i = 0; i < n && i < CPIPE_BM3D_T_COUNT; ++i) ms[i] = p->ms[i];
+    if (runs) *runs = p->runs;
+    return OK;
+}
+
+int cpipe_bm3d_compat_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_t osp, int w,
+                              int h, float sigma, int stage, void* stream) {
+    if (!in || !out) return fail(BAD_INDEX, "null image pointer");
+    if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
+    if (isp < (size_t)w || osp < (size_t)w) return fail(BAD_INDEX, "stride smaller than width");
+    if (stage != 1 && stage != 2) return fail(BAD_INDEX, "stage must be 1 or 2");
+    const dim3 rowgrid((w + 255) / 256, h);
+    k_compat_rgba16f<<<rowgrid, 256, 0, (hipStream_t)stream>>>(in, isp, out, osp, w, h, sigma, stage);
+    HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+int cpipe_bm3d_host_planes(const float* const* in, size_t in_pitch, float* const* out,
+                           size_t out_pitch, int w, int h, int channels, float sigma, int precision,
+                           int device) {
+    if (!in || !out) return fail(BAD_INDEX, "null argument");
+    cpipe_bm3d_plan_desc d{sizeof(cpipe_bm3d_plan_desc), w, h, channels, precision, device};
+    cpipe_bm3d_plan* p = nullptr;
+    int rc = cpipe_bm3d_plan_create(&d, &p);
+    if (rc) return rc;
+    DeviceGuard guard(device);
+    float* dev = nullptr;
+    const size_t npx = (size_t)w * h;
+    hipError_t e = hipMalloc((void**)&dev, sizeof(float) * npx * 2 * channels);
+    if (e != hipSuccess) {
+        free_plan(p);
+        return fail(OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    float* din[3];
+    float* dout[3];
+    for (int c = 0; c < channels; ++c) {
+        din[c] = dev + c * npx;
+        dout[c] = dev + (channels + c) * npx;
+    }
+    rc = OK;
+    for (int c = 0; c < channels && !rc; ++c) {
+        e = hipMemcpy2D(din[c], (size_t)w * sizeof(float), in[c], in_pitch * sizeof(float),
+                        (size_t)w * sizeof(float), h, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(INTERNAL, std::string("H2D: ") + hipGetErrorString(e));
+    }
+    if (!rc) rc = cpipe_bm3d_plan_run(p, din, (size_t)w, dout, (size_t)w, sigma, nullptr);
+    for (int c = 0; c < channels && !rc; ++c) {
+        e = hipMemcpy2D(out[c], out_pitch * sizeof(float), dout[c], (size_t)w * sizeof(float),
+                        (size_t)w * sizeof(float), h, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(INTERNAL, std::string("D2H: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(dev);
+    free_plan(p);
+    return rc;
+}
+
+int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_t osp, int w, int h,
+                            float sigma, int mode, int precision, int device) {
+    if (!in || !out) return fail(BAD_INDEX, "null argument");
+    if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
+    const int ndev = cpipe_bm3d_device_count();
+    if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
+    if (device < 0 || device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
+    DeviceGuard guard(device);
+    const size_t bytes = (size_t)w * h * 8;
+    uint16_t* din = nullptr;
+    uint16_t* dout = nullptr;
+    HIP_CHECK(hipMalloc((void**)&din, bytes));
+    hipError_t e = hipMalloc((void**)&dout, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(din);
+        return fail(OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    int rc = OK;
+    e = hipMemcpy2D(din, (size_t)w * 8, in, isp * 8, (size_t)w * 8, h, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(INTERNAL, std::string("H2D: ") + hipGetErrorString(e));
+    if (!rc) {
+        if (mode == 0) {
+            rc = cpipe_bm3d_compat_rgba16f(din, (size_t)w, dout, (size_t)w, w, h, sigma, 1, nullptr);
+        } else {
+            cpipe_bm3d_plan_desc d{sizeof(cpipe_bm3d_plan_desc), w, h, 3, precision, device};
+            cpipe_bm3d_plan* p = nullptr;
+            rc = cpipe_bm3d_plan_create(&d, &p);
+            if (!rc) rc = cpipe_bm3d_plan_run_rgba16f(p, din, (size_t)w, dout, (size_t)w, sigma, nullptr);
+            if (!rc && hipDeviceSynchronize() != hipSuccess) rc = fail(INTERNAL, "device sync failed");
+            free_plan(p);
+        }
+    }
+    if (!rc) {
+        e = hipMemcpy2D(out, osp * 8, dout, (size_t)w * 8, (size_t)w * 8, h, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(INTERNAL, std::string("D2H: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

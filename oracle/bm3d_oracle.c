@@ -1,0 +1,887 @@
+/*
+ * CPU oracle for the BM3D denoise node -- TEST INFRASTRUCTURE ONLY.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this code. See bm3d_oracle.h for what each function restates.
+ *
+ * Build: oracle/Makefile (-O3 -ffp-contract=off: every fused multiply-add in
+ * the FP32 mirror is an explicit fmaf(), exactly as in the GPU kernels).
+ */
+#include "bm3d_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* binary16 conversion (IEEE 754-2008, round to nearest even)               */
+/* ------------------------------------------------------------------------ */
+
+uint16_t oracle_f32_to_f16(float v) {
+    uint32_t x;
+    memcpy(&x, &v, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t a = x & 0x7fffffffu;
+    if (a >= 0x7f800000u) { /* inf / nan */
+        if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((a >> 13) & 0x3ffu));
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    if (a < 0x38800000u) { /* below 2^-14: half subnormal or zero */
+        if (a < 0x33000000u) return (uint16_t)sign; /* < 2^-25 rounds to zero */
+        const uint32_t e = a >> 23;
+        const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - e; /* units of 2^-24 */
+        uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (q & 1u))) q++;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t q = (a - 0x38000000u) >> 13; /* rebias exponent 127 -> 15 */
+    const uint32_t rem = a & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++; /* carry may reach inf: correct */
+    if (q >= 0x7c00u) q = 0x7c00u;
+    return (uint16_t)(sign | q);
+}
+
+float oracle_f16_to_f32(uint16_t h) {
+    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    uint32_t e = ((uint32_t)h >> 10) & 0x1fu;
+    uint32_t m = (uint32_t)h & 0x3ffu;
+    uint32_t x;
+    if (e == 0) {
+        if (m == 0) {
+            x = sign;
+        } else {
+            int s = 0;
+            while (!(m & 0x400u)) {
+                m <<= 1;
+                s++;
+            }
+            m &= 0x3ffu;
+            x = sign | ((uint32_t)(113 - s) << 23) | (m << 13);
+        }
+    } else if (e == 31) {
+        x = sign | 0x7f800000u | (m << 13);
+    } else {
+        x = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference-compatible arithmetic                                          */
+/* bm3d_step1_generator.cpp:9-16 (mean2x2), :42-56 (generate);              */
+/* bm3d_step2_generator.cpp:33-42 (identity). repeat_edge clamps x/y.       */
+/* ------------------------------------------------------------------------ */
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void oracle_compat_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
+                           size_t out_stride_px, int width, int height, float sigma, int stage) {
+    const float strength = fminf(1.0f, fmaxf(0.0f, sigma / 0.03f));
+    for (int y = 0; y < height; ++y) {
+        for (int x = 0; x < width; ++x) {
+            for (int c = 0; c < 4; ++c) {
+                const float v = oracle_f16_to_f32(in[(size_t)y * in_stride_px * 4 + (size_t)x * 4 + (size_t)c]);
+                float r;
+                if (stage == 2) {
+                    r = v + sigma * 0.0f;
+                } else if (c == 3) {
+                    r = v;
+                } else {
+                    const int bx = (x / 2) * 2, by = (y / 2) * 2;
+                    const int x0 = clampi(bx, 0, width - 1), x1 = clampi(bx + 1, 0, width - 1);
+                    const int y0 = clampi(by, 0, height - 1), y1 = clampi(by + 1, 0, height - 1);
+#define V(xx, yy) oracle_f16_to_f32(in[(size_t)(yy) * in_stride_px * 4 + (size_t)(xx) * 4 + (size_t)c])
+                    const float basic = (((V(x0, y0) + V(x1, y0)) + V(x0, y1)) + V(x1, y1)) * 0.25f;
+#undef V
+                    const float a = v * (1.0f - strength);
+                    const float b = basic * strength;
+                    const float den = a + b;
+                    r = fmaxf(0.0f, den);
+                }
+                out[(size_t)y * out_stride_px * 4 + (size_t)x * 4 + (size_t)c] = oracle_f32_to_f16(r);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shared geometry and constant tables                                      */
+/* ------------------------------------------------------------------------ */
+
+int oracle_ref_count(int n) {
+    if (n < ORC_K) return 0;
+    const int last = n - ORC_K;
+    return (last + ORC_STRIDE - 1) / ORC_STRIDE + 1;
+}
+
+int oracle_ref_pos(int i, int n) {
+    const int last = n - ORC_K;
+    const int p = i * ORC_STRIDE;
+    return p < last ? p : last;
+}
+
+void oracle_dct_matrix_f64(double c[64]) {
+    for (int u = 0; u < 8; ++u) {
+        const double a = (u == 0) ? sqrt(1.0 / 8.0) : sqrt(2.0 / 8.0);
+        for (int i = 0; i < 8; ++i) c[u * 8 + i] = a * cos((2.0 * i + 1.0) * u * M_PI / 16.0);
+    }
+}
+
+static double bessel_i0(double x) {
+    /* I0(x) = sum_k ((x/2)^k / k!)^2 */
+    double sum = 1.0, term = 1.0;
+    for (int k = 1; k < 200; ++k) {
+        term *= (x / (2.0 * k));
+        const double t2 = term * term;
+        sum += t2;
+        if (t2 < 1e-18 * sum) break;
+    }
+    return sum;
+}
+
+void oracle_kaiser_f64(double k[8]) {
+    const double beta = 2.0;
+    for (int n = 0; n < 8; ++n) {
+        const double r = 2.0 * n / 7.0 - 1.0;
+        k[n] = bessel_i0(beta * sqrt(1.0 - r * r)) / bessel_i0(beta);
+    }
+}
+
+typedef struct {
+    float C[64];   /* DCT matrix, row = frequency */
+    float K2[64];  /* Kaiser 8x8 outer product */
+    float haar_s;  /* 1/sqrt(2) */
+} tables_f32;
+
+static void make_tables_f32(tables_f32* t) {
+    double c[64], k[8];
+    oracle_dct_matrix_f64(c);
+    oracle_kaiser_f64(k);
+    for (int i = 0; i < 64; ++i) t->C[i] = (float)c[i];
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) t->K2[i * 8 + j] = (float)(k[i] * k[j]);
+    t->haar_s = (float)M_SQRT1_2;
+}
+
+/* Thresholds on the (un-normalised) SSD over 64 pixels: tau * k^2. */
+static const double kTauHt = 2500.0 / (255.0 * 255.0);
+static const double kTauWie = 400.0 / (255.0 * 255.0);
+
+/* ------------------------------------------------------------------------ */
+/* Thread pool helper                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*job_fn)(void* ctx, int job);
+typedef struct {
+    job_fn fn;
+    void* ctx;
+    int njobs;
+    int next;
+    pthread_mutex_t mu;
+} pool_t;
+
+static void* pool_worker(void* arg) {
+    pool_t* p = (pool_t*)arg;
+    for (;;) {
+        pthread_mutex_lock(&p->mu);
+        const int j = p->next++;
+        pthread_mutex_unlock(&p->mu);
+        if (j >= p->njobs) break;
+        p->fn(p->ctx, j);
+    }
+    return NULL;
+}
+
+static void run_jobs(job_fn fn, void* ctx, int njobs, int nthreads) {
+    if (nthreads <= 1 || njobs <= 1) {
+        for (int j = 0; j < njobs; ++j) fn(ctx, j);
+        return;
+    }
+    if (nthreads > 256) nthreads = 256;
+    pool_t p = {fn, ctx, njobs, 0, PTHREAD_MUTEX_INITIALIZER};
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, pool_worker, &p);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&p.mu);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 mirror: block matching                                              */
+/* SSD = sum_i rowsum_i, rowsum_i = fma chain over j of (r-c)^2, i ascending */
+/* Candidates scanned in (dy, dx) raster order over the clipped 39x39       */
+/* window; the reference is pinned first; insertion is strict (<), so equal */
+/* distances keep raster order. Only SSD < tau enters.                      */
+/* ------------------------------------------------------------------------ */
+
+static inline float ssd_f32(const float* ref, const float* b, int bs) {
+    float s = 0.0f;
+    for (int i = 0; i < 8; ++i) {
+        const float* br = b + (size_t)i * bs;
+        const float* ar = ref + i * 8;
+        float e = ar[0] - br[0];
+        float r = e * e;
+        for (int j = 1; j < 8; ++j) {
+            e = ar[j] - br[j];
+            r = fmaf(e, e, r);
+        }
+        s = s + r;
+    }
+    return s;
+}
+
+static void bm_ref_f32(const float* p, int w, int h, int rx, int ry, float tau, uint32_t* pos,
+                       uint8_t* count) {
+    float ref[64];
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) ref[i * 8 + j] = p[(size_t)(ry + i) * w + rx + j];
+    float bd[ORC_NMAX];
+    uint32_t bp[ORC_NMAX];
+    int n = 1;
+    bd[0] = 0.0f;
+    bp[0] = ((uint32_t)ry << 16) | (uint32_t)rx;
+    const int y0 = ry - ORC_R < 0 ? 0 : ry - ORC_R;
+    const int y1 = ry + ORC_R > h - ORC_K ? h - ORC_K : ry + ORC_R;
+    const int x0 = rx - ORC_R < 0 ? 0 : rx - ORC_R;
+    const int x1 = rx + ORC_R > w - ORC_K ? w - ORC_K : rx + ORC_R;
+    for (int qy = y0; qy <= y1; ++qy) {
+        for (int qx = x0; qx <= x1; ++qx) {
+            if (qx == rx && qy == ry) continue;
+            const float s = ssd_f32(ref, p + (size_t)qy * w + qx, w);
+            if (!(s < tau)) continue;
+            if (n == ORC_NMAX && !(s < bd[ORC_NMAX - 1])) continue;
+            int k = n < ORC_NMAX ? n : ORC_NMAX - 1;
+            while (k > 1 && s < bd[k - 1]) {
+                bd[k] = bd[k - 1];
+                bp[k] = bp[k - 1];
+                --k;
+            }
+            bd[k] = s;
+            bp[k] = ((uint32_t)qy << 16) | (uint32_t)qx;
+            if (n < ORC_NMAX) ++n;
+        }
+    }
+    for (int k = 0; k < ORC_NMAX; ++k) pos[k] = k < n ? bp[k] : 0u;
+    *count = (uint8_t)n;
+}
+
+typedef struct {
+    const float* p;
+    int w, h, nx, ny;
+    float tau;
+    uint32_t* pos;
+    uint8_t* count;
+} bm_ctx_f32;
+
+static void bm_row_job(void* c, int by) {
+    bm_ctx_f32* x = (bm_ctx_f32*)c;
+    const int ry = oracle_ref_pos(by, x->h);
+    for (int bx = 0; bx < x->nx; ++bx) {
+        const size_t r = (size_t)by * x->nx + bx;
+        bm_ref_f32(x->p, x->w, x->h, oracle_ref_pos(bx, x->w), ry, x->tau, x->pos + r * ORC_NMAX,
+                   x->count + r);
+    }
+}
+
+void oracle_bm_f32(const float* plane, int width, int height, float tau_ssd, uint32_t* pos,
+                   uint8_t* count, int nthreads) {
+    bm_ctx_f32 c = {plane, width, height, oracle_ref_count(width), oracle_ref_count(height),
+                    tau_ssd, pos, count};
+    run_jobs(bm_row_job, &c, c.ny, nthreads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 mirror: 3D transforms                                               */
+/* ------------------------------------------------------------------------ */
+
+static void dct2_fwd_f32(const float* C, const float* x, float* t) {
+    float y[64];
+    for (int i = 0; i < 8; ++i)
+        for (int v = 0; v < 8; ++v) {
+            float acc = C[v * 8] * x[i * 8];
+            for (int j = 1; j < 8; ++j) acc = fmaf(C[v * 8 + j], x[i * 8 + j], acc);
+            y[i * 8 + v] = acc;
+        }
+    for (int u = 0; u < 8; ++u)
+        for (int v = 0; v < 8; ++v) {
+            float acc = C[u * 8] * y[v];
+            for (int i = 1; i < 8; ++i) acc = fmaf(C[u * 8 + i], y[i * 8 + v], acc);
+            t[u * 8 + v] = acc;
+        }
+}
+
+static void dct2_inv_f32(const float* C, const float* t, float* x) {
+    float y[64];
+    for (int i = 0; i < 8; ++i)
+        for (int v = 0; v < 8; ++v) {
+            float acc = C[i] * t[v];
+            for (int u = 1; u < 8; ++u) acc = fmaf(C[u * 8 + i], t[u * 8 + v], acc);
+            y[i * 8 + v] = acc;
+        }
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+            float acc = C[j] * y[i * 8];
+            for (int v = 1; v < 8; ++v) acc = fmaf(C[v * 8 + j], y[i * 8 + v], acc);
+            x[i * 8 + j] = acc;
+        }
+}
+
+/* Orthonormal multi-level Haar along the stack; output [approx, d_L..d_1]. */
+static void haar_fwd_f32(float* v, int n, float s) {
+    float tmp[ORC_NMAX];
+    for (int len = n; len > 1; len >>= 1) {
+        const int h = len >> 1;
+        for (int k = 0; k < h; ++k) {
+            const float a = v[2 * k], b = v[2 * k + 1];
+            tmp[k] = (a + b) * s;
+            tmp[h + k] = (a - b) * s;
+        }
+        for (int k = 0; k < len; ++k) v[k] = tmp[k];
+    }
+}
+
+static void haar_inv_f32(float* v, int n, float s) {
+    float tmp[ORC_NMAX];
+    for (int len = 2; len <= n; len <<= 1) {
+        const int h = len >> 1;
+        for (int k = 0; k < h; ++k) {
+            const float a = v[k], d = v[h + k];
+            tmp[2 * k] = (a + d) * s;
+            tmp[2 * k + 1] = (a - d) * s;
+        }
+        for (int k = 0; k < len; ++k) v[k] = tmp[k];
+    }
+}
+
+static inline int floor_pow2(int c) {
+    int n = 1;
+    while (n * 2 <= c) n *= 2;
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 mirror: tiles, groups, aggregation, merge                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int w, h, nx, ny, tx, ty;
+    const tables_f32* tb;
+    const float* noisy[3];
+    const float* basic[3]; /* step 2 only */
+    float sig[3];          /* per-channel sigma */
+    int channels;
+    int step;
+    const uint32_t* pos;
+    const uint8_t* count;
+    float* partial; /* [c][tile][2][91*91] */
+    size_t tile_stride;
+} agg_ctx_f32;
+
+#define FOOT_MAX (15 * ORC_STRIDE + 2 * ORC_R + ORC_K) /* 91 */
+
+static void tile_box(int t, int nref, int n, int* f0, int* f1) {
+    const int first = oracle_ref_pos(t * ORC_TILE, n);
+    int lastref = t * ORC_TILE + ORC_TILE - 1;
+    if (lastref > nref - 1) lastref = nref - 1;
+    const int last = oracle_ref_pos(lastref, n);
+    *f0 = first - ORC_R < 0 ? 0 : first - ORC_R;
+    *f1 = last + ORC_R + ORC_K > n ? n : last + ORC_R + ORC_K;
+}
+
+static void group_job_f32(void* vc, int job) {
+    agg_ctx_f32* a = (agg_ctx_f32*)vc;
+    const int c = job % a->channels;
+    const int tile = job / a->channels;
+    const int tx = tile % a->tx, ty = tile / a->tx;
+    int fx0, fx1, fy0, fy1;
+    tile_box(tx, a->nx, a->w, &fx0, &fx1);
+    tile_box(ty, a->ny, a->h, &fy0, &fy1);
+    const int fw = fx1 - fx0;
+    float* num = a->partial + ((size_t)c * a->tx * a->ty + tile) * a->tile_stride;
+    float* den = num + FOOT_MAX * FOOT_MAX;
+    memset(num, 0, sizeof(float) * 2 * FOOT_MAX * FOOT_MAX);
+    const float* C = a->tb->C;
+    const float s = a->tb->haar_s;
+    const float sc = a->sig[c];
+    const float thr = 2.7f * sc;
+    const float s2 = sc * sc;
+    float g[ORC_NMAX][64], gb[ORC_NMAX][64];
+    for (int by = ty * ORC_TILE; by < ty * ORC_TILE + ORC_TILE && by < a->ny; ++by) {
+        for (int bx = tx * ORC_TILE; bx < tx * ORC_TILE + ORC_TILE && bx < a->nx; ++bx) {
+            const size_t r = (size_t)by * a->nx + bx;
+            const uint32_t* pos = a->pos + r * ORC_NMAX;
+            const int n = floor_pow2(a->count[r]);
+            for (int m = 0; m < n; ++m) {
+                const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);
+                float tmp[64];
+                for (int i = 0; i < 8; ++i)
+                    for (int j = 0; j < 8; ++j)
+                        tmp[i * 8 + j] = a->noisy[c][(size_t)(qy + i) * a->w + qx + j];
+                dct2_fwd_f32(C, tmp, g[m]);
+                if (a->step == 2) {
+                    for (int i = 0; i < 8; ++i)
+                        for (int j = 0; j < 8; ++j)
+                            tmp[i * 8 + j] = a->basic[c][(size_t)(qy + i) * a->w + qx + j];
+                    dct2_fwd_f32(C, tmp, gb[m]);
+                }
+            }
+            float wgt;
+            if (a->step == 1) {
+                int nnz = 0;
+                for (int p = 0; p < 64; ++p) {
+                    float v[ORC_NMAX];
+                    for (int m = 0; m < n; ++m) v[m] = g[m][p];
+                    haar_fwd_f32(v, n, s);
+                    for (int m = 0; m < n; ++m) {
+                        if (fabsf(v[m]) > thr) {
+                            ++nnz;
+                        } else {
+                            v[m] = 0.0f;
+                        }
+                    }
+                    haar_inv_f32(v, n, s);
+                    for (int m = 0; m < n; ++m) g[m][p] = v[m];
+                }
+                wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
+            } else {
+                float accw[64];
+                for (int p = 0; p < 64; ++p) {
+                    float v[ORC_NMAX], b[ORC_NMAX];
+                    for (int m = 0; m < n; ++m) {
+                        v[m] = g[m][p];
+                        b[m] = gb[m][p];
+                    }
+                    haar_fwd_f32(v, n, s);
+                    haar_fwd_f32(b, n, s);
+                    float acc = 0.0f;
+                    for (int m = 0; m < n; ++m) {
+                        const float bb = b[m] * b[m];
+                        const float d = bb + s2;
+                        const float W = d > 0.0f ? bb / d : 1.0f;
+                        v[m] = W * v[m];
+                        acc = (m == 0) ? W * W : fmaf(W, W, acc);
+                    }
+                    accw[p] = acc;
+                    haar_inv_f32(v, n, s);
+                    for (int m = 0; m < n; ++m) g[m][p] = v[m];
+                }
+                for (int off = 32; off >= 1; off >>= 1)
+                    for (int p = 0; p < off; ++p) accw[p] = accw[p] + accw[p + off];
+                wgt = accw[0] > 0.0f ? 1.0f / accw[0] : 1.0f;
+            }
+            for (int m = 0; m < n; ++m) {
+                const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);
+                float out[64];
+                dct2_inv_f32(C, g[m], out);
+                for (int i = 0; i < 8; ++i)
+                    for (int j = 0; j < 8; ++j) {
+                        const int p = i * 8 + j;
+                        const size_t o = (size_t)(qy + i - fy0) * fw + (size_t)(qx + j - fx0);
+                        const float wk = wgt * a->tb->K2[p];
+                        num[o] = num[o] + wk * out[p];
+                        den[o] = den[o] + wk;
+                    }
+            }
+        }
+    }
+}
+
+typedef struct {
+    const agg_ctx_f32* a;
+    int* txlo;
+    int* txhi;
+    int* tylo;
+    int* tyhi;
+    float* out[3];
+} merge_ctx_f32;
+
+static void merge_row_job(void* vc, int y) {
+    merge_ctx_f32* mc = (merge_ctx_f32*)vc;
+    const agg_ctx_f32* a = mc->a;
+    for (int c = 0; c < a->channels; ++c) {
+        for (int x = 0; x < a->w; ++x) {
+            float num = 0.0f, den = 0.0f;
+            for (int ty = mc->tylo[y]; ty <= mc->tyhi[y]; ++ty) {
+                int fy0, fy1;
+                tile_box(ty, a->ny, a->h, &fy0, &fy1);
+                for (int tx = mc->txlo[x]; tx <= mc->txhi[x]; ++tx) {
+                    int fx0, fx1;
+                    tile_box(tx, a->nx, a->w, &fx0, &fx1);
+                    const int fw = fx1 - fx0;
+                    const float* pn =
+                        a->partial + ((size_t)c * a->tx * a->ty + (size_t)ty * a->tx + tx) * a->tile_stride;
+                    const size_t o = (size_t)(y - fy0) * fw + (size_t)(x - fx0);
+                    num = num + pn[o];
+                    den = den + pn[FOOT_MAX * FOOT_MAX + o];
+                }
+            }
+            mc->out[c][(size_t)y * a->w + x] = num / den;
+        }
+    }
+}
+
+static void tile_cover(int ntile, int nref, int n, int* lo, int* hi) {
+    for (int x = 0; x < n; ++x) {
+        lo[x] = -1;
+        hi[x] = -2;
+    }
+    for (int t = 0; t < ntile; ++t) {
+        int f0, f1;
+        tile_box(t, nref, n, &f0, &f1);
+        for (int x = f0; x < f1; ++x) {
+            if (lo[x] < 0) lo[x] = t;
+            hi[x] = t;
+        }
+    }
+}
+
+static void opp_fwd_f32(const float* const* in, float* const* z, size_t npx) {
+    const float c13 = (float)(1.0 / 3.0);
+    for (size_t i = 0; i < npx; ++i) {
+        const float r = in[0][i], g = in[1][i], b = in[2][i];
+        z[0][i] = ((r + g) + b) * c13;
+        z[1][i] = (r - b) * 0.5f;
+        z[2][i] = ((r + b) - (g + g)) * 0.25f;
+    }
+}
+
+static void opp_inv_f32(const float* const* z, float* const* out, size_t npx) {
+    const float c23 = (float)(2.0 / 3.0), c43 = (float)(4.0 / 3.0);
+    for (size_t i = 0; i < npx; ++i) {
+        const float y = z[0][i], u = z[1][i], v = z[2][i];
+        out[0][i] = (y + u) + v * c23;
+        out[1][i] = y - v * c43;
+        out[2][i] = (y - u) + v * c23;
+    }
+}
+
+static void channel_sigmas(int channels, float sigma, float* sig) {
+    if (channels == 1) {
+        sig[0] = sigma;
+        return;
+    }
+    sig[0] = sigma * (float)sqrt(1.0 / 3.0);
+    sig[1] = sigma * (float)sqrt(0.5);
+    sig[2] = sigma * (float)sqrt(0.375);
+}
+
+static void aggregate_step_f32(agg_ctx_f32* a, float* const* outp, int nthreads, int* txlo,
+                               int* txhi, int* tylo, int* tyhi) {
+    run_jobs(group_job_f32, a, a->tx * a->ty * a->channels, nthreads);
+    merge_ctx_f32 mc = {a, txlo, txhi, tylo, tyhi, {outp[0], outp[1], outp[2]}};
+    run_jobs(merge_row_job, &mc, a->h, nthreads);
+}
+
+int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* basic, int width,
+                    int height, int channels, float sigma, int nthreads, uint32_t* matches1,
+                    uint8_t* counts1, uint32_t* matches2, uint8_t* counts2) {
+    if ((channels != 1 && channels != 3) || width <= 0 || height <= 0) return 1;
+    const size_t npx = (size_t)width * height;
+    if (width < ORC_K || height < ORC_K) {
+        for (int c = 0; c < channels; ++c) {
+            memcpy(out[c], in[c], npx * sizeof(float));
+            if (basic) memcpy(basic[c], in[c], npx * sizeof(float));
+        }
+        return 0;
+    }
+    tables_f32 tb;
+    make_tables_f32(&tb);
+    const int nx = oracle_ref_count(width), ny = oracle_ref_count(height);
+    const size_t nref = (size_t)nx * ny;
+    const int tx = (nx + ORC_TILE - 1) / ORC_TILE, ty = (ny + ORC_TILE - 1) / ORC_TILE;
+
+    float* zbuf = (float*)malloc(sizeof(float) * npx * 3 * 3);
+    float* z[3] = {zbuf, zbuf + npx, zbuf + 2 * npx};      /* noisy, opponent space */
+    float* bz[3] = {zbuf + 3 * npx, zbuf + 4 * npx, zbuf + 5 * npx}; /* basic */
+    float* fz[3] = {zbuf + 6 * npx, zbuf + 7 * npx, zbuf + 8 * npx}; /* final */
+    uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * nref * ORC_NMAX);
+    uint8_t* cnt = (uint8_t*)malloc(nref);
+    const size_t tile_stride = 2 * FOOT_MAX * FOOT_MAX;
+    float* partial = (float*)malloc(sizeof(float) * tile_stride * (size_t)tx * ty * channels);
+    int* cover = (int*)malloc(sizeof(int) * 2 * (size_t)(width + height));
+    int *txlo = cover, *txhi = cover + width, *tylo = cover + 2 * width, *tyhi = tylo + height;
+    if (!zbuf || !pos || !cnt || !partial || !cover) {
+        free(zbuf); free(pos); free(cnt); free(partial); free(cover);
+        return 3;
+    }
+    tile_cover(tx, nx, width, txlo, txhi);
+    tile_cover(ty, ny, height, tylo, tyhi);
+
+    if (channels == 3) {
+        opp_fwd_f32(in, z, npx);
+    } else {
+        memcpy(z[0], in[0], npx * sizeof(float));
+    }
+    float sig[3] = {0.0f, 0.0f, 0.0f};
+    channel_sigmas(channels, sigma, sig);
+
+    /* step 1 */
+    oracle_bm_f32(z[0], width, height, (float)(64.0 * kTauHt), pos, cnt, nthreads);
+    if (matches1) memcpy(matches1, pos, sizeof(uint32_t) * nref * ORC_NMAX);
+    if (counts1) memcpy(counts1, cnt, nref);
+    agg_ctx_f32 a = {width, height, nx, ny, tx, ty, &tb, {z[0], z[1], z[2]}, {0, 0, 0},
+                     {sig[0], sig[1], sig[2]}, channels, 1, pos, cnt, partial, tile_stride};
+    aggregate_step_f32(&a, bz, nthreads, txlo, txhi, tylo, tyhi);
+    if (basic) {
+        if (channels == 3) {
+            opp_inv_f32((const float* const*)bz, basic, npx);
+        } else {
+            memcpy(basic[0], bz[0], npx * sizeof(float));
+        }
+    }
+
+    /* step 2 */
+    oracle_bm_f32(bz[0], width, height, (float)(64.0 * kTauWie), pos, cnt, nthreads);
+    if (matches2) memcpy(matches2, pos, sizeof(uint32_t) * nref * ORC_NMAX);
+    if (counts2) memcpy(counts2, cnt, nref);
+    a.step = 2;
+    for (int c = 0; c < 3; ++c) a.basic[c] = bz[c];
+    aggregate_step_f32(&a, fz, nthreads, txlo, txhi, tylo, tyhi);
+    if (channels == 3) {
+        opp_inv_f32((const float* const*)fz, out, npx);
+    } else {
+        memcpy(out[0], fz[0], npx * sizeof(float));
+    }
+    free(zbuf); free(pos); free(cnt); free(partial); free(cover);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 textbook BM3D: direct loops, global accumulation in ref raster      */
+/* order, textbook weights 1/(sigma^2 N) and 1/(sigma^2 sum W^2).           */
+/* ------------------------------------------------------------------------ */
+
+static void bm_ref_f64(const double* p, int w, int h, int rx, int ry, double tau, uint32_t* pos,
+                       uint8_t* count) {
+    double bd[ORC_NMAX];
+    uint32_t bp[ORC_NMAX];
+    int n = 1;
+    bd[0] = 0.0;
+    bp[0] = ((uint32_t)ry << 16) | (uint32_t)rx;
+    for (int qy = ry - ORC_R; qy <= ry + ORC_R; ++qy) {
+        if (qy < 0 || qy > h - ORC_K) continue;
+        for (int qx = rx - ORC_R; qx <= rx + ORC_R; ++qx) {
+            if (qx < 0 || qx > w - ORC_K) continue;
+            if (qx == rx && qy == ry) continue;
+            double s = 0.0;
+            for (int i = 0; i < 8; ++i)
+                for (int j = 0; j < 8; ++j) {
+                    const double e = p[(size_t)(ry + i) * w + rx + j] - p[(size_t)(qy + i) * w + qx + j];
+                    s += e * e;
+                }
+            if (!(s < tau)) continue;
+            if (n == ORC_NMAX && !(s < bd[ORC_NMAX - 1])) continue;
+            int k = n < ORC_NMAX ? n : ORC_NMAX - 1;
+            while (k > 1 && s < bd[k - 1]) {
+                bd[k] = bd[k - 1];
+                bp[k] = bp[k - 1];
+                --k;
+            }
+            bd[k] = s;
+            bp[k] = ((uint32_t)qy << 16) | (uint32_t)qx;
+            if (n < ORC_NMAX) ++n;
+        }
+    }
+    for (int k = 0; k < ORC_NMAX; ++k) pos[k] = k < n ? bp[k] : 0u;
+    *count = (uint8_t)n;
+}
+
+void oracle_bm_f64(const double* plane, int width, int height, double tau_ssd, uint32_t* pos,
+                   uint8_t* count) {
+    const int nx = oracle_ref_count(width), ny = oracle_ref_count(height);
+    for (int by = 0; by < ny; ++by)
+        for (int bx = 0; bx < nx; ++bx) {
+            const size_t r = (size_t)by * nx + bx;
+            bm_ref_f64(plane, width, height, oracle_ref_pos(bx, width), oracle_ref_pos(by, height),
+                       tau_ssd, pos + r * ORC_NMAX, count + r);
+        }
+}
+
+static void haar_fwd_f64(double* v, int n) {
+    const double s = M_SQRT1_2;
+    double tmp[ORC_NMAX];
+    for (int len = n; len > 1; len >>= 1) {
+        const int h = len >> 1;
+        for (int k = 0; k < h; ++k) {
+            tmp[k] = (v[2 * k] + v[2 * k + 1]) * s;
+            tmp[h + k] = (v[2 * k] - v[2 * k + 1]) * s;
+        }
+        memcpy(v, tmp, sizeof(double) * len);
+    }
+}
+
+static void haar_inv_f64(double* v, int n) {
+    const double s = M_SQRT1_2;
+    double tmp[ORC_NMAX];
+    for (int len = 2; len <= n; len <<= 1) {
+        const int h = len >> 1;
+        for (int k = 0; k < h; ++k) {
+            tmp[2 * k] = (v[k] + v[h + k]) * s;
+            tmp[2 * k + 1] = (v[k] - v[h + k]) * s;
+        }
+        memcpy(v, tmp, sizeof(double) * len);
+    }
+}
+
+static void dct2_f64(const double* C, const double* x, double* t, int inverse) {
+    /* forward: t = C x C^T ; inverse: x = C^T t C */
+    for (int u = 0; u < 8; ++u)
+        for (int v = 0; v < 8; ++v) {
+            double acc = 0.0;
+            for (int i = 0; i < 8; ++i)
+                for (int j = 0; j < 8; ++j)
+                    acc += inverse ? C[i * 8 + u] * C[j * 8 + v] * x[i * 8 + j]
+                                   : C[u * 8 + i] * C[v * 8 + j] * x[i * 8 + j];
+            t[u * 8 + v] = acc;
+        }
+}
+
+static void step_f64(const double* const* z, const double* const* bz, double* const* outp,
+                     int w, int h, int channels, const double* sig, int step, const uint32_t* posall,
+                     const uint8_t* cntall) {
+    double C[64], k1[8];
+    oracle_dct_matrix_f64(C);
+    oracle_kaiser_f64(k1);
+    const int nx = oracle_ref_count(w), ny = oracle_ref_count(h);
+    const size_t npx = (size_t)w * h;
+    double* acc = (double*)calloc(2 * npx, sizeof(double));
+    double(*g)[64] = malloc(sizeof(double) * 64 * ORC_NMAX);
+    double(*gb)[64] = malloc(sizeof(double) * 64 * ORC_NMAX);
+    for (int c = 0; c < channels; ++c) {
+        memset(acc, 0, 2 * npx * sizeof(double));
+        const double sc = sig[c];
+        for (size_t r = 0; r < (size_t)nx * ny; ++r) {
+            const uint32_t* pos = posall + r * ORC_NMAX;
+            const int n = floor_pow2(cntall[r]);
+            for (int m = 0; m < n; ++m) {
+                const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);
+                double tmp[64];
+                for (int i = 0; i < 8; ++i)
+                    for (int j = 0; j < 8; ++j) tmp[i * 8 + j] = z[c][(size_t)(qy + i) * w + qx + j];
+                dct2_f64(C, tmp, g[m], 0);
+                if (step == 2) {
+                    for (int i = 0; i < 8; ++i)
+                        for (int j = 0; j < 8; ++j) tmp[i * 8 + j] = bz[c][(size_t)(qy + i) * w + qx + j];
+                    dct2_f64(C, tmp, gb[m], 0);
+                }
+            }
+            double wgt;
+            if (step == 1) {
+                int nnz = 0;
+                for (int p = 0; p < 64; ++p) {
+                    double v[ORC_NMAX];
+                    for (int m = 0; m < n; ++m) v[m] = g[m][p];
+                    haar_fwd_f64(v, n);
+                    for (int m = 0; m < n; ++m) {
+                        if (fabs(v[m]) > 2.7 * sc) ++nnz;
+                        else v[m] = 0.0;
+                    }
+                    haar_inv_f64(v, n);
+                    for (int m = 0; m < n; ++m) g[m][p] = v[m];
+                }
+                const double s2 = sc > 0.0 ? sc * sc : 1.0;
+                wgt = nnz > 0 ? 1.0 / (s2 * nnz) : 1.0;
+            } else {
+                double sw = 0.0;
+                for (int p = 0; p < 64; ++p) {
+                    double v[ORC_NMAX], b[ORC_NMAX];
+                    for (int m = 0; m < n; ++m) {
+                        v[m] = g[m][p];
+                        b[m] = gb[m][p];
+                    }
+                    haar_fwd_f64(v, n);
+                    haar_fwd_f64(b, n);
+                    for (int m = 0; m < n; ++m) {
+                        const double d = b[m] * b[m] + sc * sc;
+                        const double W = d > 0.0 ? b[m] * b[m] / d : 1.0;
+                        v[m] *= W;
+                        sw += W * W;
+                    }
+                    haar_inv_f64(v, n);
+                    for (int m = 0; m < n; ++m) g[m][p] = v[m];
+                }
+                const double s2 = sc > 0.0 ? sc * sc : 1.0;
+                wgt = sw > 0.0 ? 1.0 / (s2 * sw) : 1.0;
+            }
+            for (int m = 0; m < n; ++m) {
+                const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);
+                double out[64];
+                dct2_f64(C, g[m], out, 1);
+                for (int i = 0; i < 8; ++i)
+                    for (int j = 0; j < 8; ++j) {
+                        const size_t o = (size_t)(qy + i) * w + qx + j;
+                        const double wk = wgt * k1[i] * k1[j];
+                        acc[o] += wk * out[i * 8 + j];
+                        acc[npx + o] += wk;
+                    }
+            }
+        }
+        for (size_t i = 0; i < npx; ++i) outp[c][i] = acc[i] / acc[npx + i];
+    }
+    free(acc);
+    free(g);
+    free(gb);
+}
+
+int oracle_bm3d_f64(const double* const* in, double* const* out, double* const* basic,
+                    int width, int height, int channels, double sigma) {
+    if ((channels != 1 && channels != 3) || width <= 0 || height <= 0) return 1;
+    const size_t npx = (size_t)width * height;
+    if (width < ORC_K || height < ORC_K) {
+        for (int c = 0; c < channels; ++c) {
+            memcpy(out[c], in[c], npx * sizeof(double));
+            if (basic) memcpy(basic[c], in[c], npx * sizeof(double));
+        }
+        return 0;
+    }
+    double* buf = (double*)malloc(sizeof(double) * npx * 9);
+    double* z[3] = {buf, buf + npx, buf + 2 * npx};
+    double* bz[3] = {buf + 3 * npx, buf + 4 * npx, buf + 5 * npx};
+    double* fz[3] = {buf + 6 * npx, buf + 7 * npx, buf + 8 * npx};
+    const int nx = oracle_ref_count(width), ny = oracle_ref_count(height);
+    uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)nx * ny * ORC_NMAX);
+    uint8_t* cnt = (uint8_t*)malloc((size_t)nx * ny);
+    double sig[3] = {0.0, 0.0, 0.0};
+    if (channels == 3) {
+        for (size_t i = 0; i < npx; ++i) {
+            const double r = in[0][i], g = in[1][i], b = in[2][i];
+            z[0][i] = (r + g + b) / 3.0;
+            z[1][i] = (r - b) / 2.0;
+            z[2][i] = (r - 2.0 * g + b) / 4.0;
+        }
+        sig[0] = sigma * sqrt(1.0 / 3.0);
+        sig[1] = sigma * sqrt(0.5);
+        sig[2] = sigma * sqrt(0.375);
+    } else {
+        memcpy(z[0], in[0], npx * sizeof(double));
+        sig[0] = sigma;
+    }
+    oracle_bm_f64(z[0], width, height, 64.0 * kTauHt, pos, cnt);
+    step_f64((const double* const*)z, NULL, bz, width, height, channels, sig, 1, pos, cnt);
+    oracle_bm_f64(bz[0], width, height, 64.0 * kTauWie, pos, cnt);
+    step_f64((const double* const*)z, (const double* const*)bz, fz, width, height, channels, sig, 2,
+             pos, cnt);
+    double* const* outs[2] = {basic, out};
+    double* const* srcs[2] = {bz, fz};
+    for (int k = 0; k < 2; ++k) {
+        if (!outs[k]) continue;
+        if (channels == 3) {
+            for (size_t i = 0; i < npx; ++i) {
+                const double y = srcs[k][0][i], u = srcs[k][1][i], v = srcs[k][2][i];
+                outs[k][0][i] = y + u + v * (2.0 / 3.0);
+                outs[k][1][i] = y - v * (4.0 / 3.0);
+                outs[k][2][i] = y - u + v * (2.0 / 3.0);
+            }
+        } else {
+            memcpy(outs[k][0], srcs[k][0], npx * sizeof(double));
+        }
+    }
+    free(buf);
+    free(pos);
+    free(cnt);
+    return 0;
+}

@@ -1,0 +1,177 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
+import this module, and only as the checker (or the timed CPU baseline). The
+product package ``cpipe_amd`` never imports it.
+
+What each function restates is documented in ``oracle/bm3d_oracle.h``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+K = 8
+R = 19
+STRIDE = 3
+NMAX = 16
+TAU_HT_SSD = np.float32(64.0 * 2500.0 / (255.0 * 255.0))
+TAU_WIE_SSD = np.float32(64.0 * 400.0 / (255.0 * 255.0))
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        F = ctypes.c_float
+        D = ctypes.c_double
+        S = ctypes.c_size_t
+        L.oracle_f32_to_f16.argtypes = [F]
+        L.oracle_f32_to_f16.restype = ctypes.c_uint16
+        L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
+        L.oracle_f16_to_f32.restype = F
+        L.oracle_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I]
+        L.oracle_compat_rgba16f.restype = None
+        L.oracle_ref_count.argtypes = [I]
+        L.oracle_ref_count.restype = I
+        L.oracle_ref_pos.argtypes = [I, I]
+        L.oracle_ref_pos.restype = I
+        L.oracle_dct_matrix_f64.argtypes = [P]
+        L.oracle_kaiser_f64.argtypes = [P]
+        L.oracle_bm_f32.argtypes = [P, I, I, F, P, P, I]
+        L.oracle_bm_f64.argtypes = [P, I, I, D, P, P]
+        L.oracle_bm3d_f32.argtypes = [P, P, P, I, I, I, F, I, P, P, P, P]
+        L.oracle_bm3d_f32.restype = I
+        L.oracle_bm3d_f64.argtypes = [P, P, P, I, I, I, D]
+        L.oracle_bm3d_f64.restype = I
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _plane_ptrs(planes):
+    arr = (ctypes.c_void_p * 3)(*([_ptr(p) for p in planes] + [0] * (3 - len(planes))))
+    return arr
+
+
+def f32_to_f16_bits(x: np.ndarray) -> np.ndarray:
+    L = lib()
+    flat = np.asarray(x, dtype=np.float32).ravel()
+    return np.array([L.oracle_f32_to_f16(float(v)) for v in flat], dtype=np.uint16).reshape(np.shape(x))
+
+
+def f16_bits_to_f32(h: np.ndarray) -> np.ndarray:
+    L = lib()
+    flat = np.asarray(h, dtype=np.uint16).ravel()
+    return np.array([L.oracle_f16_to_f32(int(v)) for v in flat], dtype=np.float32).reshape(np.shape(h))
+
+
+def compat_rgba16f(rgba_bits: np.ndarray, sigma: float, stage: int = 1) -> np.ndarray:
+    """Restated reference step (stage 1: 2x2 blend, stage 2: identity) on (H, W, 4) uint16."""
+    src = np.ascontiguousarray(rgba_bits, dtype=np.uint16)
+    h, w, c = src.shape
+    assert c == 4
+    out = np.empty_like(src)
+    lib().oracle_compat_rgba16f(_ptr(src), w, _ptr(out), w, w, h, float(sigma), int(stage))
+    return out
+
+
+def ref_positions(n: int) -> np.ndarray:
+    L = lib()
+    return np.array([L.oracle_ref_pos(i, n) for i in range(L.oracle_ref_count(n))], dtype=np.int64)
+
+
+def dct_matrix() -> np.ndarray:
+    c = np.zeros(64, dtype=np.float64)
+    lib().oracle_dct_matrix_f64(_ptr(c))
+    return c.reshape(8, 8)
+
+
+def kaiser() -> np.ndarray:
+    k = np.zeros(8, dtype=np.float64)
+    lib().oracle_kaiser_f64(_ptr(k))
+    return k
+
+
+def block_match_f32(plane: np.ndarray, tau_ssd=TAU_HT_SSD, nthreads: int = 8):
+    p = np.ascontiguousarray(plane, dtype=np.float32)
+    h, w = p.shape
+    L = lib()
+    nref = L.oracle_ref_count(w) * L.oracle_ref_count(h)
+    pos = np.zeros((nref, NMAX), dtype=np.uint32)
+    cnt = np.zeros(nref, dtype=np.uint8)
+    L.oracle_bm_f32(_ptr(p), w, h, float(tau_ssd), _ptr(pos), _ptr(cnt), int(nthreads))
+    return pos, cnt
+
+
+def block_match_f64(plane: np.ndarray, tau_ssd=64.0 * 2500.0 / 255.0**2):
+    p = np.ascontiguousarray(plane, dtype=np.float64)
+    h, w = p.shape
+    L = lib()
+    nref = L.oracle_ref_count(w) * L.oracle_ref_count(h)
+    pos = np.zeros((nref, NMAX), dtype=np.uint32)
+    cnt = np.zeros(nref, dtype=np.uint8)
+    L.oracle_bm_f64(_ptr(p), w, h, float(tau_ssd), _ptr(pos), _ptr(cnt))
+    return pos, cnt
+
+
+def bm3d_f32(planes: np.ndarray, sigma: float, nthreads: int = 8, want_basic: bool = False,
+             want_matches: bool = False):
+    """planes: (C, H, W) float32, C in {1, 3}. Returns out (and basic / matches if asked)."""
+    src = np.ascontiguousarray(planes, dtype=np.float32)
+    c, h, w = src.shape
+    out = np.empty_like(src)
+    basic = np.empty_like(src) if want_basic else None
+    L = lib()
+    nref = max(L.oracle_ref_count(w) * L.oracle_ref_count(h), 1)
+    m1 = np.zeros((nref, NMAX), np.uint32) if want_matches else None
+    c1 = np.zeros(nref, np.uint8) if want_matches else None
+    m2 = np.zeros((nref, NMAX), np.uint32) if want_matches else None
+    c2 = np.zeros(nref, np.uint8) if want_matches else None
+    rc = L.oracle_bm3d_f32(
+        _plane_ptrs(list(src)), _plane_ptrs(list(out)),
+        _plane_ptrs(list(basic)) if want_basic else None, w, h, c, float(sigma), int(nthreads),
+        _ptr(m1) if want_matches else None, _ptr(c1) if want_matches else None,
+        _ptr(m2) if want_matches else None, _ptr(c2) if want_matches else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle_bm3d_f32 failed: {rc}")
+    res = [out]
+    if want_basic:
+        res.append(basic)
+    if want_matches:
+        res.append((m1, c1, m2, c2))
+    return res[0] if len(res) == 1 else tuple(res)
+
+
+def bm3d_f64(planes: np.ndarray, sigma: float, want_basic: bool = False):
+    src = np.ascontiguousarray(planes, dtype=np.float64)
+    c, h, w = src.shape
+    out = np.empty_like(src)
+    basic = np.empty_like(src) if want_basic else None
+    rc = lib().oracle_bm3d_f64(_plane_ptrs(list(src)), _plane_ptrs(list(out)),
+                               _plane_ptrs(list(basic)) if want_basic else None, w, h, c,
+                               float(sigma))
+    if rc != 0:
+        raise RuntimeError(f"oracle_bm3d_f64 failed: {rc}")
+    return (out, basic) if want_basic else out
