@@ -1,0 +1,94 @@
+"""In-tree build of the native libraries (hipcc for gfx950, g++ for host-only code).
+
+Outputs (git-ignored, shipped with the tree):
+  cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h) + the node
+                                      descriptor in the `cpipe_registry` section
+  cpipe_amd/lib/libcpipe_minihost.so  minimal cpipe host (suites, registry walk, one-node runner)
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(PKG, "_build")
+LIBDIR = os.path.join(PKG, "lib")
+ENGINE_SO = os.path.join(LIBDIR, "libcpipe_bm3d.so")
+MINIHOST_SO = os.path.join(LIBDIR, "libcpipe_minihost.so")
+
+ARCH = "gfx950"
+# Exact FP32 order: no contraction, IEEE division/sqrt, denormals kept.
+# Packed FP32 instructions are disabled for the device code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", f"-I{INCLUDE}",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+             f"-I{INCLUDE}"]
+
+
+def _hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the BM3D engine needs ROCm's hipcc to build")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed ({r.returncode}): {cmd[0]} ... {cmd[-1]}")
+    return r
+
+
+def _embed_manifest() -> str:
+    """Turn csrc/bm3d.json into a C++ string constant (like the reference's EmbedJson.cmake)."""
+    src = os.path.join(CSRC, "bm3d.json")
+    with open(src, encoding="utf-8") as f:
+        text = f.read()
+    json.loads(text)  # fail early on a malformed manifest
+    out = os.path.join(BUILD, "manifest_gen.cpp")
+    body = ", ".join(str(b) for b in text.encode("utf-8")) + ", 0"
+    with open(out, "w", encoding="utf-8") as f:
+        f.write("// generated from cpipe_amd/csrc/bm3d.json\n")
+        f.write('extern "C" const char CPIPE_BM3D_MANIFEST_JSON[];\n')
+        f.write(f'extern "C" const char CPIPE_BM3D_MANIFEST_JSON[] = {{{body}}};\n')
+    return out
+
+
+def build(verbose: bool = False) -> dict:
+    """Compile every native library of the package; returns their paths."""
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    hipcc = _hipcc()
+    manifest_cpp = _embed_manifest()
+    dev_o = os.path.join(BUILD, "bm3d_device.o")
+    node_o = os.path.join(BUILD, "node.o")
+    man_o = os.path.join(BUILD, "manifest_gen.o")
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "bm3d_device.hip"), "-o", dev_o], verbose)
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
+    _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
+    tmp = ENGINE_SO + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, node_o, man_o],
+         verbose)
+    os.replace(tmp, ENGINE_SO)
+    tmp = MINIHOST_SO + ".tmp"
+    _run(["g++", *CXX_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "minihost.cpp"), "-ldl"],
+         verbose)
+    os.replace(tmp, MINIHOST_SO)
+    return {"engine": ENGINE_SO, "minihost": MINIHOST_SO}
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -677,6 +677,7 @@ void tile_box_host(int t, int nref, int n, int* f0, int* f1) {
 void free_plan(cpipe_bm3d_plan* p) {
     if (!p) return;
     DeviceGuard g(p->device);
+    (void)hipDeviceSynchronize();  // runs may still read the scratch being freed
     (void)hipFree(p->z);
     (void)hipFree(p->bz);
     (void)hipFree(p->rgb);

@@ -1,0 +1,280 @@
+// com.cpipe.denoise.bm3d -- the cpipe node (plugin ABI 0.4) driving the MI355X engine.
+//
+// Replaces src/cpipe/nodes/denoise/bm3d.cpp (DenoiseBm3d) and its registration
+// CPIPE_REGISTER_NODE (bm3d.cpp:111, include/cpipe/sdk/registry.hpp:14-32).
+// Behaviour kept from the reference:
+//   * action dispatch and the exception barrier of sdk::detail::dispatch
+//     (include/cpipe/sdk/sdk.hpp:615-702): describe/prepare/unknown -> REPLY_DEFAULT,
+//     create writes the instance to *(void**)out_ctx, destroy takes it from in_ctx,
+//     process gets the instance as `node` and a cpipe_process_ctx as in_ctx;
+//     anything thrown -> CPIPE_INTERNAL_ERROR + host->log(host, 4, ...);
+//   * arity check -> CPIPE_BAD_INDEX (bm3d.cpp:84-87);
+//   * format gate -> CPIPE_BAD_PRECISION (bm3d.cpp:57-68);
+//   * sigma precedence sigma -> sigma_override -> metadata noise profile -> 0.03,
+//     params clamped to [0, 0.2] (bm3d.cpp:21-55);
+//   * add_applied_step("denoise.bm3d") on success (bm3d.cpp:103).
+// Added: `mode` (compat | bm3d) and `precision` params, FP32 plane I/O
+// (1 x R32_SFLOAT grey or 3 x R32_SFLOAT RGB planes), and a defined status
+// when no HIP device is visible (CPIPE_UNSUPPORTED, logged).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "cpipe_bm3d.h"
+#include "cpipe_node_abi.h"
+
+extern "C" const char CPIPE_BM3D_MANIFEST_JSON[];
+
+namespace {
+
+const char kNodeId[] = "com.cpipe.denoise.bm3d";
+const char kNodeVersion[] = "2.0.0";
+
+struct Suites {
+    const cpipe_buffer_suite_v1* buffer = nullptr;
+    const cpipe_param_suite_v1* param = nullptr;
+    const cpipe_metadata_suite_v1* metadata = nullptr;
+    const cpipe_metadata_builder_suite_v1* builder = nullptr;
+};
+
+Suites fetch_suites(cpipe_host_t* host) {
+    Suites s;
+    if (host == nullptr || host->get_suite == nullptr) return s;
+    s.buffer = static_cast<const cpipe_buffer_suite_v1*>(host->get_suite(host, "buffer", 1));
+    s.param = static_cast<const cpipe_param_suite_v1*>(host->get_suite(host, "param", 1));
+    s.metadata = static_cast<const cpipe_metadata_suite_v1*>(host->get_suite(host, "metadata", 1));
+    s.builder = static_cast<const cpipe_metadata_builder_suite_v1*>(
+        host->get_suite(host, "metadata_builder", 1));
+    return s;
+}
+
+void host_log(cpipe_host_t* host, int level, const std::string& msg) {
+    if (host != nullptr && host->log != nullptr) host->log(host, level, msg.c_str());
+}
+
+struct Bm3dNode {
+    int unused = 0;
+};
+
+// bm3d.cpp:21-55
+float resolve_sigma(const Suites& s, const cpipe_props_t* params, const cpipe_buffer_t* input) {
+    if (s.param != nullptr && s.param->get_double != nullptr && params != nullptr) {
+        double v = 0.0;
+        if (s.param->get_double(params, "sigma", &v) == CPIPE_OK)
+            return std::clamp(static_cast<float>(v), 0.0F, 0.2F);
+        double o = 0.0;
+        if (s.param->get_double(params, "sigma_override", &o) == CPIPE_OK)
+            return std::clamp(static_cast<float>(o), 0.0F, 0.2F);
+    }
+    if (s.buffer == nullptr || s.buffer->get_metadata == nullptr || s.metadata == nullptr ||
+        s.metadata->get_calibration == nullptr)
+        return 0.03F;
+    const cpipe_metadata_t* meta = nullptr;
+    if (s.buffer->get_metadata(input, &meta) != CPIPE_OK || meta == nullptr) return 0.03F;
+    cpipe_calibration_view view{};
+    if (s.metadata->get_calibration(meta, &view) != CPIPE_OK || view.get_noise_profile == nullptr)
+        return 0.03F;
+    size_t total = 0;
+    if (view.get_noise_profile(meta, 0, &total, nullptr, nullptr) != CPIPE_OK || total == 0)
+        return 0.03F;
+    std::vector<float> a(total), b(total);
+    if (view.get_noise_profile(meta, total, &total, a.data(), b.data()) != CPIPE_OK || total == 0)
+        return 0.03F;
+    double variance = 0.0;
+    for (size_t i = 0; i < total; ++i) variance += std::max(0.0F, a[i] + (b[i] * 0.25F));
+    variance /= static_cast<double>(total);
+    return static_cast<float>(std::sqrt(variance));
+}
+
+std::string enum_param(const Suites& s, const cpipe_props_t* params, const char* key,
+                       const char* env, const char* dflt) {
+    if (const char* e = std::getenv(env); e != nullptr && *e != '\0') return e;
+    if (s.param != nullptr && s.param->get_enum != nullptr && params != nullptr) {
+        const char* v = nullptr;
+        if (s.param->get_enum(params, key, &v) == CPIPE_OK && v != nullptr) return v;
+    }
+    return dflt;
+}
+
+struct Image {
+    int format = 0;
+    uint32_t w = 0, h = 0;
+    size_t row_bytes = 0;
+};
+
+int describe_buffer(const Suites& s, const cpipe_buffer_t* b, Image* out) {
+    if (s.buffer == nullptr || s.buffer->get_format == nullptr || s.buffer->get_dims == nullptr)
+        return CPIPE_BAD_INDEX;
+    int rc = s.buffer->get_format(b, &out->format);
+    if (rc != CPIPE_OK) return rc;
+    uint8_t ndim = 0;
+    uint32_t dims[8] = {};
+    rc = s.buffer->get_dims(b, &ndim, dims);
+    if (rc != CPIPE_OK) return rc;
+    if (ndim < 2) return CPIPE_BAD_INDEX;
+    out->w = dims[0];
+    out->h = dims[1];
+    const size_t bpp = out->format == CPIPE_FORMAT_R16G16B16A16_SFLOAT ? 8 : 4;
+    out->row_bytes = (size_t)out->w * bpp;
+    if (s.buffer->get_stride != nullptr) {
+        uint64_t st[8] = {};
+        if (s.buffer->get_stride(b, st) == CPIPE_OK) {
+            if (st[0] != 0 && st[0] != bpp) return CPIPE_UNSUPPORTED;
+            if (st[1] != 0) out->row_bytes = st[1];
+        }
+    }
+    if (out->row_bytes % bpp != 0 || out->row_bytes < (size_t)out->w * bpp) return CPIPE_BAD_INDEX;
+    return CPIPE_OK;
+}
+
+int map_engine_status(int rc, cpipe_host_t* host) {
+    if (rc != CPIPE_OK) host_log(host, 3, std::string("denoise.bm3d: ") + cpipe_bm3d_last_error());
+    return rc;
+}
+
+int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process_ctx* ctx) {
+    const Suites s = fetch_suites(host);
+    if (ctx == nullptr || ctx->inputs == nullptr || ctx->outputs == nullptr ||
+        ctx->out_metadata == nullptr || ctx->n_in == 0 || ctx->n_in != ctx->n_out ||
+        (ctx->n_in != 1 && ctx->n_in != 3) || ctx->out_metadata[0] == nullptr)
+        return CPIPE_BAD_INDEX;
+    for (size_t i = 0; i < ctx->n_in; ++i)
+        if (ctx->inputs[i] == nullptr || ctx->outputs[i] == nullptr) return CPIPE_BAD_INDEX;
+    if (s.buffer == nullptr || s.buffer->lock_cpu == nullptr || s.buffer->unlock_cpu == nullptr)
+        return CPIPE_BAD_INDEX;
+
+    std::vector<Image> in(ctx->n_in), out(ctx->n_out);
+    for (size_t i = 0; i < ctx->n_in; ++i) {
+        int rc = describe_buffer(s, ctx->inputs[i], &in[i]);
+        if (rc != CPIPE_OK) return rc;
+        rc = describe_buffer(s, ctx->outputs[i], &out[i]);
+        if (rc != CPIPE_OK) return rc;
+    }
+    const std::string mode = enum_param(s, params, "mode", "CPIPE_BM3D_MODE", "compat");
+    const std::string prec = enum_param(s, params, "precision", "CPIPE_BM3D_PRECISION", "fp32");
+    if (mode != "compat" && mode != "bm3d") return CPIPE_BAD_INDEX;
+    int precision = CPIPE_BM3D_PRECISION_FP32;
+    if (prec == "bf16") {
+        precision = CPIPE_BM3D_PRECISION_BF16;
+    } else if (prec == "fp16") {
+        precision = CPIPE_BM3D_PRECISION_FP16;
+    } else if (prec != "fp32") {
+        return CPIPE_BAD_INDEX;
+    }
+
+    const bool rgba = ctx->n_in == 1 && in[0].format == CPIPE_FORMAT_R16G16B16A16_SFLOAT &&
+                      out[0].format == CPIPE_FORMAT_R16G16B16A16_SFLOAT;
+    bool planes = true;
+    for (size_t i = 0; i < ctx->n_in; ++i)
+        planes = planes && in[i].format == CPIPE_FORMAT_R32_SFLOAT &&
+                 out[i].format == CPIPE_FORMAT_R32_SFLOAT;
+    if (!rgba && !(planes && mode == "bm3d")) return CPIPE_BAD_PRECISION;
+    for (size_t i = 0; i < ctx->n_in; ++i)
+        if (in[i].w != in[0].w || in[i].h != in[0].h || out[i].w != in[0].w || out[i].h != in[0].h)
+            return CPIPE_BAD_INDEX;
+
+    const float sigma = resolve_sigma(s, params, ctx->inputs[0]);
+    if (cpipe_bm3d_device_count() <= 0) {
+        host_log(host, 3, "denoise.bm3d: no HIP device visible; the node runs on the GPU only");
+        return CPIPE_UNSUPPORTED;
+    }
+
+    std::vector<void*> inp(ctx->n_in, nullptr), outp(ctx->n_out, nullptr);
+    size_t locked_in = 0, locked_out = 0;
+    int rc = CPIPE_OK;
+    for (size_t i = 0; i < ctx->n_in && rc == CPIPE_OK; ++i) {
+        rc = s.buffer->lock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[i]), CPIPE_CPU_ACCESS_READ,
+                                &inp[i]);
+        if (rc == CPIPE_OK) ++locked_in;
+    }
+    for (size_t i = 0; i < ctx->n_out && rc == CPIPE_OK; ++i) {
+        rc = s.buffer->lock_cpu(ctx->outputs[i], CPIPE_CPU_ACCESS_WRITE, &outp[i]);
+        if (rc == CPIPE_OK) ++locked_out;
+    }
+    if (rc == CPIPE_OK) {
+        const int w = (int)in[0].w, h = (int)in[0].h;
+        if (rgba) {
+            rc = cpipe_bm3d_host_rgba16f(static_cast<const uint16_t*>(inp[0]), in[0].row_bytes / 8,
+                                         static_cast<uint16_t*>(outp[0]), out[0].row_bytes / 8, w, h,
+                                         sigma, mode == "bm3d" ? 1 : 0, precision, 0);
+        } else {
+            const float* ip[3] = {};
+            float* op[3] = {};
+            for (size_t c = 0; c < ctx->n_in; ++c) {
+                ip[c] = static_cast<const float*>(inp[c]);
+                op[c] = static_cast<float*>(outp[c]);
+            }
+            size_t ipitch = in[0].row_bytes / 4, opitch = out[0].row_bytes / 4;
+            for (size_t c = 1; c < ctx->n_in; ++c)
+                if (in[c].row_bytes != in[0].row_bytes || out[c].row_bytes != out[0].row_bytes)
+                    rc = CPIPE_UNSUPPORTED;
+            if (rc == CPIPE_OK)
+                rc = cpipe_bm3d_host_planes(ip, ipitch, op, opitch, w, h, (int)ctx->n_in, sigma,
+                                            precision, 0);
+        }
+        rc = map_engine_status(rc, host);
+    }
+    for (size_t i = 0; i < locked_out; ++i) {
+        s.buffer->unlock_cpu(ctx->outputs[i]);
+        if (rc == CPIPE_OK && s.buffer->flush_cpu_writes != nullptr)
+            s.buffer->flush_cpu_writes(ctx->outputs[i]);
+    }
+    for (size_t i = 0; i < locked_in; ++i)
+        s.buffer->unlock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[i]));
+    if (rc != CPIPE_OK) return rc;
+    if (s.builder == nullptr || s.builder->add_applied_step == nullptr) return CPIPE_BAD_INDEX;
+    return s.builder->add_applied_step(ctx->out_metadata[0], "denoise.bm3d");
+}
+
+int node_main_entry(const char* action, cpipe_host_t* host, cpipe_node_t* node,
+                    cpipe_props_t* params, void* in_ctx, void* out_ctx) {
+    try {
+        if (action == nullptr) return CPIPE_REPLY_DEFAULT;
+        if (std::strcmp(action, CPIPE_ACTION_DESCRIBE) == 0) return CPIPE_REPLY_DEFAULT;
+        if (std::strcmp(action, CPIPE_ACTION_CREATE) == 0) {
+            if (out_ctx == nullptr) return CPIPE_BAD_INDEX;
+            *static_cast<void**>(out_ctx) = new Bm3dNode();
+            return CPIPE_OK;
+        }
+        if (std::strcmp(action, CPIPE_ACTION_DESTROY) == 0) {
+            delete static_cast<Bm3dNode*>(in_ctx);
+            return CPIPE_OK;
+        }
+        if (std::strcmp(action, CPIPE_ACTION_PROCESS) == 0) {
+            if (node == nullptr) return CPIPE_BAD_INDEX;
+            return process(host, params, static_cast<const cpipe_process_ctx*>(in_ctx));
+        }
+    } catch (...) {
+        host_log(host, 4, "plugin dispatch threw an exception");
+        return CPIPE_INTERNAL_ERROR;
+    }
+    return CPIPE_REPLY_DEFAULT;
+}
+
+__attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_desc_t kBm3dDescriptor = {
+    CPIPE_ABI_MAJOR, CPIPE_ABI_MINOR, kNodeId, kNodeVersion, CPIPE_BM3D_MANIFEST_JSON,
+    &node_main_entry};
+
+}  // namespace
+
+// Section bounds, defined by the linker for the `cpipe_registry` section.
+extern "C" const cpipe_plugin_desc_t __start_cpipe_registry[];
+extern "C" const cpipe_plugin_desc_t __stop_cpipe_registry[];
+
+extern "C" const cpipe_plugin_desc_t* cpipe_bm3d_node_descriptor(void) { return &kBm3dDescriptor; }
+
+// Descriptors of this library's `cpipe_registry` section, for hosts that dlopen
+// the node instead of linking it (the reference host walks the section of its
+// own executable, src/cpipe/runtime/Registry.cpp:6-26).
+extern "C" size_t cpipe_registry_entries(const cpipe_plugin_desc_t** first) {
+    if (first != nullptr) *first = __start_cpipe_registry;
+    return (size_t)(__stop_cpipe_registry - __start_cpipe_registry);
+}
+
+// Link anchor with the reference node's name (src/cpipe/nodes/denoise/bm3d.cpp:115-117),
+// so a host's builtin_link.cpp pulls this object in from the static archive.
+extern "C" void cpipe_link_builtin_denoise_bm3d(void) {}

@@ -1,0 +1,203 @@
+"""Python binding of the MI355X BM3D engine (include/cpipe_bm3d.h) via ctypes.
+
+torch is used only for device memory and streams. Every call goes to the
+native library cpipe_amd/lib/libcpipe_bm3d.so; if it is missing or no HIP
+device is visible, the calls raise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ENGINE_SO = os.path.join(PKG, "lib", "libcpipe_bm3d.so")
+
+PRECISION_FP32 = 0
+PRECISION_BF16 = 1
+PRECISION_FP16 = 2
+_PRECISIONS = {"fp32": PRECISION_FP32, "bf16": PRECISION_BF16, "fp16": PRECISION_FP16}
+
+STAGES = ("color", "bm1", "group1", "merge1", "bm2", "group2", "merge2")
+
+# Every symbol the C ABI header declares (checked by the CPU test suite).
+EXPORTS = (
+    "cpipe_bm3d_last_error", "cpipe_bm3d_device_count", "cpipe_bm3d_plan_create",
+    "cpipe_bm3d_plan_destroy", "cpipe_bm3d_plan_run", "cpipe_bm3d_plan_run_rgba16f",
+    "cpipe_bm3d_plan_basic", "cpipe_bm3d_plan_matches", "cpipe_bm3d_plan_ref_grid",
+    "cpipe_bm3d_plan_set_timing", "cpipe_bm3d_plan_timing", "cpipe_bm3d_compat_rgba16f",
+    "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_node_descriptor",
+    "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d",
+)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"cpipe_bm3d status {status}: {msg}")
+        self.status = status
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("channels", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the native engine; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ENGINE_SO):
+        raise RuntimeError(f"native engine missing: {ENGINE_SO} (run cpipe_amd.build.build())")
+    L = ctypes.CDLL(ENGINE_SO)
+    P, I, F, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+    L.cpipe_bm3d_last_error.restype = ctypes.c_char_p
+    L.cpipe_bm3d_device_count.restype = I
+    L.cpipe_bm3d_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(P)]
+    L.cpipe_bm3d_plan_destroy.argtypes = [P]
+    L.cpipe_bm3d_plan_destroy.restype = None
+    L.cpipe_bm3d_plan_run.argtypes = [P, P, S, P, S, F, P]
+    L.cpipe_bm3d_plan_run_rgba16f.argtypes = [P, P, S, P, S, F, P]
+    L.cpipe_bm3d_plan_basic.argtypes = [P, P, S, P]
+    L.cpipe_bm3d_plan_matches.argtypes = [P, I, P, P, P]
+    L.cpipe_bm3d_plan_ref_grid.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I)]
+    L.cpipe_bm3d_plan_set_timing.argtypes = [P, I]
+    L.cpipe_bm3d_plan_timing.argtypes = [P, ctypes.POINTER(F), I, ctypes.POINTER(I)]
+    L.cpipe_bm3d_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I, P]
+    L.cpipe_bm3d_host_planes.argtypes = [P, S, P, S, I, I, I, F, I, I]
+    L.cpipe_bm3d_host_rgba16f.argtypes = [P, S, P, S, I, I, F, I, I, I]
+    L.cpipe_bm3d_node_descriptor.restype = P
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise EngineError(rc, lib().cpipe_bm3d_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    return int(lib().cpipe_bm3d_device_count())
+
+
+def _ptr_array(ptrs):
+    return (ctypes.c_void_p * 3)(*(list(ptrs) + [0] * (3 - len(ptrs))))
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Plan:
+    """Device scratch for one image size; runs both BM3D steps on device planes."""
+
+    def __init__(self, width: int, height: int, channels: int = 3, precision="fp32",
+                 device: int = 0):
+        if isinstance(precision, str):
+            precision = _PRECISIONS[precision]
+        self.width, self.height, self.channels = int(width), int(height), int(channels)
+        self.precision, self.device = int(precision), int(device)
+        d = PlanDesc(ctypes.sizeof(PlanDesc), self.width, self.height, self.channels,
+                     self.precision, self.device)
+        h = ctypes.c_void_p()
+        _check(lib().cpipe_bm3d_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cpipe_bm3d_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ref_grid(self):
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        _check(lib().cpipe_bm3d_plan_ref_grid(self._h, ctypes.byref(nx), ctypes.byref(ny)))
+        return nx.value, ny.value
+
+    def run_ptrs(self, in_ptrs, in_pitch, out_ptrs, out_pitch, sigma, stream=None) -> None:
+        _check(lib().cpipe_bm3d_plan_run(self._h, _ptr_array(in_ptrs), in_pitch,
+                                         _ptr_array(out_ptrs), out_pitch, float(sigma),
+                                         _stream_handle(stream)))
+
+    def run(self, src, dst, sigma, stream=None) -> None:
+        """src, dst: (C, H, W) float32 CUDA tensors with a unit stride along W."""
+        self._check_planes(src)
+        self._check_planes(dst)
+        self.run_ptrs([src[c].data_ptr() for c in range(self.channels)], src.stride(1),
+                      [dst[c].data_ptr() for c in range(self.channels)], dst.stride(1), sigma,
+                      stream)
+
+    def run_rgba16f(self, src, dst, sigma, stream=None) -> None:
+        """src, dst: (H, W, 4) int16/float16 CUDA tensors (interleaved RGBA16F)."""
+        for t in (src, dst):
+            if tuple(t.shape) != (self.height, self.width, 4) or t.stride(2) != 1 or t.stride(1) != 4:
+                raise ValueError("expected (H, W, 4) interleaved RGBA16F")
+        _check(lib().cpipe_bm3d_plan_run_rgba16f(self._h, src.data_ptr(), src.stride(0) // 4,
+                                                 dst.data_ptr(), dst.stride(0) // 4, float(sigma),
+                                                 _stream_handle(stream)))
+
+    def basic(self, dst, stream=None) -> None:
+        self._check_planes(dst)
+        _check(lib().cpipe_bm3d_plan_basic(self._h, _ptr_array([dst[c].data_ptr() for c in range(self.channels)]),
+                                           dst.stride(1), _stream_handle(stream)))
+
+    def matches(self, step: int, stream=None):
+        import torch
+        nx, ny = self.ref_grid
+        pos = torch.zeros((nx * ny, 16), dtype=torch.int32, device=f"cuda:{self.device}")
+        cnt = torch.zeros((nx * ny,), dtype=torch.uint8, device=f"cuda:{self.device}")
+        _check(lib().cpipe_bm3d_plan_matches(self._h, int(step), pos.data_ptr(), cnt.data_ptr(),
+                                             _stream_handle(stream)))
+        return pos, cnt
+
+    def set_timing(self, enable: bool) -> None:
+        _check(lib().cpipe_bm3d_plan_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        ms = (ctypes.c_float * len(STAGES))()
+        runs = ctypes.c_int()
+        _check(lib().cpipe_bm3d_plan_timing(self._h, ms, len(STAGES), ctypes.byref(runs)))
+        return {k: float(v) for k, v in zip(STAGES, ms)}, runs.value
+
+    def _check_planes(self, t) -> None:
+        if tuple(t.shape) != (self.channels, self.height, self.width):
+            raise ValueError(f"expected planes of shape {(self.channels, self.height, self.width)}, "
+                             f"got {tuple(t.shape)}")
+        if t.dtype.__repr__() != "torch.float32" or t.stride(2) != 1 or not t.is_cuda:
+            raise ValueError("expected float32 CUDA planes with unit stride along W")
+        if t.stride(0) < t.stride(1) * self.height:
+            raise ValueError("planes overlap")
+
+
+def compat_rgba16f(src, dst, sigma: float, stage: int = 1, stream=None) -> None:
+    """Reference-compatible filter on (H, W, 4) RGBA16F CUDA tensors."""
+    h, w, c = src.shape
+    if c != 4 or tuple(dst.shape) != (h, w, 4):
+        raise ValueError("expected (H, W, 4) RGBA16F tensors")
+    _check(lib().cpipe_bm3d_compat_rgba16f(src.data_ptr(), src.stride(0) // 4, dst.data_ptr(),
+                                           dst.stride(0) // 4, w, h, float(sigma), int(stage),
+                                           _stream_handle(stream)))
+
+
+def denoise(planes, sigma: float, precision="fp32", stream=None):
+    """BM3D both steps on (C, H, W) float32 CUDA planes; returns new planes."""
+    import torch
+    c, h, w = planes.shape
+    plan = Plan(w, h, c, precision, planes.device.index or 0)
+    out = torch.empty_like(planes)
+    plan.run(planes.contiguous(), out, sigma, stream)
+    return out

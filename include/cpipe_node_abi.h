@@ -225,6 +225,15 @@ typedef struct {
  * src/cpipe/nodes/denoise/bm3d.cpp:111). */
 const cpipe_plugin_desc_t* cpipe_bm3d_node_descriptor(void);
 
+/* The library's `cpipe_registry` section as an array (for hosts that dlopen
+ * the node; the reference host walks __start/__stop_cpipe_registry of its own
+ * executable, src/cpipe/runtime/Registry.cpp:6-26). Returns the entry count. */
+size_t cpipe_registry_entries(const cpipe_plugin_desc_t** first);
+
+/* Link anchor of the same name as the reference's
+ * src/cpipe/nodes/denoise/bm3d.cpp:115-117 (called by builtin_link.cpp:32-60). */
+void cpipe_link_builtin_denoise_bm3d(void);
+
 #if defined(__cplusplus) && defined(__x86_64__)
 static_assert(sizeof(cpipe_plugin_desc_t) == 40, "cpipe_plugin_desc_t layout");
 static_assert(offsetof(cpipe_plugin_desc_t, node_id) == 8, "node_id offset");

@@ -1,0 +1,131 @@
+"""The C ABI library and the node's plugin contract, checked without a GPU.
+
+No compute call reaches a kernel here: without a HIP device the engine must
+refuse with defined statuses (CPIPE_UNSUPPORTED), never fall back to the CPU.
+"""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+from cpipe_amd import engine, host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def _declared_functions(header):
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(cpipe_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if not n.endswith("_t")))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.lib()
+    declared = _declared_functions("cpipe_bm3d.h") + _declared_functions("cpipe_node_abi.h")
+    assert "cpipe_bm3d_plan_run" in declared and "cpipe_bm3d_node_descriptor" in declared
+    for name in declared:
+        assert hasattr(lib, name), f"missing export {name}"
+    for name in engine.EXPORTS:
+        assert hasattr(lib, name), f"missing export {name}"
+
+
+def test_abi_header_layout_static_asserts(tmp_path):
+    src = tmp_path / "abi.cpp"
+    src.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\nint main(){return 0;}\n')
+    subprocess.run(["g++", "-std=c++17", f"-I{INCLUDE}", "-fsyntax-only", str(src)], check=True)
+    csrc = tmp_path / "abi.c"
+    csrc.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\nint main(void){return 0;}\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{INCLUDE}", "-fsyntax-only", str(csrc)],
+                   check=True)
+
+
+def test_registry_section_holds_the_node():
+    assert host.mh().mh_registry_count(engine.ENGINE_SO.encode()) == 1
+    d = host.find_node()
+    assert d.contents.abi_major == 0 and d.contents.abi_minor == 4
+    assert d.contents.node_id == b"com.cpipe.denoise.bm3d"
+    direct = engine.lib().cpipe_bm3d_node_descriptor()
+    assert direct == ctypes.addressof(d.contents)
+
+
+def test_manifest_contract():
+    """Assertions of test_node_denoise_bm3d.cpp:137-144, with the engine changed to HIP."""
+    d = host.find_node()
+    m = json.loads(d.contents.manifest_json.decode())
+    assert m["id"] == "com.cpipe.denoise.bm3d"
+    assert m["params"][0]["name"] == "sigma"
+    assert m["params"][1]["name"] == "sigma_override"
+    assert m["params"][0]["range"] == {"min": 0, "max": 0.2}
+    assert m["ports"][1]["metadata"]["sets_steps_applied"] == ["denoise.bm3d"]
+    assert m["ports"][0]["metadata"]["requires_steps_applied"] == ["color_matrix"]
+    assert m["compute"]["device"] == "GPU" and m["compute"]["engine"] == "HIP"
+    assert "f16" in m["ports"][0]["caps"]["precision"] and "f32" in m["ports"][0]["caps"]["precision"]
+    modes = [p for p in m["params"] if p["name"] == "mode"][0]
+    assert modes["enum_values"] == ["compat", "bm3d"] and modes["default"] == "compat"
+
+
+def test_actions_without_work():
+    d = host.find_node()
+    assert host.mh().mh_call_action(d, b"describe") == host.CPIPE_REPLY_DEFAULT
+    assert host.mh().mh_call_action(d, b"prepare") == host.CPIPE_REPLY_DEFAULT
+    assert host.mh().mh_call_action(d, b"no-such-action") == host.CPIPE_REPLY_DEFAULT
+
+
+def _rgba_buffers(w=4, h=4):
+    a = host.Buffer(host.FORMAT_RGBA16F, w, h)
+    b = host.Buffer(host.FORMAT_RGBA16F, w, h)
+    return a, b
+
+
+def test_status_contract_without_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("this checks the no-device status")
+    d = host.find_node()
+    a, b = _rgba_buffers()
+    host.mh().mh_clear_log()
+    assert host.run_node(d, [a], [b]) == host.CPIPE_UNSUPPORTED
+    assert "no HIP device" in host.host_log()
+    assert engine.device_count() == 0
+    with pytest.raises(engine.EngineError) as e:
+        engine.Plan(64, 64, 3)
+    assert e.value.status == host.CPIPE_UNSUPPORTED
+
+
+def test_format_gate_and_arity():
+    d = host.find_node()
+    # compat mode takes RGBA16F only (bm3d.cpp:57-68 -> CPIPE_BAD_PRECISION)
+    f1 = host.Buffer(host.FORMAT_R32F, 8, 8)
+    f2 = host.Buffer(host.FORMAT_R32F, 8, 8)
+    assert host.run_node(d, [f1], [f2]) == host.CPIPE_BAD_PRECISION
+    a, _ = _rgba_buffers()
+    # mixed formats
+    assert host.run_node(d, [a], [f2], {"mode": "bm3d"}) == host.CPIPE_BAD_PRECISION
+    # two inputs, two outputs is not an accepted arity
+    g1 = host.Buffer(host.FORMAT_R32F, 8, 8)
+    g2 = host.Buffer(host.FORMAT_R32F, 8, 8)
+    assert host.run_node(d, [f1, g1], [f2, g2], {"mode": "bm3d"}) == host.CPIPE_BAD_INDEX
+    # size mismatch
+    big = host.Buffer(host.FORMAT_RGBA16F, 8, 4)
+    assert host.run_node(d, [a], [big]) == host.CPIPE_BAD_INDEX
+    # unknown enum values
+    b = host.Buffer(host.FORMAT_RGBA16F, 4, 4)
+    assert host.run_node(d, [a], [b], {"mode": "fast"}) == host.CPIPE_BAD_INDEX
+    assert host.run_node(d, [a], [b], {"precision": "int4"}) == host.CPIPE_BAD_INDEX
+
+
+def test_engine_argument_checks_without_device():
+    L = engine.lib()
+    desc = engine.PlanDesc(ctypes.sizeof(engine.PlanDesc), 64, 64, 2, 0, 0)
+    h = ctypes.c_void_p()
+    assert L.cpipe_bm3d_plan_create(ctypes.byref(desc), ctypes.byref(h)) == host.CPIPE_BAD_INDEX
+    desc = engine.PlanDesc(4, 64, 64, 3, 0, 0)
+    assert L.cpipe_bm3d_plan_create(ctypes.byref(desc), ctypes.byref(h)) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_plan_create(None, ctypes.byref(h)) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_compat_rgba16f(None, 4, None, 4, 4, 4, ctypes.c_float(0.03), 1, None) == host.CPIPE_BAD_INDEX
+    assert b"null" in L.cpipe_bm3d_last_error()
