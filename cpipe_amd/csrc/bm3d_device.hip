@@ -40,8 +40,11 @@ constexpr int NMAX = 16;      // max group size
 constexpr int TILE = 16;      // refs per tile edge
 constexpr int FOOT = (TILE - 1) * STRIDE + 2 * R + K;  // 91: max tile footprint edge
 constexpr int FOOT2 = FOOT * FOOT;
-constexpr int BM_LW = FOOT + 1;  // block-matching LDS row: +1 column for the dx=20 overrun
+constexpr int BM_COLS = FOOT + 1;  // block-matching LDS columns: +1 for the dx=20 overrun
+constexpr int BM_LW = 144;         // LDS row pitch: 144 = 16 mod 64, so the 4 ref rows x 16
+                                   // ref columns (stride 3) of a wave hit 64 distinct banks
 constexpr int BM_LH = FOOT;
+constexpr int PEND = 12;           // per-lane queue of candidates awaiting top-16 insertion
 
 enum Status { OK = 0, FAILED = 1, OOM = 3, BAD_INDEX = 5, INTERNAL = 7, UNSUPPORTED = 8 };
 
@@ -194,7 +197,9 @@ __global__ void k_compat_rgba16f(const uint16_t* __restrict__ in, size_t isp,
 // scans its clipped 39x39 window in (dy, dx) raster order, 4 candidates at a
 // time (11 LDS reads per patch row serve 4 SSDs), and keeps a sorted top-16
 // in registers (reference pinned first, strict insertion => ties keep raster
-// order). Only SSD < tau enters.
+// order). Only SSD < tau enters. Qualifying candidates wait in a short LDS
+// queue and are inserted in batches, so the wave runs the insertion network
+// once per batch instead of once per candidate.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void insert16(float (&bd)[NMAX], uint32_t (&bp)[NMAX], float s,
@@ -214,15 +219,16 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
                                                      uint32_t* __restrict__ pos,
                                                      uint8_t* __restrict__ cnt) {
     __shared__ float tile[BM_LW * BM_LH];
+    __shared__ float2 pend[PEND * 256];  // [slot][lane]: (ssd, packed position)
     const int bx0 = blockIdx.x * TILE, by0 = blockIdx.y * TILE;
     const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
     const int Y0 = ref_pos(by0, h) - R;
-    for (int idx = threadIdx.x; idx < BM_LW * BM_LH; idx += blockDim.x) {
-        const int yy = idx / BM_LW, xx = idx - yy * BM_LW;
+    for (int idx = threadIdx.x; idx < BM_COLS * BM_LH; idx += blockDim.x) {
+        const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
         const int gx = X0 + xx, gy = Y0 + yy;
         float v = 0.0f;
         if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = plane[(size_t)gy * w + gx];
-        tile[idx] = v;
+        tile[yy * BM_LW + xx] = v;
     }
     __syncthreads();
 
@@ -248,11 +254,28 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
     bd[0] = 0.0f;
     bp[0] = ((uint32_t)ry << 16) | (uint32_t)rx;
 
+    // Candidates below the current 16th best are queued (in scan order) and
+    // merged in batches; the queue filter only uses a stale, looser bound, so
+    // the result equals inserting each candidate immediately.
+    float2* q = pend + threadIdx.x;
+    int np = 0;
+    auto flush = [&]() {
+#pragma unroll
+        for (int k = 0; k < PEND; ++k) {
+            if (k < np) {
+                const float2 e = q[k * 256];
+                if (e.x < bd[NMAX - 1]) insert16(bd, bp, e.x, __float_as_uint(e.y));
+            }
+        }
+        np = 0;
+    };
+
     for (int dy = -R; dy <= R; ++dy) {
         const int qy = ry + dy;
         if (qy < 0 || qy > h - K) continue;
         const float* rowp = tile + (lry + dy) * BM_LW + lrx;
         for (int dx0 = -R; dx0 <= R; dx0 += 4) {
+            if (__any(np > PEND - 4)) flush();
             float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -276,11 +299,14 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
                 const int dx = dx0 + k;
                 const int qx = rx + dx;
                 const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
-                if (valid && s[k] < tau && s[k] < bd[NMAX - 1])
-                    insert16(bd, bp, s[k], ((uint32_t)qy << 16) | (uint32_t)qx);
+                if (valid && s[k] < tau && s[k] < bd[NMAX - 1]) {
+                    q[np * 256] = make_float2(s[k], __uint_as_float(((uint32_t)qy << 16) | (uint32_t)qx));
+                    ++np;
+                }
             }
         }
     }
+    flush();
     int n = 1;
 #pragma unroll
     for (int k = 1; k < NMAX; ++k) n += bd[k] < inf ? 1 : 0;
@@ -302,10 +328,11 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
 // Group transform + shrinkage + tile aggregation (fused), exact FP32.
 // One workgroup (4 waves) = one (tile, channel). Lane p = (i, j) holds pixel
 // (i, j) of every member patch of its wave's group in registers; the 2D DCT
-// passes gather the 8 row/column values with ds_bpermute; the Haar transform
+// passes go through a per-wave LDS scratch (16-byte row reads); the Haar transform
 // along the stack is in-register. Groups are taken in ref raster order, four
 // at a time (one per wave), and added to the LDS num/den footprint in wave
-// order, one wave per barrier phase, so the accumulation order is fixed.
+// order, one wave per barrier phase, so the accumulation order is fixed
+// (plain read-add-write: LDS float atomics cost ~180 cycles per instruction here).
 // ---------------------------------------------------------------------------
 
 struct GroupParams {
@@ -322,26 +349,61 @@ struct GroupParams {
     float* partial;
 };
 
-__device__ __forceinline__ float dct_fwd(float x, int li, int lj, const float (&Cr)[8],
-                                         const float (&Cc)[8]) {
-    float y = Cr[0] * __shfl(x, li * 8);
-#pragma unroll
-    for (int q = 1; q < 8; ++q) y = __builtin_fmaf(Cr[q], __shfl(x, li * 8 + q), y);
-    float t = Cc[0] * __shfl(y, lj);
-#pragma unroll
-    for (int q = 1; q < 8; ++q) t = __builtin_fmaf(Cc[q], __shfl(y, q * 8 + lj), t);
-    return t;
+// Separable 2D DCT of the N member patches a wave holds (lane p = (i, j) holds
+// pixel p of every member), through a per-wave LDS scratch of 8 patches: each
+// pass writes one value per lane and reads the lane's row (or column, stored
+// transposed) back as two 16-byte reads. Per member the operation order is a
+// row pass then a column pass, each an fma chain in index order.
+constexpr int SCR = 8;  // members per LDS scratch chunk
+
+__device__ __forceinline__ float row8(const float* p, const float (&c)[8]) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    const float4 b = *reinterpret_cast<const float4*>(p + 4);
+    float r = c[0] * a.x;
+    r = __builtin_fmaf(c[1], a.y, r);
+    r = __builtin_fmaf(c[2], a.z, r);
+    r = __builtin_fmaf(c[3], a.w, r);
+    r = __builtin_fmaf(c[4], b.x, r);
+    r = __builtin_fmaf(c[5], b.y, r);
+    r = __builtin_fmaf(c[6], b.z, r);
+    r = __builtin_fmaf(c[7], b.w, r);
+    return r;
 }
 
-__device__ __forceinline__ float dct_inv(float t, int li, int lj, const float (&Ci1)[8],
-                                         const float (&Ci2)[8]) {
-    float y = Ci1[0] * __shfl(t, lj);
+template <int N>
+__device__ __forceinline__ void dct_fwd_n(float (&v)[NMAX], float* sb, int lane, int li, int lj,
+                                          const float (&Cr)[8], const float (&Cc)[8]) {
 #pragma unroll
-    for (int q = 1; q < 8; ++q) y = __builtin_fmaf(Ci1[q], __shfl(t, q * 8 + lj), y);
-    float x = Ci2[0] * __shfl(y, li * 8);
+    for (int c0 = 0; c0 < N; c0 += SCR) {
+        constexpr int CH = N < SCR ? N : SCR;
+        float y[CH];
 #pragma unroll
-    for (int q = 1; q < 8; ++q) x = __builtin_fmaf(Ci2[q], __shfl(y, li * 8 + q), x);
-    return x;
+        for (int m = 0; m < CH; ++m) sb[m * 64 + lane] = v[c0 + m];
+#pragma unroll
+        for (int m = 0; m < CH; ++m) y[m] = row8(sb + m * 64 + li * 8, Cr);  // y[i][v=j]
+#pragma unroll
+        for (int m = 0; m < CH; ++m) sb[m * 64 + lj * 8 + li] = y[m];        // stored [v][i]
+#pragma unroll
+        for (int m = 0; m < CH; ++m) v[c0 + m] = row8(sb + m * 64 + lj * 8, Cc);  // t[u=i][v=j]
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void dct_inv_n(float (&v)[NMAX], float* sb, int lane, int li, int lj,
+                                          const float (&Ci1)[8], const float (&Ci2)[8]) {
+#pragma unroll
+    for (int c0 = 0; c0 < N; c0 += SCR) {
+        constexpr int CH = N < SCR ? N : SCR;
+        float y[CH];
+#pragma unroll
+        for (int m = 0; m < CH; ++m) sb[m * 64 + lj * 8 + li] = v[c0 + m];   // t stored [v][u]
+#pragma unroll
+        for (int m = 0; m < CH; ++m) y[m] = row8(sb + m * 64 + lj * 8, Ci1);  // y[i][v=j]
+#pragma unroll
+        for (int m = 0; m < CH; ++m) sb[m * 64 + lane] = y[m];               // stored [i][v]
+#pragma unroll
+        for (int m = 0; m < CH; ++m) v[c0 + m] = row8(sb + m * 64 + li * 8, Ci2);  // x[i][j]
+    }
 }
 
 template <int N>
@@ -378,33 +440,70 @@ __device__ __forceinline__ void haar_inv_n(float (&v)[NMAX], float s) {
     }
 }
 
-__device__ __forceinline__ void haar_fwd(float (&v)[NMAX], int n, float s) {
-    switch (n) {
-        case 16: haar_fwd_n<16>(v, s); break;
-        case 8: haar_fwd_n<8>(v, s); break;
-        case 4: haar_fwd_n<4>(v, s); break;
-        case 2: haar_fwd_n<2>(v, s); break;
-        default: break;
-    }
-}
-
-__device__ __forceinline__ void haar_inv(float (&v)[NMAX], int n, float s) {
-    switch (n) {
-        case 16: haar_inv_n<16>(v, s); break;
-        case 8: haar_inv_n<8>(v, s); break;
-        case 4: haar_inv_n<4>(v, s); break;
-        case 2: haar_inv_n<2>(v, s); break;
-        default: break;
-    }
-}
-
 __device__ __forceinline__ int floor_pow2(int c) {
     return c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+}
+
+struct LaneTables {
+    float Cr[8], Cc[8], Ci1[8], Ci2[8];
+};
+
+// Filters one group in place: loads the members' pixels (noisy, and the basic
+// estimate for step 2), 3D transform, shrinkage, inverse; returns the group's
+// aggregation weight (1/N_nonzero for the hard threshold, 1/sum W^2 for Wiener).
+template <int STEP, int N>
+__device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (&mp)[NMAX],
+                                              const float* __restrict__ src,
+                                              const float* __restrict__ bsrc, int w, float* sb,
+                                              int lane, int li, int lj, const LaneTables& T,
+                                              float hs, float thr, float s2) {
+    float b[NMAX];
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
+        const size_t off = (size_t)(qy + li) * w + qx + lj;
+        v[m] = src[off];
+        if (STEP == 2) b[m] = bsrc[off];
+    }
+    dct_fwd_n<N>(v, sb, lane, li, lj, T.Cr, T.Cc);
+    haar_fwd_n<N>(v, hs);
+    float wgt;
+    if (STEP == 1) {
+        int nnz = 0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            const bool keep = fabsf(v[m]) > thr;
+            nnz += keep ? 1 : 0;
+            v[m] = keep ? v[m] : 0.0f;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
+        wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
+    } else {
+        dct_fwd_n<N>(b, sb, lane, li, lj, T.Cr, T.Cc);
+        haar_fwd_n<N>(b, hs);
+        float aw = 0.0f;
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            const float bb = b[m] * b[m];
+            const float d = bb + s2;
+            const float W = d > 0.0f ? bb / d : 1.0f;
+            v[m] = W * v[m];
+            aw = (m == 0) ? W * W : __builtin_fmaf(W, W, aw);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
+        wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
+    }
+    haar_inv_n<N>(v, hs);
+    dct_inv_n<N>(v, sb, lane, li, lj, T.Ci1, T.Ci2);
+    return wgt;
 }
 
 template <int STEP>
 __global__ __launch_bounds__(256) void k_group(GroupParams p) {
     __shared__ float acc[2 * FOOT2];
+    __shared__ __align__(16) float scratch[4][SCR * 64];
     const int tile = blockIdx.x;
     const int c = blockIdx.y;
     const int tx = tile % p.ntx, ty = tile / p.ntx;
@@ -415,13 +514,14 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane >> 3, lj = lane & 7;
-    float Cr[8], Cc[8], Ci1[8], Ci2[8];
+    float* sb = scratch[wave];
+    LaneTables T;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        Cr[q] = c_dct[lj * 8 + q];
-        Cc[q] = c_dct[li * 8 + q];
-        Ci1[q] = c_dct[q * 8 + li];
-        Ci2[q] = c_dct[q * 8 + lj];
+        T.Cr[q] = c_dct[lj * 8 + q];
+        T.Cc[q] = c_dct[li * 8 + q];
+        T.Ci1[q] = c_dct[q * 8 + li];
+        T.Ci2[q] = c_dct[q * 8 + lj];
     }
     const float kw = c_kaiser2[lane];
     const float hs = (float)M_SQRT1_2;
@@ -454,61 +554,13 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
                 mp[4 * q + 2] = t.z;
                 mp[4 * q + 3] = t.w;
             }
-#pragma unroll
-            for (int m = 0; m < NMAX; ++m) {
-                v[m] = 0.0f;
-                if (m < n) {
-                    const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
-                    v[m] = src[(size_t)(qy + li) * w + qx + lj];
-                }
+            switch (n) {
+                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
+                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
+                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
+                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
+                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
             }
-#pragma unroll
-            for (int m = 0; m < NMAX; ++m)
-                if (m < n) v[m] = dct_fwd(v[m], li, lj, Cr, Cc);
-            haar_fwd(v, n, hs);
-            if (STEP == 1) {
-                int nnz = 0;
-#pragma unroll
-                for (int m = 0; m < NMAX; ++m) {
-                    if (m < n) {
-                        const bool keep = fabsf(v[m]) > thr;
-                        nnz += keep ? 1 : 0;
-                        v[m] = keep ? v[m] : 0.0f;
-                    }
-                }
-#pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
-                wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
-            } else {
-                float b[NMAX];
-#pragma unroll
-                for (int m = 0; m < NMAX; ++m) {
-                    b[m] = 0.0f;
-                    if (m < n) {
-                        const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
-                        b[m] = dct_fwd(bsrc[(size_t)(qy + li) * w + qx + lj], li, lj, Cr, Cc);
-                    }
-                }
-                haar_fwd(b, n, hs);
-                float aw = 0.0f;
-#pragma unroll
-                for (int m = 0; m < NMAX; ++m) {
-                    if (m < n) {
-                        const float bb = b[m] * b[m];
-                        const float d = bb + s2;
-                        const float W = d > 0.0f ? bb / d : 1.0f;
-                        v[m] = W * v[m];
-                        aw = (m == 0) ? W * W : __builtin_fmaf(W, W, aw);
-                    }
-                }
-#pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
-                wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
-            }
-            haar_inv(v, n, hs);
-#pragma unroll
-            for (int m = 0; m < NMAX; ++m)
-                if (m < n) v[m] = dct_inv(v[m], li, lj, Ci1, Ci2);
         }
         for (int k = 0; k < 4; ++k) {
             __syncthreads();
@@ -519,10 +571,10 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
                     if (m < n) {
                         const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
                         const int o = (qy + li - fy0) * fw + (qx + lj - fx0);
-                        __hip_atomic_fetch_add(&acc[o], wk * v[m], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(&acc[FOOT2 + o], wk, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        // one member per step: its 64 lanes touch 64 distinct pixels,
+                        // and LDS executes a wave's accesses in order
+                        acc[o] = acc[o] + wk * v[m];
+                        acc[FOOT2 + o] = acc[FOOT2 + o] + wk;
                     }
                 }
             }

@@ -54,13 +54,13 @@ def test_bm3d_fp32_matches_cpu_fp32(cuda, oracle_lib, name, seed):
     clean, noisy, sigma = synth.workload(name, seed=seed)
     print(f"case {name} seed {seed} sigma {sigma}", flush=True)
     g = _run_gpu(torch, noisy, sigma, want_basic=True, want_matches=True)
-    out, basic, (m1, m2) = oracle_lib.bm3d_f32(noisy, sigma, nthreads=8, want_basic=True,
-                                               want_matches=True)
+    out, basic, (p1, c1, p2, c2) = oracle_lib.bm3d_f32(noisy, sigma, nthreads=8, want_basic=True,
+                                                           want_matches=True)
     # match lists: bit-exact
-    np.testing.assert_array_equal(g["m1"][1], m1[1])
-    np.testing.assert_array_equal(g["m1"][0].view(np.uint32), m1[0])
-    np.testing.assert_array_equal(g["m2"][1], m2[1])
-    np.testing.assert_array_equal(g["m2"][0].view(np.uint32), m2[0])
+    np.testing.assert_array_equal(g["m1"][1], c1)
+    np.testing.assert_array_equal(g["m1"][0].view(np.uint32), p1)
+    np.testing.assert_array_equal(g["m2"][1], c2)
+    np.testing.assert_array_equal(g["m2"][0].view(np.uint32), p2)
     # basic and final estimates
     assert synth.max_abs(g["basic"], basic) <= 1e-5
     err = synth.max_abs(g["out"], out)
