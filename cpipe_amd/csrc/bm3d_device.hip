@@ -40,11 +40,16 @@ constexpr int NMAX = 16;      // max group size
 constexpr int TILE = 16;      // refs per tile edge
 constexpr int FOOT = (TILE - 1) * STRIDE + 2 * R + K;  // 91: max tile footprint edge
 constexpr int FOOT2 = FOOT * FOOT;
-constexpr int BM_COLS = FOOT + 1;  // block-matching LDS columns: +1 for the dx=20 overrun
-constexpr int BM_LW = 144;         // LDS row pitch: 144 = 16 mod 64, so the 4 ref rows x 16
-                                   // ref columns (stride 3) of a wave hit 64 distinct banks
-constexpr int BM_LH = FOOT;
-constexpr int PEND = 12;           // per-lane queue of candidates awaiting top-16 insertion
+// Block matching workgroup: 16 ref columns x 14 ref rows. Each wave owns 4
+// columns; a column is 16 consecutive lanes (one DPP row): 14 refs + 2 helper
+// lanes that only compute patch-row sums for the refs above them.
+constexpr int BM_TX = 16;
+constexpr int BM_TY = 14;
+constexpr int BM_COLS = (BM_TX - 1) * STRIDE + 2 * R + K + 1;  // 92: +1 for the dx=20 overrun
+constexpr int BM_LH = (BM_TY + 1) * STRIDE + 2 * R + 3 + 1;    // 86: helper rows included
+constexpr int BM_LW = 132;  // LDS row pitch = 4 mod 64: a wave's 16 rows x 4 columns of refs
+                            // (stride 3) read 64 distinct banks
+constexpr int PEND = 12;    // per-lane queue of candidates awaiting top-16 insertion
 
 enum Status { OK = 0, FAILED = 1, OOM = 3, BAD_INDEX = 5, INTERNAL = 7, UNSUPPORTED = 8 };
 
@@ -192,14 +197,16 @@ __global__ void k_compat_rgba16f(const uint16_t* __restrict__ in, size_t isp,
 
 // ---------------------------------------------------------------------------
 // Block matching, exact FP32.
-// One workgroup = one tile of 16x16 reference positions, one lane per ref.
-// The tile's search region (<= 91x91 px) is staged in LDS once; every lane
+// One workgroup = 16 x 14 reference positions, one lane per ref (+ helpers).
+// The tile's search region (<= 92x86 px) is staged in LDS once; every lane
 // scans its clipped 39x39 window in (dy, dx) raster order, 4 candidates at a
 // time (11 LDS reads per patch row serve 4 SSDs), and keeps a sorted top-16
 // in registers (reference pinned first, strict insertion => ties keep raster
 // order). Only SSD < tau enters. Qualifying candidates wait in a short LDS
 // queue and are inserted in batches, so the wave runs the insertion network
-// once per batch instead of once per candidate.
+// once per batch instead of once per candidate. Refs of a column are 3 rows
+// apart, so each lane computes the row sums of its own 3 patch rows and takes
+// rows 3..7 from the two lanes below (DPP row shifts): same sums, same order.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void insert16(float (&bd)[NMAX], uint32_t (&bp)[NMAX], float s,
@@ -214,13 +221,21 @@ __device__ __forceinline__ void insert16(float (&bd)[NMAX], uint32_t (&bp)[NMAX]
     }
 }
 
+__device__ __forceinline__ float dpp_row_shl(float v, int n) {
+    // lane l receives lane l+n of its 16-lane row (0 past the row end)
+    const int x = __builtin_bit_cast(int, v);
+    const int r = n == 1 ? __builtin_amdgcn_update_dpp(0, x, 0x101, 0xf, 0xf, true)
+                         : __builtin_amdgcn_update_dpp(0, x, 0x102, 0xf, 0xf, true);
+    return __builtin_bit_cast(float, r);
+}
+
 __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ plane, int w, int h,
                                                      int nx, int ny, float tau,
                                                      uint32_t* __restrict__ pos,
                                                      uint8_t* __restrict__ cnt) {
     __shared__ float tile[BM_LW * BM_LH];
-    __shared__ float2 pend[PEND * 256];  // [slot][lane]: (ssd, packed position)
-    const int bx0 = blockIdx.x * TILE, by0 = blockIdx.y * TILE;
+    __shared__ float2 pend[PEND * 256];  // [slot][thread]: (ssd, packed position)
+    const int bx0 = blockIdx.x * BM_TX, by0 = blockIdx.y * BM_TY;
     const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
     const int Y0 = ref_pos(by0, h) - R;
     for (int idx = threadIdx.x; idx < BM_COLS * BM_LH; idx += blockDim.x) {
@@ -232,16 +247,21 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
     }
     __syncthreads();
 
-    const int bx = bx0 + (threadIdx.x & 15), by = by0 + (threadIdx.x >> 4);
-    if (bx >= nx || by >= ny) return;  // no barrier follows
-    const int rx = ref_pos(bx, w), ry = ref_pos(by, h);
-    const int lrx = rx - X0, lry = ry - Y0;
-
-    float ref[64];
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) ref[i * 8 + j] = tile[(lry + i) * BM_LW + lrx + j];
+    const int l = threadIdx.x & 63;
+    const int a = l & 15;                               // ref row within the tile (14, 15: helpers)
+    const int bx = bx0 + (threadIdx.x >> 6) * 4 + (l >> 4);
+    const int by = by0 + a;
+    const bool real = a < BM_TY && bx < nx && by < ny;
+    const int rx = ref_pos(min(bx, nx - 1), w);
+    const int lrx = rx - X0;
+    // Regular tile: refs of a column sit exactly 3 rows apart, so a ref's
+    // patch rows 3..7 are the rows 0..2 / 0..1 of the refs one and two below.
+    // Only a tile holding the image's last ref row at a shorter stride is not.
+    const int last_in = ny - 1 - by0;
+    const bool regular = !(last_in >= 0 && last_in < BM_TY &&
+                           ref_pos(ny - 1, h) != ref_pos(by0, h) + STRIDE * last_in);
+    const int ry = regular ? ref_pos(by0, h) + STRIDE * a : ref_pos(min(by, ny - 1), h);
+    const int lry = ry - Y0;
 
     const float inf = __builtin_inff();
     float bd[NMAX];
@@ -269,44 +289,97 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
         }
         np = 0;
     };
+    auto offer = [&](float s, int dx, int dy) {
+        const int qx = rx + dx, qy = ry + dy;
+        const bool valid = real && dx <= R && qx >= 0 && qx <= w - K && qy >= 0 && qy <= h - K &&
+                           (dx != 0 || dy != 0);
+        if (valid && s < tau && s < bd[NMAX - 1]) {
+            q[np * 256] = make_float2(s, __uint_as_float(((uint32_t)qy << 16) | (uint32_t)qx));
+            ++np;
+        }
+    };
 
-    for (int dy = -R; dy <= R; ++dy) {
-        const int qy = ry + dy;
-        if (qy < 0 || qy > h - K) continue;
-        const float* rowp = tile + (lry + dy) * BM_LW + lrx;
-        for (int dx0 = -R; dx0 <= R; dx0 += 4) {
-            if (__any(np > PEND - 4)) flush();
-            float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (regular) {
+        float ref[3][8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                float c[11];
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+            for (int j = 0; j < 8; ++j) ref[i][j] = tile[(lry + i) * BM_LW + lrx + j];
+        for (int dy = -R; dy <= R; ++dy) {
+            const float* rowp = tile + (lry + dy) * BM_LW + lrx;
+            for (int dx0 = -R; dx0 <= R; dx0 += 4) {
+                if (__any(np > PEND - 4)) flush();
+                float rs[3][4];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    float c[11];
+#pragma unroll
+                    for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        float e = ref[i][0] - c[k];
+                        float r = e * e;
+#pragma unroll
+                        for (int j = 1; j < 8; ++j) {
+                            e = ref[i][j] - c[k + j];
+                            r = __builtin_fmaf(e, e, r);
+                        }
+                        rs[i][k] = r;
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    float e = ref[i * 8] - c[k];
-                    float r = e * e;
-#pragma unroll
-                    for (int j = 1; j < 8; ++j) {
-                        e = ref[i * 8 + j] - c[k + j];
-                        r = __builtin_fmaf(e, e, r);
-                    }
-                    s[k] = s[k] + r;
+                    const float r3 = dpp_row_shl(rs[0][k], 1), r4 = dpp_row_shl(rs[1][k], 1);
+                    const float r5 = dpp_row_shl(rs[2][k], 1), r6 = dpp_row_shl(rs[0][k], 2);
+                    const float r7 = dpp_row_shl(rs[1][k], 2);
+                    float sum = rs[0][k] + rs[1][k];
+                    sum = sum + rs[2][k];
+                    sum = sum + r3;
+                    sum = sum + r4;
+                    sum = sum + r5;
+                    sum = sum + r6;
+                    sum = sum + r7;
+                    offer(sum, dx0 + k, dy);
                 }
             }
+        }
+    } else {
+        float ref[64];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int dx = dx0 + k;
-                const int qx = rx + dx;
-                const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
-                if (valid && s[k] < tau && s[k] < bd[NMAX - 1]) {
-                    q[np * 256] = make_float2(s[k], __uint_as_float(((uint32_t)qy << 16) | (uint32_t)qx));
-                    ++np;
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ref[i * 8 + j] = tile[(lry + i) * BM_LW + lrx + j];
+        for (int dy = -R; dy <= R; ++dy) {
+            const int qy = ry + dy;
+            if (!real || qy < 0 || qy > h - K) continue;
+            const float* rowp = tile + (lry + dy) * BM_LW + lrx;
+            for (int dx0 = -R; dx0 <= R; dx0 += 4) {
+                if (__any(np > PEND - 4)) flush();
+                float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    float c[11];
+#pragma unroll
+                    for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        float e = ref[i * 8] - c[k];
+                        float r = e * e;
+#pragma unroll
+                        for (int j = 1; j < 8; ++j) {
+                            e = ref[i * 8 + j] - c[k + j];
+                            r = __builtin_fmaf(e, e, r);
+                        }
+                        s[k] = s[k] + r;
+                    }
                 }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) offer(s[k], dx0 + k, dy);
             }
         }
     }
     flush();
+    if (!real) return;
     int n = 1;
 #pragma unroll
     for (int k = 1; k < NMAX; ++k) n += bd[k] < inf ? 1 : 0;
@@ -314,12 +387,12 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
     uint32_t* o = pos + r * NMAX;
 #pragma unroll
     for (int k = 0; k < NMAX; k += 4) {
-        uint4 q;
-        q.x = k + 0 < n ? bp[k + 0] : 0u;
-        q.y = k + 1 < n ? bp[k + 1] : 0u;
-        q.z = k + 2 < n ? bp[k + 2] : 0u;
-        q.w = k + 3 < n ? bp[k + 3] : 0u;
-        *reinterpret_cast<uint4*>(o + k) = q;
+        uint4 v;
+        v.x = k + 0 < n ? bp[k + 0] : 0u;
+        v.y = k + 1 < n ? bp[k + 1] : 0u;
+        v.z = k + 2 < n ? bp[k + 2] : 0u;
+        v.w = k + 3 < n ? bp[k + 3] : 0u;
+        *reinterpret_cast<uint4*>(o + k) = v;
     }
     cnt[r] = (uint8_t)n;
 }
@@ -549,10 +622,10 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint4 t = pp[q];
-                mp[4 * q + 0] = t.x;
-                mp[4 * q + 1] = t.y;
-                mp[4 * q + 2] = t.z;
-                mp[4 * q + 3] = t.w;
+                mp[4 * q + 0] = __builtin_amdgcn_readfirstlane(t.x);  // wave-uniform
+                mp[4 * q + 1] = __builtin_amdgcn_readfirstlane(t.y);
+                mp[4 * q + 2] = __builtin_amdgcn_readfirstlane(t.z);
+                mp[4 * q + 3] = __builtin_amdgcn_readfirstlane(t.w);
             }
             switch (n) {
                 case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
@@ -566,15 +639,48 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
             __syncthreads();
             if (wave == k && has) {
                 const float wk = wgt * kw;
+                const int lo = (li - fy0) * fw + (lj - fx0);
+                // One member per step: its 64 lanes touch 64 distinct pixels, and
+                // LDS executes a wave's accesses in order. Four consecutive members
+                // whose patches are pairwise disjoint share one read/add/write
+                // round trip; the per-pixel order is unchanged either way.
 #pragma unroll
-                for (int m = 0; m < NMAX; ++m) {
-                    if (m < n) {
-                        const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
-                        const int o = (qy + li - fy0) * fw + (qx + lj - fx0);
-                        // one member per step: its 64 lanes touch 64 distinct pixels,
-                        // and LDS executes a wave's accesses in order
-                        acc[o] = acc[o] + wk * v[m];
-                        acc[FOOT2 + o] = acc[FOOT2 + o] + wk;
+                for (int m0 = 0; m0 < NMAX; m0 += 4) {
+                    if (m0 < n) {
+                        int o[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            o[j] = (int)(mp[m0 + j] >> 16) * fw + (int)(mp[m0 + j] & 0xffffu) + lo;
+                        bool disjoint = n >= 4;
+#pragma unroll
+                        for (int a = 0; a < 4; ++a)
+#pragma unroll
+                            for (int b = a + 1; b < 4; ++b) {
+                                const int dx = (int)(mp[m0 + a] & 0xffffu) - (int)(mp[m0 + b] & 0xffffu);
+                                const int dy = (int)(mp[m0 + a] >> 16) - (int)(mp[m0 + b] >> 16);
+                                disjoint = disjoint && (dx >= K || dx <= -K || dy >= K || dy <= -K);
+                            }
+                        if (disjoint) {
+                            float an[4], ad[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                an[j] = acc[o[j]];
+                                ad[j] = acc[FOOT2 + o[j]];
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                acc[o[j]] = an[j] + wk * v[m0 + j];
+                                acc[FOOT2 + o[j]] = ad[j] + wk;
+                            }
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                if (m0 + j < n) {
+                                    acc[o[j]] = acc[o[j]] + wk * v[m0 + j];
+                                    acc[FOOT2 + o[j]] = acc[FOOT2 + o[j]] + wk;
+                                }
+                            }
+                        }
                     }
                 }
             }
@@ -791,7 +897,8 @@ int enqueue(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float* 
     }
     for (int step = 1; step <= 2; ++step) {
         const float* bmplane = step == 1 ? z[0] : bz[0];
-        k_block_match<<<dim3(p->ntx, p->nty), 256, 0, st>>>(bmplane, w, h, p->nx, p->ny,
+        const dim3 bmgrid((p->nx + BM_TX - 1) / BM_TX, (p->ny + BM_TY - 1) / BM_TY);
+        k_block_match<<<bmgrid, 256, 0, st>>>(bmplane, w, h, p->nx, p->ny,
                                                             tau_ssd(step), p->pos[step - 1],
                                                             p->cnt[step - 1]);
         HIP_CHECK(hipGetLastError());
