@@ -100,6 +100,7 @@ def main():
         plan.run(src, dst, sigma, stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    plan.check()
     stage_ms, runs = plan.timing()
     ms_per_step = wall * 1e3 / args.steps
     mpx = w * h / (ms_per_step / 1e3) / 1e6

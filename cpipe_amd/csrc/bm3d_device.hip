@@ -403,8 +403,9 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
 // (i, j) of every member patch of its wave's group in registers; the 2D DCT
 // passes go through a per-wave LDS scratch (16-byte row reads); the Haar transform
 // along the stack is in-register. Groups are taken in ref raster order, four
-// at a time (one per wave), and added to the LDS num/den footprint in wave
-// order, one wave per barrier phase, so the accumulation order is fixed
+// at a time (one per wave), and added to the LDS num/den footprint strictly in
+// group order (an LDS turn token passed from wave to wave), so the accumulation
+// order is fixed while the other waves keep filtering their next groups
 // (plain read-add-write: LDS float atomics cost ~180 cycles per instruction here).
 // ---------------------------------------------------------------------------
 
@@ -420,6 +421,7 @@ struct GroupParams {
     const int* fy0;
     const int* fy1;
     float* partial;
+    int* err;  // set to 1 if an ordering wait timed out (never expected)
 };
 
 // Separable 2D DCT of the N member patches a wave holds (lane p = (i, j) holds
@@ -577,8 +579,11 @@ template <int STEP>
 __global__ __launch_bounds__(256) void k_group(GroupParams p) {
     __shared__ float acc[2 * FOOT2];
     __shared__ __align__(16) float scratch[4][SCR * 64];
+    __shared__ int turn;  // number of groups of this tile already accumulated
     const int tile = blockIdx.x;
     const int c = blockIdx.y;
+    if (threadIdx.x == 0) turn = 0;
+    bool timed_out = false;
     const int tx = tile % p.ntx, ty = tile / p.ntx;
     const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
     const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
@@ -635,9 +640,22 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
                 default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
             }
         }
-        for (int k = 0; k < 4; ++k) {
-            __syncthreads();
-            if (wave == k && has) {
+        // Group g is accumulated right after group g-1: wait for the turn token
+        // (in LDS) to reach g. Every wave takes its turns in increasing g and
+        // turn g-1 never waits on turn g, so the chain always completes; the
+        // spin is bounded anyway and reports a timeout instead of hanging.
+        {
+            int spins = 0;
+            while (__hip_atomic_load(&turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1 << 24)) {
+                    timed_out = true;
+                    break;
+                }
+            }
+        }
+        {
+            if (has) {
                 const float wk = wgt * kw;
                 const int lo = (li - fy0) * fw + (lj - fx0);
                 // One member per step: its 64 lanes touch 64 distinct pixels, and
@@ -684,8 +702,10 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
                     }
                 }
             }
+            __hip_atomic_store(&turn, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
+    if (timed_out && lane == 0) atomicOr(p.err, 1);
     __syncthreads();
     float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * (2 * FOOT2);
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
@@ -814,6 +834,7 @@ struct cpipe_bm3d_plan_s {
     uint32_t* pos[2] = {nullptr, nullptr};
     uint8_t* cnt[2] = {nullptr, nullptr};
     float* partial = nullptr;
+    int* err = nullptr;  // device flag: an ordering wait timed out
     int* geo = nullptr;  // txlo[w] txhi[w] tylo[h] tyhi[h] fx0[ntx] fx1[ntx] fy0[nty] fy1[nty]
     bool timing = false;
     hipEvent_t ev[CPIPE_BM3D_T_COUNT + 1] = {};
@@ -844,6 +865,7 @@ void free_plan(cpipe_bm3d_plan* p) {
         (void)hipFree(p->cnt[s]);
     }
     (void)hipFree(p->partial);
+    (void)hipFree(p->err);
     (void)hipFree(p->geo);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
@@ -922,6 +944,7 @@ int enqueue(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float* 
         gp.fy0 = fy0;
         gp.fy1 = fy1;
         gp.partial = p->partial;
+        gp.err = p->err;
         const dim3 ggrid(p->ntx * p->nty, C);
         if (step == 1) {
             k_group<1><<<ggrid, 256, 0, st>>>(gp);
@@ -1018,6 +1041,7 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
             (rc = alloc((void**)&p->pos[1], sizeof(uint32_t) * nref * NMAX)) ||
             (rc = alloc((void**)&p->cnt[0], nref)) || (rc = alloc((void**)&p->cnt[1], nref)) ||
             (rc = alloc((void**)&p->partial, sizeof(float) * 2 * FOOT2 * ntiles * p->c)) ||
+            (rc = alloc((void**)&p->err, sizeof(int))) ||
             (rc = alloc((void**)&p->geo, sizeof(int) * ngeo))) {
             free_plan(p);
             return rc;
@@ -1061,7 +1085,8 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
                 free_plan(p);
                 return fail(INTERNAL, "tile cover has a gap in y");
             }
-        hipError_t e = hipMemcpy(p->geo, geo.data(), sizeof(int) * ngeo, hipMemcpyHostToDevice);
+        hipError_t e = hipMemset(p->err, 0, sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(p->geo, geo.data(), sizeof(int) * ngeo, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             free_plan(p);
             return fail(INTERNAL, std::string("hipMemcpy geo: ") + hipGetErrorString(e));
@@ -1153,6 +1178,17 @@ int cpipe_bm3d_plan_matches(cpipe_bm3d_plan* p, int step, uint32_t* pos, uint8_t
     HIP_CHECK(hipMemcpyAsync(pos, p->pos[step - 1], nref * NMAX * sizeof(uint32_t),
                              hipMemcpyDeviceToDevice, st));
     HIP_CHECK(hipMemcpyAsync(count, p->cnt[step - 1], nref, hipMemcpyDeviceToDevice, st));
+    return OK;
+}
+
+int cpipe_bm3d_plan_check(cpipe_bm3d_plan* p) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    if (!p->err) return OK;
+    DeviceGuard guard(p->device);
+    int flag = 0;
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(&flag, p->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) return fail(INTERNAL, "an aggregation ordering wait timed out; results are invalid");
     return OK;
 }
 

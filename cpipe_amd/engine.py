@@ -24,6 +24,7 @@ EXPORTS = (
     "cpipe_bm3d_last_error", "cpipe_bm3d_device_count", "cpipe_bm3d_plan_create",
     "cpipe_bm3d_plan_destroy", "cpipe_bm3d_plan_run", "cpipe_bm3d_plan_run_rgba16f",
     "cpipe_bm3d_plan_basic", "cpipe_bm3d_plan_matches", "cpipe_bm3d_plan_ref_grid",
+    "cpipe_bm3d_plan_check",
     "cpipe_bm3d_plan_set_timing", "cpipe_bm3d_plan_timing", "cpipe_bm3d_compat_rgba16f",
     "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_node_descriptor",
     "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d",
@@ -64,6 +65,7 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_plan_basic.argtypes = [P, P, S, P]
     L.cpipe_bm3d_plan_matches.argtypes = [P, I, P, P, P]
     L.cpipe_bm3d_plan_ref_grid.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I)]
+    L.cpipe_bm3d_plan_check.argtypes = [P]
     L.cpipe_bm3d_plan_set_timing.argtypes = [P, I]
     L.cpipe_bm3d_plan_timing.argtypes = [P, ctypes.POINTER(F), I, ctypes.POINTER(I)]
     L.cpipe_bm3d_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I, P]
@@ -163,6 +165,10 @@ class Plan:
         _check(lib().cpipe_bm3d_plan_matches(self._h, int(step), pos.data_ptr(), cnt.data_ptr(),
                                              _stream_handle(stream)))
         return pos, cnt
+
+    def check(self) -> None:
+        """Synchronise and raise if a run hit an internal consistency check."""
+        _check(lib().cpipe_bm3d_plan_check(self._h))
 
     def set_timing(self, enable: bool) -> None:
         _check(lib().cpipe_bm3d_plan_set_timing(self._h, 1 if enable else 0))

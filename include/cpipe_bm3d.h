@@ -92,6 +92,10 @@ int cpipe_bm3d_plan_basic(cpipe_bm3d_plan* plan, float* const* out_planes, size_
                           void* stream);
 int cpipe_bm3d_plan_matches(cpipe_bm3d_plan* plan, int step, uint32_t* pos, uint8_t* count,
                             void* stream);
+/* Synchronises the device and reports whether any run of this plan hit an
+ * internal consistency check (CPIPE_INTERNAL_ERROR; results invalid). */
+int cpipe_bm3d_plan_check(cpipe_bm3d_plan* plan);
+
 /* Reference grid of the plan: number of reference positions along x and y. */
 int cpipe_bm3d_plan_ref_grid(const cpipe_bm3d_plan* plan, int* nx, int* ny);
 

@@ -29,6 +29,7 @@ def _run_gpu(torch, noisy, sigma, want_basic=False, want_matches=False):
         res["m1"] = plan.matches(1)
         res["m2"] = plan.matches(2)
     torch.cuda.synchronize()
+    plan.check()
     res["out"] = out.cpu().numpy()
     if want_basic:
         res["basic"] = res["basic"].cpu().numpy()
