@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", default="2048x1536")
+    ap.add_argument("--pilot", default=None,
+                    help="learned-pilot weight blob (default: the packaged weights if present)")
+    ap.add_argument("--no-pilot", action="store_true", help="skip the learned-pilot form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -137,6 +140,38 @@ def main():
                 "flop_per_launch": flop, "launch_ms": round(ms, 4),
                 "end_to_end_tflops": round(total_flop / (ms_per_step / 1e3) / 1e12, 3)}
 
+    learned = None
+    from cpipe_amd import pilot as pilot_mod
+    blob = args.pilot or (pilot_mod.DEFAULT_WEIGHTS if os.path.exists(pilot_mod.DEFAULT_WEIGHTS) else None)
+    if blob and not args.no_pilot and c == 3:
+        net = pilot_mod.Pilot(seed=None, weights=blob)
+        plan.set_pilot(net)
+        dst2 = torch.empty_like(src)
+        plan.set_timing(False)
+        for _ in range(args.warmup):
+            plan.run(src, dst2, sigma, stream)
+        torch.cuda.synchronize()
+        plan.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run(src, dst2, sigma, stream)
+        torch.cuda.synchronize()
+        wall2 = time.perf_counter() - t0
+        plan.check()
+        st2, runs2 = plan.timing()
+        rep = plan.pilot_report()
+        out2 = dst2.cpu().numpy()
+        ms2 = wall2 * 1e3 / args.steps
+        psnr2 = synth.psnr(out2, clean)
+        learned = {"value": round(w * h / (ms2 / 1e3) / 1e6, 3), "unit": "Mpixel/s",
+                   "ms_per_step": round(ms2, 3), "psnr_out_db": round(psnr2, 3),
+                   "delta_psnr_db_vs_fp32": round(psnr_out - psnr2, 4),
+                   "speedup_vs_fp32": round(ms_per_step / ms2, 3),
+                   "stage_ms": {k: round(v / runs2, 4) for k, v in st2.items()},
+                   "spot_checks": rep, "weights": os.path.basename(blob)}
+        plan.set_pilot(None)
+        print(f"# learned pilot: {json.dumps(learned)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         sw, sh = (int(v) for v in args.cpu_sample.split("x"))
@@ -155,6 +190,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_pilot": learned,
     }
     if cpu:
         line["gpu_over_cpu"] = round(mpx / cpu["value"], 1)

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "cpipe_bm3d.h"
+#include "pilot.h"
 
 namespace {
 
@@ -232,10 +233,15 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
 __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ plane, int w, int h,
                                                      int nx, int ny, float tau,
                                                      uint32_t* __restrict__ pos,
-                                                     uint8_t* __restrict__ cnt) {
+                                                     uint8_t* __restrict__ cnt, int btx0, int bty0,
+                                                     const int2* __restrict__ tiles,
+                                                     const int* __restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;  // step skipped (decided on the device)
     __shared__ float tile[BM_LW * BM_LH];
     __shared__ float2 pend[PEND * 256];  // [slot][thread]: (ssd, packed position)
-    const int bx0 = blockIdx.x * BM_TX, by0 = blockIdx.y * BM_TY;
+    // tile rectangle (btx0, bty0) + block index, or an explicit tile list
+    const int2 bt = tiles ? tiles[blockIdx.x] : make_int2(blockIdx.x + btx0, blockIdx.y + bty0);
+    const int bx0 = bt.x * BM_TX, by0 = bt.y * BM_TY;
     const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
     const int Y0 = ref_pos(by0, h) - R;
     for (int idx = threadIdx.x; idx < BM_COLS * BM_LH; idx += blockDim.x) {
@@ -422,6 +428,9 @@ struct GroupParams {
     const int* fy1;
     float* partial;
     int* err;  // set to 1 if an ordering wait timed out (never expected)
+    int tx0, ty0, ntx_sub;  // tile sub-rectangle processed by this launch
+    const int* tiles;       // or an explicit list of tile indices (ty * ntx + tx)
+    const int* gate;        // null, or run only if *gate != 0
 };
 
 // Separable 2D DCT of the N member patches a wave holds (lane p = (i, j) holds
@@ -580,11 +589,14 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
     __shared__ float acc[2 * FOOT2];
     __shared__ __align__(16) float scratch[4][SCR * 64];
     __shared__ int turn;  // number of groups of this tile already accumulated
-    const int tile = blockIdx.x;
+    if (p.gate != nullptr && *p.gate == 0) return;
+    const int tile = p.tiles ? p.tiles[blockIdx.x]
+                             : (p.ty0 + (int)blockIdx.x / p.ntx_sub) * p.ntx + p.tx0 +
+                                   (int)blockIdx.x % p.ntx_sub;
+    const int tx = tile % p.ntx, ty = tile / p.ntx;
     const int c = blockIdx.y;
     if (threadIdx.x == 0) turn = 0;
     bool timed_out = false;
-    const int tx = tile % p.ntx, ty = tile / p.ntx;
     const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
     const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
     for (int idx = threadIdx.x; idx < 2 * FOOT2; idx += blockDim.x) acc[idx] = 0.0f;
@@ -734,12 +746,17 @@ struct MergeParams {
     float* out[3];
     size_t out_pitch;
     int inverse_colour;
+    int x0, y0, x1, y1;      // pixel rectangle of this launch
+    const int4* rects;       // or a list of rectangles (x0, y0, x1, y1), one per blockIdx.z
+    const int* gate;         // null, or run only if *gate != 0
 };
 
 __global__ void k_merge(MergeParams p) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= p.w) return;
+    if (p.gate != nullptr && *p.gate == 0) return;
+    const int4 rc = p.rects ? p.rects[blockIdx.z] : make_int4(p.x0, p.y0, p.x1, p.y1);
+    const int x = rc.x + blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = rc.y + blockIdx.y;
+    if (x >= rc.z || y >= rc.w) return;
     const int txl = p.txlo[x], txh = p.txhi[x], tyl = p.tylo[y], tyh = p.tyhi[y];
     float o[3] = {0.0f, 0.0f, 0.0f};
     for (int c = 0; c < p.channels; ++c) {
@@ -766,6 +783,77 @@ __global__ void k_merge(MergeParams p) {
     } else {
         for (int c = 0; c < p.channels; ++c) p.out[c][oi] = o[c];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Learned-pilot spot checks. For each checked rectangle the plain step-1
+// estimate (bzp) is computed exactly; the method-noise ratio
+// delta = |10 log10(|z - pilot|^2 / |z - plain|^2)| (three channels) decides
+// whether the rectangle falls back to the plain estimate. If more than a set
+// fraction of the checked rectangles fall back, the gate enables a full plain
+// step 1.
+// ---------------------------------------------------------------------------
+
+struct CheckRect {
+    int x0, y0, x1, y1;
+};
+static_assert(sizeof(CheckRect) == sizeof(int4), "check rectangles double as merge rectangles");
+
+__global__ __launch_bounds__(256) void k_pilot_check(const float* z0, const float* z1,
+                                                     const float* z2, float* b0, float* b1,
+                                                     float* b2, const float* p0, const float* p1,
+                                                     const float* p2, int w,
+                                                     const CheckRect* __restrict__ rects,
+                                                     float thr_db, float* __restrict__ report) {
+    __shared__ float ra[256], rb[256];
+    const CheckRect r = rects[blockIdx.x];
+    const int rw = r.x1 - r.x0, n = rw * (r.y1 - r.y0);
+    float a = 0.0f, b = 0.0f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const size_t o = (size_t)(r.y0 + i / rw) * w + r.x0 + i % rw;
+        const float zs[3] = {z0[o], z1[o], z2[o]};
+        const float ls[3] = {b0[o], b1[o], b2[o]};
+        const float ps[3] = {p0[o], p1[o], p2[o]};
+        for (int c = 0; c < 3; ++c) {
+            const float dl = zs[c] - ls[c], dp = zs[c] - ps[c];
+            a = __builtin_fmaf(dl, dl, a);
+            b = __builtin_fmaf(dp, dp, b);
+        }
+    }
+    ra[threadIdx.x] = a;
+    rb[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            ra[threadIdx.x] += ra[threadIdx.x + off];
+            rb[threadIdx.x] += rb[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    const float sa = ra[0], sb = rb[0];
+    const float delta = (sa > 0.0f && sb > 0.0f) ? fabsf(10.0f * log10f(sa / sb))
+                                                  : (sa == sb ? 0.0f : 1e30f);
+    const bool fall = delta > thr_db;
+    if (fall) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const size_t o = (size_t)(r.y0 + i / rw) * w + r.x0 + i % rw;
+            b0[o] = p0[o];
+            b1[o] = p1[o];
+            b2[o] = p2[o];
+        }
+    }
+    if (threadIdx.x == 0) {
+        report[2 * blockIdx.x] = delta;
+        report[2 * blockIdx.x + 1] = fall ? 1.0f : 0.0f;
+    }
+}
+
+__global__ void k_pilot_decide(const float* __restrict__ report, int n, float frac,
+                               int* __restrict__ gate) {
+    if (threadIdx.x != 0) return;
+    int falls = 0;
+    for (int i = 0; i < n; ++i) falls += report[2 * i + 1] > 0.5f ? 1 : 0;
+    *gate = ((float)falls > frac * (float)n) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -837,10 +925,30 @@ struct cpipe_bm3d_plan_s {
     int* err = nullptr;  // device flag: an ordering wait timed out
     int* geo = nullptr;  // txlo[w] txhi[w] tylo[h] tyhi[h] fx0[ntx] fx1[ntx] fy0[nty] fy1[nty]
     bool timing = false;
-    hipEvent_t ev[CPIPE_BM3D_T_COUNT + 1] = {};
+    hipEvent_t ev[2 * CPIPE_BM3D_T_COUNT] = {};  // start/end per stage
+    bool stage_ran[CPIPE_BM3D_T_COUNT] = {};
     float ms[CPIPE_BM3D_T_COUNT] = {};
     int runs = 0;
     bool ran = false;
+    std::vector<int> geo_host;
+    // learned pilot (replaces step 1) and its spot checks
+    cpipe_bm3d_pilot* pilot = nullptr;
+    float pilot_thr_db = 0.5f;
+    float pilot_full_frac = 0.5f;
+    struct Check {
+        int btx0, bty0, btx1, bty1;  // block-matching tiles
+        int tx0, ty0, tx1, ty1;      // aggregation tiles
+        int x0, y0, x1, y1;          // pixels
+    };
+    std::vector<Check> checks;
+    void* rects = nullptr;     // CheckRect[n] on the device
+    // the checked rectangles' work as lists, so each kernel runs once for all checks
+    int2* chk_bm = nullptr;    // distinct block-matching tiles
+    int* chk_tiles = nullptr;  // distinct aggregation tiles (ty * ntx + tx)
+    int n_chk_bm = 0, n_chk_tiles = 0, chk_w = 0, chk_h = 0;  // counts, largest rectangle
+    float* report = nullptr;   // [n][2]: delta dB, fell back
+    int* gate = nullptr;       // 1: run the plain step 1 on the whole frame
+    float* bzp = nullptr;      // plain step-1 planes on checked rectangles
 };
 
 namespace {
@@ -867,9 +975,139 @@ void free_plan(cpipe_bm3d_plan* p) {
     (void)hipFree(p->partial);
     (void)hipFree(p->err);
     (void)hipFree(p->geo);
+    (void)hipFree(p->rects);
+    (void)hipFree(p->chk_bm);
+    (void)hipFree(p->chk_tiles);
+    (void)hipFree(p->report);
+    (void)hipFree(p->gate);
+    (void)hipFree(p->bzp);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
+}
+
+struct StepRange {
+    int btx0, bty0, btx1, bty1;  // block-matching tiles
+    int tx0, ty0, tx1, ty1;      // aggregation tiles
+    int x0, y0, x1, y1;          // merged pixels
+    // or explicit lists (when bm_list is set): tiles, and rectangles of at most
+    // x1 x y1 pixels
+    const int2* bm_list = nullptr;
+    const int* tile_list = nullptr;
+    const int4* rect_list = nullptr;
+    int n_bm = 0, n_tiles = 0, n_rects = 0;
+};
+
+StepRange full_range(const cpipe_bm3d_plan* p) {
+    return {0, 0, (p->nx + BM_TX - 1) / BM_TX, (p->ny + BM_TY - 1) / BM_TY, 0, 0, p->ntx, p->nty,
+            0, 0, p->w, p->h};
+}
+
+struct RunCtx {
+    cpipe_bm3d_plan* p;
+    hipStream_t st;
+    float* z[3];
+    float* bz[3];
+    float sig[3];
+    int mark(int stage, bool end) {
+        if (!p->timing) return OK;
+        if (!end) p->stage_ran[stage] = true;
+        HIP_CHECK(hipEventRecord(p->ev[2 * stage + (end ? 1 : 0)], st));
+        return OK;
+    }
+};
+
+// One BM3D step over a tile sub-rectangle: block matching, group filtering and
+// aggregation, merge of the pixel rectangle into `out` (opponent planes, or
+// RGB with inverse_colour). `gate` (device) skips the kernels when zero.
+int run_step(RunCtx& r, int step, const StepRange& g, float* const* out, size_t out_pitch,
+             int inverse_colour, const int* gate, bool timed) {
+    cpipe_bm3d_plan* p = r.p;
+    const int w = p->w, h = p->h, C = p->c;
+    const int* txlo = p->geo;
+    const int* txhi = txlo + w;
+    const int* tylo = txhi + w;
+    const int* tyhi = tylo + h;
+    const int* fx0 = tyhi + h;
+    const int* fx1 = fx0 + p->ntx;
+    const int* fy0 = fx1 + p->ntx;
+    const int* fy1 = fy0 + p->nty;
+    const int sbm = step == 1 ? CPIPE_BM3D_T_BM1 : CPIPE_BM3D_T_BM2;
+    const int sgr = step == 1 ? CPIPE_BM3D_T_GROUP1 : CPIPE_BM3D_T_GROUP2;
+    const int smg = step == 1 ? CPIPE_BM3D_T_MERGE1 : CPIPE_BM3D_T_MERGE2;
+    int rc;
+    if (timed && (rc = r.mark(sbm, false))) return rc;
+    const float* bmplane = step == 1 ? r.z[0] : r.bz[0];
+    const dim3 bmgrid = g.bm_list ? dim3(g.n_bm) : dim3(g.btx1 - g.btx0, g.bty1 - g.bty0);
+    k_block_match<<<bmgrid, 256, 0, r.st>>>(bmplane, w, h, p->nx, p->ny, tau_ssd(step),
+                                            p->pos[step - 1], p->cnt[step - 1], g.btx0, g.bty0,
+                                            g.bm_list, gate);
+    HIP_CHECK(hipGetLastError());
+    if (timed && (rc = r.mark(sbm, true))) return rc;
+    if (timed && (rc = r.mark(sgr, false))) return rc;
+    GroupParams gp{};
+    for (int c = 0; c < 3; ++c) {
+        gp.noisy[c] = c < C ? r.z[c] : r.z[0];
+        gp.basic[c] = c < C ? r.bz[c] : r.bz[0];
+        gp.sig[c] = r.sig[c];
+    }
+    gp.w = w;
+    gp.h = h;
+    gp.nx = p->nx;
+    gp.ny = p->ny;
+    gp.ntx = p->ntx;
+    gp.nty = p->nty;
+    gp.pos = p->pos[step - 1];
+    gp.cnt = p->cnt[step - 1];
+    gp.fx0 = fx0;
+    gp.fx1 = fx1;
+    gp.fy0 = fy0;
+    gp.fy1 = fy1;
+    gp.partial = p->partial;
+    gp.err = p->err;
+    gp.tx0 = g.tx0;
+    gp.ty0 = g.ty0;
+    gp.ntx_sub = g.tx1 - g.tx0;
+    gp.tiles = g.bm_list ? g.tile_list : nullptr;
+    gp.gate = gate;
+    const dim3 ggrid(g.bm_list ? g.n_tiles : (g.tx1 - g.tx0) * (g.ty1 - g.ty0), C);
+    if (step == 1) {
+        k_group<1><<<ggrid, 256, 0, r.st>>>(gp);
+    } else {
+        k_group<2><<<ggrid, 256, 0, r.st>>>(gp);
+    }
+    HIP_CHECK(hipGetLastError());
+    if (timed && (rc = r.mark(sgr, true))) return rc;
+    if (timed && (rc = r.mark(smg, false))) return rc;
+    MergeParams mp{};
+    mp.partial = p->partial;
+    mp.w = w;
+    mp.h = h;
+    mp.ntx = p->ntx;
+    mp.nty = p->nty;
+    mp.channels = C;
+    mp.txlo = txlo;
+    mp.txhi = txhi;
+    mp.tylo = tylo;
+    mp.tyhi = tyhi;
+    mp.fx0 = fx0;
+    mp.fx1 = fx1;
+    mp.fy0 = fy0;
+    for (int c = 0; c < 3; ++c) mp.out[c] = out[c < C ? c : 0];
+    mp.out_pitch = out_pitch;
+    mp.inverse_colour = inverse_colour;
+    mp.x0 = g.x0;
+    mp.y0 = g.y0;
+    mp.x1 = g.x1;
+    mp.y1 = g.y1;
+    mp.rects = g.bm_list ? g.rect_list : nullptr;
+    mp.gate = gate;
+    const dim3 mgrid = g.bm_list ? dim3((g.x1 + 255) / 256, g.y1, g.n_rects)
+                                 : dim3((g.x1 - g.x0 + 255) / 256, g.y1 - g.y0);
+    k_merge<<<mgrid, 256, 0, r.st>>>(mp);
+    HIP_CHECK(hipGetLastError());
+    if (timed && (rc = r.mark(smg, true))) return rc;
+    return OK;
 }
 
 int enqueue(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float* const* out,
@@ -884,103 +1122,58 @@ int enqueue(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float* 
         return OK;
     }
     const size_t npx = p->npx;
-    float* z[3] = {p->z, p->z + npx, p->z + 2 * npx};
-    float* bz[3] = {p->bz, p->bz + npx, p->bz + 2 * npx};
-    auto mark = [&](int i) -> int {
-        if (p->timing) HIP_CHECK(hipEventRecord(p->ev[i], st));
-        return OK;
-    };
-    int rc;
-    if ((rc = mark(0))) return rc;
-    if (C == 3) {
-        k_opp_fwd<<<rowgrid, 256, 0, st>>>(in[0], in[1], in[2], in_pitch, z[0], z[1], z[2], w, h);
+    RunCtx r{p, st, {p->z, p->z + npx, p->z + 2 * npx}, {p->bz, p->bz + npx, p->bz + 2 * npx}, {}};
+    for (bool& b : p->stage_ran) b = false;
+    if (C == 1) {
+        r.sig[0] = r.sig[1] = r.sig[2] = sigma;
     } else {
-        k_copy_plane<<<rowgrid, 256, 0, st>>>(in[0], in_pitch, z[0], (size_t)w, w);
+        r.sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
+        r.sig[1] = sigma * (float)std::sqrt(0.5);
+        r.sig[2] = sigma * (float)std::sqrt(0.375);
+    }
+    int rc;
+    if ((rc = r.mark(CPIPE_BM3D_T_COLOR, false))) return rc;
+    if (C == 3) {
+        k_opp_fwd<<<rowgrid, 256, 0, st>>>(in[0], in[1], in[2], in_pitch, r.z[0], r.z[1], r.z[2], w, h);
+    } else {
+        k_copy_plane<<<rowgrid, 256, 0, st>>>(in[0], in_pitch, r.z[0], (size_t)w, w);
     }
     HIP_CHECK(hipGetLastError());
-    if ((rc = mark(1))) return rc;
+    if ((rc = r.mark(CPIPE_BM3D_T_COLOR, true))) return rc;
 
-    const int* txlo = p->geo;
-    const int* txhi = txlo + w;
-    const int* tylo = txhi + w;
-    const int* tyhi = tylo + h;
-    const int* fx0 = tyhi + h;
-    const int* fx1 = fx0 + p->ntx;
-    const int* fy0 = fx1 + p->ntx;
-    const int* fy1 = fy0 + p->nty;
-
-    float sig[3];
-    if (C == 1) {
-        sig[0] = sig[1] = sig[2] = sigma;
+    const StepRange all = full_range(p);
+    const int* gate = nullptr;
+    if (p->pilot) {
+        // learned pilot in place of step 1, then spot checks against the plain step 1
+        if ((rc = r.mark(CPIPE_BM3D_T_PILOT, false))) return rc;
+        rc = cpipe_internal::pilot_forward(p->pilot, r.z, (size_t)w, 0, w, h, 1, 10.0f * sigma,
+                                           nullptr, r.bz, st);
+        if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
+        float* bzp[3] = {p->bzp, p->bzp + npx, p->bzp + 2 * npx};
+        const int n = (int)p->checks.size();
+        if (n > 0) {
+            StepRange g{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, p->chk_w, p->chk_h};
+            g.bm_list = p->chk_bm;
+            g.tile_list = p->chk_tiles;
+            g.rect_list = (const int4*)p->rects;
+            g.n_bm = p->n_chk_bm;
+            g.n_tiles = p->n_chk_tiles;
+            g.n_rects = n;
+            if ((rc = run_step(r, 1, g, bzp, (size_t)w, 0, nullptr, false))) return rc;
+            k_pilot_check<<<n, 256, 0, st>>>(r.z[0], r.z[1], r.z[2], r.bz[0], r.bz[1], r.bz[2], bzp[0],
+                                             bzp[1], bzp[2], w, (const CheckRect*)p->rects,
+                                             p->pilot_thr_db, p->report);
+            HIP_CHECK(hipGetLastError());
+            k_pilot_decide<<<1, 64, 0, st>>>(p->report, n, p->pilot_full_frac, p->gate);
+            HIP_CHECK(hipGetLastError());
+            gate = p->gate;
+        }
+        if ((rc = r.mark(CPIPE_BM3D_T_PILOT, true))) return rc;
+        if (gate && (rc = run_step(r, 1, all, r.bz, (size_t)w, 0, gate, true))) return rc;
     } else {
-        sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
-        sig[1] = sigma * (float)std::sqrt(0.5);
-        sig[2] = sigma * (float)std::sqrt(0.375);
+        if ((rc = run_step(r, 1, all, r.bz, (size_t)w, 0, nullptr, true))) return rc;
     }
-    for (int step = 1; step <= 2; ++step) {
-        const float* bmplane = step == 1 ? z[0] : bz[0];
-        const dim3 bmgrid((p->nx + BM_TX - 1) / BM_TX, (p->ny + BM_TY - 1) / BM_TY);
-        k_block_match<<<bmgrid, 256, 0, st>>>(bmplane, w, h, p->nx, p->ny,
-                                                            tau_ssd(step), p->pos[step - 1],
-                                                            p->cnt[step - 1]);
-        HIP_CHECK(hipGetLastError());
-        if ((rc = mark(step == 1 ? 2 : 5))) return rc;
-        GroupParams gp{};
-        for (int c = 0; c < 3; ++c) {
-            gp.noisy[c] = c < C ? z[c] : z[0];
-            gp.basic[c] = c < C ? bz[c] : bz[0];
-            gp.sig[c] = sig[c];
-        }
-        gp.w = w;
-        gp.h = h;
-        gp.nx = p->nx;
-        gp.ny = p->ny;
-        gp.ntx = p->ntx;
-        gp.nty = p->nty;
-        gp.pos = p->pos[step - 1];
-        gp.cnt = p->cnt[step - 1];
-        gp.fx0 = fx0;
-        gp.fx1 = fx1;
-        gp.fy0 = fy0;
-        gp.fy1 = fy1;
-        gp.partial = p->partial;
-        gp.err = p->err;
-        const dim3 ggrid(p->ntx * p->nty, C);
-        if (step == 1) {
-            k_group<1><<<ggrid, 256, 0, st>>>(gp);
-        } else {
-            k_group<2><<<ggrid, 256, 0, st>>>(gp);
-        }
-        HIP_CHECK(hipGetLastError());
-        if ((rc = mark(step == 1 ? 3 : 6))) return rc;
-        MergeParams mp{};
-        mp.partial = p->partial;
-        mp.w = w;
-        mp.h = h;
-        mp.ntx = p->ntx;
-        mp.nty = p->nty;
-        mp.channels = C;
-        mp.txlo = txlo;
-        mp.txhi = txhi;
-        mp.tylo = tylo;
-        mp.tyhi = tyhi;
-        mp.fx0 = fx0;
-        mp.fx1 = fx1;
-        mp.fy0 = fy0;
-        if (step == 1) {
-            for (int c = 0; c < 3; ++c) mp.out[c] = bz[c < C ? c : 0];
-            mp.out_pitch = (size_t)w;
-            mp.inverse_colour = 0;
-        } else {
-            for (int c = 0; c < 3; ++c) mp.out[c] = out[c < C ? c : 0];
-            mp.out_pitch = out_pitch;
-            mp.inverse_colour = C == 3 ? 1 : 0;
-        }
-        k_merge<<<rowgrid, 256, 0, st>>>(mp);
-        HIP_CHECK(hipGetLastError());
-        if ((rc = mark(step == 1 ? 4 : 7))) return rc;
-    }
-    return OK;
+    return run_step(r, 2, all, out, out_pitch, C == 3 ? 1 : 0, nullptr, true);
 }
 
 }  // namespace
@@ -1085,6 +1278,7 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
                 free_plan(p);
                 return fail(INTERNAL, "tile cover has a gap in y");
             }
+        p->geo_host = geo;
         hipError_t e = hipMemset(p->err, 0, sizeof(int));
         if (e == hipSuccess) e = hipMemcpy(p->geo, geo.data(), sizeof(int) * ngeo, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -1119,10 +1313,11 @@ int cpipe_bm3d_plan_run(cpipe_bm3d_plan* p, const float* const* in, size_t in_pi
     if (rc) return rc;
     p->ran = true;
     if (p->timing && p->w >= K && p->h >= K) {
-        HIP_CHECK(hipEventSynchronize(p->ev[CPIPE_BM3D_T_COUNT]));
+        HIP_CHECK(hipEventSynchronize(p->ev[2 * CPIPE_BM3D_T_MERGE2 + 1]));
         for (int i = 0; i < CPIPE_BM3D_T_COUNT; ++i) {
+            if (!p->stage_ran[i]) continue;
             float ms = 0.0f;
-            HIP_CHECK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+            HIP_CHECK(hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]));
             p->ms[i] += ms;
         }
         p->runs++;
@@ -1178,6 +1373,157 @@ int cpipe_bm3d_plan_matches(cpipe_bm3d_plan* p, int step, uint32_t* pos, uint8_t
     HIP_CHECK(hipMemcpyAsync(pos, p->pos[step - 1], nref * NMAX * sizeof(uint32_t),
                              hipMemcpyDeviceToDevice, st));
     HIP_CHECK(hipMemcpyAsync(count, p->cnt[step - 1], nref, hipMemcpyDeviceToDevice, st));
+    return OK;
+}
+
+int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch,
+                                   float sigma, float* const* noisy_out, float* const* basic_out,
+                                   size_t out_pitch, void* stream) {
+    if (!p || !in || !noisy_out || !basic_out) return fail(BAD_INDEX, "null argument");
+    if (p->c != 3) return fail(UNSUPPORTED, "opponent planes exist for 3-channel plans only");
+    if (p->w < K || p->h < K) return fail(UNSUPPORTED, "image smaller than one patch");
+    if (in_pitch < (size_t)p->w || out_pitch < (size_t)p->w) return fail(BAD_INDEX, "pitch smaller than width");
+    DeviceGuard guard(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t npx = p->npx;
+    const int w = p->w, h = p->h;
+    RunCtx r{p, st, {p->z, p->z + npx, p->z + 2 * npx}, {p->bz, p->bz + npx, p->bz + 2 * npx}, {}};
+    r.sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
+    r.sig[1] = sigma * (float)std::sqrt(0.5);
+    r.sig[2] = sigma * (float)std::sqrt(0.375);
+    const dim3 rowgrid((w + 255) / 256, h);
+    k_opp_fwd<<<rowgrid, 256, 0, st>>>(in[0], in[1], in[2], in_pitch, r.z[0], r.z[1], r.z[2], w, h);
+    HIP_CHECK(hipGetLastError());
+    const int rc = run_step(r, 1, full_range(p), r.bz, (size_t)w, 0, nullptr, false);
+    if (rc) return rc;
+    for (int c = 0; c < 3; ++c) {
+        HIP_CHECK(hipMemcpy2DAsync(noisy_out[c], out_pitch * sizeof(float), r.z[c], (size_t)w * sizeof(float),
+                                   (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpy2DAsync(basic_out[c], out_pitch * sizeof(float), r.bz[c], (size_t)w * sizeof(float),
+                                   (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, st));
+    }
+    return OK;
+}
+
+int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
+                              const cpipe_bm3d_pilot_opts* opts) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    DeviceGuard guard(p->device);
+    HIP_CHECK(hipDeviceSynchronize());
+    (void)hipFree(p->rects);
+    (void)hipFree(p->chk_bm);
+    (void)hipFree(p->chk_tiles);
+    (void)hipFree(p->report);
+    (void)hipFree(p->gate);
+    (void)hipFree(p->bzp);
+    p->rects = nullptr;
+    p->chk_bm = nullptr;
+    p->chk_tiles = nullptr;
+    p->n_chk_bm = p->n_chk_tiles = p->chk_w = p->chk_h = 0;
+    p->report = nullptr;
+    p->gate = nullptr;
+    p->bzp = nullptr;
+    p->checks.clear();
+    p->pilot = nullptr;
+    if (!net) return OK;
+    if (p->c != 3) return fail(UNSUPPORTED, "the learned pilot is trained for RGB (3-plane) images");
+    if (p->w < K || p->h < K) return fail(UNSUPPORTED, "image smaller than one patch");
+    cpipe_bm3d_pilot_opts o{sizeof(cpipe_bm3d_pilot_opts), 0.5f, 256, 16, 0.5f};
+    if (opts) {
+        if (opts->struct_size < sizeof(cpipe_bm3d_pilot_opts)) return fail(BAD_INDEX, "opts.struct_size too small");
+        o = *opts;
+    }
+    if (!(o.fallback_db >= 0.0f) || o.check_block < 16 || o.check_every < 0)
+        return fail(BAD_INDEX, "bad pilot options");
+    const int w = p->w, h = p->h;
+    const int* txlo = p->geo_host.data();
+    const int* txhi = txlo + w;
+    const int* tylo = txhi + w;
+    const int* tyhi = tylo + h;
+    const int B = o.check_block;
+    const int nbx = (w + B - 1) / B, nby = (h + B - 1) / B;
+    std::vector<CheckRect> rects;
+    std::vector<char> bm_used, tile_used((size_t)p->ntx * p->nty, 0);
+    const int nbtx = (p->nx + BM_TX - 1) / BM_TX, nbty = (p->ny + BM_TY - 1) / BM_TY;
+    bm_used.assign((size_t)nbtx * nbty, 0);
+    for (int b = 0; o.check_every > 0 && b < nbx * nby; ++b) {
+        // every check_every-th block in raster order, the phase shifted by one per block row
+        if ((b + b / nbx) % o.check_every != o.check_every / 2) continue;
+        cpipe_bm3d_plan_s::Check c{};
+        c.x0 = (b % nbx) * B;
+        c.y0 = (b / nbx) * B;
+        c.x1 = std::min(w, c.x0 + B);
+        c.y1 = std::min(h, c.y0 + B);
+        c.tx0 = txlo[c.x0];
+        c.tx1 = txhi[c.x1 - 1] + 1;
+        c.ty0 = tylo[c.y0];
+        c.ty1 = tyhi[c.y1 - 1] + 1;
+        const int rx0 = c.tx0 * TILE, rx1 = std::min(c.tx1 * TILE, p->nx);
+        const int ry0 = c.ty0 * TILE, ry1 = std::min(c.ty1 * TILE, p->ny);
+        c.btx0 = rx0 / BM_TX;
+        c.btx1 = (rx1 + BM_TX - 1) / BM_TX;
+        c.bty0 = ry0 / BM_TY;
+        c.bty1 = (ry1 + BM_TY - 1) / BM_TY;
+        p->checks.push_back(c);
+        rects.push_back({c.x0, c.y0, c.x1, c.y1});
+        for (int ty = c.bty0; ty < c.bty1; ++ty)
+            for (int tx = c.btx0; tx < c.btx1; ++tx) bm_used[(size_t)ty * nbtx + tx] = 1;
+        for (int ty = c.ty0; ty < c.ty1; ++ty)
+            for (int tx = c.tx0; tx < c.tx1; ++tx) tile_used[(size_t)ty * p->ntx + tx] = 1;
+        p->chk_w = std::max(p->chk_w, c.x1 - c.x0);
+        p->chk_h = std::max(p->chk_h, c.y1 - c.y0);
+    }
+    // each tile once: overlapping checks must not run one tile twice in a launch
+    std::vector<int2> bm_list;
+    std::vector<int> tile_list;
+    for (int i = 0; i < nbtx * nbty; ++i)
+        if (bm_used[i]) bm_list.push_back(make_int2(i % nbtx, i / nbtx));
+    for (int i = 0; i < p->ntx * p->nty; ++i)
+        if (tile_used[i]) tile_list.push_back(i);
+    p->n_chk_bm = (int)bm_list.size();
+    p->n_chk_tiles = (int)tile_list.size();
+    const size_t n = rects.size();
+    HIP_CHECK(hipMalloc((void**)&p->chk_bm, std::max<size_t>(1, bm_list.size()) * sizeof(int2)));
+    HIP_CHECK(hipMalloc((void**)&p->chk_tiles, std::max<size_t>(1, tile_list.size()) * sizeof(int)));
+    if (!bm_list.empty())
+        HIP_CHECK(hipMemcpy(p->chk_bm, bm_list.data(), bm_list.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!tile_list.empty())
+        HIP_CHECK(hipMemcpy(p->chk_tiles, tile_list.data(), tile_list.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    HIP_CHECK(hipMalloc(&p->rects, std::max<size_t>(1, n) * sizeof(CheckRect)));
+    HIP_CHECK(hipMalloc((void**)&p->report, std::max<size_t>(1, n) * 2 * sizeof(float)));
+    HIP_CHECK(hipMalloc((void**)&p->gate, sizeof(int)));
+    HIP_CHECK(hipMalloc((void**)&p->bzp, 3 * p->npx * sizeof(float)));
+    if (n) HIP_CHECK(hipMemcpy(p->rects, rects.data(), n * sizeof(CheckRect), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(p->report, 0, std::max<size_t>(1, n) * 2 * sizeof(float)));
+    HIP_CHECK(hipMemset(p->gate, 0, sizeof(int)));
+    p->pilot = net;
+    p->pilot_thr_db = o.fallback_db;
+    p->pilot_full_frac = o.full_fraction;
+    return OK;
+}
+
+int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* p, int* n_checked, int* n_fallback,
+                                 float* max_delta_db, int* full_fallback) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    int nc = 0, nf = 0, full = 0;
+    float md = 0.0f;
+    if (p->pilot) {
+        DeviceGuard guard(p->device);
+        HIP_CHECK(hipDeviceSynchronize());
+        nc = (int)p->checks.size();
+        std::vector<float> rep(2 * std::max(1, nc));
+        if (nc) HIP_CHECK(hipMemcpy(rep.data(), p->report, 2 * nc * sizeof(float), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nc; ++i) {
+            md = std::max(md, rep[2 * i]);
+            nf += rep[2 * i + 1] > 0.5f ? 1 : 0;
+        }
+        if (nc) HIP_CHECK(hipMemcpy(&full, p->gate, sizeof(int), hipMemcpyDeviceToHost));
+    }
+    if (n_checked) *n_checked = nc;
+    if (n_fallback) *n_fallback = nf;
+    if (max_delta_db) *max_delta_db = md;
+    if (full_fallback) *full_fallback = full;
     return OK;
 }
 

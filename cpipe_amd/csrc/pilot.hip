@@ -1,0 +1,775 @@
+// Learned pilot for BM3D step 2: a 6-layer 3x3 convolutional network that
+// maps the noisy opponent planes (+ a noise-level channel) to the basic
+// estimate, replacing BM3D step 1. Forward, backward and Adam are HIP kernels
+// for gfx950; convolutions run on MFMA (FP16 operands, FP32 accumulation).
+//
+// Layers (channels padded to multiples of 16 for the 16x16x16 MFMA):
+//   L1 conv 4(->16) -> 32, ReLU     input: Y, U, V, 10*sigma
+//   L2..L5 conv 32 -> 32, ReLU
+//   L6 conv 32 -> 3(->16), output = input[0..2] + L6 (residual)
+// Activations: NHWC FP16, channel stride = padded channel count.
+// Weights: W[tap 0..8][cin][cout] FP32 master copy + FP16 copy, bias FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "cpipe_bm3d.h"
+#include "pilot.h"
+
+namespace pilot {
+
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr int TW = 64;  // output tile width (pixels) per workgroup
+constexpr int TH = 4;   // output tile height: one row per wave
+
+// Layer geometry (padded channel counts).
+constexpr int NL = 6;
+constexpr int CIN[NL] = {16, 32, 32, 32, 32, 32};
+constexpr int COUT[NL] = {32, 32, 32, 32, 32, 16};
+constexpr int CIN_REAL[NL] = {4, 32, 32, 32, 32, 32};
+constexpr int COUT_REAL[NL] = {32, 32, 32, 32, 32, 3};
+
+inline size_t layer_w(int l) { return (size_t)9 * CIN[l] * COUT[l]; }
+inline size_t layer_params(int l) { return layer_w(l) + COUT[l]; }
+inline size_t total_params() {
+    size_t n = 0;
+    for (int l = 0; l < NL; ++l) n += layer_params(l);
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// Packing: opponent planes (FP32) + noise level -> NHWC FP16 (16 channels)
+// ---------------------------------------------------------------------------
+
+__global__ void k_pack_input(const float* __restrict__ y, const float* __restrict__ u,
+                             const float* __restrict__ v, size_t pitch, float level,
+                             const float* __restrict__ levels, int w, int h,
+                             size_t img_stride_planes, _Float16* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy = blockIdx.y;
+    const int img = blockIdx.z;
+    if (x >= w) return;
+    const size_t i = img * img_stride_planes + (size_t)yy * pitch + x;
+    _Float16* o = out + ((size_t)img * w * h + (size_t)yy * w + x) * 16;
+    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    const float lv = levels ? levels[img] : level;
+    half8 a = {(_Float16)y[i], (_Float16)u[i], (_Float16)v[i], (_Float16)lv, 0, 0, 0, 0};
+    half8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+    *reinterpret_cast<half8*>(o) = a;
+    *reinterpret_cast<half8*>(o + 8) = b;
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, stride 1, zero padding, on MFMA 16x16x16 (FP16 -> FP32).
+// Workgroup: 64x4 output pixels, 4 waves (one output row each), 4 segments of
+// 16 pixels per wave. The input tile (66x6 pixels) is staged in LDS with a
+// padded pixel pitch (CIN + 4 halves) to spread banks.
+// MFMA operand layouts (wave64, 16x16x16): A[i][k] at lane i + 16*(k/4),
+// element k%4; B[k][j] at lane j + 16*(k/4), element k%4; D[i][j] at lane
+// j + 16*(i/4), element i%4.
+// EPI 0: out = relu(acc + bias) (FP16 NHWC)
+// EPI 1: out planes (FP32) = residual planes + acc + bias, channels < 3
+// EPI 2: out = acc * (mask > 0) (FP16 NHWC) -- backward through a ReLU
+// ---------------------------------------------------------------------------
+
+struct ConvArgs {
+    const _Float16* in;    // NHWC, channel stride CIN, images stacked
+    const _Float16* wt;    // FP16 weights [tap][cin][cout] (forward layout)
+    const float* bias;     // [cout] (EPI 0/1)
+    _Float16* out;         // NHWC, channel stride COUT (EPI 0/2)
+    const _Float16* mask;  // NHWC, channel stride COUT (EPI 2)
+    const float* res[3];   // residual planes (EPI 1)
+    float* outp[3];        // output planes (EPI 1)
+    size_t pitch;          // plane row pitch (EPI 1)
+    size_t img_stride_planes;
+    int w, h;
+    int transposed;  // 1: use weights as the backward (flipped, cin<->cout) operator
+};
+
+template <int CI, int CO, int EPI>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+    constexpr int PS = CI + 4;  // LDS pixel pitch in halves
+    __shared__ __align__(16) _Float16 tile[(TH + 2) * (TW + 2) * PS];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, img = blockIdx.z;
+    const int w = a.w, h = a.h;
+    const _Float16* in = a.in + (size_t)img * w * h * CI;
+    // stage (TW+2) x (TH+2) pixels, CI channels, 8 bytes per thread-step
+    constexpr int QPP = CI / 4;  // quads per pixel
+    for (int idx = threadIdx.x; idx < (TH + 2) * (TW + 2) * QPP; idx += blockDim.x) {
+        const int q = idx % QPP, pix = idx / QPP;
+        const int ty = pix / (TW + 2), tx = pix - ty * (TW + 2);
+        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+        half4 v = {0, 0, 0, 0};
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h)
+            v = *reinterpret_cast<const half4*>(in + ((size_t)gy * w + gx) * CI + q * 4);
+        *reinterpret_cast<half4*>(&tile[pix * PS + q * 4]) = v;
+    }
+    // weights as B fragments: B[k = cin][j = cout] for tap t (forward), or
+    // B[k = cin'][j = cout'] = W[8 - t][j][k] for the backward operator
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    constexpr int KB = CI / 16, NB = CO / 16;
+    half4 bw[9][KB][NB];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                half4 f;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k = kb * 16 + lq * 4 + e, j = nb * 16 + lr;
+                    // forward weights are [tap][CI][CO]; the backward operator reads
+                    // the forward tensor of shape [tap][CO][CI] flipped in the taps
+                    f[e] = a.transposed ? a.wt[((size_t)(8 - t) * CO + j) * CI + k]
+                                        : a.wt[((size_t)t * CI + k) * CO + j];
+                }
+                bw[t][kb][nb] = f;
+            }
+    __syncthreads();
+
+    const int oy = y0 + wave;
+#pragma unroll 1
+    for (int seg = 0; seg < TW / 16; ++seg) {
+        float4v acc[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4v){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int ky = t / 3, kx = t % 3;
+            const _Float16* row = &tile[((wave + ky) * (TW + 2) + seg * 16 + lr + kx) * PS];
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const half4 av = *reinterpret_cast<const half4*>(row + kb * 16 + lq * 4);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+            }
+        }
+        if (oy >= h) continue;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int co = nb * 16 + lr;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int ox = x0 + seg * 16 + lq * 4 + e;
+                if (ox >= w) continue;
+                const size_t p = (size_t)img * w * h + (size_t)oy * w + ox;
+                if (EPI == 0) {
+                    const float v = acc[nb][e] + a.bias[co];
+                    a.out[p * CO + co] = (_Float16)(v > 0.0f ? v : 0.0f);
+                } else if (EPI == 1) {
+                    if (co < 3) {
+                        const size_t q = img * a.img_stride_planes + (size_t)oy * a.pitch + ox;
+                        a.outp[co][q] = a.res[co][q] + (acc[nb][e] + a.bias[co]);
+                    }
+                } else {
+                    const float m = (float)a.mask[p * CO + co];
+                    a.out[p * CO + co] = (_Float16)(m > 0.0f ? acc[nb][e] : 0.0f);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradients: dW[t][ci][co] = sum_p X[p + t][ci] * dY[p][co], db[co] =
+// sum_p dY[p][co], as MFMA with K = pixels. Each workgroup reduces its own
+// pixel tile (TW x TH) and writes FP32 partials; k_reduce sums them in a
+// fixed order.
+// ---------------------------------------------------------------------------
+
+struct WgradArgs {
+    const _Float16* x;   // NHWC, stride CI (layer input)
+    const _Float16* dy;  // NHWC, stride CO (gradient at the layer output, pre-activation)
+    float* partial;      // [wg][9*CI*CO + CO]
+    int w, h;
+};
+
+template <int CI, int CO>
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(WgradArgs a) {
+    // channel-major staging so that 4 consecutive pixels are contiguous
+    constexpr int XW = TW + 2, XH = TH + 2;
+    __shared__ _Float16 xs[CI * XH * XW];
+    __shared__ _Float16 gs[CO * TH * TW];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, img = blockIdx.z;
+    const int w = a.w, h = a.h;
+    const _Float16* xin = a.x + (size_t)img * w * h * CI;
+    const _Float16* gin = a.dy + (size_t)img * w * h * CO;
+    for (int idx = threadIdx.x; idx < XH * XW * CI; idx += blockDim.x) {
+        const int c = idx % CI, pix = idx / CI;
+        const int ty = pix / XW, tx = pix - ty * XW;
+        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+        _Float16 v = 0;
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = xin[((size_t)gy * w + gx) * CI + c];
+        xs[c * XH * XW + pix] = v;
+    }
+    for (int idx = threadIdx.x; idx < TH * TW * CO; idx += blockDim.x) {
+        const int c = idx % CO, pix = idx / CO;
+        const int ty = pix / TW, tx = pix - ty * TW;
+        const int gx = x0 + tx, gy = y0 + ty;
+        _Float16 v = 0;
+        if (gx < w && gy < h) v = gin[((size_t)gy * w + gx) * CO + c];
+        gs[c * TH * TW + pix] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    constexpr int MB = CI / 16, NB = CO / 16;
+    const size_t wg = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    float* part = a.partial + wg * (9 * CI * CO + CO);
+    // 9 taps x MB x NB output blocks, spread over the 4 waves
+    for (int job = wave; job < 9 * MB * NB; job += 4) {
+        const int t = job / (MB * NB), mb = (job / NB) % MB, nb = job % NB;
+        const int ky = t / 3, kx = t % 3;
+        float4v acc = {0.f, 0.f, 0.f, 0.f};
+        // K = the TH*TW pixels of the tile, 16 at a time (4 per lane)
+#pragma unroll 4
+        for (int k0 = 0; k0 < TH * TW; k0 += 16) {
+            const int py = k0 / TW, px = k0 % TW + lq * 4;
+            half4 av, bv;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                av[e] = xs[(mb * 16 + lr) * XH * XW + (py + ky) * XW + px + e + kx];
+                bv[e] = gs[(nb * 16 + lr) * TH * TW + py * TW + px + e];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc, 0, 0, 0);
+        }
+        // D[i = ci][j = co]: lane holds rows lq*4 + e, column lr
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ci = mb * 16 + lq * 4 + e, co = nb * 16 + lr;
+            part[((size_t)t * CI + ci) * CO + co] = acc[e];
+        }
+    }
+    if (threadIdx.x < CO) {
+        float s = 0.0f;
+        for (int pix = 0; pix < TH * TW; ++pix) s += (float)gs[threadIdx.x * TH * TW + pix];
+        part[9 * CI * CO + threadIdx.x] = s;
+    }
+}
+
+// Sum partials in workgroup order (deterministic), scale, write FP32 grads.
+__global__ void k_reduce(const float* __restrict__ partial, int nwg, int n, float scale,
+                         float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < nwg; ++k) s += partial[(size_t)k * n + i];
+    out[i] = s * scale;
+}
+
+// Loss gradient at the output planes (layout [img][c][plane] for both the
+// output and the target): dY (FP16 NHWC, 16 ch) = out - target for channels
+// < 3, and per-block squared-error sums for the loss value.
+__global__ void k_loss_grad(const float* __restrict__ out, const float* __restrict__ tgt,
+                            size_t plane, _Float16* __restrict__ dy, float* __restrict__ sq_partial) {
+    __shared__ float red[256];
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t img = blockIdx.y;
+    float s = 0.0f;
+    if (p < plane) {
+        typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+        half8 g = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+            const size_t q = (img * 3 + c) * plane + p;
+            const float d = out[q] - tgt[q];
+            g[c] = (_Float16)d;
+            s += d * d;
+        }
+        _Float16* o = dy + (img * plane + p) * 16;
+        *reinterpret_cast<half8*>(o) = g;
+        *reinterpret_cast<half8*>(o + 8) = (half8){0, 0, 0, 0, 0, 0, 0, 0};
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sq_partial[img * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Adam on the FP32 master weights; refreshes the FP16 copy; padded entries
+// (mask == 0) stay zero.
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, const uint8_t* __restrict__ live,
+                       _Float16* __restrict__ w16, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!live[i]) {
+        w[i] = 0.0f;
+        w16[i] = 0;
+        return;
+    }
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    w16[i] = (_Float16)wi;
+}
+
+__global__ void k_to_half(const float* __restrict__ w, _Float16* __restrict__ w16, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w16[i] = (_Float16)w[i];
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+namespace {
+
+thread_local std::string g_perr;
+
+int perr(int code, const std::string& msg) {
+    g_perr = msg;
+    return code;
+}
+
+#define PHIP(expr)                                                                      \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return perr(e_ == hipErrorOutOfMemory ? 3 : 7,                              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+constexpr uint32_t kBlobMagic = 0x4C504350u;  // "CPPL"
+constexpr uint32_t kBlobVersion = 1;
+
+struct BlobHeader {
+    uint32_t magic;
+    uint32_t version;
+    uint32_t nparam;
+    uint32_t flags;  // bit 0: optimizer state follows the weights
+    uint64_t step;
+    uint64_t seed;
+};
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double uniform01(uint64_t& s) { return ((splitmix64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+}  // namespace
+}  // namespace pilot
+
+using namespace pilot;
+
+struct cpipe_bm3d_pilot_s {
+    int device = 0;
+    size_t nparam = 0;
+    size_t off[NL] = {};
+    float* w32 = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    float* g = nullptr;
+    _Float16* w16 = nullptr;
+    uint8_t* live = nullptr;
+    uint64_t step = 0;
+    uint64_t seed = 0;
+    std::vector<uint8_t> live_host;
+    // workspace (grown on demand)
+    size_t cap_px = 0;
+    bool cap_train = false;
+    _Float16* act[NL] = {};
+    _Float16* dyb[2] = {};
+    float* outp = nullptr;  // 3 planes (training)
+    float* partial = nullptr;
+    size_t partial_cap = 0;
+    float* sq = nullptr;
+    size_t sq_cap = 0;
+    float* levels = nullptr;
+    size_t levels_cap = 0;
+};
+
+namespace {
+
+class PDevice {
+public:
+    explicit PDevice(int dev) {
+        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+        if (prev_ != dev) (void)hipSetDevice(dev);
+    }
+    ~PDevice() {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+
+private:
+    int prev_ = -1;
+};
+
+void free_ws(cpipe_bm3d_pilot* n) {
+    for (auto& a : n->act) {
+        (void)hipFree(a);
+        a = nullptr;
+    }
+    for (auto& d : n->dyb) {
+        (void)hipFree(d);
+        d = nullptr;
+    }
+    (void)hipFree(n->outp);
+    n->outp = nullptr;
+    n->cap_px = 0;
+}
+
+// Workspace for `px` pixels: inference needs the input and two ping-pong
+// activations; training keeps every activation plus two gradient buffers.
+int ensure_ws(cpipe_bm3d_pilot* n, size_t px, bool train) {
+    if (px <= n->cap_px && (!train || n->cap_train)) return 0;
+    (void)hipDeviceSynchronize();
+    free_ws(n);
+    const size_t want = px;
+    PHIP(hipMalloc((void**)&n->act[0], want * 16 * sizeof(_Float16)));
+    const int nact = train ? NL : 3;
+    for (int l = 1; l < nact; ++l) PHIP(hipMalloc((void**)&n->act[l], want * 32 * sizeof(_Float16)));
+    if (train) {
+        for (auto& d : n->dyb) PHIP(hipMalloc((void**)&d, want * 32 * sizeof(_Float16)));
+        PHIP(hipMalloc((void**)&n->outp, want * 3 * sizeof(float)));
+    }
+    n->cap_px = want;
+    n->cap_train = train;
+    return 0;
+}
+
+int ensure_buf(float** p, size_t* cap, size_t n) {
+    if (n <= *cap) return 0;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(*p);
+    *p = nullptr;
+    PHIP(hipMalloc((void**)p, n * sizeof(float)));
+    *cap = n;
+    return 0;
+}
+
+template <int CI, int CO, int EPI>
+int launch_conv(const ConvArgs& a, int n_img, hipStream_t st) {
+    const dim3 grid((a.w + TW - 1) / TW, (a.h + TH - 1) / TH, n_img);
+    k_conv3x3<CI, CO, EPI><<<grid, 256, 0, st>>>(a);
+    PHIP(hipGetLastError());
+    return 0;
+}
+
+int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t img_stride, int w,
+            int h, int n_img, float level, const float* levels, float* const* out, bool train,
+            hipStream_t st) {
+    const size_t px = (size_t)w * h * n_img;
+    int rc = ensure_ws(n, px, train);
+    if (rc) return rc;
+    k_pack_input<<<dim3((w + 255) / 256, h, n_img), 256, 0, st>>>(in[0], in[1], in[2], pitch, level,
+                                                                   levels, w, h, img_stride, n->act[0]);
+    PHIP(hipGetLastError());
+    ConvArgs a{};
+    a.w = w;
+    a.h = h;
+    a.pitch = pitch;
+    a.img_stride_planes = img_stride;
+    // L1
+    a.in = n->act[0];
+    a.wt = n->w16 + n->off[0];
+    a.bias = n->w32 + n->off[0] + layer_w(0);
+    a.out = n->act[1];
+    if ((rc = launch_conv<16, 32, 0>(a, n_img, st))) return rc;
+    int cur = 1;
+    for (int l = 1; l < NL - 1; ++l) {
+        const int nxt = train ? l + 1 : (cur == 1 ? 2 : 1);
+        a.in = n->act[cur];
+        a.wt = n->w16 + n->off[l];
+        a.bias = n->w32 + n->off[l] + layer_w(l);
+        a.out = n->act[nxt];
+        if ((rc = launch_conv<32, 32, 0>(a, n_img, st))) return rc;
+        cur = nxt;
+    }
+    a.in = n->act[cur];
+    a.wt = n->w16 + n->off[NL - 1];
+    a.bias = n->w32 + n->off[NL - 1] + layer_w(NL - 1);
+    for (int c = 0; c < 3; ++c) {
+        a.res[c] = in[c];
+        a.outp[c] = out[c];
+    }
+    return launch_conv<32, 16, 1>(a, n_img, st);
+}
+
+template <int CI, int CO>
+int wgrad(cpipe_bm3d_pilot* n, int l, const _Float16* x, const _Float16* dy, int w, int h,
+          int n_img, float scale, hipStream_t st) {
+    const dim3 grid((w + TW - 1) / TW, (h + TH - 1) / TH, n_img);
+    const size_t nwg = (size_t)grid.x * grid.y * grid.z;
+    const size_t per = (size_t)9 * CI * CO + CO;
+    int rc = ensure_buf(&n->partial, &n->partial_cap, nwg * per);
+    if (rc) return rc;
+    WgradArgs a{x, dy, n->partial, w, h};
+    k_conv3x3_wgrad<CI, CO><<<grid, 256, 0, st>>>(a);
+    PHIP(hipGetLastError());
+    k_reduce<<<(unsigned)((per + 255) / 256), 256, 0, st>>>(n->partial, (int)nwg, (int)per, scale,
+                                                             n->g + n->off[l]);
+    PHIP(hipGetLastError());
+    return 0;
+}
+
+// forward + loss + backward into n->g; returns the loss (host).
+// Inputs and targets are laid out [img][c][plane] (in[c] = base + c * plane).
+int backward(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tgt, int w, int h,
+             int n_img, const float* levels_dev, float* loss, hipStream_t st) {
+    const size_t plane = (size_t)w * h;
+    const size_t px = plane * n_img;
+    int rc = ensure_ws(n, px, true);
+    if (rc) return rc;
+    float* outp[3] = {n->outp, n->outp + plane, n->outp + 2 * plane};
+    if ((rc = forward(n, in, (size_t)w, 3 * plane, w, h, n_img, 0.0f, levels_dev, outp, true, st)))
+        return rc;
+    const size_t nb_img = (plane + 255) / 256;
+    const size_t nblk = nb_img * n_img;
+    if ((rc = ensure_buf(&n->sq, &n->sq_cap, nblk))) return rc;
+    k_loss_grad<<<dim3((unsigned)nb_img, n_img), 256, 0, st>>>(n->outp, tgt[0], plane, n->dyb[0], n->sq);
+    PHIP(hipGetLastError());
+    std::vector<float> sqh(nblk);
+    const float scale = 1.0f / (float)(px * 3);
+    // layer 6 (index 5): x = act[5], dy = dyb[0] (16 ch)
+    if ((rc = wgrad<32, 16>(n, NL - 1, n->act[NL - 1], n->dyb[0], w, h, n_img, scale, st))) return rc;
+    ConvArgs a{};
+    a.w = w;
+    a.h = h;
+    a.transposed = 1;
+    a.in = n->dyb[0];
+    a.wt = n->w16 + n->off[NL - 1];
+    a.mask = n->act[NL - 1];
+    a.out = n->dyb[1];
+    if ((rc = launch_conv<16, 32, 2>(a, n_img, st))) return rc;
+    int cur = 1;
+    for (int l = NL - 2; l >= 1; --l) {
+        if ((rc = wgrad<32, 32>(n, l, n->act[l], n->dyb[cur], w, h, n_img, scale, st))) return rc;
+        a.in = n->dyb[cur];
+        a.wt = n->w16 + n->off[l];
+        a.mask = n->act[l];
+        a.out = n->dyb[1 - cur];
+        if ((rc = launch_conv<32, 32, 2>(a, n_img, st))) return rc;
+        cur = 1 - cur;
+    }
+    if ((rc = wgrad<16, 32>(n, 0, n->act[0], n->dyb[cur], w, h, n_img, scale, st))) return rc;
+    PHIP(hipMemcpyAsync(sqh.data(), n->sq, nblk * sizeof(float), hipMemcpyDeviceToHost, st));
+    PHIP(hipStreamSynchronize(st));
+    double acc = 0.0;
+    for (size_t i = 0; i < nblk; ++i) acc += sqh[i];
+    if (loss) *loss = (float)(0.5 * acc / (double)(px * 3));
+    return 0;
+}
+
+int upload_levels(cpipe_bm3d_pilot* n, const float* sigma, int n_img, hipStream_t st) {
+    int rc = ensure_buf(&n->levels, &n->levels_cap, (size_t)n_img);
+    if (rc) return rc;
+    std::vector<float> lv(n_img);
+    for (int i = 0; i < n_img; ++i) lv[i] = 10.0f * sigma[i];
+    PHIP(hipMemcpyAsync(n->levels, lv.data(), n_img * sizeof(float), hipMemcpyHostToDevice, st));
+    PHIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace
+
+namespace cpipe_internal {
+int pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size_t pitch, size_t img_stride,
+                  int w, int h, int n_img, float level, const float* levels, float* const* out,
+                  hipStream_t st) {
+    return forward(net, in, pitch, img_stride, w, h, n_img, level, levels, out, false, st);
+}
+}  // namespace cpipe_internal
+
+extern "C" {
+
+const char* cpipe_bm3d_pilot_last_error(void) { return g_perr.c_str(); }
+
+size_t cpipe_bm3d_pilot_param_count(void) { return total_params(); }
+
+int cpipe_bm3d_pilot_create(int device, cpipe_bm3d_pilot** out) {
+    if (!out) return perr(5, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return perr(8, "no HIP device visible");
+    if (device < 0 || device >= ndev) return perr(5, "device ordinal out of range");
+    PDevice g(device);
+    auto* n = new cpipe_bm3d_pilot_s();
+    n->device = device;
+    size_t o = 0;
+    for (int l = 0; l < NL; ++l) {
+        n->off[l] = o;
+        o += layer_params(l);
+    }
+    n->nparam = o;
+    n->live_host.assign(o, 1);
+    for (int l = 0; l < NL; ++l) {
+        for (int t = 0; t < 9; ++t)
+            for (int ci = 0; ci < CIN[l]; ++ci)
+                for (int co = 0; co < COUT[l]; ++co)
+                    if (ci >= CIN_REAL[l] || co >= COUT_REAL[l])
+                        n->live_host[n->off[l] + ((size_t)t * CIN[l] + ci) * COUT[l] + co] = 0;
+        for (int co = COUT_REAL[l]; co < COUT[l]; ++co) n->live_host[n->off[l] + layer_w(l) + co] = 0;
+    }
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMalloc((void**)&n->w32, o * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&n->m, o * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&n->v, o * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&n->g, o * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&n->w16, o * sizeof(_Float16));
+    if (e == hipSuccess) e = hipMalloc((void**)&n->live, o);
+    if (e == hipSuccess) e = hipMemcpy(n->live, n->live_host.data(), o, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(n->w32, 0, o * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(n->m, 0, o * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(n->v, 0, o * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(n->w16, 0, o * sizeof(_Float16));
+    if (e != hipSuccess) {
+        cpipe_bm3d_pilot_destroy(n);
+        return perr(e == hipErrorOutOfMemory ? 3 : 7, std::string("pilot alloc: ") + hipGetErrorString(e));
+    }
+    *out = n;
+    return 0;
+}
+
+void cpipe_bm3d_pilot_destroy(cpipe_bm3d_pilot* n) {
+    if (!n) return;
+    PDevice g(n->device);
+    (void)hipDeviceSynchronize();
+    free_ws(n);
+    for (void* p : {(void*)n->w32, (void*)n->m, (void*)n->v, (void*)n->g, (void*)n->w16,
+                    (void*)n->live, (void*)n->partial, (void*)n->sq, (void*)n->levels})
+        (void)hipFree(p);
+    delete n;
+}
+
+int cpipe_bm3d_pilot_init(cpipe_bm3d_pilot* n, uint64_t seed) {
+    if (!n) return perr(5, "null pilot");
+    PDevice g(n->device);
+    std::vector<float> w(n->nparam, 0.0f);
+    uint64_t s = seed;
+    for (int l = 0; l < NL; ++l) {
+        const double fan_in = 9.0 * CIN_REAL[l];
+        double std = std::sqrt(2.0 / fan_in);  // He init for ReLU layers
+        if (l == NL - 1) std *= 0.1;           // residual output starts near zero
+        for (size_t i = 0; i < layer_w(l); ++i) {
+            const double u1 = uniform01(s), u2 = uniform01(s);
+            const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+            if (n->live_host[n->off[l] + i]) w[n->off[l] + i] = (float)(z * std);
+        }
+    }
+    n->step = 0;
+    n->seed = seed;
+    PHIP(hipMemcpy(n->w32, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    PHIP(hipMemset(n->m, 0, n->nparam * sizeof(float)));
+    PHIP(hipMemset(n->v, 0, n->nparam * sizeof(float)));
+    k_to_half<<<(unsigned)((n->nparam + 255) / 256), 256>>>(n->w32, n->w16, (int)n->nparam);
+    PHIP(hipGetLastError());
+    PHIP(hipDeviceSynchronize());
+    return 0;
+}
+
+size_t cpipe_bm3d_pilot_blob_size(int with_optimizer) {
+    return sizeof(BlobHeader) + total_params() * sizeof(float) * (with_optimizer ? 3 : 1);
+}
+
+int cpipe_bm3d_pilot_save(const cpipe_bm3d_pilot* n, void* blob, size_t cap, int with_optimizer) {
+    if (!n || !blob) return perr(5, "null argument");
+    const size_t need = cpipe_bm3d_pilot_blob_size(with_optimizer);
+    if (cap < need) return perr(5, "blob buffer too small");
+    PDevice g(n->device);
+    BlobHeader h{kBlobMagic, kBlobVersion, (uint32_t)n->nparam, with_optimizer ? 1u : 0u, n->step, n->seed};
+    std::memcpy(blob, &h, sizeof(h));
+    auto* f = reinterpret_cast<float*>(static_cast<char*>(blob) + sizeof(h));
+    PHIP(hipDeviceSynchronize());
+    PHIP(hipMemcpy(f, n->w32, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
+    if (with_optimizer) {
+        PHIP(hipMemcpy(f + n->nparam, n->m, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
+        PHIP(hipMemcpy(f + 2 * n->nparam, n->v, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int cpipe_bm3d_pilot_load(cpipe_bm3d_pilot* n, const void* blob, size_t size) {
+    if (!n || !blob) return perr(5, "null argument");
+    if (size < sizeof(BlobHeader)) return perr(5, "blob too small");
+    BlobHeader h;
+    std::memcpy(&h, blob, sizeof(h));
+    if (h.magic != kBlobMagic || h.version != kBlobVersion) return perr(5, "not a pilot weight blob (magic/version)");
+    if (h.nparam != n->nparam) return perr(5, "blob was written for another network shape");
+    const bool opt = (h.flags & 1u) != 0;
+    if (size < cpipe_bm3d_pilot_blob_size(opt ? 1 : 0)) return perr(5, "blob truncated");
+    PDevice g(n->device);
+    const float* f = reinterpret_cast<const float*>(static_cast<const char*>(blob) + sizeof(h));
+    PHIP(hipDeviceSynchronize());
+    PHIP(hipMemcpy(n->w32, f, n->nparam * sizeof(float), hipMemcpyHostToDevice));
+    if (opt) {
+        PHIP(hipMemcpy(n->m, f + n->nparam, n->nparam * sizeof(float), hipMemcpyHostToDevice));
+        PHIP(hipMemcpy(n->v, f + 2 * n->nparam, n->nparam * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        PHIP(hipMemset(n->m, 0, n->nparam * sizeof(float)));
+        PHIP(hipMemset(n->v, 0, n->nparam * sizeof(float)));
+    }
+    n->step = opt ? h.step : 0;
+    n->seed = h.seed;
+    k_to_half<<<(unsigned)((n->nparam + 255) / 256), 256>>>(n->w32, n->w16, (int)n->nparam);
+    PHIP(hipGetLastError());
+    PHIP(hipDeviceSynchronize());
+    return 0;
+}
+
+int cpipe_bm3d_pilot_forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch,
+                             size_t img_stride, int w, int h, int n_img, float sigma,
+                             float* const* out, void* stream) {
+    if (!n || !in || !out) return perr(5, "null argument");
+    if (w <= 0 || h <= 0 || n_img <= 0 || pitch < (size_t)w) return perr(5, "bad geometry");
+    PDevice g(n->device);
+    return forward(n, in, pitch, img_stride, w, h, n_img, 10.0f * sigma, nullptr, out, false,
+                   (hipStream_t)stream);
+}
+
+int cpipe_bm3d_pilot_gradients(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tgt,
+                               int w, int h, int n_img, const float* sigma, float* grads_host,
+                               float* loss) {
+    if (!n || !in || !tgt || !sigma) return perr(5, "null argument");
+    PDevice g(n->device);
+    hipStream_t st = nullptr;
+    int rc = upload_levels(n, sigma, n_img, st);
+    if (rc) return rc;
+    if ((rc = backward(n, in, tgt, w, h, n_img, n->levels, loss, st))) return rc;
+    if (grads_host) {
+        PHIP(hipStreamSynchronize(st));
+        PHIP(hipMemcpy(grads_host, n->g, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int cpipe_bm3d_pilot_train_step(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tgt,
+                                int w, int h, int n_img, const float* sigma, float lr, float* loss) {
+    if (!n || !in || !tgt || !sigma) return perr(5, "null argument");
+    PDevice g(n->device);
+    hipStream_t st = nullptr;
+    int rc = upload_levels(n, sigma, n_img, st);
+    if (rc) return rc;
+    if ((rc = backward(n, in, tgt, w, h, n_img, n->levels, loss, st))) return rc;
+    n->step++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
+    k_adam<<<(unsigned)((n->nparam + 255) / 256), 256, 0, st>>>(n->w32, n->m, n->v, n->g, n->live,
+                                                                 n->w16, (int)n->nparam, lr, b1, b2,
+                                                                 eps, c1, c2);
+    PHIP(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

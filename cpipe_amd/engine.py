@@ -17,14 +17,18 @@ PRECISION_BF16 = 1
 PRECISION_FP16 = 2
 _PRECISIONS = {"fp32": PRECISION_FP32, "bf16": PRECISION_BF16, "fp16": PRECISION_FP16}
 
-STAGES = ("color", "bm1", "group1", "merge1", "bm2", "group2", "merge2")
+STAGES = ("color", "bm1", "group1", "merge1", "bm2", "group2", "merge2", "pilot")
 
 # Every symbol the C ABI header declares (checked by the CPU test suite).
 EXPORTS = (
     "cpipe_bm3d_last_error", "cpipe_bm3d_device_count", "cpipe_bm3d_plan_create",
     "cpipe_bm3d_plan_destroy", "cpipe_bm3d_plan_run", "cpipe_bm3d_plan_run_rgba16f",
     "cpipe_bm3d_plan_basic", "cpipe_bm3d_plan_matches", "cpipe_bm3d_plan_ref_grid",
-    "cpipe_bm3d_plan_check",
+    "cpipe_bm3d_plan_check", "cpipe_bm3d_plan_step1_opponent", "cpipe_bm3d_plan_set_pilot",
+    "cpipe_bm3d_plan_pilot_report", "cpipe_bm3d_pilot_create", "cpipe_bm3d_pilot_destroy",
+    "cpipe_bm3d_pilot_init", "cpipe_bm3d_pilot_save", "cpipe_bm3d_pilot_load",
+    "cpipe_bm3d_pilot_forward", "cpipe_bm3d_pilot_gradients", "cpipe_bm3d_pilot_train_step",
+    "cpipe_bm3d_pilot_blob_size", "cpipe_bm3d_pilot_param_count", "cpipe_bm3d_pilot_last_error",
     "cpipe_bm3d_plan_set_timing", "cpipe_bm3d_plan_timing", "cpipe_bm3d_compat_rgba16f",
     "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_node_descriptor",
     "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d",
@@ -66,6 +70,10 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_plan_matches.argtypes = [P, I, P, P, P]
     L.cpipe_bm3d_plan_ref_grid.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I)]
     L.cpipe_bm3d_plan_check.argtypes = [P]
+    L.cpipe_bm3d_plan_step1_opponent.argtypes = [P, P, S, F, P, P, S, P]
+    L.cpipe_bm3d_plan_set_pilot.argtypes = [P, P, P]
+    L.cpipe_bm3d_plan_pilot_report.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I),
+                                               ctypes.POINTER(F), ctypes.POINTER(I)]
     L.cpipe_bm3d_plan_set_timing.argtypes = [P, I]
     L.cpipe_bm3d_plan_timing.argtypes = [P, ctypes.POINTER(F), I, ctypes.POINTER(I)]
     L.cpipe_bm3d_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I, P]
@@ -96,6 +104,12 @@ def _stream_handle(stream) -> int:
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
+
+
+class PilotOpts(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("fallback_db", ctypes.c_float),
+                ("check_block", ctypes.c_int32), ("check_every", ctypes.c_int32),
+                ("full_fraction", ctypes.c_float)]
 
 
 class Plan:
@@ -165,6 +179,34 @@ class Plan:
         _check(lib().cpipe_bm3d_plan_matches(self._h, int(step), pos.data_ptr(), cnt.data_ptr(),
                                              _stream_handle(stream)))
         return pos, cnt
+
+    def step1_opponent(self, src, noisy_out, basic_out, sigma, stream=None) -> None:
+        """Plain step 1 for training pairs: opponent noisy and basic planes (3, H, W)."""
+        self._check_planes(src)
+        self._check_planes(noisy_out)
+        self._check_planes(basic_out)
+        _check(lib().cpipe_bm3d_plan_step1_opponent(
+            self._h, _ptr_array([src[c].data_ptr() for c in range(3)]), src.stride(1), float(sigma),
+            _ptr_array([noisy_out[c].data_ptr() for c in range(3)]),
+            _ptr_array([basic_out[c].data_ptr() for c in range(3)]), noisy_out.stride(1),
+            _stream_handle(stream)))
+
+    def set_pilot(self, net, fallback_db: float = 0.5, check_block: int = 256,
+                  check_every: int = 16, full_fraction: float = 0.5) -> None:
+        """Use a learned pilot (cpipe_amd.pilot.Pilot) in place of step 1; None restores it."""
+        opts = PilotOpts(ctypes.sizeof(PilotOpts), fallback_db, check_block, check_every,
+                         full_fraction)
+        _check(lib().cpipe_bm3d_plan_set_pilot(self._h, net.handle if net is not None else None,
+                                               ctypes.byref(opts)))
+        self._pilot = net  # the plan references the network
+
+    def pilot_report(self):
+        nc, nf, full = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        md = ctypes.c_float()
+        _check(lib().cpipe_bm3d_plan_pilot_report(self._h, ctypes.byref(nc), ctypes.byref(nf),
+                                                  ctypes.byref(md), ctypes.byref(full)))
+        return {"checked": nc.value, "fallback": nf.value, "max_delta_db": md.value,
+                "full_fallback": bool(full.value)}
 
     def check(self) -> None:
         """Synchronise and raise if a run hit an internal consistency check."""

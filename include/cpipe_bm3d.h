@@ -50,7 +50,8 @@ enum {
     CPIPE_BM3D_T_BM2 = 4,     /* block matching, step 2 */
     CPIPE_BM3D_T_GROUP2 = 5,  /* transform + Wiener + tile aggregation */
     CPIPE_BM3D_T_MERGE2 = 6,  /* tile merge + normalisation + inverse colour */
-    CPIPE_BM3D_T_COUNT = 7
+    CPIPE_BM3D_T_PILOT = 7,   /* learned pilot + spot checks (replaces step 1) */
+    CPIPE_BM3D_T_COUNT = 8
 };
 
 typedef struct cpipe_bm3d_plan_s cpipe_bm3d_plan;
@@ -108,6 +109,73 @@ int cpipe_bm3d_plan_timing(cpipe_bm3d_plan* plan, float* ms, int n, int* runs);
 int cpipe_bm3d_compat_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
                               size_t out_stride_px, int width, int height, float sigma, int stage,
                               void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned pilot: a 6-layer 3x3 convolutional network (32 channels, FP16
+ * weights and activations, FP32 accumulation, residual output) mapping the
+ * noisy opponent planes and the noise level to the step-1 basic estimate.
+ * No counterpart in the reference (build-only component of the node).
+ * Planes are FP32 opponent-space Y, U, V; image i of a stack starts at
+ * plane[c] + i * img_stride.
+ * ------------------------------------------------------------------------- */
+typedef struct cpipe_bm3d_pilot_s cpipe_bm3d_pilot;
+
+const char* cpipe_bm3d_pilot_last_error(void);
+size_t cpipe_bm3d_pilot_param_count(void);
+int cpipe_bm3d_pilot_create(int device, cpipe_bm3d_pilot** out);
+void cpipe_bm3d_pilot_destroy(cpipe_bm3d_pilot* net);
+/* He-normal initialisation from a seed (deterministic). */
+int cpipe_bm3d_pilot_init(cpipe_bm3d_pilot* net, uint64_t seed);
+/* Versioned weight blob (magic "CPPL", version 1); with_optimizer adds the
+ * Adam moments and step count (a training checkpoint). */
+size_t cpipe_bm3d_pilot_blob_size(int with_optimizer);
+int cpipe_bm3d_pilot_save(const cpipe_bm3d_pilot* net, void* blob, size_t cap, int with_optimizer);
+int cpipe_bm3d_pilot_load(cpipe_bm3d_pilot* net, const void* blob, size_t size);
+/* Inference on device planes (row pitch `pitch`). */
+int cpipe_bm3d_pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size_t pitch,
+                             size_t img_stride, int width, int height, int n_img, float sigma,
+                             float* const* out, void* stream);
+/* Training primitives on a stack of n_img images laid out [img][c][plane]
+ * (in[c] = base + c * width * height, image stride 3 * width * height), with
+ * one sigma per image (host array). Loss = 0.5 * mean squared error over
+ * pixels and channels. gradients: forward + backward only (FP32 dLoss/dparam
+ * copied to grads_host, cpipe_bm3d_pilot_param_count() floats); train_step:
+ * forward + backward + one Adam update. */
+int cpipe_bm3d_pilot_gradients(cpipe_bm3d_pilot* net, const float* const* in,
+                               const float* const* target, int width, int height, int n_img,
+                               const float* sigma, float* grads_host, float* loss);
+int cpipe_bm3d_pilot_train_step(cpipe_bm3d_pilot* net, const float* const* in,
+                                const float* const* target, int width, int height, int n_img,
+                                const float* sigma, float lr, float* loss);
+
+/* Step 1 only, for training the pilot: writes the opponent-space noisy planes
+ * and the plain basic estimate (3 planes each, device, row pitch out_pitch). */
+int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* plan, const float* const* in, size_t in_pitch,
+                                   float sigma, float* const* noisy_out, float* const* basic_out,
+                                   size_t out_pitch, void* stream);
+
+/* Learned pilot in a plan (3-channel plans): step 1 is replaced by the
+ * network; blocks of check_block x check_block pixels, one in check_every
+ * (block b = by * nbx + bx in raster order is checked when
+ * (b + by) % check_every == check_every / 2), are spot-checked against the
+ * plain step 1 computed exactly on that block. A block whose method-noise
+ * ratio |10 log10(|z - pilot|^2 / |z - plain|^2)| exceeds fallback_db uses the
+ * plain estimate; if more than full_fraction of the checked blocks fall back,
+ * the plain step 1 runs on the whole frame (decided on the device, no host sync).
+ * net = NULL restores the plain path. The plan keeps a reference to net. */
+typedef struct {
+    uint32_t struct_size;
+    float fallback_db;    /* default 0.5 */
+    int32_t check_block;  /* default 256 */
+    int32_t check_every;  /* default 16; 0 disables spot checks */
+    float full_fraction;  /* whole-frame fallback when more than this fraction of the
+                             checked blocks fall back; default 0.5 (>= 1 disables it) */
+} cpipe_bm3d_pilot_opts;
+int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* plan, cpipe_bm3d_pilot* net,
+                              const cpipe_bm3d_pilot_opts* opts);
+/* Outcome of the last run's spot checks (synchronises the device). */
+int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* plan, int* n_checked, int* n_fallback,
+                                 float* max_delta_db, int* full_fallback);
 
 /* Host-memory conveniences used by the node: copy in, run, copy out, sync. */
 int cpipe_bm3d_host_planes(const float* const* in_planes, size_t in_pitch, float* const* out_planes,

@@ -62,6 +62,12 @@ def lib() -> ctypes.CDLL:
         L.oracle_bm3d_f32.restype = I
         L.oracle_bm3d_f64.argtypes = [P, P, P, I, I, I, D]
         L.oracle_bm3d_f64.restype = I
+        L.oracle_pilot_param_count.restype = S
+        L.oracle_pilot_forward_f64.argtypes = [P, P, P, I, I, I, P]
+        L.oracle_pilot_loss_f64.argtypes = [P, P, P, P, I, I, I]
+        L.oracle_pilot_loss_f64.restype = D
+        L.oracle_pilot_grad_f64.argtypes = [P, P, P, P, I, I, I, P]
+        L.oracle_pilot_grad_f64.restype = D
         _lib = L
     return _lib
 
@@ -175,3 +181,39 @@ def bm3d_f64(planes: np.ndarray, sigma: float, want_basic: bool = False):
     if rc != 0:
         raise RuntimeError(f"oracle_bm3d_f64 failed: {rc}")
     return (out, basic) if want_basic else out
+
+
+# ---------------------------------------------------------------- learned pilot
+def pilot_param_count() -> int:
+    return int(lib().oracle_pilot_param_count())
+
+
+def pilot_forward_f64(params: np.ndarray, inp: np.ndarray, sigma) -> np.ndarray:
+    """inp: (N, 3, H, W) opponent planes; sigma: (N,). Returns (N, 3, H, W) float64."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(inp, dtype=np.float64)
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    n, c, h, w = x.shape
+    out = np.empty_like(x)
+    lib().oracle_pilot_forward_f64(_ptr(p), _ptr(x), _ptr(s), w, h, n, _ptr(out))
+    return out
+
+
+def pilot_loss_f64(params, inp, target, sigma) -> float:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(inp, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    n, c, h, w = x.shape
+    return float(lib().oracle_pilot_loss_f64(_ptr(p), _ptr(x), _ptr(t), _ptr(s), w, h, n))
+
+
+def pilot_grad_f64(params, inp, target, sigma):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(inp, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    n, c, h, w = x.shape
+    g = np.zeros(pilot_param_count(), dtype=np.float64)
+    loss = lib().oracle_pilot_grad_f64(_ptr(p), _ptr(x), _ptr(t), _ptr(s), w, h, n, _ptr(g))
+    return float(loss), g

@@ -1,0 +1,99 @@
+"""The learned pilot's HIP kernels against the FP64 CPU oracle.
+
+Tolerances reflect FP16 activations and weights with FP32 accumulation:
+forward max-abs <= 5e-3 on O(1) planes; per-layer gradient relative L2
+error <= 5e-2 and cosine >= 0.995.
+"""
+import numpy as np
+import pytest
+
+from cpipe_amd import pilot
+
+pytestmark = pytest.mark.gpu
+
+CINP = [16, 32, 32, 32, 32, 32]
+COUTP = [32, 32, 32, 32, 32, 16]
+
+
+def _slices():
+    off, out = 0, []
+    for l in range(6):
+        nw = 9 * CINP[l] * COUTP[l]
+        out.append((off, nw + COUTP[l]))
+        off += nw + COUTP[l]
+    return out
+
+
+def _data(torch, n, h, w, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0.0, 0.25, size=(n, 3, h, w)).astype(np.float32)
+    x[:, 0] += 0.4
+    t = (x + rng.normal(0, 0.05, size=x.shape)).astype(np.float32)
+    sig = rng.uniform(0.01, 0.1, size=n).astype(np.float32)
+    return x, t, sig, torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+
+
+def test_forward_matches_oracle(cuda, oracle_lib):
+    torch = cuda
+    net = pilot.Pilot(seed=7)
+    x, _, sig, xd, _ = _data(torch, 2, 37, 70, 0)
+    print("pilot forward 2x37x70", flush=True)
+    outs = [net.forward(xd[i:i + 1].contiguous(), float(sig[i])) for i in range(2)]
+    torch.cuda.synchronize()
+    got = torch.cat(outs).cpu().numpy()
+    want = oracle_lib.pilot_forward_f64(net.params(), x, sig.astype(np.float64))
+    err = float(np.max(np.abs(got - want)))
+    assert err <= 5e-3, err
+    assert float(np.max(np.abs(got - x))) > 1e-3  # the network does something
+
+
+def test_gradients_match_oracle(cuda, oracle_lib):
+    torch = cuda
+    net = pilot.Pilot(seed=11)
+    x, t, sig, xd, td = _data(torch, 3, 20, 70, 1)
+    print("pilot gradients 3x20x70", flush=True)
+    loss, g = net.gradients(xd, td, sig)
+    loss64, g64 = oracle_lib.pilot_grad_f64(net.params(), x, t, sig.astype(np.float64))
+    assert loss == pytest.approx(loss64, rel=2e-2)
+    for l, (off, n) in enumerate(_slices()):
+        a, b = g[off:off + n].astype(np.float64), g64[off:off + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+        print(f"layer {l}: rel {rel:.3e} cos {cos:.6f}")
+        assert rel <= 5e-2 and cos >= 0.995, (l, rel, cos)
+
+
+def test_blob_round_trip_and_checkpoint(cuda):
+    torch = cuda
+    a = pilot.Pilot(seed=3)
+    x, t, sig, xd, td = _data(torch, 2, 16, 64, 2)
+    a.train_step(xd, td, sig, 1e-3)
+    blob = a.save(with_optimizer=True)
+    b = pilot.Pilot(seed=None, weights=blob)
+    np.testing.assert_array_equal(a.params(), b.params())
+    ya = a.forward(xd[:1].contiguous(), 0.05)
+    yb = b.forward(xd[:1].contiguous(), 0.05)
+    torch.cuda.synchronize()
+    assert torch.equal(ya, yb)
+    # resuming from the checkpoint continues identically
+    la = a.train_step(xd, td, sig, 1e-3)
+    lb = b.train_step(xd, td, sig, 1e-3)
+    assert la == lb
+    np.testing.assert_array_equal(a.params(), b.params())
+    with pytest.raises(Exception):
+        b.load(b"not a blob" * 8)
+
+
+def test_training_reduces_loss_deterministically(cuda):
+    torch = cuda
+    x, _, sig, xd, _ = _data(torch, 4, 32, 64, 3)
+    td = 0.8 * xd  # a mapping the residual network can learn
+    runs = []
+    for _ in range(2):
+        net = pilot.Pilot(seed=5)
+        losses = [net.train_step(xd, td, sig, 2e-3) for _ in range(30)]
+        runs.append((losses, net.params()))
+    print("losses", runs[0][0][0], runs[0][0][-1])
+    assert runs[0][0][-1] < 0.7 * runs[0][0][0]
+    assert runs[0][0] == runs[1][0]
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])

@@ -1,0 +1,87 @@
+"""The learned pilot inside a BM3D plan: spot checks and fallbacks.
+
+- every block checked with a 0 dB threshold: all fall back, the device gate
+  runs the plain step 1 on the whole frame, and the output is bit-identical
+  to the plain plan;
+- no fallback: the step-2 input is exactly the network's output;
+- partial fallback: checked blocks that fall back carry the plain basic
+  estimate exactly.
+"""
+import numpy as np
+import pytest
+
+from cpipe_amd import engine, pilot, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame(torch, name="rgb200x136", seed=4):
+    clean, noisy, sigma = synth.workload(name, seed=seed)
+    return clean, noisy, sigma, torch.from_numpy(noisy).cuda()
+
+
+def _run(torch, src, sigma, net=None, **opts):
+    c, h, w = src.shape
+    plan = engine.Plan(w, h, 3)
+    if net is not None:
+        plan.set_pilot(net, **opts)
+    out = torch.empty_like(src)
+    basic = torch.empty_like(src)
+    plan.run(src, out, sigma)
+    plan.basic(basic)
+    torch.cuda.synchronize()
+    plan.check()
+    rep = plan.pilot_report()
+    res = out.cpu().numpy(), basic.cpu().numpy(), rep
+    plan.close()
+    return res
+
+
+def test_full_fallback_is_bit_identical_to_plain(cuda):
+    torch = cuda
+    _, _, sigma, src = _frame(torch)
+    plain, plain_basic, _ = _run(torch, src, sigma)
+    net = pilot.Pilot(seed=1)  # untrained: a poor pilot
+    out, basic, rep = _run(torch, src, sigma, net, fallback_db=0.0, check_block=64, check_every=1)
+    print(rep)
+    assert rep["checked"] == 4 * 3 and rep["fallback"] == rep["checked"] and rep["full_fallback"]
+    assert np.array_equal(out.view(np.uint32), plain.view(np.uint32))
+    assert np.array_equal(basic.view(np.uint32), plain_basic.view(np.uint32))
+
+
+def test_no_fallback_uses_the_network(cuda):
+    torch = cuda
+    _, noisy, sigma, src = _frame(torch)
+    net = pilot.Pilot(seed=2)
+    out, basic, rep = _run(torch, src, sigma, net, fallback_db=1e9, check_block=64, check_every=2)
+    assert rep["checked"] > 0 and rep["fallback"] == 0 and not rep["full_fallback"]
+    # the basic estimate is the network applied to the opponent planes
+    plan = engine.Plan(200, 136, 3)
+    z = torch.empty_like(src)
+    b = torch.empty_like(src)
+    plan.step1_opponent(src, z, b, sigma)
+    y = net.forward(z.unsqueeze(0).contiguous(), sigma)[0]
+    torch.cuda.synchronize()
+    y = y.cpu().numpy()
+    c13, c23, c43 = np.float32(1 / 3), np.float32(2 / 3), np.float32(4 / 3)
+    rgb = np.stack([(y[0] + y[1]) + y[2] * c23, y[0] - y[2] * c43, (y[0] - y[1]) + y[2] * c23])
+    np.testing.assert_array_equal(basic, rgb)
+    plan.close()
+
+
+def test_partial_fallback_blocks_carry_the_plain_estimate(cuda):
+    torch = cuda
+    _, _, sigma, src = _frame(torch, "rgb256x192", seed=6)
+    _, plain_basic, _ = _run(torch, src, sigma)
+    net = pilot.Pilot(seed=3)
+    out, basic, rep = _run(torch, src, sigma, net, fallback_db=0.0, check_block=64, check_every=3,
+                           full_fraction=2.0)
+    # every checked block falls back (threshold 0); whole-frame escalation disabled
+    assert rep["fallback"] == rep["checked"] == 4 and not rep["full_fallback"]
+    nbx = 4
+    for b in range(12):
+        if (b + b // nbx) % 3 != 1:
+            continue
+        x0, y0 = (b % nbx) * 64, (b // nbx) * 64
+        np.testing.assert_array_equal(basic[:, y0:y0 + 64, x0:x0 + 64],
+                                      plain_basic[:, y0:y0 + 64, x0:x0 + 64])
