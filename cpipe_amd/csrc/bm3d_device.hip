@@ -29,18 +29,13 @@
 #include <string>
 #include <vector>
 
+#include "bm3d_shared.h"
 #include "cpipe_bm3d.h"
 #include "pilot.h"
 
-namespace {
+using namespace cpipe_internal;
 
-constexpr int K = 8;          // patch edge
-constexpr int R = 19;         // search radius (39x39 window)
-constexpr int STRIDE = 3;     // reference stride
-constexpr int NMAX = 16;      // max group size
-constexpr int TILE = 16;      // refs per tile edge
-constexpr int FOOT = (TILE - 1) * STRIDE + 2 * R + K;  // 91: max tile footprint edge
-constexpr int FOOT2 = FOOT * FOOT;
+namespace {
 // Block matching workgroup: 16 ref columns x 14 ref rows. Each wave owns 4
 // columns; a column is 16 consecutive lanes (one DPP row): 14 refs + 2 helper
 // lanes that only compute patch-row sums for the refs above them.
@@ -415,23 +410,6 @@ __global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ p
 // (plain read-add-write: LDS float atomics cost ~180 cycles per instruction here).
 // ---------------------------------------------------------------------------
 
-struct GroupParams {
-    const float* noisy[3];
-    const float* basic[3];
-    float sig[3];
-    int w, h, nx, ny, ntx, nty;
-    const uint32_t* pos;
-    const uint8_t* cnt;
-    const int* fx0;
-    const int* fx1;
-    const int* fy0;
-    const int* fy1;
-    float* partial;
-    int* err;  // set to 1 if an ordering wait timed out (never expected)
-    int tx0, ty0, ntx_sub;  // tile sub-rectangle processed by this launch
-    const int* tiles;       // or an explicit list of tile indices (ty * ntx + tx)
-    const int* gate;        // null, or run only if *gate != 0
-};
 
 // Separable 2D DCT of the N member patches a wave holds (lane p = (i, j) holds
 // pixel p of every member), through a per-wave LDS scratch of 8 patches: each
@@ -524,9 +502,6 @@ __device__ __forceinline__ void haar_inv_n(float (&v)[NMAX], float s) {
     }
 }
 
-__device__ __forceinline__ int floor_pow2(int c) {
-    return c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
-}
 
 struct LaneTables {
     float Cr[8], Cc[8], Ci1[8], Ci2[8];
@@ -652,70 +627,8 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
                 default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
             }
         }
-        // Group g is accumulated right after group g-1: wait for the turn token
-        // (in LDS) to reach g. Every wave takes its turns in increasing g and
-        // turn g-1 never waits on turn g, so the chain always completes; the
-        // spin is bounded anyway and reports a timeout instead of hanging.
-        {
-            int spins = 0;
-            while (__hip_atomic_load(&turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != g) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1 << 24)) {
-                    timed_out = true;
-                    break;
-                }
-            }
-        }
-        {
-            if (has) {
-                const float wk = wgt * kw;
-                const int lo = (li - fy0) * fw + (lj - fx0);
-                // One member per step: its 64 lanes touch 64 distinct pixels, and
-                // LDS executes a wave's accesses in order. Four consecutive members
-                // whose patches are pairwise disjoint share one read/add/write
-                // round trip; the per-pixel order is unchanged either way.
-#pragma unroll
-                for (int m0 = 0; m0 < NMAX; m0 += 4) {
-                    if (m0 < n) {
-                        int o[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            o[j] = (int)(mp[m0 + j] >> 16) * fw + (int)(mp[m0 + j] & 0xffffu) + lo;
-                        bool disjoint = n >= 4;
-#pragma unroll
-                        for (int a = 0; a < 4; ++a)
-#pragma unroll
-                            for (int b = a + 1; b < 4; ++b) {
-                                const int dx = (int)(mp[m0 + a] & 0xffffu) - (int)(mp[m0 + b] & 0xffffu);
-                                const int dy = (int)(mp[m0 + a] >> 16) - (int)(mp[m0 + b] >> 16);
-                                disjoint = disjoint && (dx >= K || dx <= -K || dy >= K || dy <= -K);
-                            }
-                        if (disjoint) {
-                            float an[4], ad[4];
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                an[j] = acc[o[j]];
-                                ad[j] = acc[FOOT2 + o[j]];
-                            }
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                acc[o[j]] = an[j] + wk * v[m0 + j];
-                                acc[FOOT2 + o[j]] = ad[j] + wk;
-                            }
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                if (m0 + j < n) {
-                                    acc[o[j]] = acc[o[j]] + wk * v[m0 + j];
-                                    acc[FOOT2 + o[j]] = acc[FOOT2 + o[j]] + wk;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            __hip_atomic_store(&turn, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+        const int lo = (li - fy0) * fw + (lj - fx0);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
     }
     if (timed_out && lane == 0) atomicOr(p.err, 1);
     __syncthreads();
@@ -892,6 +805,7 @@ int ensure_tables(int device) {
         for (int j = 0; j < 8; ++j) kf[i * 8 + j] = (float)(k1[i] * k1[j]);
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dct), cf, sizeof(cf)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_kaiser2), kf, sizeof(kf)));
+    HIP_CHECK((hipError_t)group_mx_tables(device, cf, kf));
     g_tables_ready[device] = true;
     return OK;
 }
@@ -1071,7 +985,9 @@ int run_step(RunCtx& r, int step, const StepRange& g, float* const* out, size_t 
     gp.tiles = g.bm_list ? g.tile_list : nullptr;
     gp.gate = gate;
     const dim3 ggrid(g.bm_list ? g.n_tiles : (g.tx1 - g.tx0) * (g.ty1 - g.ty0), C);
-    if (step == 1) {
+    if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
+        HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, gp, ggrid, r.st));
+    } else if (step == 1) {
         k_group<1><<<ggrid, 256, 0, r.st>>>(gp);
     } else {
         k_group<2><<<ggrid, 256, 0, r.st>>>(gp);
@@ -1196,8 +1112,9 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
     if (d->width <= 0 || d->height <= 0 || d->width > 65535 || d->height > 65535)
         return fail(BAD_INDEX, "width/height out of range (1..65535)");
     if (d->channels != 1 && d->channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
-    if (d->precision != CPIPE_BM3D_PRECISION_FP32)
-        return fail(UNSUPPORTED, "only CPIPE_BM3D_PRECISION_FP32 is built");
+    if (d->precision != CPIPE_BM3D_PRECISION_FP32 && d->precision != CPIPE_BM3D_PRECISION_FP16 &&
+        d->precision != CPIPE_BM3D_PRECISION_BF16)
+        return fail(BAD_INDEX, "unknown precision");
     const int ndev = cpipe_bm3d_device_count();
     if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
     if (d->device < 0 || d->device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");

@@ -1,0 +1,156 @@
+// Definitions shared by the BM3D engine's translation units (plan + exact FP32
+// kernels in bm3d_device.hip, mixed-precision group kernels in group_mx.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cpipe_internal {
+
+constexpr int K = 8;          // patch edge
+constexpr int R = 19;         // search radius (39x39 window)
+constexpr int STRIDE = 3;     // reference stride
+constexpr int NMAX = 16;      // max group size
+constexpr int TILE = 16;      // refs per tile edge
+constexpr int FOOT = (TILE - 1) * STRIDE + 2 * R + K;  // 91: max tile footprint edge
+constexpr int FOOT2 = FOOT * FOOT;
+
+// Arguments of the group kernels (one workgroup per (tile, channel)).
+struct GroupParams {
+    const float* noisy[3];
+    const float* basic[3];
+    float sig[3];
+    int w, h, nx, ny, ntx, nty;
+    const uint32_t* pos;
+    const uint8_t* cnt;
+    const int* fx0;
+    const int* fx1;
+    const int* fy0;
+    const int* fy1;
+    float* partial;
+    int* err;  // set to 1 if an ordering wait timed out (never expected)
+    int tx0, ty0, ntx_sub;  // tile sub-rectangle processed by this launch
+    const int* tiles;       // or an explicit list of tile indices (ty * ntx + tx)
+    const int* gate;        // null, or run only if *gate != 0
+};
+
+// Adds one filtered group to a tile's LDS num/den footprint (acc[0, FOOT2) and
+// acc[FOOT2, 2 FOOT2)) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
+// member m in v[m]; mp[m] (wave-uniform) is the member's footprint position
+// (y << 16 | x); lo = (i - fy0) * fw + (j - fx0). Group g is accumulated right
+// after group g-1: the wave waits for the LDS turn token to reach g, adds
+// num += wk * v and den += wk, then passes the token on. Every wave takes its
+// turns in increasing g and turn g-1 never waits on turn g, so the chain always
+// completes; the spin is bounded anyway and returns true (timed out) instead of
+// hanging. `has` false: only pass the token.
+//
+// The turns form the kernel's critical path, so everything that does not
+// touch the footprint (offsets, products, the batching plan) is computed
+// before the wait. Members are added in read/add/write rounds: a block of
+// four consecutive members whose patches are pairwise disjoint in one round,
+// else each disjoint pair in one round, else one member per round. A pixel's
+// members are added in member order either way.
+__device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, bool has, int n,
+                                                  const uint32_t (&mp)[NMAX],
+                                                  const float (&v)[NMAX], float wk, int lo,
+                                                  int fw) {
+    int o[NMAX];
+    float c[NMAX];
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) {
+        o[m] = (int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu) + lo;
+        c[m] = wk * v[m];
+    }
+    auto overlap = [&](int a, int b) {
+        const int dx = (int)(mp[a] & 0xffffu) - (int)(mp[b] & 0xffffu);
+        const int dy = (int)(mp[a] >> 16) - (int)(mp[b] >> 16);
+        return dx < K && dx > -K && dy < K && dy > -K;
+    };
+    // bit b: block b (members 4b .. 4b+3) is one round; bit 4 + k: pair k is one round
+    uint32_t plan = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int m0 = 4 * b;
+        const bool p0 = !overlap(m0, m0 + 1), p1 = !overlap(m0 + 2, m0 + 3);
+        const bool all = p0 && p1 && !overlap(m0, m0 + 2) && !overlap(m0, m0 + 3) &&
+                         !overlap(m0 + 1, m0 + 2) && !overlap(m0 + 1, m0 + 3);
+        plan |= (all && m0 + 4 <= n ? 1u : 0u) << b;
+        plan |= (p0 && m0 + 2 <= n ? 1u : 0u) << (4 + 2 * b);
+        plan |= (p1 && m0 + 4 <= n ? 1u : 0u) << (5 + 2 * b);
+    }
+    plan = __builtin_amdgcn_readfirstlane(plan);
+    // keep the precomputation ahead of the wait
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) asm volatile("" : "+v"(o[m]), "+v"(c[m]));
+    asm volatile("" : "+s"(plan));
+
+    bool timed_out = false;
+    int spins = 0;
+    while (__hip_atomic_load(turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != g) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1 << 24)) {
+            timed_out = true;
+            break;
+        }
+    }
+    if (has) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int m0 = 4 * b;
+            if (m0 < n) {
+                if (plan & (1u << b)) {
+                    float an[4], ad[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        an[j] = acc[o[m0 + j]];
+                        ad[j] = acc[FOOT2 + o[m0 + j]];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        acc[o[m0 + j]] = an[j] + c[m0 + j];
+                        acc[FOOT2 + o[m0 + j]] = ad[j] + wk;
+                    }
+                } else {
+#pragma unroll
+                    for (int pr = 0; pr < 2; ++pr) {
+                        const int q0 = m0 + 2 * pr;
+                        if (q0 < n) {
+                            if (plan & (1u << (4 + 2 * b + pr))) {
+                                const float a0 = acc[o[q0]], a1 = acc[o[q0 + 1]];
+                                const float d0 = acc[FOOT2 + o[q0]], d1 = acc[FOOT2 + o[q0 + 1]];
+                                acc[o[q0]] = a0 + c[q0];
+                                acc[o[q0 + 1]] = a1 + c[q0 + 1];
+                                acc[FOOT2 + o[q0]] = d0 + wk;
+                                acc[FOOT2 + o[q0 + 1]] = d1 + wk;
+                            } else {
+#pragma unroll
+                                for (int j = 0; j < 2; ++j) {
+                                    if (q0 + j < n) {
+                                        acc[o[q0 + j]] = acc[o[q0 + j]] + c[q0 + j];
+                                        acc[FOOT2 + o[q0 + j]] = acc[FOOT2 + o[q0 + j]] + wk;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __hip_atomic_store(turn, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return timed_out;
+}
+
+__device__ __forceinline__ int floor_pow2(int c) {
+    return c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+}
+
+// Mixed-precision group kernels (group_mx.hip). group_mx_tables uploads the
+// 8x8 DCT-II matrix C[u][i] and the 8x8 Kaiser window (row-major, FP32) to the
+// current device, once per device; group_mx_launch enqueues the step-1 or
+// step-2 group kernel with FP16 or BF16 MFMA transforms on the GroupParams
+// launch grid (tiles, channels). Both return 0 or a hipError_t.
+int group_mx_tables(int device, const float* dct64, const float* kaiser64);
+int group_mx_launch(int step, int precision, const GroupParams& p, dim3 grid, hipStream_t st);
+
+}  // namespace cpipe_internal

@@ -1,0 +1,327 @@
+// Mixed-precision BM3D group filtering for gfx950.
+//
+// The 3D transform of a group (2D DCT-II of each 8x8 member, 1D orthonormal
+// Haar along the stack) and its inverse run as 16x16x16 MFMA tiles with FP16
+// or BF16 operands and FP32 accumulation. Shrinkage (hard threshold, Wiener),
+// group weights and the tile aggregation stay FP32, and the aggregation uses
+// the same fixed order as the exact kernels (groups in ref raster order via
+// the LDS turn token, members in match order), so a run is reproducible.
+//
+// Operand layouts of v_mfma_f32_16x16x16 (f16 and bf16_1k), lane l:
+//   A[i][k]: lane i + 16 (k / 4), element k % 4
+//   B[k][j]: lane j + 16 (k / 4), element k % 4
+//   D[i][j]: lane j + 16 (i / 4), element i % 4
+// A group of up to 16 members is the 16 x 64 matrix X (member x pixel,
+// pixel = 8 row + col); rows past the group size are zero.
+//   forward DCT   Y  = X Kt    A = X (lane r = member, 4 consecutive pixels), B = blocks of Kt
+//   forward Haar  Z  = H Y     A = H_n, B = Y (the D layout of the previous product)
+//   inverse Haar  Vt = Zt H    A = Zt (Z's D layout read as an A operand), B = H_n
+//   inverse DCT   Xt = Kt Vt   A = blocks of Kt, B = Vt (D layout of the previous product)
+// with K = C (x) C, K[u][p] = C[u / 8][p / 8] C[u % 8][p % 8], and H_n the
+// n-point multi-level orthonormal Haar matrix (the exact kernels' butterfly,
+// applied to unit vectors) zero-padded to 16 x 16. Xt ends in D layout (lane
+// r = member, 4 consecutive pixels of one patch row); a per-wave LDS scratch
+// turns it into lane = pixel for the ordered aggregation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bm3d_shared.h"
+#include "cpipe_bm3d.h"
+
+namespace cpipe_internal {
+namespace {
+
+__constant__ float c_mx_dct[64];        // C[u][i]
+__constant__ float c_mx_kaiser[64];     // Kaiser(8, beta=2) outer product
+__constant__ float c_mx_haar[5][256];   // H_n for n = 1, 2, 4, 8, 16, [row][col], zero-padded
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int PREC>
+struct Mx;
+
+template <>
+struct Mx<CPIPE_BM3D_PRECISION_FP16> {
+    typedef _Float16 vec __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ vec pack(float a, float b, float c, float d) {
+        vec r;
+        r[0] = (_Float16)a;
+        r[1] = (_Float16)b;
+        r[2] = (_Float16)c;
+        r[3] = (_Float16)d;
+        return r;
+    }
+    static __device__ __forceinline__ f4 mma(vec a, vec b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mx<CPIPE_BM3D_PRECISION_BF16> {
+    typedef short vec __attribute__((ext_vector_type(4)));
+    // round to nearest even (inputs are finite)
+    static __device__ __forceinline__ short bf(float x) {
+        uint32_t u = __float_as_uint(x);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (short)(u >> 16);
+    }
+    static __device__ __forceinline__ vec pack(float a, float b, float c, float d) {
+        vec r;
+        r[0] = bf(a);
+        r[1] = bf(b);
+        r[2] = bf(c);
+        r[3] = bf(d);
+        return r;
+    }
+    static __device__ __forceinline__ f4 mma(vec a, vec b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+};
+
+template <int PREC>
+__device__ __forceinline__ typename Mx<PREC>::vec pack4(const f4& x) {
+    return Mx<PREC>::pack(x[0], x[1], x[2], x[3]);
+}
+
+__device__ __forceinline__ float kmat(int u, int p) {
+    return c_mx_dct[(u >> 3) * 8 + (p >> 3)] * c_mx_dct[(u & 7) * 8 + (p & 7)];
+}
+
+// Orders this wave's LDS accesses before/after the point (lanes exchange data).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward 3D transform of one stack: z[nb] = (H_n X Kt)[:, 16 nb .. 16 nb + 15] in D layout.
+template <int PREC>
+__device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ src, int w,
+                                          uint32_t pm, bool live, int q,
+                                          const typename Mx<PREC>::vec (&kf)[4][4],
+                                          const typename Mx<PREC>::vec& ha) {
+    typedef Mx<PREC> M;
+    const int qx = (int)(pm & 0xffffu), qy = (int)(pm >> 16);
+    typename M::vec xa[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (live) {
+            const float* s = src + (size_t)(qy + 2 * kb + (q >> 1)) * w + qx + 4 * (q & 1);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[t] = s[t];
+        }
+        xa[kb] = M::pack(a[0], a[1], a[2], a[3]);
+    }
+    const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        f4 y = zero;
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) y = M::mma(xa[kb], kf[kb][nb], y);
+        z[nb] = M::mma(ha, pack4<PREC>(y), zero);
+    }
+}
+
+// Filters one group: 3D transform, shrinkage, inverse; leaves lane p = pixel p
+// of member m in v[m] (m < n) and returns the group's aggregation weight.
+template <int STEP, int PREC>
+__device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm, int r, int q,
+                                           int lane, const float* __restrict__ src,
+                                           const float* __restrict__ bsrc, int w, float* sb,
+                                           const typename Mx<PREC>::vec (&kf)[4][4],
+                                           const typename Mx<PREC>::vec (&ki)[4][4], float thr,
+                                           float s2) {
+    typedef Mx<PREC> M;
+    const bool live = r < n;
+    const float* hm = c_mx_haar[31 - __builtin_clz((unsigned)n)];
+    const typename M::vec ha =
+        M::pack(hm[r * 16 + 4 * q], hm[r * 16 + 4 * q + 1], hm[r * 16 + 4 * q + 2], hm[r * 16 + 4 * q + 3]);
+    const typename M::vec hb = M::pack(hm[(4 * q) * 16 + r], hm[(4 * q + 1) * 16 + r],
+                                       hm[(4 * q + 2) * 16 + r], hm[(4 * q + 3) * 16 + r]);
+    f4 z[4];
+    forward3d<PREC>(z, src, w, pm, live, q, kf, ha);
+    float wgt;
+    if (STEP == 1) {
+        // coefficient (member 4q + t, frequency 16 nb + r); rows >= n are exactly zero
+        int nnz = 0;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bool keep = fabsf(z[nb][t]) > thr;
+                z[nb][t] = keep ? z[nb][t] : 0.0f;
+                nnz += __popcll(__ballot(keep));
+            }
+        wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
+    } else {
+        f4 zb[4];
+        forward3d<PREC>(zb, bsrc, w, pm, live, q, kf, ha);
+        float aw = 0.0f;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float bb = zb[nb][t] * zb[nb][t];
+                const float d = bb + s2;
+                float W = d > 0.0f ? bb / d : 1.0f;
+                W = (4 * q + t) < n ? W : 0.0f;
+                z[nb][t] = W * z[nb][t];
+                aw = __builtin_fmaf(W, W, aw);
+            }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
+        wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
+    }
+    const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    typename M::vec vt[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) vt[nb] = pack4<PREC>(M::mma(pack4<PREC>(z[nb]), hb, zero));
+    f4 xt[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f4 x = zero;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) x = M::mma(ki[mb][nb], vt[nb], x);
+        xt[mb] = x;
+    }
+    // lane (member r, quarter q) holds pixels 16 mb + 4 q + t: transpose through
+    // LDS, 8 members at a time
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        if (hf * 8 < n) {
+            if ((r >> 3) == hf) {
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb)
+                    *reinterpret_cast<f4*>(sb + (r & 7) * 64 + 16 * mb + 4 * q) = xt[mb];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                if (hf * 8 + m < n) v[hf * 8 + m] = sb[m * 64 + lane];
+            wave_lds_sync();
+        }
+    }
+    return wgt;
+}
+
+template <int STEP, int PREC>
+__global__ __launch_bounds__(256) void k_group_mx(GroupParams p) {
+    typedef Mx<PREC> M;
+    __shared__ float acc[2 * FOOT2];
+    __shared__ __align__(16) float scratch[4][8 * 64];
+    __shared__ int turn;  // number of groups of this tile already accumulated
+    if (p.gate != nullptr && *p.gate == 0) return;
+    const int tile = p.tiles ? p.tiles[blockIdx.x]
+                             : (p.ty0 + (int)blockIdx.x / p.ntx_sub) * p.ntx + p.tx0 +
+                                   (int)blockIdx.x % p.ntx_sub;
+    const int tx = tile % p.ntx, ty = tile / p.ntx;
+    const int c = blockIdx.y;
+    if (threadIdx.x == 0) turn = 0;
+    bool timed_out = false;
+    const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
+    const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
+    for (int idx = threadIdx.x; idx < 2 * FOOT2; idx += blockDim.x) acc[idx] = 0.0f;
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, q = lane >> 4;
+    const int li = lane >> 3, lj = lane & 7;
+    float* sb = scratch[wave];
+    typename M::vec kf[4][4], ki[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            // B[k = pixel][j = freq] = K[freq][pixel]
+            kf[a][nb] = M::pack(kmat(16 * nb + r, 16 * a + 4 * q), kmat(16 * nb + r, 16 * a + 4 * q + 1),
+                                kmat(16 * nb + r, 16 * a + 4 * q + 2), kmat(16 * nb + r, 16 * a + 4 * q + 3));
+            // A[i = pixel][k = freq] = K[freq][pixel]
+            ki[a][nb] = M::pack(kmat(16 * nb + 4 * q, 16 * a + r), kmat(16 * nb + 4 * q + 1, 16 * a + r),
+                                kmat(16 * nb + 4 * q + 2, 16 * a + r), kmat(16 * nb + 4 * q + 3, 16 * a + r));
+        }
+    const float kw = c_mx_kaiser[lane];
+    const float* __restrict__ src = p.noisy[c];
+    const float* __restrict__ bsrc = p.basic[c];
+    const float sc = p.sig[c];
+    const float thr = 2.7f * sc;
+    const float s2 = sc * sc;
+    const int w = p.w;
+    const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TILE, p.ny - ty * TILE);
+    const int G = nbx * nby;
+
+    for (int rnd = 0; rnd < (G + 3) / 4; ++rnd) {
+        const int g = rnd * 4 + wave;
+        const bool has = g < G;
+        int n = 0;
+        float wgt = 0.0f;
+        float v[NMAX];
+        uint32_t mp[NMAX];
+        if (has) {
+            const int bx = tx * TILE + g % nbx, by = ty * TILE + g / nbx;
+            const size_t ref = (size_t)by * p.nx + bx;
+            n = floor_pow2(p.cnt[ref]);
+            const uint32_t pm = p.pos[ref * NMAX + r];  // this lane's member (rows of X)
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m) mp[m] = __builtin_amdgcn_readlane(pm, m);
+            wgt = filter_mx<STEP, PREC>(v, n, pm, r, q, lane, src, bsrc, w, sb, kf, ki, thr, s2);
+        }
+        const int lo = (li - fy0) * fw + (lj - fx0);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
+    }
+    if (timed_out && lane == 0) atomicOr(p.err, 1);
+    __syncthreads();
+    float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * (2 * FOOT2);
+    for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
+        dst[idx] = acc[idx];
+        dst[FOOT2 + idx] = acc[FOOT2 + idx];
+    }
+}
+
+}  // namespace
+
+int group_mx_tables(int device, const float* dct64, const float* kaiser64) {
+    (void)device;  // the caller has made `device` current
+    float haar[5][256] = {};
+    for (int lvl = 0; lvl < 5; ++lvl) {
+        const int n = 1 << lvl;
+        for (int j = 0; j < n; ++j) {
+            double v[16] = {};
+            v[j] = 1.0;
+            const double s = std::sqrt(0.5);
+            for (int len = n; len > 1; len >>= 1) {
+                double tmp[16];
+                const int hh = len >> 1;
+                for (int k = 0; k < hh; ++k) {
+                    tmp[k] = (v[2 * k] + v[2 * k + 1]) * s;
+                    tmp[hh + k] = (v[2 * k] - v[2 * k + 1]) * s;
+                }
+                for (int k = 0; k < len; ++k) v[k] = tmp[k];
+            }
+            for (int i = 0; i < n; ++i) haar[lvl][i * 16 + j] = (float)v[i];
+        }
+    }
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mx_dct), dct64, 64 * sizeof(float))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mx_kaiser), kaiser64, 64 * sizeof(float))) != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_mx_haar), haar, sizeof(haar));
+}
+
+int group_mx_launch(int step, int precision, const GroupParams& p, dim3 grid, hipStream_t st) {
+    const bool bf = precision == CPIPE_BM3D_PRECISION_BF16;
+    if (step == 1) {
+        if (bf)
+            k_group_mx<1, CPIPE_BM3D_PRECISION_BF16><<<grid, 256, 0, st>>>(p);
+        else
+            k_group_mx<1, CPIPE_BM3D_PRECISION_FP16><<<grid, 256, 0, st>>>(p);
+    } else {
+        if (bf)
+            k_group_mx<2, CPIPE_BM3D_PRECISION_BF16><<<grid, 256, 0, st>>>(p);
+        else
+            k_group_mx<2, CPIPE_BM3D_PRECISION_FP16><<<grid, 256, 0, st>>>(p);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cpipe_internal

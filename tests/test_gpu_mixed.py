@@ -1,0 +1,75 @@
+"""Mixed-precision plans (FP16 / BF16 MFMA group transforms) through the C ABI.
+
+Bar: the output's PSNR against the clean image within 0.1 dB of the FP64
+textbook oracle's (both steps), PSNR >= 40 dB (FP16) / 34 dB (BF16) against
+the exact FP32 path, match lists of step 1 identical to the FP32 path (block
+matching is exact FP32 in every precision), and bit-identical output on a
+repeated run (fixed aggregation order).
+"""
+import numpy as np
+import pytest
+
+from cpipe_amd import engine, synth
+
+pytestmark = pytest.mark.gpu
+
+MIN_PSNR_VS_FP32 = {"fp16": 40.0, "bf16": 34.0}
+MAX_DELTA_VS_FP64_DB = 0.1
+
+
+def _run(torch, noisy, sigma, precision):
+    c, h, w = noisy.shape
+    src = torch.from_numpy(np.ascontiguousarray(noisy)).to("cuda:0")
+    out = torch.empty_like(src)
+    basic = torch.empty_like(src)
+    plan = engine.Plan(w, h, c, precision, 0)
+    plan.run(src, out, sigma)
+    plan.basic(basic)
+    p1, c1 = plan.matches(1)
+    torch.cuda.synchronize()
+    plan.check()
+    res = out.cpu().numpy(), basic.cpu().numpy(), p1.cpu().numpy(), c1.cpu().numpy()
+    plan.close()
+    return res
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("name,seed", [("rgb256x192", 0), ("gray200x150", 1), ("rgb61x47", 2)])
+def test_mixed_close_to_fp32_and_fp64(cuda, oracle_lib, precision, name, seed):
+    torch = cuda
+    clean, noisy, sigma = synth.workload(name, seed=seed)
+    print(f"case {name} seed {seed} precision {precision}", flush=True)
+    out, basic, p1, c1 = _run(torch, noisy, sigma, precision)
+    ref, rbasic, rp1, rc1 = _run(torch, noisy, sigma, "fp32")
+    assert np.all(np.isfinite(out))
+    np.testing.assert_array_equal(c1, rc1)
+    np.testing.assert_array_equal(p1, rp1)
+    vs32 = synth.psnr(out, ref)
+    vs32_basic = synth.psnr(basic, rbasic)
+    o64 = oracle_lib.bm3d_f64(noisy, sigma)
+    d64 = synth.psnr(out, clean) - synth.psnr(o64, clean)
+    print(f"psnr vs fp32 {vs32:.2f} dB (basic {vs32_basic:.2f}), delta vs fp64 {d64:+.4f} dB")
+    assert vs32 >= MIN_PSNR_VS_FP32[precision]
+    assert abs(d64) <= MAX_DELTA_VS_FP64_DB
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_mixed_is_deterministic(cuda, precision):
+    torch = cuda
+    _, noisy, sigma = synth.workload("rgb300x200", seed=5)
+    a = _run(torch, noisy, sigma, precision)[0]
+    b = _run(torch, noisy, sigma, precision)[0]
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_mixed_constant_image_closed_form(cuda):
+    """A constant image: every group's members are identical, so only DC terms
+    survive; the Wiener output is v * b^2 / (b^2 + sigma^2) with b = 32 v
+    (DC of the 16-member 3D transform), within FP16 rounding."""
+    torch = cuda
+    v, sigma = 0.4, 0.05
+    img = np.full((1, 64, 80), v, np.float32)
+    out = _run(torch, img, sigma, "fp16")[0]
+    b = 32.0 * v
+    want = v * b * b / (b * b + sigma * sigma)
+    assert np.max(np.abs(out - want)) <= 2e-3
