@@ -628,14 +628,14 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
             }
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
-        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw, FOOT2);
     }
     if (timed_out && lane == 0) atomicOr(p.err, 1);
     __syncthreads();
-    float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * (2 * FOOT2);
+    float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * p.tile_stride;
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
         dst[idx] = acc[idx];
-        dst[FOOT2 + idx] = acc[FOOT2 + idx];
+        dst[p.dofs + idx] = acc[FOOT2 + idx];
     }
 }
 
@@ -649,6 +649,8 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
 struct MergeParams {
     const float* partial;
     int w, h, ntx, nty, channels;
+    int dofs;            // den offset in a partial block
+    size_t tile_stride;  // floats per (channel, tile) partial block
     const int* txlo;
     const int* txhi;
     const int* tylo;
@@ -679,10 +681,10 @@ __global__ void k_merge(MergeParams p) {
             for (int tx = txl; tx <= txh; ++tx) {
                 const int fw = p.fx1[tx] - p.fx0[tx];
                 const float* pn =
-                    p.partial + ((size_t)c * p.ntx * p.nty + (size_t)ty * p.ntx + tx) * (2 * FOOT2);
+                    p.partial + ((size_t)c * p.ntx * p.nty + (size_t)ty * p.ntx + tx) * p.tile_stride;
                 const int off = ry * fw + (x - p.fx0[tx]);
                 num = num + pn[off];
-                den = den + pn[FOOT2 + off];
+                den = den + pn[p.dofs + off];
             }
         }
         o[c] = num / den;
@@ -829,6 +831,9 @@ private:
 struct cpipe_bm3d_plan_s {
     int w = 0, h = 0, c = 0, precision = 0, device = 0;
     int nx = 0, ny = 0, ntx = 0, nty = 0;
+    int tile_y = TILE;         // refs per tile column (16 exact FP32, 8 mixed precision)
+    int dofs = FOOT2;          // den offset in a partial block
+    size_t tile_stride = 2 * (size_t)FOOT2;  // floats per (channel, tile) partial block
     size_t npx = 0;
     float* z = nullptr;        // channels planes
     float* bz = nullptr;       // channels planes
@@ -867,9 +872,9 @@ struct cpipe_bm3d_plan_s {
 
 namespace {
 
-void tile_box_host(int t, int nref, int n, int* f0, int* f1) {
-    const int first = ref_pos(t * TILE, n);
-    const int lastref = std::min(t * TILE + TILE - 1, nref - 1);
+void tile_box_host(int t, int tile, int nref, int n, int* f0, int* f1) {
+    const int first = ref_pos(t * tile, n);
+    const int lastref = std::min(t * tile + tile - 1, nref - 1);
     const int last = ref_pos(lastref, n);
     *f0 = std::max(0, first - R);
     *f1 = std::min(n, last + R + K);
@@ -982,11 +987,13 @@ int run_step(RunCtx& r, int step, const StepRange& g, float* const* out, size_t 
     gp.tx0 = g.tx0;
     gp.ty0 = g.ty0;
     gp.ntx_sub = g.tx1 - g.tx0;
+    gp.dofs = p->dofs;
+    gp.tile_stride = p->tile_stride;
     gp.tiles = g.bm_list ? g.tile_list : nullptr;
     gp.gate = gate;
     const dim3 ggrid(g.bm_list ? g.n_tiles : (g.tx1 - g.tx0) * (g.ty1 - g.ty0), C);
     if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
-        HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, gp, ggrid, r.st));
+        HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, gp, ggrid, r.st));
     } else if (step == 1) {
         k_group<1><<<ggrid, 256, 0, r.st>>>(gp);
     } else {
@@ -1002,6 +1009,8 @@ int run_step(RunCtx& r, int step, const StepRange& g, float* const* out, size_t 
     mp.ntx = p->ntx;
     mp.nty = p->nty;
     mp.channels = C;
+    mp.dofs = p->dofs;
+    mp.tile_stride = p->tile_stride;
     mp.txlo = txlo;
     mp.txhi = txhi;
     mp.tylo = tylo;
@@ -1132,8 +1141,14 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
     if (p->w >= K && p->h >= K) {
         p->nx = ref_count(p->w);
         p->ny = ref_count(p->h);
+        // exact FP32 plans use 16 x 16-ref tiles (the CPU FP32 restatement's tiling);
+        // mixed-precision plans 16 x 8, whose smaller LDS footprint fits three
+        // workgroups per CU (three ordered aggregation chains instead of two)
+        p->tile_y = p->precision == CPIPE_BM3D_PRECISION_FP32 ? TILE : TILE / 2;
+        p->dofs = FOOT * foot_edge(p->tile_y);
+        p->tile_stride = 2 * (size_t)p->dofs;
         p->ntx = (p->nx + TILE - 1) / TILE;
-        p->nty = (p->ny + TILE - 1) / TILE;
+        p->nty = (p->ny + p->tile_y - 1) / p->tile_y;
         const size_t nref = (size_t)p->nx * p->ny;
         auto alloc = [&](void** ptr, size_t bytes) -> int {
             hipError_t e = hipMalloc(ptr, bytes);
@@ -1150,7 +1165,7 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
             (rc = alloc((void**)&p->pos[0], sizeof(uint32_t) * nref * NMAX)) ||
             (rc = alloc((void**)&p->pos[1], sizeof(uint32_t) * nref * NMAX)) ||
             (rc = alloc((void**)&p->cnt[0], nref)) || (rc = alloc((void**)&p->cnt[1], nref)) ||
-            (rc = alloc((void**)&p->partial, sizeof(float) * 2 * FOOT2 * ntiles * p->c)) ||
+            (rc = alloc((void**)&p->partial, sizeof(float) * p->tile_stride * ntiles * p->c)) ||
             (rc = alloc((void**)&p->err, sizeof(int))) ||
             (rc = alloc((void**)&p->geo, sizeof(int) * ngeo))) {
             free_plan(p);
@@ -1165,8 +1180,8 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
         int* fx1 = fx0 + p->ntx;
         int* fy0 = fx1 + p->ntx;
         int* fy1 = fy0 + p->nty;
-        for (int t = 0; t < p->ntx; ++t) tile_box_host(t, p->nx, p->w, &fx0[t], &fx1[t]);
-        for (int t = 0; t < p->nty; ++t) tile_box_host(t, p->ny, p->h, &fy0[t], &fy1[t]);
+        for (int t = 0; t < p->ntx; ++t) tile_box_host(t, TILE, p->nx, p->w, &fx0[t], &fx1[t]);
+        for (int t = 0; t < p->nty; ++t) tile_box_host(t, p->tile_y, p->ny, p->h, &fy0[t], &fy1[t]);
         for (int x = 0; x < p->w; ++x) {
             txlo[x] = -1;
             txhi[x] = -2;
@@ -1376,7 +1391,7 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
         c.ty0 = tylo[c.y0];
         c.ty1 = tyhi[c.y1 - 1] + 1;
         const int rx0 = c.tx0 * TILE, rx1 = std::min(c.tx1 * TILE, p->nx);
-        const int ry0 = c.ty0 * TILE, ry1 = std::min(c.ty1 * TILE, p->ny);
+        const int ry0 = c.ty0 * p->tile_y, ry1 = std::min(c.ty1 * p->tile_y, p->ny);
         c.btx0 = rx0 / BM_TX;
         c.btx1 = (rx1 + BM_TX - 1) / BM_TX;
         c.bty0 = ry0 / BM_TY;

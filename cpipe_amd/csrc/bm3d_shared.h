@@ -11,9 +11,12 @@ constexpr int K = 8;          // patch edge
 constexpr int R = 19;         // search radius (39x39 window)
 constexpr int STRIDE = 3;     // reference stride
 constexpr int NMAX = 16;      // max group size
-constexpr int TILE = 16;      // refs per tile edge
-constexpr int FOOT = (TILE - 1) * STRIDE + 2 * R + K;  // 91: max tile footprint edge
-constexpr int FOOT2 = FOOT * FOOT;
+constexpr int TILE = 16;      // refs per tile row (tile width); tile height is 16 or 8
+// Footprint edge of a tile of t refs: every pixel a member patch of one of its
+// groups can touch (t refs at stride 3, +- 19 px search, + 8 px patch).
+constexpr int foot_edge(int t) { return (t - 1) * STRIDE + 2 * R + K; }
+constexpr int FOOT = foot_edge(TILE);  // 91
+constexpr int FOOT2 = FOOT * FOOT;     // largest footprint (16 x 16 refs)
 
 // Arguments of the group kernels (one workgroup per (tile, channel)).
 struct GroupParams {
@@ -29,13 +32,15 @@ struct GroupParams {
     const int* fy1;
     float* partial;
     int* err;  // set to 1 if an ordering wait timed out (never expected)
+    int dofs;               // den offset in a tile's num/den footprint (FOOT x foot height)
+    size_t tile_stride;     // floats per (channel, tile) partial block
     int tx0, ty0, ntx_sub;  // tile sub-rectangle processed by this launch
     const int* tiles;       // or an explicit list of tile indices (ty * ntx + tx)
     const int* gate;        // null, or run only if *gate != 0
 };
 
-// Adds one filtered group to a tile's LDS num/den footprint (acc[0, FOOT2) and
-// acc[FOOT2, 2 FOOT2)) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
+// Adds one filtered group to a tile's LDS num/den footprint (acc[0, dofs) and
+// acc[dofs, 2 dofs)) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
 // member m in v[m]; mp[m] (wave-uniform) is the member's footprint position
 // (y << 16 | x); lo = (i - fy0) * fw + (j - fx0). Group g is accumulated right
 // after group g-1: the wave waits for the LDS turn token to reach g, adds
@@ -53,7 +58,7 @@ struct GroupParams {
 __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, bool has, int n,
                                                   const uint32_t (&mp)[NMAX],
                                                   const float (&v)[NMAX], float wk, int lo,
-                                                  int fw) {
+                                                  int fw, int dofs) {
     int o[NMAX];
     float c[NMAX];
 #pragma unroll
@@ -103,12 +108,12 @@ __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, 
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         an[j] = acc[o[m0 + j]];
-                        ad[j] = acc[FOOT2 + o[m0 + j]];
+                        ad[j] = acc[dofs + o[m0 + j]];
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         acc[o[m0 + j]] = an[j] + c[m0 + j];
-                        acc[FOOT2 + o[m0 + j]] = ad[j] + wk;
+                        acc[dofs + o[m0 + j]] = ad[j] + wk;
                     }
                 } else {
 #pragma unroll
@@ -117,17 +122,17 @@ __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, 
                         if (q0 < n) {
                             if (plan & (1u << (4 + 2 * b + pr))) {
                                 const float a0 = acc[o[q0]], a1 = acc[o[q0 + 1]];
-                                const float d0 = acc[FOOT2 + o[q0]], d1 = acc[FOOT2 + o[q0 + 1]];
+                                const float d0 = acc[dofs + o[q0]], d1 = acc[dofs + o[q0 + 1]];
                                 acc[o[q0]] = a0 + c[q0];
                                 acc[o[q0 + 1]] = a1 + c[q0 + 1];
-                                acc[FOOT2 + o[q0]] = d0 + wk;
-                                acc[FOOT2 + o[q0 + 1]] = d1 + wk;
+                                acc[dofs + o[q0]] = d0 + wk;
+                                acc[dofs + o[q0 + 1]] = d1 + wk;
                             } else {
 #pragma unroll
                                 for (int j = 0; j < 2; ++j) {
                                     if (q0 + j < n) {
                                         acc[o[q0 + j]] = acc[o[q0 + j]] + c[q0 + j];
-                                        acc[FOOT2 + o[q0 + j]] = acc[FOOT2 + o[q0 + j]] + wk;
+                                        acc[dofs + o[q0 + j]] = acc[dofs + o[q0 + j]] + wk;
                                     }
                                 }
                             }
@@ -149,8 +154,10 @@ __device__ __forceinline__ int floor_pow2(int c) {
 // 8x8 DCT-II matrix C[u][i] and the 8x8 Kaiser window (row-major, FP32) to the
 // current device, once per device; group_mx_launch enqueues the step-1 or
 // step-2 group kernel with FP16 or BF16 MFMA transforms on the GroupParams
-// launch grid (tiles, channels). Both return 0 or a hipError_t.
+// launch grid (tiles, channels) for tiles of 16 x tile_y refs (tile_y = 8).
+// Both return 0 or a hipError_t.
 int group_mx_tables(int device, const float* dct64, const float* kaiser64);
-int group_mx_launch(int step, int precision, const GroupParams& p, dim3 grid, hipStream_t st);
+int group_mx_launch(int step, int precision, int tile_y, const GroupParams& p, dim3 grid,
+                    hipStream_t st);
 
 }  // namespace cpipe_internal

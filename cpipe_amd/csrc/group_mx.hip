@@ -124,6 +124,8 @@ __device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ 
     }
 }
 
+constexpr int MX_SCR = 4;  // members per LDS transposition chunk
+
 // Filters one group: 3D transform, shrinkage, inverse; leaves lane p = pixel p
 // of member m in v[m] (m < n) and returns the group's aggregation weight.
 template <int STEP, int PREC>
@@ -187,30 +189,34 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm,
         xt[mb] = x;
     }
     // lane (member r, quarter q) holds pixels 16 mb + 4 q + t: transpose through
-    // LDS, 8 members at a time
+    // LDS, MX_SCR members at a time
 #pragma unroll
-    for (int hf = 0; hf < 2; ++hf) {
-        if (hf * 8 < n) {
-            if ((r >> 3) == hf) {
+    for (int hf = 0; hf < NMAX / MX_SCR; ++hf) {
+        if (hf * MX_SCR < n) {
+            if (r / MX_SCR == hf) {
 #pragma unroll
                 for (int mb = 0; mb < 4; ++mb)
-                    *reinterpret_cast<f4*>(sb + (r & 7) * 64 + 16 * mb + 4 * q) = xt[mb];
+                    *reinterpret_cast<f4*>(sb + (r % MX_SCR) * 64 + 16 * mb + 4 * q) = xt[mb];
             }
             wave_lds_sync();
 #pragma unroll
-            for (int m = 0; m < 8; ++m)
-                if (hf * 8 + m < n) v[hf * 8 + m] = sb[m * 64 + lane];
+            for (int m = 0; m < MX_SCR; ++m)
+                if (hf * MX_SCR + m < n) v[hf * MX_SCR + m] = sb[m * 64 + lane];
             wave_lds_sync();
         }
     }
     return wgt;
 }
 
-template <int STEP, int PREC>
-__global__ __launch_bounds__(256) void k_group_mx(GroupParams p) {
+// One workgroup per (tile of 16 x TY refs, channel); 16 x 8 tiles keep the LDS
+// footprint small enough for three workgroups per CU.
+template <int STEP, int PREC, int TY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_group_mx(GroupParams p) {
     typedef Mx<PREC> M;
-    __shared__ float acc[2 * FOOT2];
-    __shared__ __align__(16) float scratch[4][8 * 64];
+    constexpr int DOFS = FOOT * foot_edge(TY);
+    __shared__ float acc[2 * DOFS];
+    __shared__ __align__(16) float scratch[4][MX_SCR * 64];
     __shared__ int turn;  // number of groups of this tile already accumulated
     if (p.gate != nullptr && *p.gate == 0) return;
     const int tile = p.tiles ? p.tiles[blockIdx.x]
@@ -222,7 +228,7 @@ __global__ __launch_bounds__(256) void k_group_mx(GroupParams p) {
     bool timed_out = false;
     const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
     const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
-    for (int idx = threadIdx.x; idx < 2 * FOOT2; idx += blockDim.x) acc[idx] = 0.0f;
+    for (int idx = threadIdx.x; idx < 2 * DOFS; idx += blockDim.x) acc[idx] = 0.0f;
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -248,7 +254,7 @@ __global__ __launch_bounds__(256) void k_group_mx(GroupParams p) {
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
     const int w = p.w;
-    const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TILE, p.ny - ty * TILE);
+    const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
 
     for (int rnd = 0; rnd < (G + 3) / 4; ++rnd) {
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(256) void k_group_mx(GroupParams p) {
         float v[NMAX];
         uint32_t mp[NMAX];
         if (has) {
-            const int bx = tx * TILE + g % nbx, by = ty * TILE + g / nbx;
+            const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
             const size_t ref = (size_t)by * p.nx + bx;
             n = floor_pow2(p.cnt[ref]);
             const uint32_t pm = p.pos[ref * NMAX + r];  // this lane's member (rows of X)
@@ -268,14 +274,14 @@ __global__ __launch_bounds__(256) void k_group_mx(GroupParams p) {
             wgt = filter_mx<STEP, PREC>(v, n, pm, r, q, lane, src, bsrc, w, sb, kf, ki, thr, s2);
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
-        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw, DOFS);
     }
     if (timed_out && lane == 0) atomicOr(p.err, 1);
     __syncthreads();
-    float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * (2 * FOOT2);
+    float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * p.tile_stride;
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
         dst[idx] = acc[idx];
-        dst[FOOT2 + idx] = acc[FOOT2 + idx];
+        dst[p.dofs + idx] = acc[DOFS + idx];
     }
 }
 
@@ -308,18 +314,21 @@ int group_mx_tables(int device, const float* dct64, const float* kaiser64) {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_mx_haar), haar, sizeof(haar));
 }
 
-int group_mx_launch(int step, int precision, const GroupParams& p, dim3 grid, hipStream_t st) {
+int group_mx_launch(int step, int precision, int tile_y, const GroupParams& p, dim3 grid,
+                    hipStream_t st) {
+    if (tile_y != TILE / 2 || p.dofs != FOOT * foot_edge(TILE / 2)) return hipErrorInvalidValue;
+    constexpr int TY = TILE / 2;
     const bool bf = precision == CPIPE_BM3D_PRECISION_BF16;
     if (step == 1) {
         if (bf)
-            k_group_mx<1, CPIPE_BM3D_PRECISION_BF16><<<grid, 256, 0, st>>>(p);
+            k_group_mx<1, CPIPE_BM3D_PRECISION_BF16, TY><<<grid, 256, 0, st>>>(p);
         else
-            k_group_mx<1, CPIPE_BM3D_PRECISION_FP16><<<grid, 256, 0, st>>>(p);
+            k_group_mx<1, CPIPE_BM3D_PRECISION_FP16, TY><<<grid, 256, 0, st>>>(p);
     } else {
         if (bf)
-            k_group_mx<2, CPIPE_BM3D_PRECISION_BF16><<<grid, 256, 0, st>>>(p);
+            k_group_mx<2, CPIPE_BM3D_PRECISION_BF16, TY><<<grid, 256, 0, st>>>(p);
         else
-            k_group_mx<2, CPIPE_BM3D_PRECISION_FP16><<<grid, 256, 0, st>>>(p);
+            k_group_mx<2, CPIPE_BM3D_PRECISION_FP16, TY><<<grid, 256, 0, st>>>(p);
     }
     return hipGetLastError();
 }
