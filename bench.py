@@ -2,9 +2,15 @@
 
 Workload (default): 4080x3072 RGB float32 linear planes in [0, 1] from the
 seeded shape+texture generator, Poisson-Gaussian noise (gain 0.01, read 0.002),
-node sigma from the noise-profile rule (0.05004), both BM3D steps, plain FP32.
-A step is one full BM3D pass over the frame with the input already resident in
-HBM. Prints one JSON line.
+node sigma from the noise-profile rule (0.05004), both BM3D steps. A step is
+one full BM3D pass over the frame with the input already resident in HBM.
+
+The headline `value` is the plain exact-FP32 form. The same frame is also run
+in the mixed-precision forms (FP16 / BF16 MFMA group transforms) and with the
+learned pilot replacing step 1; each form reports its throughput, its PSNR
+against the clean frame and against the FP32 output, and, on a 512x384 crop
+of the same noisy frame, its PSNR difference to the FP64 CPU oracle.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -23,7 +29,9 @@ from cpipe_amd import engine, synth  # noqa: E402
 
 K, R, STRIDE = 8, 19, 3
 PEAK_FP32_TFLOPS = 157.3
+PEAK_F16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "group_wiener_fp32_rgb12mp_traffic.json")
 
 
 def ref_positions(n: int) -> np.ndarray:
@@ -64,18 +72,38 @@ def cpu_baseline(noisy: np.ndarray, sigma: float, sample_w: int, sample_h: int):
                       f"CPU FP32 BM3D (same algorithm and operation order), {dt:.2f} s"}
 
 
+def time_form(torch, plan, src, dst, sigma, steps, warmup, stream):
+    plan.set_timing(False)
+    for _ in range(warmup):
+        plan.run(src, dst, sigma, stream)
+    torch.cuda.synchronize()
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run(src, dst, sigma, stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    plan.check()
+    stage_ms, runs = plan.timing()
+    plan.set_timing(False)
+    return wall * 1e3 / steps, {k: v / runs for k, v in stage_ms.items() if v > 0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rgb12mp")
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--forms", default="fp16,bf16,learned_fp16,learned_fp32",
+                    help="extra forms besides the plain FP32 headline ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--cpu-sample", default="2048x1536")
+    ap.add_argument("--fp64-crop", default="512x384")
     ap.add_argument("--pilot", default=None,
-                    help="learned-pilot weight blob (default: the packaged weights if present)")
-    ap.add_argument("--no-pilot", action="store_true", help="skip the learned-pilot form")
+                    help="learned-pilot weight blob (default: the packaged weights)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -88,37 +116,24 @@ def main():
     c, h, w = noisy.shape
     print(f"# workload {args.workload}: {w}x{h}x{c} sigma={sigma:.5f}", file=sys.stderr, flush=True)
     src = torch.from_numpy(noisy).to(dev)
-    dst = torch.empty_like(src)
-    plan = engine.Plan(w, h, c, args.precision, 0)
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        plan.run(src, dst, sigma, stream)
-    torch.cuda.synchronize()
-    print("# warmup done", file=sys.stderr, flush=True)
-
-    plan.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.run(src, dst, sigma, stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    plan.check()
-    stage_ms, runs = plan.timing()
-    ms_per_step = wall * 1e3 / args.steps
-    mpx = w * h / (ms_per_step / 1e3) / 1e6
-    per_stage = {k: v / runs for k, v in stage_ms.items()}
-    print(f"# per-stage ms: {json.dumps({k: round(v, 3) for k, v in per_stage.items()})}",
-          file=sys.stderr, flush=True)
-
-    out = dst.cpu().numpy()
-    psnr_out = synth.psnr(out, clean)
     psnr_in = synth.psnr(noisy, clean)
 
+    # ---- headline: plain exact FP32
+    plan = engine.Plan(w, h, c, "fp32", 0)
+    dst = torch.empty_like(src)
+    ms_per_step, per_stage = time_form(torch, plan, src, dst, sigma, args.steps, args.warmup, stream)
+    mpx = w * h / (ms_per_step / 1e3) / 1e6
+    print(f"# fp32 {ms_per_step:.3f} ms; per-stage ms: "
+          f"{json.dumps({k: round(v, 3) for k, v in per_stage.items()})}", file=sys.stderr, flush=True)
+    out32 = dst.cpu().numpy()
+    psnr_out = synth.psnr(out32, clean)
     _, cnt1 = plan.matches(1)
     _, cnt2 = plan.matches(2)
     cnt1 = cnt1.cpu().numpy()
     cnt2 = cnt2.cpu().numpy()
+    plan.close()
+
     cand = bm_candidates(w, h)
     bm_flop = 192.0 * cand
     g1_flop = group_flops(cnt1, c, 1)
@@ -134,43 +149,82 @@ def main():
     flop, ms = kernels[dominant]
     achieved = flop / (ms / 1e3) / 1e12
     total_flop = 2 * bm_flop + g1_flop + g2_flop
+    traffic = None
+    if dominant == "group_wiener" and os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "mfma", "kernel": dominant, "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                 "flop_per_launch": flop, "launch_ms": round(ms, 4),
                 "end_to_end_tflops": round(total_flop / (ms_per_step / 1e3) / 1e12, 3)}
 
-    learned = None
-    from cpipe_amd import pilot as pilot_mod
-    blob = args.pilot or (pilot_mod.DEFAULT_WEIGHTS if os.path.exists(pilot_mod.DEFAULT_WEIGHTS) else None)
-    if blob and not args.no_pilot and c == 3:
+    # ---- other forms on the same frame
+    forms = {}
+    names = [f for f in args.forms.split(",") if f]
+    net = None
+    if any(f.startswith("learned") for f in names) and c == 3:
+        from cpipe_amd import pilot as pilot_mod
+        blob = args.pilot or pilot_mod.DEFAULT_WEIGHTS
         net = pilot_mod.Pilot(seed=None, weights=blob)
-        plan.set_pilot(net)
-        dst2 = torch.empty_like(src)
-        plan.set_timing(False)
-        for _ in range(args.warmup):
-            plan.run(src, dst2, sigma, stream)
-        torch.cuda.synchronize()
-        plan.set_timing(True)
+    for name in names:
+        learned = name.startswith("learned")
+        if learned and net is None:
+            continue
+        prec = name.split("_")[-1]
+        fplan = engine.Plan(w, h, c, prec, 0)
+        if learned:
+            fplan.set_pilot(net)
+        fdst = torch.empty_like(src)
+        fms, fstage = time_form(torch, fplan, src, fdst, sigma, args.steps, args.warmup, stream)
+        fout = fdst.cpu().numpy()
+        entry = {"value": round(w * h / (fms / 1e3) / 1e6, 3), "unit": "Mpixel/s",
+                 "ms_per_step": round(fms, 3), "precision": prec,
+                 "psnr_out_db": round(synth.psnr(fout, clean), 3),
+                 "delta_psnr_db_vs_fp32": round(synth.psnr(fout, clean) - psnr_out, 4),
+                 "psnr_vs_fp32_db": round(synth.psnr(fout, out32), 2),
+                 "max_abs_vs_fp32": round(synth.max_abs(fout, out32), 5),
+                 "speedup_vs_fp32": round(ms_per_step / fms, 3),
+                 "stage_ms": {k: round(v, 4) for k, v in fstage.items()}}
+        if learned:
+            entry["spot_checks"] = fplan.pilot_report()
+        fplan.close()
+        forms[name] = entry
+        print(f"# {name}: {json.dumps(entry)}", file=sys.stderr, flush=True)
+
+    # ---- FP64 oracle on a crop: delta PSNR of every form
+    fp64 = None
+    if not args.no_fp64:
+        from oracle import oracle
+        cw, ch = (int(v) for v in args.fp64_crop.split("x"))
+        cw, ch = min(cw, w), min(ch, h)
+        cn = np.ascontiguousarray(noisy[:, :ch, :cw])
+        cc = clean[:, :ch, :cw]
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            plan.run(src, dst2, sigma, stream)
-        torch.cuda.synchronize()
-        wall2 = time.perf_counter() - t0
-        plan.check()
-        st2, runs2 = plan.timing()
-        rep = plan.pilot_report()
-        out2 = dst2.cpu().numpy()
-        ms2 = wall2 * 1e3 / args.steps
-        psnr2 = synth.psnr(out2, clean)
-        learned = {"value": round(w * h / (ms2 / 1e3) / 1e6, 3), "unit": "Mpixel/s",
-                   "ms_per_step": round(ms2, 3), "psnr_out_db": round(psnr2, 3),
-                   "delta_psnr_db_vs_plain": round(psnr2 - psnr_out, 4),
-                   "speedup_vs_fp32": round(ms_per_step / ms2, 3),
-                   "stage_ms": {k: round(v / runs2, 4) for k, v in st2.items()},
-                   "spot_checks": rep, "weights": os.path.basename(blob)}
-        plan.set_pilot(None)
-        print(f"# learned pilot: {json.dumps(learned)}", file=sys.stderr, flush=True)
+        o64 = oracle.bm3d_f64(cn, sigma)
+        t64 = time.perf_counter() - t0
+        p64 = synth.psnr(o64, cc)
+        csrc = torch.from_numpy(cn).to(dev)
+        fp64 = {"crop": f"{cw}x{ch}x{c} top-left of the same noisy frame", "psnr_fp64_db": round(p64, 4),
+                "oracle_seconds": round(t64, 2), "forms": {}}
+        for name in ["fp32"] + names:
+            learned = name.startswith("learned")
+            if learned and net is None:
+                continue
+            prec = name.split("_")[-1]
+            cplan = engine.Plan(cw, ch, c, prec, 0)
+            if learned:
+                cplan.set_pilot(net)
+            cdst = torch.empty_like(csrc)
+            cplan.run(csrc, cdst, sigma)
+            torch.cuda.synchronize()
+            cplan.check()
+            co = cdst.cpu().numpy()
+            cplan.close()
+            fp64["forms"][name] = {"delta_psnr_db": round(synth.psnr(co, cc) - p64, 4),
+                                   "psnr_vs_fp64_db": round(synth.psnr(co, o64), 2),
+                                   "max_abs_vs_fp64": round(synth.max_abs(co, o64), 5)}
+        print(f"# fp64 crop: {json.dumps(fp64)}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -185,17 +239,17 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload} {w}x{h}x{c} float32 planes, Poisson-Gaussian "
                                f"gain 0.01 read 0.002, sigma {sigma:.5f}, both steps",
-                   "precision": args.precision, "frames": 1},
+                   "precision": "fp32", "frames": 1},
         "quality": {"psnr_noisy_db": round(psnr_in, 3), "psnr_out_db": round(psnr_out, 3)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "learned_pilot": learned,
+        "forms": forms,
+        "fp64_crop": fp64,
     }
     if cpu:
         line["gpu_over_cpu"] = round(mpx / cpu["value"], 1)
     print(json.dumps(line), flush=True)
-    plan.close()
 
 
 if __name__ == "__main__":

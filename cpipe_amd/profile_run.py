@@ -1,0 +1,46 @@
+"""One BM3D pass for profilers (rocprofv3 kernel trace / PMC counters).
+
+Runs the plan on the named workload `--runs` times, then a 256 MiB
+device-to-device copy whose counter readings calibrate FETCH_SIZE and
+WRITE_SIZE (the copy reads and writes exactly 268435456 bytes).
+
+Usage: python -m cpipe_amd.profile_run --workload rgb12mp --precision fp32 [--pilot]
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="rgb12mp")
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--runs", type=int, default=1)
+    ap.add_argument("--pilot", action="store_true", help="learned pilot with the packaged weights")
+    args = ap.parse_args(argv)
+    import torch
+    from . import engine, synth
+    clean, noisy, sigma = synth.workload(args.workload, seed=0)
+    c, h, w = noisy.shape
+    src = torch.from_numpy(noisy).cuda()
+    dst = torch.empty_like(src)
+    plan = engine.Plan(w, h, c, args.precision, 0)
+    if args.pilot:
+        from . import pilot
+        plan.set_pilot(pilot.Pilot(seed=None, weights=pilot.DEFAULT_WEIGHTS))
+    for i in range(args.runs):
+        plan.run(src, dst, sigma)
+        torch.cuda.synchronize()
+        print(f"run {i} done", file=sys.stderr, flush=True)
+    plan.check()
+    a = torch.full((64 * 1024 * 1024,), 1.0, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("calibration copy done (268435456 bytes read and written)", file=sys.stderr, flush=True)
+
+
+if __name__ == "__main__":
+    main()
