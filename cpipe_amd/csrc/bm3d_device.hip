@@ -25,8 +25,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "bm3d_shared.h"
@@ -99,6 +101,37 @@ __global__ void k_opp_fwd(const float* __restrict__ r, const float* __restrict__
     z0[o] = ((rv + gv) + bv) * c13;
     z1[o] = (rv - bv) * 0.5f;
     z2[o] = ((rv + bv) - (gv + gv)) * 0.25f;
+}
+
+// Input colour transform for every frame of a plan: one block row per pixel
+// row of a frame rectangle (blockIdx.z = rectangle, blockIdx.y = row in it).
+__global__ void k_colour_in(const FrameTab* __restrict__ tab, const FrameRect* __restrict__ rects) {
+    const FrameRect rc = rects[blockIdx.z];
+    const FrameTab& F = tab[rc.f];
+    const int x = rc.x0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = rc.y0 + blockIdx.y;
+    if (x >= rc.x1 || y >= rc.y1) return;
+    const size_t i = (size_t)y * F.in_pitch + x;
+    const size_t o = (size_t)y * F.w + x;
+    if (F.c == 3) {
+        const float rv = F.in[0][i], gv = F.in[1][i], bv = F.in[2][i];
+        const float c13 = (float)(1.0 / 3.0);
+        F.z[0][o] = ((rv + gv) + bv) * c13;
+        F.z[1][o] = (rv - bv) * 0.5f;
+        F.z[2][o] = ((rv + bv) - (gv + gv)) * 0.25f;
+    } else {
+        F.z[0][o] = F.in[0][i];
+    }
+}
+
+// Frames smaller than one patch: output = input (blockIdx.z = frame rectangle).
+__global__ void k_copy_through(const FrameTab* __restrict__ tab, const FrameRect* __restrict__ rects) {
+    const FrameRect rc = rects[blockIdx.z];
+    const FrameTab& F = tab[rc.f];
+    const int x = rc.x0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = rc.y0 + blockIdx.y;
+    if (x >= rc.x1 || y >= rc.y1) return;
+    for (int c = 0; c < F.c; ++c) F.out[c][(size_t)y * F.out_pitch + x] = F.in[c][(size_t)y * F.in_pitch + x];
 }
 
 __global__ void k_copy_plane(const float* __restrict__ src, size_t pitch, float* __restrict__ dst,
@@ -225,18 +258,20 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
     return __builtin_bit_cast(float, r);
 }
 
-__global__ __launch_bounds__(256) void k_block_match(const float* __restrict__ plane, int w, int h,
-                                                     int nx, int ny, float tau,
-                                                     uint32_t* __restrict__ pos,
-                                                     uint8_t* __restrict__ cnt, int btx0, int bty0,
-                                                     const int2* __restrict__ tiles,
-                                                     const int* __restrict__ gate) {
-    if (gate != nullptr && *gate == 0) return;  // step skipped (decided on the device)
+// One workgroup per item (frame, tile column, tile row) of `items`.
+__global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict__ tab,
+                                                     const int4* __restrict__ items, int step,
+                                                     float tau, int use_gate) {
     __shared__ float tile[BM_LW * BM_LH];
     __shared__ float2 pend[PEND * 256];  // [slot][thread]: (ssd, packed position)
-    // tile rectangle (btx0, bty0) + block index, or an explicit tile list
-    const int2 bt = tiles ? tiles[blockIdx.x] : make_int2(blockIdx.x + btx0, blockIdx.y + bty0);
-    const int bx0 = bt.x * BM_TX, by0 = bt.y * BM_TY;
+    const int4 it = items[blockIdx.x];
+    const FrameTab& F = tab[it.x];
+    if (use_gate && *F.gate == 0) return;  // step skipped (decided on the device)
+    const float* __restrict__ plane = step == 1 ? F.z[0] : F.bz[0];
+    const int w = F.w, h = F.h, nx = F.nx, ny = F.ny;
+    uint32_t* __restrict__ pos = F.pos[step - 1];
+    uint8_t* __restrict__ cnt = F.cnt[step - 1];
+    const int bx0 = it.y * BM_TX, by0 = it.z * BM_TY;
     const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
     const int Y0 = ref_pos(by0, h) - R;
     for (int idx = threadIdx.x; idx < BM_COLS * BM_LH; idx += blockDim.x) {
@@ -560,14 +595,15 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
 }
 
 template <int STEP>
-__global__ __launch_bounds__(256) void k_group(GroupParams p) {
+__global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
     __shared__ float acc[2 * FOOT2];
     __shared__ __align__(16) float scratch[4][SCR * 64];
     __shared__ int turn;  // number of groups of this tile already accumulated
-    if (p.gate != nullptr && *p.gate == 0) return;
-    const int tile = p.tiles ? p.tiles[blockIdx.x]
-                             : (p.ty0 + (int)blockIdx.x / p.ntx_sub) * p.ntx + p.tx0 +
-                                   (int)blockIdx.x % p.ntx_sub;
+    const int2 item = L.items[blockIdx.x];
+    const FrameTab& F = L.tab[item.x];
+    if (L.use_gate && *F.gate == 0) return;
+    const GroupView p = group_view(F, STEP);
+    const int tile = item.y;
     const int tx = tile % p.ntx, ty = tile / p.ntx;
     const int c = blockIdx.y;
     if (threadIdx.x == 0) turn = 0;
@@ -590,9 +626,9 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
     }
     const float kw = c_kaiser2[lane];
     const float hs = (float)M_SQRT1_2;
-    const float* __restrict__ src = p.noisy[c];
-    const float* __restrict__ bsrc = p.basic[c];
-    const float sc = p.sig[c];
+    const float* __restrict__ src = F.z[c];
+    const float* __restrict__ bsrc = F.bz[c];
+    const float sc = F.sig[c];
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
     const int w = p.w;
@@ -630,7 +666,7 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
         const int lo = (li - fy0) * fw + (lj - fx0);
         timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw, FOOT2);
     }
-    if (timed_out && lane == 0) atomicOr(p.err, 1);
+    if (timed_out && lane == 0) atomicOr(L.err, 1);
     __syncthreads();
     float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * p.tile_stride;
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
@@ -646,57 +682,52 @@ __global__ __launch_bounds__(256) void k_group(GroupParams p) {
 // inverse colour transform and writes the caller's output planes.
 // ---------------------------------------------------------------------------
 
-struct MergeParams {
-    const float* partial;
-    int w, h, ntx, nty, channels;
-    int dofs;            // den offset in a partial block
-    size_t tile_stride;  // floats per (channel, tile) partial block
-    const int* txlo;
-    const int* txhi;
-    const int* tylo;
-    const int* tyhi;
-    const int* fx0;
-    const int* fx1;
-    const int* fy0;
-    float* out[3];
-    size_t out_pitch;
-    int inverse_colour;
-    int x0, y0, x1, y1;      // pixel rectangle of this launch
-    const int4* rects;       // or a list of rectangles (x0, y0, x1, y1), one per blockIdx.z
-    const int* gate;         // null, or run only if *gate != 0
-};
+// Destinations of a merge: the step-1 basic estimate (opponent planes), the
+// plain step-1 planes on spot-checked rectangles, or the caller's output
+// (inverse colour transform for 3 channels).
+enum MergeDest { TO_BASIC = 0, TO_CHECK = 1, TO_OUTPUT = 2 };
 
-__global__ void k_merge(MergeParams p) {
-    if (p.gate != nullptr && *p.gate == 0) return;
-    const int4 rc = p.rects ? p.rects[blockIdx.z] : make_int4(p.x0, p.y0, p.x1, p.y1);
-    const int x = rc.x + blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = rc.y + blockIdx.y;
-    if (x >= rc.z || y >= rc.w) return;
-    const int txl = p.txlo[x], txh = p.txhi[x], tyl = p.tylo[y], tyh = p.tyhi[y];
+// One block row per pixel row of a frame rectangle (blockIdx.z = rectangle).
+__global__ void k_merge(const FrameTab* __restrict__ tab, const FrameRect* __restrict__ rects,
+                        int dest, int use_gate) {
+    const FrameRect rc = rects[blockIdx.z];
+    const FrameTab& F = tab[rc.f];
+    if (use_gate && *F.gate == 0) return;
+    const int x = rc.x0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = rc.y0 + blockIdx.y;
+    if (x >= rc.x1 || y >= rc.y1) return;
+    const int txl = F.txlo[x], txh = F.txhi[x], tyl = F.tylo[y], tyh = F.tyhi[y];
+    const size_t ntiles = (size_t)F.ntx * F.nty;
     float o[3] = {0.0f, 0.0f, 0.0f};
-    for (int c = 0; c < p.channels; ++c) {
+    for (int c = 0; c < F.c; ++c) {
         float num = 0.0f, den = 0.0f;
         for (int ty = tyl; ty <= tyh; ++ty) {
-            const int ry = y - p.fy0[ty];
+            const int ry = y - F.fy0[ty];
             for (int tx = txl; tx <= txh; ++tx) {
-                const int fw = p.fx1[tx] - p.fx0[tx];
+                const int fw = F.fx1[tx] - F.fx0[tx];
                 const float* pn =
-                    p.partial + ((size_t)c * p.ntx * p.nty + (size_t)ty * p.ntx + tx) * p.tile_stride;
-                const int off = ry * fw + (x - p.fx0[tx]);
+                    F.partial + ((size_t)c * ntiles + (size_t)ty * F.ntx + tx) * F.tile_stride;
+                const int off = ry * fw + (x - F.fx0[tx]);
                 num = num + pn[off];
-                den = den + pn[p.dofs + off];
+                den = den + pn[F.dofs + off];
             }
         }
         o[c] = num / den;
     }
-    const size_t oi = (size_t)y * p.out_pitch + x;
-    if (p.inverse_colour) {
-        const float c23 = (float)(2.0 / 3.0), c43 = (float)(4.0 / 3.0);
-        p.out[0][oi] = (o[0] + o[1]) + o[2] * c23;
-        p.out[1][oi] = o[0] - o[2] * c43;
-        p.out[2][oi] = (o[0] - o[1]) + o[2] * c23;
+    if (dest == TO_OUTPUT) {
+        const size_t oi = (size_t)y * F.out_pitch + x;
+        if (F.c == 3) {
+            const float c23 = (float)(2.0 / 3.0), c43 = (float)(4.0 / 3.0);
+            F.out[0][oi] = (o[0] + o[1]) + o[2] * c23;
+            F.out[1][oi] = o[0] - o[2] * c43;
+            F.out[2][oi] = (o[0] - o[1]) + o[2] * c23;
+        } else {
+            F.out[0][oi] = o[0];
+        }
     } else {
-        for (int c = 0; c < p.channels; ++c) p.out[c][oi] = o[c];
+        float* const* d = dest == TO_BASIC ? F.bz : F.bzp;
+        const size_t oi = (size_t)y * F.w + x;
+        for (int c = 0; c < F.c; ++c) d[c][oi] = o[c];
     }
 }
 
@@ -709,28 +740,21 @@ __global__ void k_merge(MergeParams p) {
 // step 1.
 // ---------------------------------------------------------------------------
 
-struct CheckRect {
-    int x0, y0, x1, y1;
-};
-static_assert(sizeof(CheckRect) == sizeof(int4), "check rectangles double as merge rectangles");
-
-__global__ __launch_bounds__(256) void k_pilot_check(const float* z0, const float* z1,
-                                                     const float* z2, float* b0, float* b1,
-                                                     float* b2, const float* p0, const float* p1,
-                                                     const float* p2, int w,
-                                                     const CheckRect* __restrict__ rects,
+// One block per checked rectangle (report entry blockIdx.x).
+__global__ __launch_bounds__(256) void k_pilot_check(const FrameTab* __restrict__ tab,
+                                                     const FrameRect* __restrict__ rects,
                                                      float thr_db, float* __restrict__ report) {
     __shared__ float ra[256], rb[256];
-    const CheckRect r = rects[blockIdx.x];
+    const FrameRect r = rects[blockIdx.x];
+    const FrameTab& F = tab[r.f];
+    const int w = F.w;
     const int rw = r.x1 - r.x0, n = rw * (r.y1 - r.y0);
     float a = 0.0f, b = 0.0f;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const size_t o = (size_t)(r.y0 + i / rw) * w + r.x0 + i % rw;
-        const float zs[3] = {z0[o], z1[o], z2[o]};
-        const float ls[3] = {b0[o], b1[o], b2[o]};
-        const float ps[3] = {p0[o], p1[o], p2[o]};
         for (int c = 0; c < 3; ++c) {
-            const float dl = zs[c] - ls[c], dp = zs[c] - ps[c];
+            const float zs = F.z[c][o];
+            const float dl = zs - F.bz[c][o], dp = zs - F.bzp[c][o];
             a = __builtin_fmaf(dl, dl, a);
             b = __builtin_fmaf(dp, dp, b);
         }
@@ -752,9 +776,7 @@ __global__ __launch_bounds__(256) void k_pilot_check(const float* z0, const floa
     if (fall) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const size_t o = (size_t)(r.y0 + i / rw) * w + r.x0 + i % rw;
-            b0[o] = p0[o];
-            b1[o] = p1[o];
-            b2[o] = p2[o];
+            for (int c = 0; c < 3; ++c) F.bz[c][o] = F.bzp[c][o];
         }
     }
     if (threadIdx.x == 0) {
@@ -763,12 +785,16 @@ __global__ __launch_bounds__(256) void k_pilot_check(const float* z0, const floa
     }
 }
 
-__global__ void k_pilot_decide(const float* __restrict__ report, int n, float frac,
-                               int* __restrict__ gate) {
-    if (threadIdx.x != 0) return;
+// Per frame: the gate runs the plain step 1 on the whole frame when more than
+// `frac` of its checked rectangles fell back.
+__global__ void k_pilot_decide(const FrameTab* __restrict__ tab, int n_frames,
+                               const float* __restrict__ report, float frac) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_frames) return;
+    const FrameTab& F = tab[f];
     int falls = 0;
-    for (int i = 0; i < n; ++i) falls += report[2 * i + 1] > 0.5f ? 1 : 0;
-    *gate = ((float)falls > frac * (float)n) ? 1 : 0;
+    for (int i = F.chk0; i < F.chk0 + F.nchk; ++i) falls += report[2 * i + 1] > 0.5f ? 1 : 0;
+    *F.gate = ((float)falls > frac * (float)F.nchk) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -828,46 +854,77 @@ private:
 
 }  // namespace
 
+// A plan owns one or more frames (a batch). Every stage is one launch over a
+// work list that spans all frames: block-matching tiles (frame, tx, ty),
+// aggregation tiles (frame, tile) and pixel-row bands (frame rectangles); the
+// kernels read each frame's geometry, buffers, sigma and I/O planes from the
+// device frame table. A single-image plan is a batch of one frame.
 struct cpipe_bm3d_plan_s {
-    int w = 0, h = 0, c = 0, precision = 0, device = 0;
-    int nx = 0, ny = 0, ntx = 0, nty = 0;
-    int tile_y = TILE;         // refs per tile column (16 exact FP32, 8 mixed precision)
-    int dofs = FOOT2;          // den offset in a partial block
-    size_t tile_stride = 2 * (size_t)FOOT2;  // floats per (channel, tile) partial block
-    size_t npx = 0;
-    float* z = nullptr;        // channels planes
-    float* bz = nullptr;       // channels planes
-    float* rgb = nullptr;      // 6 planes for the RGBA16F path (in 3 + out 3)
-    uint32_t* pos[2] = {nullptr, nullptr};
-    uint8_t* cnt[2] = {nullptr, nullptr};
-    float* partial = nullptr;
-    int* err = nullptr;  // device flag: an ordering wait timed out
-    int* geo = nullptr;  // txlo[w] txhi[w] tylo[h] tyhi[h] fx0[ntx] fx1[ntx] fy0[nty] fy1[nty]
-    bool timing = false;
-    hipEvent_t ev[2 * CPIPE_BM3D_T_COUNT] = {};  // start/end per stage
-    bool stage_ran[CPIPE_BM3D_T_COUNT] = {};
-    float ms[CPIPE_BM3D_T_COUNT] = {};
-    int runs = 0;
-    bool ran = false;
-    std::vector<int> geo_host;
-    // learned pilot (replaces step 1) and its spot checks
-    cpipe_bm3d_pilot* pilot = nullptr;
-    float pilot_thr_db = 0.5f;
-    float pilot_full_frac = 0.5f;
+    struct Frame {
+        int w = 0, h = 0, nx = 0, ny = 0, ntx = 0, nty = 0;
+        size_t npx = 0;
+        bool small = false;         // smaller than one patch: copied through
+        std::vector<int> geo;       // txlo[w] txhi[w] tylo[h] tyhi[h] fx0[ntx] fx1[ntx] fy0[nty] fy1[nty]
+        float* z = nullptr;         // c planes each
+        float* bz = nullptr;
+        float* bzp = nullptr;       // learned pilot: plain step-1 planes on checked rectangles
+        uint32_t* pos[2] = {nullptr, nullptr};
+        uint8_t* cnt[2] = {nullptr, nullptr};
+        float* partial = nullptr;
+        int* geo_dev = nullptr;
+    };
     struct Check {
+        int f;
         int btx0, bty0, btx1, bty1;  // block-matching tiles
         int tx0, ty0, tx1, ty1;      // aggregation tiles
         int x0, y0, x1, y1;          // pixels
     };
+    struct Lists {  // one stage's work over a set of frames / rectangles
+        int4* bm = nullptr;
+        int2* grp = nullptr;
+        FrameRect* rects = nullptr;
+        int n_bm = 0, n_grp = 0, n_rects = 0;
+        int rect_w = 0, rect_h = 0;  // largest rectangle
+    };
+
+    int c = 0, precision = 0, device = 0;
+    int tile_y = TILE;               // refs per tile column (16 exact FP32, 8 or 16 mixed precision)
+    int dofs = FOOT2;                // den offset in a partial block
+    size_t tile_stride = 2 * (size_t)FOOT2;
+    int order = 0;                   // work-list order: 0 frame raster, 1 XCD-contiguous chunks
+    int band_rows = 32;              // rows per colour / merge rectangle
+    std::vector<Frame> fr;
+    std::vector<FrameTab> tab;       // host image of the device frame table
+    FrameTab* tab_dev = nullptr;     // followed by one float noise level per frame
+    void* staging = nullptr;         // pinned host copy of the table, uploaded every run
+    size_t staging_bytes = 0;
+    hipEvent_t staged = nullptr;     // the last upload (or graph replay) has read `staging`
+    bool staged_pending = false;
+    int* gates = nullptr;            // per frame
+    int* err = nullptr;              // device flag: an ordering wait timed out
+    Lists full;                      // every frame, whole image
+    FrameRect* small_rects = nullptr;  // frames smaller than a patch (copied through)
+    int n_small = 0, small_w = 0, small_h = 0;
+    float* rgb = nullptr;            // 6 planes for the RGBA16F path (single-frame plans)
+    bool ran = false;
+    // timing (eager runs)
+    bool timing = false;
+    hipEvent_t ev[2 * CPIPE_BM3D_T_COUNT] = {};
+    bool stage_ran[CPIPE_BM3D_T_COUNT] = {};
+    float ms[CPIPE_BM3D_T_COUNT] = {};
+    int runs = 0;
+    // learned pilot (replaces step 1) and its spot checks
+    cpipe_bm3d_pilot* pilot = nullptr;
+    float pilot_thr_db = 0.5f;
+    float pilot_full_frac = 0.5f;
+    cpipe_bm3d_pilot_opts pilot_opts{};
     std::vector<Check> checks;
-    void* rects = nullptr;     // CheckRect[n] on the device
-    // the checked rectangles' work as lists, so each kernel runs once for all checks
-    int2* chk_bm = nullptr;    // distinct block-matching tiles
-    int* chk_tiles = nullptr;  // distinct aggregation tiles (ty * ntx + tx)
-    int n_chk_bm = 0, n_chk_tiles = 0, chk_w = 0, chk_h = 0;  // counts, largest rectangle
-    float* report = nullptr;   // [n][2]: delta dB, fell back
-    int* gate = nullptr;       // 1: run the plain step 1 on the whole frame
-    float* bzp = nullptr;      // plain step-1 planes on checked rectangles
+    Lists chk;                       // the checked rectangles' work, all checks in one launch per kernel
+    float* report = nullptr;         // [n][2]: delta dB, fell back
+    // one graph per plan: the whole batch as a single submission
+    bool use_graph = false;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;               // work lists or pilot changed since the graph was captured
 };
 
 namespace {
@@ -880,225 +937,602 @@ void tile_box_host(int t, int tile, int nref, int n, int* f0, int* f1) {
     *f1 = std::min(n, last + R + K);
 }
 
+void free_lists(cpipe_bm3d_plan_s::Lists& l) {
+    (void)hipFree(l.bm);
+    (void)hipFree(l.grp);
+    (void)hipFree(l.rects);
+    l = cpipe_bm3d_plan_s::Lists{};
+}
+
+void drop_graph(cpipe_bm3d_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+    p->dirty = true;
+}
+
+void free_frame_buffers(cpipe_bm3d_plan_s::Frame& f) {
+    (void)hipFree(f.z);
+    (void)hipFree(f.bz);
+    (void)hipFree(f.bzp);
+    for (int s = 0; s < 2; ++s) {
+        (void)hipFree(f.pos[s]);
+        (void)hipFree(f.cnt[s]);
+    }
+    (void)hipFree(f.partial);
+    (void)hipFree(f.geo_dev);
+    f.z = f.bz = f.bzp = f.partial = nullptr;
+    f.pos[0] = f.pos[1] = nullptr;
+    f.cnt[0] = f.cnt[1] = nullptr;
+    f.geo_dev = nullptr;
+}
+
 void free_plan(cpipe_bm3d_plan* p) {
     if (!p) return;
     DeviceGuard g(p->device);
     (void)hipDeviceSynchronize();  // runs may still read the scratch being freed
-    (void)hipFree(p->z);
-    (void)hipFree(p->bz);
-    (void)hipFree(p->rgb);
-    for (int s = 0; s < 2; ++s) {
-        (void)hipFree(p->pos[s]);
-        (void)hipFree(p->cnt[s]);
-    }
-    (void)hipFree(p->partial);
+    drop_graph(p);
+    for (auto& f : p->fr) free_frame_buffers(f);
+    free_lists(p->full);
+    free_lists(p->chk);
+    (void)hipFree(p->small_rects);
+    (void)hipFree(p->tab_dev);
+    if (p->staging) (void)hipHostFree(p->staging);
+    if (p->staged) (void)hipEventDestroy(p->staged);
+    (void)hipFree(p->gates);
     (void)hipFree(p->err);
-    (void)hipFree(p->geo);
-    (void)hipFree(p->rects);
-    (void)hipFree(p->chk_bm);
-    (void)hipFree(p->chk_tiles);
+    (void)hipFree(p->rgb);
     (void)hipFree(p->report);
-    (void)hipFree(p->gate);
-    (void)hipFree(p->bzp);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
 }
 
-struct StepRange {
-    int btx0, bty0, btx1, bty1;  // block-matching tiles
-    int tx0, ty0, tx1, ty1;      // aggregation tiles
-    int x0, y0, x1, y1;          // merged pixels
-    // or explicit lists (when bm_list is set): tiles, and rectangles of at most
-    // x1 x y1 pixels
-    const int2* bm_list = nullptr;
-    const int* tile_list = nullptr;
-    const int4* rect_list = nullptr;
-    int n_bm = 0, n_tiles = 0, n_rects = 0;
-};
+int dev_alloc(void** ptr, size_t bytes) {
+    hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? OOM : INTERNAL, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return OK;
+}
 
-StepRange full_range(const cpipe_bm3d_plan* p) {
-    return {0, 0, (p->nx + BM_TX - 1) / BM_TX, (p->ny + BM_TY - 1) / BM_TY, 0, 0, p->ntx, p->nty,
-            0, 0, p->w, p->h};
+template <typename T>
+int upload(T** dst, const std::vector<T>& v) {
+    int rc = dev_alloc((void**)dst, v.size() * sizeof(T));
+    if (rc) return rc;
+    if (!v.empty()) HIP_CHECK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return OK;
+}
+
+// Work-list order. Workgroups are handed to the 8 XCDs round robin; order 1
+// permutes a list so that each XCD receives one contiguous run of it
+// (neighbouring tiles, whose footprints overlap, share that XCD's L2).
+template <typename T>
+std::vector<T> apply_order(const std::vector<T>& v, int order) {
+    if (order != 1 || v.size() < 16) return v;
+    const size_t n = v.size();
+    constexpr size_t X = 8;
+    size_t start[X], size[X], acc = 0;
+    for (size_t x = 0; x < X; ++x) {
+        size[x] = (n > x) ? (n - x + X - 1) / X : 0;
+        start[x] = acc;
+        acc += size[x];
+    }
+    std::vector<T> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = v[start[i % X] + i / X];
+    return out;
+}
+
+// Geometry, buffers and the device table entry of frame f for the plan's tile shape.
+int build_frame(cpipe_bm3d_plan* p, int f) {
+    auto& F = p->fr[f];
+    free_frame_buffers(F);
+    F.npx = (size_t)F.w * F.h;
+    F.small = F.w < K || F.h < K;
+    FrameTab& T = p->tab[f];
+    T = FrameTab{};
+    T.w = F.w;
+    T.h = F.h;
+    T.c = p->c;
+    T.gate = p->gates + f;
+    if (F.small) return OK;
+    F.nx = ref_count(F.w);
+    F.ny = ref_count(F.h);
+    F.ntx = (F.nx + TILE - 1) / TILE;
+    F.nty = (F.ny + p->tile_y - 1) / p->tile_y;
+    const size_t nref = (size_t)F.nx * F.ny;
+    const size_t ntiles = (size_t)F.ntx * F.nty;
+    int rc;
+    if ((rc = dev_alloc((void**)&F.z, sizeof(float) * F.npx * p->c)) ||
+        (rc = dev_alloc((void**)&F.bz, sizeof(float) * F.npx * p->c)) ||
+        (rc = dev_alloc((void**)&F.pos[0], sizeof(uint32_t) * nref * NMAX)) ||
+        (rc = dev_alloc((void**)&F.pos[1], sizeof(uint32_t) * nref * NMAX)) ||
+        (rc = dev_alloc((void**)&F.cnt[0], nref)) || (rc = dev_alloc((void**)&F.cnt[1], nref)) ||
+        (rc = dev_alloc((void**)&F.partial, sizeof(float) * p->tile_stride * ntiles * p->c)))
+        return rc;
+    const size_t ngeo = 2 * (size_t)F.w + 2 * (size_t)F.h + 2 * (size_t)F.ntx + 2 * (size_t)F.nty;
+    F.geo.assign(ngeo, 0);
+    int* txlo = F.geo.data();
+    int* txhi = txlo + F.w;
+    int* tylo = txhi + F.w;
+    int* tyhi = tylo + F.h;
+    int* fx0 = tyhi + F.h;
+    int* fx1 = fx0 + F.ntx;
+    int* fy0 = fx1 + F.ntx;
+    int* fy1 = fy0 + F.nty;
+    for (int t = 0; t < F.ntx; ++t) tile_box_host(t, TILE, F.nx, F.w, &fx0[t], &fx1[t]);
+    for (int t = 0; t < F.nty; ++t) tile_box_host(t, p->tile_y, F.ny, F.h, &fy0[t], &fy1[t]);
+    for (int x = 0; x < F.w; ++x) {
+        txlo[x] = -1;
+        txhi[x] = -2;
+    }
+    for (int t = 0; t < F.ntx; ++t)
+        for (int x = fx0[t]; x < fx1[t]; ++x) {
+            if (txlo[x] < 0) txlo[x] = t;
+            txhi[x] = t;
+        }
+    for (int y = 0; y < F.h; ++y) {
+        tylo[y] = -1;
+        tyhi[y] = -2;
+    }
+    for (int t = 0; t < F.nty; ++t)
+        for (int y = fy0[t]; y < fy1[t]; ++y) {
+            if (tylo[y] < 0) tylo[y] = t;
+            tyhi[y] = t;
+        }
+    for (int x = 0; x < F.w; ++x)
+        if (txlo[x] < 0) return fail(INTERNAL, "tile cover has a gap in x");
+    for (int y = 0; y < F.h; ++y)
+        if (tylo[y] < 0) return fail(INTERNAL, "tile cover has a gap in y");
+    if ((rc = upload(&F.geo_dev, F.geo))) return rc;
+    T.nx = F.nx;
+    T.ny = F.ny;
+    T.ntx = F.ntx;
+    T.nty = F.nty;
+    T.tile_y = p->tile_y;
+    T.dofs = p->dofs;
+    T.tile_stride = p->tile_stride;
+    for (int ch = 0; ch < 3; ++ch) {
+        const int cc = ch < p->c ? ch : 0;
+        T.z[ch] = F.z + cc * F.npx;
+        T.bz[ch] = F.bz + cc * F.npx;
+    }
+    T.pos[0] = F.pos[0];
+    T.pos[1] = F.pos[1];
+    T.cnt[0] = F.cnt[0];
+    T.cnt[1] = F.cnt[1];
+    T.partial = F.partial;
+    const int* g = F.geo_dev;
+    T.txlo = g;
+    T.txhi = g + F.w;
+    T.tylo = T.txhi + F.w;
+    T.tyhi = T.tylo + F.h;
+    T.fx0 = T.tyhi + F.h;
+    T.fx1 = T.fx0 + F.ntx;
+    T.fy0 = T.fx1 + F.ntx;
+    T.fy1 = T.fy0 + F.nty;
+    return OK;
+}
+
+// Whole-frame work lists of every frame, in the plan's order.
+int build_full_lists(cpipe_bm3d_plan* p) {
+    free_lists(p->full);
+    (void)hipFree(p->small_rects);
+    p->small_rects = nullptr;
+    std::vector<int4> bm;
+    std::vector<int2> grp;
+    std::vector<FrameRect> bands, small;
+    int maxw = 0, sw = 0, sh = 0;
+    for (int f = 0; f < (int)p->fr.size(); ++f) {
+        const auto& F = p->fr[f];
+        if (F.small) {
+            small.push_back({f, 0, 0, F.w, F.h, 0, 0, 0});
+            sw = std::max(sw, F.w);
+            sh = std::max(sh, F.h);
+            continue;
+        }
+        const int nbtx = (F.nx + BM_TX - 1) / BM_TX, nbty = (F.ny + BM_TY - 1) / BM_TY;
+        for (int ty = 0; ty < nbty; ++ty)
+            for (int tx = 0; tx < nbtx; ++tx) bm.push_back(make_int4(f, tx, ty, 0));
+        for (int t = 0; t < F.ntx * F.nty; ++t) grp.push_back(make_int2(f, t));
+        for (int y0 = 0; y0 < F.h; y0 += p->band_rows)
+            bands.push_back({f, 0, y0, F.w, std::min(F.h, y0 + p->band_rows), 0, 0, 0});
+        maxw = std::max(maxw, F.w);
+    }
+    int rc;
+    auto& L = p->full;
+    if ((rc = upload(&L.bm, apply_order(bm, p->order))) || (rc = upload(&L.grp, apply_order(grp, p->order))) ||
+        (rc = upload(&L.rects, bands)) || (rc = upload(&p->small_rects, small)))
+        return rc;
+    L.n_bm = (int)bm.size();
+    L.n_grp = (int)grp.size();
+    L.n_rects = (int)bands.size();
+    L.rect_w = maxw;
+    L.rect_h = p->band_rows;
+    p->n_small = (int)small.size();
+    p->small_w = sw;
+    p->small_h = sh;
+    p->dirty = true;
+    return OK;
+}
+
+// Tile shape: exact FP32 plans use 16 x 16-ref tiles (the CPU FP32
+// restatement's tiling); mixed-precision plans default to 16 x 8, whose
+// smaller LDS footprint fits three workgroups per CU (three ordered
+// aggregation chains instead of two).
+int set_tile_shape(cpipe_bm3d_plan* p, int tile_y) {
+    p->tile_y = tile_y;
+    p->dofs = FOOT * foot_edge(tile_y);
+    p->tile_stride = 2 * (size_t)p->dofs;
+    for (int f = 0; f < (int)p->fr.size(); ++f) {
+        const int rc = build_frame(p, f);
+        if (rc) return rc;
+    }
+    return build_full_lists(p);
+}
+
+int create_core(cpipe_bm3d_plan** out, const int* widths, const int* heights, int n, int channels,
+                int precision, int device) {
+    *out = nullptr;
+    if (n <= 0 || n > 4096) return fail(BAD_INDEX, "frame count out of range (1..4096)");
+    for (int f = 0; f < n; ++f)
+        if (widths[f] <= 0 || heights[f] <= 0 || widths[f] > 65535 || heights[f] > 65535)
+            return fail(BAD_INDEX, "width/height out of range (1..65535)");
+    if (channels != 1 && channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
+    if (precision != CPIPE_BM3D_PRECISION_FP32 && precision != CPIPE_BM3D_PRECISION_FP16 &&
+        precision != CPIPE_BM3D_PRECISION_BF16)
+        return fail(BAD_INDEX, "unknown precision");
+    const int ndev = cpipe_bm3d_device_count();
+    if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
+    if (device < 0 || device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
+    DeviceGuard guard(device);
+    int rc = ensure_tables(device);
+    if (rc) return rc;
+    auto* p = new cpipe_bm3d_plan_s();
+    p->c = channels;
+    p->precision = precision;
+    p->device = device;
+    p->fr.resize(n);
+    p->tab.resize(n);
+    for (int f = 0; f < n; ++f) {
+        p->fr[f].w = widths[f];
+        p->fr[f].h = heights[f];
+    }
+    const size_t tab_bytes = sizeof(FrameTab) * n + sizeof(float) * n;
+    p->staging_bytes = tab_bytes;
+    hipError_t e = hipHostMalloc(&p->staging, tab_bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_plan(p);
+        return fail(OOM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    if ((rc = dev_alloc((void**)&p->tab_dev, tab_bytes)) || (rc = dev_alloc((void**)&p->gates, sizeof(int) * n)) ||
+        (rc = dev_alloc((void**)&p->err, sizeof(int)))) {
+        free_plan(p);
+        return rc;
+    }
+    e = hipMemset(p->err, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(p->gates, 0, sizeof(int) * n);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->staged, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        free_plan(p);
+        return fail(INTERNAL, std::string("init: ") + hipGetErrorString(e));
+    }
+    rc = set_tile_shape(p, precision == CPIPE_BM3D_PRECISION_FP32 ? TILE : TILE / 2);
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    *out = p;
+    return OK;
+}
+
+float* levels_dev(cpipe_bm3d_plan* p) {
+    return reinterpret_cast<float*>(p->tab_dev + p->fr.size());
+}
+
+// Per-run fields of frame f (sigma, noise level, I/O planes).
+void set_run(cpipe_bm3d_plan* p, int f, const float* const* in, size_t in_pitch, float* const* out,
+             size_t out_pitch, float sigma) {
+    FrameTab& T = p->tab[f];
+    T.sigma = sigma;
+    if (p->c == 1) {
+        T.sig[0] = T.sig[1] = T.sig[2] = sigma;
+    } else {
+        T.sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
+        T.sig[1] = sigma * (float)std::sqrt(0.5);
+        T.sig[2] = sigma * (float)std::sqrt(0.375);
+    }
+    for (int ch = 0; ch < 3; ++ch) {
+        const int cc = ch < p->c ? ch : 0;
+        T.in[ch] = in ? in[cc] : nullptr;
+        T.out[ch] = out ? out[cc] : nullptr;
+    }
+    T.in_pitch = in_pitch;
+    T.out_pitch = out_pitch;
+}
+
+// Copies the frame table into the pinned staging buffer once the previous
+// upload has consumed it.
+int fill_staging(cpipe_bm3d_plan* p) {
+    if (p->staged_pending) {
+        HIP_CHECK(hipEventSynchronize(p->staged));
+        p->staged_pending = false;
+    }
+    const size_t n = p->fr.size();
+    std::memcpy(p->staging, p->tab.data(), sizeof(FrameTab) * n);
+    float* lv = reinterpret_cast<float*>(static_cast<char*>(p->staging) + sizeof(FrameTab) * n);
+    for (size_t f = 0; f < n; ++f) lv[f] = 10.0f * p->tab[f].sigma;
+    return OK;
 }
 
 struct RunCtx {
     cpipe_bm3d_plan* p;
     hipStream_t st;
-    float* z[3];
-    float* bz[3];
-    float sig[3];
+    bool timed;
     int mark(int stage, bool end) {
-        if (!p->timing) return OK;
+        if (!timed) return OK;
         if (!end) p->stage_ran[stage] = true;
         HIP_CHECK(hipEventRecord(p->ev[2 * stage + (end ? 1 : 0)], st));
         return OK;
     }
 };
 
-// One BM3D step over a tile sub-rectangle: block matching, group filtering and
-// aggregation, merge of the pixel rectangle into `out` (opponent planes, or
-// RGB with inverse_colour). `gate` (device) skips the kernels when zero.
-int run_step(RunCtx& r, int step, const StepRange& g, float* const* out, size_t out_pitch,
-             int inverse_colour, const int* gate, bool timed) {
+// One BM3D step over a work list: block matching, group filtering and
+// aggregation, merge into `dest`. With use_gate a frame runs only if its
+// gate is set (decided on the device).
+int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, int use_gate, bool timed) {
     cpipe_bm3d_plan* p = r.p;
-    const int w = p->w, h = p->h, C = p->c;
-    const int* txlo = p->geo;
-    const int* txhi = txlo + w;
-    const int* tylo = txhi + w;
-    const int* tyhi = tylo + h;
-    const int* fx0 = tyhi + h;
-    const int* fx1 = fx0 + p->ntx;
-    const int* fy0 = fx1 + p->ntx;
-    const int* fy1 = fy0 + p->nty;
     const int sbm = step == 1 ? CPIPE_BM3D_T_BM1 : CPIPE_BM3D_T_BM2;
     const int sgr = step == 1 ? CPIPE_BM3D_T_GROUP1 : CPIPE_BM3D_T_GROUP2;
     const int smg = step == 1 ? CPIPE_BM3D_T_MERGE1 : CPIPE_BM3D_T_MERGE2;
+    timed = timed && r.timed;
+    RunCtx rt{p, r.st, timed};
     int rc;
-    if (timed && (rc = r.mark(sbm, false))) return rc;
-    const float* bmplane = step == 1 ? r.z[0] : r.bz[0];
-    const dim3 bmgrid = g.bm_list ? dim3(g.n_bm) : dim3(g.btx1 - g.btx0, g.bty1 - g.bty0);
-    k_block_match<<<bmgrid, 256, 0, r.st>>>(bmplane, w, h, p->nx, p->ny, tau_ssd(step),
-                                            p->pos[step - 1], p->cnt[step - 1], g.btx0, g.bty0,
-                                            g.bm_list, gate);
-    HIP_CHECK(hipGetLastError());
-    if (timed && (rc = r.mark(sbm, true))) return rc;
-    if (timed && (rc = r.mark(sgr, false))) return rc;
-    GroupParams gp{};
-    for (int c = 0; c < 3; ++c) {
-        gp.noisy[c] = c < C ? r.z[c] : r.z[0];
-        gp.basic[c] = c < C ? r.bz[c] : r.bz[0];
-        gp.sig[c] = r.sig[c];
+    if ((rc = rt.mark(sbm, false))) return rc;
+    if (L.n_bm > 0) {
+        k_block_match<<<L.n_bm, 256, 0, r.st>>>(p->tab_dev, L.bm, step, tau_ssd(step), use_gate);
+        HIP_CHECK(hipGetLastError());
     }
-    gp.w = w;
-    gp.h = h;
-    gp.nx = p->nx;
-    gp.ny = p->ny;
-    gp.ntx = p->ntx;
-    gp.nty = p->nty;
-    gp.pos = p->pos[step - 1];
-    gp.cnt = p->cnt[step - 1];
-    gp.fx0 = fx0;
-    gp.fx1 = fx1;
-    gp.fy0 = fy0;
-    gp.fy1 = fy1;
-    gp.partial = p->partial;
-    gp.err = p->err;
-    gp.tx0 = g.tx0;
-    gp.ty0 = g.ty0;
-    gp.ntx_sub = g.tx1 - g.tx0;
-    gp.dofs = p->dofs;
-    gp.tile_stride = p->tile_stride;
-    gp.tiles = g.bm_list ? g.tile_list : nullptr;
-    gp.gate = gate;
-    const dim3 ggrid(g.bm_list ? g.n_tiles : (g.tx1 - g.tx0) * (g.ty1 - g.ty0), C);
-    if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
-        HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, gp, ggrid, r.st));
-    } else if (step == 1) {
-        k_group<1><<<ggrid, 256, 0, r.st>>>(gp);
-    } else {
-        k_group<2><<<ggrid, 256, 0, r.st>>>(gp);
+    if ((rc = rt.mark(sbm, true))) return rc;
+    if ((rc = rt.mark(sgr, false))) return rc;
+    if (L.n_grp > 0) {
+        const GroupLaunch gl{p->tab_dev, L.grp, use_gate, p->err};
+        const dim3 ggrid(L.n_grp, p->c);
+        if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
+            HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, gl, ggrid, r.st));
+        } else if (step == 1) {
+            k_group<1><<<ggrid, 256, 0, r.st>>>(gl);
+        } else {
+            k_group<2><<<ggrid, 256, 0, r.st>>>(gl);
+        }
+        HIP_CHECK(hipGetLastError());
     }
-    HIP_CHECK(hipGetLastError());
-    if (timed && (rc = r.mark(sgr, true))) return rc;
-    if (timed && (rc = r.mark(smg, false))) return rc;
-    MergeParams mp{};
-    mp.partial = p->partial;
-    mp.w = w;
-    mp.h = h;
-    mp.ntx = p->ntx;
-    mp.nty = p->nty;
-    mp.channels = C;
-    mp.dofs = p->dofs;
-    mp.tile_stride = p->tile_stride;
-    mp.txlo = txlo;
-    mp.txhi = txhi;
-    mp.tylo = tylo;
-    mp.tyhi = tyhi;
-    mp.fx0 = fx0;
-    mp.fx1 = fx1;
-    mp.fy0 = fy0;
-    for (int c = 0; c < 3; ++c) mp.out[c] = out[c < C ? c : 0];
-    mp.out_pitch = out_pitch;
-    mp.inverse_colour = inverse_colour;
-    mp.x0 = g.x0;
-    mp.y0 = g.y0;
-    mp.x1 = g.x1;
-    mp.y1 = g.y1;
-    mp.rects = g.bm_list ? g.rect_list : nullptr;
-    mp.gate = gate;
-    const dim3 mgrid = g.bm_list ? dim3((g.x1 + 255) / 256, g.y1, g.n_rects)
-                                 : dim3((g.x1 - g.x0 + 255) / 256, g.y1 - g.y0);
-    k_merge<<<mgrid, 256, 0, r.st>>>(mp);
-    HIP_CHECK(hipGetLastError());
-    if (timed && (rc = r.mark(smg, true))) return rc;
+    if ((rc = rt.mark(sgr, true))) return rc;
+    if ((rc = rt.mark(smg, false))) return rc;
+    if (L.n_rects > 0) {
+        const dim3 mgrid((L.rect_w + 255) / 256, L.rect_h, L.n_rects);
+        k_merge<<<mgrid, 256, 0, r.st>>>(p->tab_dev, L.rects, dest, use_gate);
+        HIP_CHECK(hipGetLastError());
+    }
+    if ((rc = rt.mark(smg, true))) return rc;
     return OK;
 }
 
-int enqueue(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float* const* out,
-            size_t out_pitch, float sigma, hipStream_t st) {
-    const int w = p->w, h = p->h, C = p->c;
-    const dim3 rowgrid((w + 255) / 256, h);
-    if (w < K || h < K) {
-        for (int c = 0; c < C; ++c)
-            HIP_CHECK(hipMemcpy2DAsync(out[c], out_pitch * sizeof(float), in[c],
-                                       in_pitch * sizeof(float), (size_t)w * sizeof(float), h,
-                                       hipMemcpyDeviceToDevice, st));
-        return OK;
-    }
-    const size_t npx = p->npx;
-    RunCtx r{p, st, {p->z, p->z + npx, p->z + 2 * npx}, {p->bz, p->bz + npx, p->bz + 2 * npx}, {}};
+// The whole run: table upload, colour, step 1 (or the learned pilot with its
+// spot checks and device-side fallback), step 2 into the callers' planes.
+int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
+    RunCtx r{p, st, timed};
     for (bool& b : p->stage_ran) b = false;
-    if (C == 1) {
-        r.sig[0] = r.sig[1] = r.sig[2] = sigma;
-    } else {
-        r.sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
-        r.sig[1] = sigma * (float)std::sqrt(0.5);
-        r.sig[2] = sigma * (float)std::sqrt(0.375);
+    const size_t n = p->fr.size();
+    HIP_CHECK(hipMemcpyAsync(p->tab_dev, p->staging, p->staging_bytes, hipMemcpyHostToDevice, st));
+    if (!capturing) {
+        HIP_CHECK(hipEventRecord(p->staged, st));
+        p->staged_pending = true;
     }
+    if (p->n_small > 0) {  // frames smaller than one patch pass through
+        const dim3 g((p->small_w + 255) / 256, p->small_h, p->n_small);
+        k_copy_through<<<g, 256, 0, st>>>(p->tab_dev, p->small_rects);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (p->full.n_rects == 0) return OK;
     int rc;
     if ((rc = r.mark(CPIPE_BM3D_T_COLOR, false))) return rc;
-    if (C == 3) {
-        k_opp_fwd<<<rowgrid, 256, 0, st>>>(in[0], in[1], in[2], in_pitch, r.z[0], r.z[1], r.z[2], w, h);
-    } else {
-        k_copy_plane<<<rowgrid, 256, 0, st>>>(in[0], in_pitch, r.z[0], (size_t)w, w);
+    {
+        const dim3 g((p->full.rect_w + 255) / 256, p->full.rect_h, p->full.n_rects);
+        k_colour_in<<<g, 256, 0, st>>>(p->tab_dev, p->full.rects);
+        HIP_CHECK(hipGetLastError());
     }
-    HIP_CHECK(hipGetLastError());
     if ((rc = r.mark(CPIPE_BM3D_T_COLOR, true))) return rc;
 
-    const StepRange all = full_range(p);
-    const int* gate = nullptr;
     if (p->pilot) {
         // learned pilot in place of step 1, then spot checks against the plain step 1
         if ((rc = r.mark(CPIPE_BM3D_T_PILOT, false))) return rc;
-        rc = cpipe_internal::pilot_forward(p->pilot, r.z, (size_t)w, 0, w, h, 1, 10.0f * sigma,
-                                           nullptr, r.bz, st);
-        if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
-        float* bzp[3] = {p->bzp, p->bzp + npx, p->bzp + 2 * npx};
-        const int n = (int)p->checks.size();
-        if (n > 0) {
-            StepRange g{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, p->chk_w, p->chk_h};
-            g.bm_list = p->chk_bm;
-            g.tile_list = p->chk_tiles;
-            g.rect_list = (const int4*)p->rects;
-            g.n_bm = p->n_chk_bm;
-            g.n_tiles = p->n_chk_tiles;
-            g.n_rects = n;
-            if ((rc = run_step(r, 1, g, bzp, (size_t)w, 0, nullptr, false))) return rc;
-            k_pilot_check<<<n, 256, 0, st>>>(r.z[0], r.z[1], r.z[2], r.bz[0], r.bz[1], r.bz[2], bzp[0],
-                                             bzp[1], bzp[2], w, (const CheckRect*)p->rects,
-                                             p->pilot_thr_db, p->report);
+        for (int f = 0; f < (int)n; ++f) {
+            const auto& F = p->fr[f];
+            if (F.small) continue;
+            float* zin[3] = {p->tab[f].z[0], p->tab[f].z[1], p->tab[f].z[2]};
+            float* bout[3] = {p->tab[f].bz[0], p->tab[f].bz[1], p->tab[f].bz[2]};
+            rc = cpipe_internal::pilot_forward(p->pilot, zin, (size_t)F.w, 0, F.w, F.h, 1, 0.0f,
+                                               levels_dev(p) + f, bout, st);
+            if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
+        }
+        const int nchk = p->chk.n_rects;
+        if (nchk > 0) {
+            if ((rc = run_step(r, 1, p->chk, TO_CHECK, 0, false))) return rc;
+            k_pilot_check<<<nchk, 256, 0, st>>>(p->tab_dev, p->chk.rects, p->pilot_thr_db, p->report);
             HIP_CHECK(hipGetLastError());
-            k_pilot_decide<<<1, 64, 0, st>>>(p->report, n, p->pilot_full_frac, p->gate);
+            k_pilot_decide<<<((int)n + 63) / 64, 64, 0, st>>>(p->tab_dev, (int)n, p->report, p->pilot_full_frac);
             HIP_CHECK(hipGetLastError());
-            gate = p->gate;
         }
         if ((rc = r.mark(CPIPE_BM3D_T_PILOT, true))) return rc;
-        if (gate && (rc = run_step(r, 1, all, r.bz, (size_t)w, 0, gate, true))) return rc;
+        if (nchk > 0 && (rc = run_step(r, 1, p->full, TO_BASIC, 1, true))) return rc;
     } else {
-        if ((rc = run_step(r, 1, all, r.bz, (size_t)w, 0, nullptr, true))) return rc;
+        if ((rc = run_step(r, 1, p->full, TO_BASIC, 0, true))) return rc;
     }
-    return run_step(r, 2, all, out, out_pitch, C == 3 ? 1 : 0, nullptr, true);
+    return run_step(r, 2, p->full, TO_OUTPUT, 0, true);
+}
+
+int collect_timing(cpipe_bm3d_plan* p) {
+    HIP_CHECK(hipEventSynchronize(p->ev[2 * CPIPE_BM3D_T_MERGE2 + 1]));
+    for (int i = 0; i < CPIPE_BM3D_T_COUNT; ++i) {
+        if (!p->stage_ran[i]) continue;
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]));
+        p->ms[i] += ms;
+    }
+    p->runs++;
+    return OK;
+}
+
+// Runs the plan with the frame table already set: eagerly (timed if asked),
+// or as one replay of the captured graph (captured on first use after an
+// eager run has sized every lazily allocated buffer).
+int run_plan(cpipe_bm3d_plan* p, hipStream_t st) {
+    int rc = fill_staging(p);
+    if (rc) return rc;
+    const bool timed = p->timing && p->full.n_rects > 0;
+    if (!p->use_graph || timed || p->dirty) {
+        if ((rc = enqueue(p, st, timed, false))) return rc;
+        p->ran = true;
+        if (timed && (rc = collect_timing(p))) return rc;
+        if (p->use_graph && p->dirty) {
+            // capture the next runs' graph now (buffers exist after this eager run)
+            HIP_CHECK(hipEventSynchronize(p->staged));
+            p->staged_pending = false;
+            drop_graph(p);
+            hipStream_t cs;
+            HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                rc = enqueue(p, cs, false, true);
+                hipError_t e2 = hipStreamEndCapture(cs, &g);
+                if (!rc && e2 != hipSuccess) rc = fail(INTERNAL, std::string("graph capture: ") + hipGetErrorString(e2));
+                if (!rc) {
+                    e2 = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+                    if (e2 != hipSuccess) rc = fail(INTERNAL, std::string("graph instantiate: ") + hipGetErrorString(e2));
+                }
+                if (g) (void)hipGraphDestroy(g);
+            } else {
+                rc = fail(INTERNAL, std::string("begin capture: ") + hipGetErrorString(e));
+            }
+            (void)hipStreamDestroy(cs);
+            if (rc) return rc;
+            p->dirty = false;
+        }
+        return OK;
+    }
+    HIP_CHECK(hipGraphLaunch(p->exec, st));
+    HIP_CHECK(hipEventRecord(p->staged, st));
+    p->staged_pending = true;
+    p->ran = true;
+    return OK;
+}
+
+// Spot-check rectangles of every frame and their work lists.
+int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
+    free_lists(p->chk);
+    (void)hipFree(p->report);
+    p->report = nullptr;
+    p->checks.clear();
+    std::vector<FrameRect> rects;
+    std::vector<int4> bm;
+    std::vector<int2> grp;
+    int cw = 0, chh = 0;
+    const int B = o.check_block;
+    for (int f = 0; f < (int)p->fr.size(); ++f) {
+        const auto& F = p->fr[f];
+        FrameTab& T = p->tab[f];
+        T.chk0 = (int)p->checks.size();
+        T.nchk = 0;
+        if (F.small || o.check_every <= 0) continue;
+        const int* txlo = F.geo.data();
+        const int* txhi = txlo + F.w;
+        const int* tylo = txhi + F.w;
+        const int* tyhi = tylo + F.h;
+        const int nbx = (F.w + B - 1) / B, nby = (F.h + B - 1) / B;
+        const int nbtx = (F.nx + BM_TX - 1) / BM_TX, nbty = (F.ny + BM_TY - 1) / BM_TY;
+        std::vector<char> bm_used((size_t)nbtx * nbty, 0), tile_used((size_t)F.ntx * F.nty, 0);
+        for (int b = 0; b < nbx * nby; ++b) {
+            // every check_every-th block in raster order, the phase shifted by one per block row
+            if ((b + b / nbx) % o.check_every != o.check_every / 2) continue;
+            cpipe_bm3d_plan_s::Check c{};
+            c.f = f;
+            c.x0 = (b % nbx) * B;
+            c.y0 = (b / nbx) * B;
+            c.x1 = std::min(F.w, c.x0 + B);
+            c.y1 = std::min(F.h, c.y0 + B);
+            c.tx0 = txlo[c.x0];
+            c.tx1 = txhi[c.x1 - 1] + 1;
+            c.ty0 = tylo[c.y0];
+            c.ty1 = tyhi[c.y1 - 1] + 1;
+            const int rx0 = c.tx0 * TILE, rx1 = std::min(c.tx1 * TILE, F.nx);
+            const int ry0 = c.ty0 * p->tile_y, ry1 = std::min(c.ty1 * p->tile_y, F.ny);
+            c.btx0 = rx0 / BM_TX;
+            c.btx1 = (rx1 + BM_TX - 1) / BM_TX;
+            c.bty0 = ry0 / BM_TY;
+            c.bty1 = (ry1 + BM_TY - 1) / BM_TY;
+            p->checks.push_back(c);
+            rects.push_back({f, c.x0, c.y0, c.x1, c.y1, 0, 0, 0});
+            T.nchk++;
+            for (int ty = c.bty0; ty < c.bty1; ++ty)
+                for (int tx = c.btx0; tx < c.btx1; ++tx) bm_used[(size_t)ty * nbtx + tx] = 1;
+            for (int ty = c.ty0; ty < c.ty1; ++ty)
+                for (int tx = c.tx0; tx < c.tx1; ++tx) tile_used[(size_t)ty * F.ntx + tx] = 1;
+            cw = std::max(cw, c.x1 - c.x0);
+            chh = std::max(chh, c.y1 - c.y0);
+        }
+        // each tile once: overlapping checks must not run one tile twice in a launch
+        for (int i = 0; i < nbtx * nbty; ++i)
+            if (bm_used[i]) bm.push_back(make_int4(f, i % nbtx, i / nbtx, 0));
+        for (int i = 0; i < F.ntx * F.nty; ++i)
+            if (tile_used[i]) grp.push_back(make_int2(f, i));
+    }
+    int rc;
+    auto& L = p->chk;
+    if ((rc = upload(&L.bm, bm)) || (rc = upload(&L.grp, grp)) || (rc = upload(&L.rects, rects)) ||
+        (rc = dev_alloc((void**)&p->report, std::max<size_t>(1, rects.size()) * 2 * sizeof(float))))
+        return rc;
+    HIP_CHECK(hipMemset(p->report, 0, std::max<size_t>(1, rects.size()) * 2 * sizeof(float)));
+    L.n_bm = (int)bm.size();
+    L.n_grp = (int)grp.size();
+    L.n_rects = (int)rects.size();
+    L.rect_w = cw;
+    L.rect_h = chh;
+    return OK;
+}
+
+// Autotuner cache: best (tile height, work-list order, band rows) per plan shape.
+struct TuneKey {
+    int precision, c;
+    std::vector<int> dims;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(precision, c, dims) < std::tie(o.precision, o.c, o.dims);
+    }
+};
+struct TuneChoice {
+    int tile_y, order, band_rows;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoice> g_tune_cache;
+
+TuneKey tune_key(const cpipe_bm3d_plan* p) {
+    TuneKey k{p->precision, p->c, {}};
+    for (const auto& f : p->fr) {
+        k.dims.push_back(f.w);
+        k.dims.push_back(f.h);
+    }
+    return k;
+}
+
+int apply_choice(cpipe_bm3d_plan* p, const TuneChoice& ch) {
+    const bool reshape = ch.tile_y != p->tile_y;
+    p->order = ch.order;
+    p->band_rows = ch.band_rows;
+    drop_graph(p);
+    int rc = reshape ? set_tile_shape(p, ch.tile_y) : build_full_lists(p);
+    if (rc) return rc;
+    // a new tile shape rebuilds the frames: re-create the pilot's buffers and checks
+    if (reshape && p->pilot) rc = cpipe_bm3d_plan_set_pilot(p, p->pilot, &p->pilot_opts);
+    return rc;
 }
 
 }  // namespace
@@ -1118,114 +1552,17 @@ int cpipe_bm3d_plan_create(const cpipe_bm3d_plan_desc* d, cpipe_bm3d_plan** out_
     *out_plan = nullptr;
     if (d->struct_size < sizeof(cpipe_bm3d_plan_desc))
         return fail(BAD_INDEX, "cpipe_bm3d_plan_desc.struct_size too small");
-    if (d->width <= 0 || d->height <= 0 || d->width > 65535 || d->height > 65535)
-        return fail(BAD_INDEX, "width/height out of range (1..65535)");
-    if (d->channels != 1 && d->channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
-    if (d->precision != CPIPE_BM3D_PRECISION_FP32 && d->precision != CPIPE_BM3D_PRECISION_FP16 &&
-        d->precision != CPIPE_BM3D_PRECISION_BF16)
-        return fail(BAD_INDEX, "unknown precision");
-    const int ndev = cpipe_bm3d_device_count();
-    if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
-    if (d->device < 0 || d->device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
-    DeviceGuard guard(d->device);
-    int rc = ensure_tables(d->device);
+    int rc = create_core(out_plan, &d->width, &d->height, 1, d->channels, d->precision, d->device);
     if (rc) return rc;
-
-    auto* p = new cpipe_bm3d_plan_s();
-    p->w = d->width;
-    p->h = d->height;
-    p->c = d->channels;
-    p->precision = d->precision;
-    p->device = d->device;
-    p->npx = (size_t)p->w * p->h;
-    if (p->w >= K && p->h >= K) {
-        p->nx = ref_count(p->w);
-        p->ny = ref_count(p->h);
-        // exact FP32 plans use 16 x 16-ref tiles (the CPU FP32 restatement's tiling);
-        // mixed-precision plans 16 x 8, whose smaller LDS footprint fits three
-        // workgroups per CU (three ordered aggregation chains instead of two)
-        p->tile_y = p->precision == CPIPE_BM3D_PRECISION_FP32 ? TILE : TILE / 2;
-        p->dofs = FOOT * foot_edge(p->tile_y);
-        p->tile_stride = 2 * (size_t)p->dofs;
-        p->ntx = (p->nx + TILE - 1) / TILE;
-        p->nty = (p->ny + p->tile_y - 1) / p->tile_y;
-        const size_t nref = (size_t)p->nx * p->ny;
-        auto alloc = [&](void** ptr, size_t bytes) -> int {
-            hipError_t e = hipMalloc(ptr, bytes);
-            if (e != hipSuccess) {
-                return fail(e == hipErrorOutOfMemory ? OOM : INTERNAL,
-                            std::string("hipMalloc: ") + hipGetErrorString(e));
-            }
-            return OK;
-        };
-        const size_t ntiles = (size_t)p->ntx * p->nty;
-        const size_t ngeo = 2 * (size_t)p->w + 2 * (size_t)p->h + 2 * (size_t)p->ntx + 2 * (size_t)p->nty;
-        if ((rc = alloc((void**)&p->z, sizeof(float) * p->npx * p->c)) ||
-            (rc = alloc((void**)&p->bz, sizeof(float) * p->npx * p->c)) ||
-            (rc = alloc((void**)&p->pos[0], sizeof(uint32_t) * nref * NMAX)) ||
-            (rc = alloc((void**)&p->pos[1], sizeof(uint32_t) * nref * NMAX)) ||
-            (rc = alloc((void**)&p->cnt[0], nref)) || (rc = alloc((void**)&p->cnt[1], nref)) ||
-            (rc = alloc((void**)&p->partial, sizeof(float) * p->tile_stride * ntiles * p->c)) ||
-            (rc = alloc((void**)&p->err, sizeof(int))) ||
-            (rc = alloc((void**)&p->geo, sizeof(int) * ngeo))) {
+    cpipe_bm3d_plan* p = *out_plan;
+    if (p->c == 3) {
+        DeviceGuard guard(p->device);
+        if ((rc = dev_alloc((void**)&p->rgb, sizeof(float) * 6 * (size_t)d->width * d->height))) {
             free_plan(p);
+            *out_plan = nullptr;
             return rc;
         }
-        std::vector<int> geo(ngeo);
-        int* txlo = geo.data();
-        int* txhi = txlo + p->w;
-        int* tylo = txhi + p->w;
-        int* tyhi = tylo + p->h;
-        int* fx0 = tyhi + p->h;
-        int* fx1 = fx0 + p->ntx;
-        int* fy0 = fx1 + p->ntx;
-        int* fy1 = fy0 + p->nty;
-        for (int t = 0; t < p->ntx; ++t) tile_box_host(t, TILE, p->nx, p->w, &fx0[t], &fx1[t]);
-        for (int t = 0; t < p->nty; ++t) tile_box_host(t, p->tile_y, p->ny, p->h, &fy0[t], &fy1[t]);
-        for (int x = 0; x < p->w; ++x) {
-            txlo[x] = -1;
-            txhi[x] = -2;
-        }
-        for (int t = 0; t < p->ntx; ++t)
-            for (int x = fx0[t]; x < fx1[t]; ++x) {
-                if (txlo[x] < 0) txlo[x] = t;
-                txhi[x] = t;
-            }
-        for (int y = 0; y < p->h; ++y) {
-            tylo[y] = -1;
-            tyhi[y] = -2;
-        }
-        for (int t = 0; t < p->nty; ++t)
-            for (int y = fy0[t]; y < fy1[t]; ++y) {
-                if (tylo[y] < 0) tylo[y] = t;
-                tyhi[y] = t;
-            }
-        for (int x = 0; x < p->w; ++x)
-            if (txlo[x] < 0) {
-                free_plan(p);
-                return fail(INTERNAL, "tile cover has a gap in x");
-            }
-        for (int y = 0; y < p->h; ++y)
-            if (tylo[y] < 0) {
-                free_plan(p);
-                return fail(INTERNAL, "tile cover has a gap in y");
-            }
-        p->geo_host = geo;
-        hipError_t e = hipMemset(p->err, 0, sizeof(int));
-        if (e == hipSuccess) e = hipMemcpy(p->geo, geo.data(), sizeof(int) * ngeo, hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            free_plan(p);
-            return fail(INTERNAL, std::string("hipMemcpy geo: ") + hipGetErrorString(e));
-        }
     }
-    if (p->c == 3) {
-        hipError_t e = hipMalloc((void**)&p->rgb, sizeof(float) * p->npx * 6);
-        if (e != hipSuccess) {
-            free_plan(p);
-            return fail(OOM, std::string("hipMalloc rgb: ") + hipGetErrorString(e));
-        }
-    }
-    *out_plan = p;
     return OK;
 }
 
@@ -1234,45 +1571,165 @@ void cpipe_bm3d_plan_destroy(cpipe_bm3d_plan* plan) { free_plan(plan); }
 int cpipe_bm3d_plan_run(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch,
                         float* const* out, size_t out_pitch, float sigma, void* stream) {
     if (!p || !in || !out) return fail(BAD_INDEX, "null argument");
+    if (p->fr.size() != 1) return fail(BAD_INDEX, "a batch runs with cpipe_bm3d_batch_run");
     for (int c = 0; c < p->c; ++c)
         if (!in[c] || !out[c]) return fail(BAD_INDEX, "null plane pointer");
-    if (in_pitch < (size_t)p->w || out_pitch < (size_t)p->w)
+    if (in_pitch < (size_t)p->fr[0].w || out_pitch < (size_t)p->fr[0].w)
         return fail(BAD_INDEX, "pitch smaller than width");
     if (!(sigma >= 0.0f) || !std::isfinite(sigma)) return fail(BAD_INDEX, "sigma must be finite and >= 0");
     DeviceGuard guard(p->device);
-    hipStream_t st = (hipStream_t)stream;
-    const int rc = enqueue(p, in, in_pitch, out, out_pitch, sigma, st);
+    set_run(p, 0, in, in_pitch, out, out_pitch, sigma);
+    return run_plan(p, (hipStream_t)stream);
+}
+
+int cpipe_bm3d_batch_create(const cpipe_bm3d_batch_desc* d, cpipe_bm3d_batch** out) {
+    if (!d || !out) return fail(BAD_INDEX, "null argument");
+    *out = nullptr;
+    if (d->struct_size < sizeof(cpipe_bm3d_batch_desc))
+        return fail(BAD_INDEX, "cpipe_bm3d_batch_desc.struct_size too small");
+    if (!d->widths || !d->heights) return fail(BAD_INDEX, "null frame sizes");
+    int rc = create_core(out, d->widths, d->heights, d->n_frames, d->channels, d->precision, d->device);
     if (rc) return rc;
-    p->ran = true;
-    if (p->timing && p->w >= K && p->h >= K) {
-        HIP_CHECK(hipEventSynchronize(p->ev[2 * CPIPE_BM3D_T_MERGE2 + 1]));
-        for (int i = 0; i < CPIPE_BM3D_T_COUNT; ++i) {
-            if (!p->stage_ran[i]) continue;
-            float ms = 0.0f;
-            HIP_CHECK(hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]));
-            p->ms[i] += ms;
+    cpipe_bm3d_plan* p = *out;
+    p->use_graph = d->use_graph != 0;
+    TuneChoice ch{};
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(tune_key(p));
+        if (it != g_tune_cache.end()) {
+            ch = it->second;
+            cached = true;
         }
-        p->runs++;
     }
+    if (cached) {
+        DeviceGuard guard(p->device);
+        if ((rc = apply_choice(p, ch))) {
+            free_plan(p);
+            *out = nullptr;
+            return rc;
+        }
+    }
+    return OK;
+}
+
+void cpipe_bm3d_batch_destroy(cpipe_bm3d_batch* b) { free_plan(b); }
+
+int cpipe_bm3d_batch_run(cpipe_bm3d_batch* b, const float* const* in, const size_t* in_pitch,
+                         float* const* out, const size_t* out_pitch, const float* sigmas,
+                         void* stream) {
+    if (!b || !in || !out || !in_pitch || !out_pitch || !sigmas) return fail(BAD_INDEX, "null argument");
+    const int n = (int)b->fr.size(), C = b->c;
+    for (int f = 0; f < n; ++f) {
+        for (int c = 0; c < C; ++c)
+            if (!in[f * C + c] || !out[f * C + c]) return fail(BAD_INDEX, "null plane pointer");
+        if (in_pitch[f] < (size_t)b->fr[f].w || out_pitch[f] < (size_t)b->fr[f].w)
+            return fail(BAD_INDEX, "pitch smaller than width");
+        if (!(sigmas[f] >= 0.0f) || !std::isfinite(sigmas[f]))
+            return fail(BAD_INDEX, "sigma must be finite and >= 0");
+    }
+    DeviceGuard guard(b->device);
+    for (int f = 0; f < n; ++f) set_run(b, f, in + f * C, in_pitch[f], out + f * C, out_pitch[f], sigmas[f]);
+    return run_plan(b, (hipStream_t)stream);
+}
+
+int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* b, const float* const* in, const size_t* in_pitch,
+                          float* const* out, const size_t* out_pitch, const float* sigmas, int iters,
+                          void* stream, cpipe_bm3d_tune_report* report) {
+    if (!b || !in || !out || !in_pitch || !out_pitch || !sigmas) return fail(BAD_INDEX, "null argument");
+    if (iters < 1) iters = 1;
+    DeviceGuard guard(b->device);
+    hipStream_t st = (hipStream_t)stream;
+    const int n = (int)b->fr.size(), C = b->c;
+    auto set_all = [&]() {
+        for (int f = 0; f < n; ++f) set_run(b, f, in + f * C, in_pitch[f], out + f * C, out_pitch[f], sigmas[f]);
+    };
+    const bool graph = b->use_graph, timing = b->timing;
+    b->use_graph = false;
+    b->timing = false;
+    std::vector<TuneChoice> cands;
+    std::vector<int> tys = {b->tile_y};
+    if (b->precision != CPIPE_BM3D_PRECISION_FP32) tys = {TILE / 2, TILE};
+    for (int ty : tys)
+        for (int order : {0, 1})
+            for (int rows : {16, 64}) cands.push_back({ty, order, rows});
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    TuneChoice best = cands[0];
+    int rc = OK, k = 0;
+    for (const auto& ch : cands) {
+        if ((rc = apply_choice(b, ch))) break;
+        set_all();
+        if ((rc = fill_staging(b)) || (rc = enqueue(b, st, false, false))) break;  // warm-up
+        HIP_CHECK(hipEventRecord(e0, st));
+        for (int i = 0; i < iters && !rc; ++i) {
+            if ((rc = fill_staging(b)) || (rc = enqueue(b, st, false, false))) break;
+        }
+        if (rc) break;
+        HIP_CHECK(hipEventRecord(e1, st));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)iters;
+        if (report && k < CPIPE_BM3D_TUNE_MAX) {
+            report->tile_y[k] = ch.tile_y;
+            report->order[k] = ch.order;
+            report->band_rows[k] = ch.band_rows;
+            report->ms[k] = ms;
+        }
+        ++k;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = ch;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->use_graph = graph;
+    b->timing = timing;
+    if (rc) return rc;
+    if ((rc = apply_choice(b, best))) return rc;
+    set_all();
+    if (report) {
+        report->n = std::min(k, (int)CPIPE_BM3D_TUNE_MAX);
+        report->best = -1;
+        for (int i = 0; i < report->n; ++i)
+            if (report->tile_y[i] == best.tile_y && report->order[i] == best.order &&
+                report->band_rows[i] == best.band_rows)
+                report->best = i;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[tune_key(b)] = best;
+    return OK;
+}
+
+int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* b, int* tile_y, int* order, int* band_rows) {
+    if (!b) return fail(BAD_INDEX, "null batch");
+    if (tile_y) *tile_y = b->tile_y;
+    if (order) *order = b->order;
+    if (band_rows) *band_rows = b->band_rows;
     return OK;
 }
 
 int cpipe_bm3d_plan_run_rgba16f(cpipe_bm3d_plan* p, const uint16_t* in, size_t isp, uint16_t* out,
                                 size_t osp, float sigma, void* stream) {
     if (!p || !in || !out) return fail(BAD_INDEX, "null argument");
-    if (p->c != 3) return fail(BAD_INDEX, "RGBA16F needs a 3-channel plan");
-    if (isp < (size_t)p->w || osp < (size_t)p->w) return fail(BAD_INDEX, "stride smaller than width");
+    if (p->fr.size() != 1 || p->c != 3) return fail(BAD_INDEX, "RGBA16F needs a 3-channel single-frame plan");
+    const int w = p->fr[0].w, h = p->fr[0].h;
+    if (isp < (size_t)w || osp < (size_t)w) return fail(BAD_INDEX, "stride smaller than width");
     DeviceGuard guard(p->device);
     hipStream_t st = (hipStream_t)stream;
-    const size_t npx = p->npx;
+    const dim3 rowgrid((w + 255) / 256, h);
+    const size_t npx = p->fr[0].npx;
     float* rin[3] = {p->rgb, p->rgb + npx, p->rgb + 2 * npx};
     float* rout[3] = {p->rgb + 3 * npx, p->rgb + 4 * npx, p->rgb + 5 * npx};
-    const dim3 rowgrid((p->w + 255) / 256, p->h);
-    k_unpack_rgba16f<<<rowgrid, 256, 0, st>>>(in, isp, rin[0], rin[1], rin[2], p->w);
+    k_unpack_rgba16f<<<rowgrid, 256, 0, st>>>(in, isp, rin[0], rin[1], rin[2], w);
     HIP_CHECK(hipGetLastError());
-    const int rc = cpipe_bm3d_plan_run(p, rin, (size_t)p->w, rout, (size_t)p->w, sigma, stream);
+    const int rc = cpipe_bm3d_plan_run(p, rin, (size_t)w, rout, (size_t)w, sigma, stream);
     if (rc) return rc;
-    k_pack_rgba16f<<<rowgrid, 256, 0, st>>>(rout[0], rout[1], rout[2], in, isp, out, osp, p->w);
+    k_pack_rgba16f<<<rowgrid, 256, 0, st>>>(rout[0], rout[1], rout[2], in, isp, out, osp, w);
     HIP_CHECK(hipGetLastError());
     return OK;
 }
@@ -1280,16 +1737,17 @@ int cpipe_bm3d_plan_run_rgba16f(cpipe_bm3d_plan* p, const uint16_t* in, size_t i
 int cpipe_bm3d_plan_basic(cpipe_bm3d_plan* p, float* const* out, size_t out_pitch, void* stream) {
     if (!p || !out) return fail(BAD_INDEX, "null argument");
     if (!p->ran) return fail(BAD_INDEX, "plan has not run yet");
-    if (p->w < K || p->h < K) return fail(UNSUPPORTED, "image smaller than one patch has no basic estimate");
+    const auto& F = p->fr[0];
+    if (F.small) return fail(UNSUPPORTED, "image smaller than one patch has no basic estimate");
     DeviceGuard guard(p->device);
     hipStream_t st = (hipStream_t)stream;
-    const size_t npx = p->npx;
-    const dim3 rowgrid((p->w + 255) / 256, p->h);
+    const size_t npx = F.npx;
+    const dim3 rowgrid((F.w + 255) / 256, F.h);
     if (p->c == 3) {
-        k_opp_inv<<<rowgrid, 256, 0, st>>>(p->bz, p->bz + npx, p->bz + 2 * npx, out[0], out[1],
-                                           out[2], out_pitch, p->w);
+        k_opp_inv<<<rowgrid, 256, 0, st>>>(F.bz, F.bz + npx, F.bz + 2 * npx, out[0], out[1], out[2],
+                                           out_pitch, F.w);
     } else {
-        k_copy_plane<<<rowgrid, 256, 0, st>>>(p->bz, (size_t)p->w, out[0], out_pitch, p->w);
+        k_copy_plane<<<rowgrid, 256, 0, st>>>(F.bz, (size_t)F.w, out[0], out_pitch, F.w);
     }
     HIP_CHECK(hipGetLastError());
     return OK;
@@ -1298,13 +1756,14 @@ int cpipe_bm3d_plan_basic(cpipe_bm3d_plan* p, float* const* out, size_t out_pitc
 int cpipe_bm3d_plan_matches(cpipe_bm3d_plan* p, int step, uint32_t* pos, uint8_t* count,
                             void* stream) {
     if (!p || !pos || !count || (step != 1 && step != 2)) return fail(BAD_INDEX, "bad argument");
-    if (!p->ran || p->nx == 0) return fail(BAD_INDEX, "no match lists");
+    const auto& F = p->fr[0];
+    if (!p->ran || F.nx == 0) return fail(BAD_INDEX, "no match lists");
     DeviceGuard guard(p->device);
     hipStream_t st = (hipStream_t)stream;
-    const size_t nref = (size_t)p->nx * p->ny;
-    HIP_CHECK(hipMemcpyAsync(pos, p->pos[step - 1], nref * NMAX * sizeof(uint32_t),
+    const size_t nref = (size_t)F.nx * F.ny;
+    HIP_CHECK(hipMemcpyAsync(pos, F.pos[step - 1], nref * NMAX * sizeof(uint32_t),
                              hipMemcpyDeviceToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(count, p->cnt[step - 1], nref, hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(count, F.cnt[step - 1], nref, hipMemcpyDeviceToDevice, st));
     return OK;
 }
 
@@ -1312,27 +1771,29 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* p, const float* const* in, s
                                    float sigma, float* const* noisy_out, float* const* basic_out,
                                    size_t out_pitch, void* stream) {
     if (!p || !in || !noisy_out || !basic_out) return fail(BAD_INDEX, "null argument");
-    if (p->c != 3) return fail(UNSUPPORTED, "opponent planes exist for 3-channel plans only");
-    if (p->w < K || p->h < K) return fail(UNSUPPORTED, "image smaller than one patch");
-    if (in_pitch < (size_t)p->w || out_pitch < (size_t)p->w) return fail(BAD_INDEX, "pitch smaller than width");
+    if (p->fr.size() != 1 || p->c != 3) return fail(UNSUPPORTED, "opponent planes exist for 3-channel single-frame plans");
+    const auto& F = p->fr[0];
+    if (F.small) return fail(UNSUPPORTED, "image smaller than one patch");
+    if (in_pitch < (size_t)F.w || out_pitch < (size_t)F.w) return fail(BAD_INDEX, "pitch smaller than width");
     DeviceGuard guard(p->device);
     hipStream_t st = (hipStream_t)stream;
-    const size_t npx = p->npx;
-    const int w = p->w, h = p->h;
-    RunCtx r{p, st, {p->z, p->z + npx, p->z + 2 * npx}, {p->bz, p->bz + npx, p->bz + 2 * npx}, {}};
-    r.sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
-    r.sig[1] = sigma * (float)std::sqrt(0.5);
-    r.sig[2] = sigma * (float)std::sqrt(0.375);
-    const dim3 rowgrid((w + 255) / 256, h);
-    k_opp_fwd<<<rowgrid, 256, 0, st>>>(in[0], in[1], in[2], in_pitch, r.z[0], r.z[1], r.z[2], w, h);
-    HIP_CHECK(hipGetLastError());
-    const int rc = run_step(r, 1, full_range(p), r.bz, (size_t)w, 0, nullptr, false);
+    set_run(p, 0, in, in_pitch, nullptr, 0, sigma);
+    int rc = fill_staging(p);
     if (rc) return rc;
+    HIP_CHECK(hipMemcpyAsync(p->tab_dev, p->staging, p->staging_bytes, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipEventRecord(p->staged, st));
+    p->staged_pending = true;
+    p->ran = true;
+    const dim3 g((p->full.rect_w + 255) / 256, p->full.rect_h, p->full.n_rects);
+    k_colour_in<<<g, 256, 0, st>>>(p->tab_dev, p->full.rects);
+    HIP_CHECK(hipGetLastError());
+    RunCtx r{p, st, false};
+    if ((rc = run_step(r, 1, p->full, TO_BASIC, 0, false))) return rc;
     for (int c = 0; c < 3; ++c) {
-        HIP_CHECK(hipMemcpy2DAsync(noisy_out[c], out_pitch * sizeof(float), r.z[c], (size_t)w * sizeof(float),
-                                   (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, st));
-        HIP_CHECK(hipMemcpy2DAsync(basic_out[c], out_pitch * sizeof(float), r.bz[c], (size_t)w * sizeof(float),
-                                   (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpy2DAsync(noisy_out[c], out_pitch * sizeof(float), p->tab[0].z[c], (size_t)F.w * sizeof(float),
+                                   (size_t)F.w * sizeof(float), F.h, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpy2DAsync(basic_out[c], out_pitch * sizeof(float), p->tab[0].bz[c], (size_t)F.w * sizeof(float),
+                                   (size_t)F.w * sizeof(float), F.h, hipMemcpyDeviceToDevice, st));
     }
     return OK;
 }
@@ -1342,24 +1803,25 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     if (!p) return fail(BAD_INDEX, "null plan");
     DeviceGuard guard(p->device);
     HIP_CHECK(hipDeviceSynchronize());
-    (void)hipFree(p->rects);
-    (void)hipFree(p->chk_bm);
-    (void)hipFree(p->chk_tiles);
+    drop_graph(p);
+    free_lists(p->chk);
     (void)hipFree(p->report);
-    (void)hipFree(p->gate);
-    (void)hipFree(p->bzp);
-    p->rects = nullptr;
-    p->chk_bm = nullptr;
-    p->chk_tiles = nullptr;
-    p->n_chk_bm = p->n_chk_tiles = p->chk_w = p->chk_h = 0;
     p->report = nullptr;
-    p->gate = nullptr;
-    p->bzp = nullptr;
+    for (auto& f : p->fr) {
+        (void)hipFree(f.bzp);
+        f.bzp = nullptr;
+    }
+    for (auto& t : p->tab) {
+        t.bzp[0] = t.bzp[1] = t.bzp[2] = nullptr;
+        t.chk0 = t.nchk = 0;
+    }
     p->checks.clear();
     p->pilot = nullptr;
     if (!net) return OK;
     if (p->c != 3) return fail(UNSUPPORTED, "the learned pilot is trained for RGB (3-plane) images");
-    if (p->w < K || p->h < K) return fail(UNSUPPORTED, "image smaller than one patch");
+    bool any = false;
+    for (const auto& f : p->fr) any = any || !f.small;
+    if (!any) return fail(UNSUPPORTED, "image smaller than one patch");
     cpipe_bm3d_pilot_opts o{sizeof(cpipe_bm3d_pilot_opts), 0.5f, 256, 16, 0.5f};
     if (opts) {
         if (opts->struct_size < sizeof(cpipe_bm3d_pilot_opts)) return fail(BAD_INDEX, "opts.struct_size too small");
@@ -1367,69 +1829,16 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     }
     if (!(o.fallback_db >= 0.0f) || o.check_block < 16 || o.check_every < 0)
         return fail(BAD_INDEX, "bad pilot options");
-    const int w = p->w, h = p->h;
-    const int* txlo = p->geo_host.data();
-    const int* txhi = txlo + w;
-    const int* tylo = txhi + w;
-    const int* tyhi = tylo + h;
-    const int B = o.check_block;
-    const int nbx = (w + B - 1) / B, nby = (h + B - 1) / B;
-    std::vector<CheckRect> rects;
-    std::vector<char> bm_used, tile_used((size_t)p->ntx * p->nty, 0);
-    const int nbtx = (p->nx + BM_TX - 1) / BM_TX, nbty = (p->ny + BM_TY - 1) / BM_TY;
-    bm_used.assign((size_t)nbtx * nbty, 0);
-    for (int b = 0; o.check_every > 0 && b < nbx * nby; ++b) {
-        // every check_every-th block in raster order, the phase shifted by one per block row
-        if ((b + b / nbx) % o.check_every != o.check_every / 2) continue;
-        cpipe_bm3d_plan_s::Check c{};
-        c.x0 = (b % nbx) * B;
-        c.y0 = (b / nbx) * B;
-        c.x1 = std::min(w, c.x0 + B);
-        c.y1 = std::min(h, c.y0 + B);
-        c.tx0 = txlo[c.x0];
-        c.tx1 = txhi[c.x1 - 1] + 1;
-        c.ty0 = tylo[c.y0];
-        c.ty1 = tyhi[c.y1 - 1] + 1;
-        const int rx0 = c.tx0 * TILE, rx1 = std::min(c.tx1 * TILE, p->nx);
-        const int ry0 = c.ty0 * p->tile_y, ry1 = std::min(c.ty1 * p->tile_y, p->ny);
-        c.btx0 = rx0 / BM_TX;
-        c.btx1 = (rx1 + BM_TX - 1) / BM_TX;
-        c.bty0 = ry0 / BM_TY;
-        c.bty1 = (ry1 + BM_TY - 1) / BM_TY;
-        p->checks.push_back(c);
-        rects.push_back({c.x0, c.y0, c.x1, c.y1});
-        for (int ty = c.bty0; ty < c.bty1; ++ty)
-            for (int tx = c.btx0; tx < c.btx1; ++tx) bm_used[(size_t)ty * nbtx + tx] = 1;
-        for (int ty = c.ty0; ty < c.ty1; ++ty)
-            for (int tx = c.tx0; tx < c.tx1; ++tx) tile_used[(size_t)ty * p->ntx + tx] = 1;
-        p->chk_w = std::max(p->chk_w, c.x1 - c.x0);
-        p->chk_h = std::max(p->chk_h, c.y1 - c.y0);
+    int rc;
+    for (int f = 0; f < (int)p->fr.size(); ++f) {
+        auto& F = p->fr[f];
+        if (F.small) continue;
+        if ((rc = dev_alloc((void**)&F.bzp, sizeof(float) * 3 * F.npx))) return rc;
+        for (int c = 0; c < 3; ++c) p->tab[f].bzp[c] = F.bzp + c * F.npx;
     }
-    // each tile once: overlapping checks must not run one tile twice in a launch
-    std::vector<int2> bm_list;
-    std::vector<int> tile_list;
-    for (int i = 0; i < nbtx * nbty; ++i)
-        if (bm_used[i]) bm_list.push_back(make_int2(i % nbtx, i / nbtx));
-    for (int i = 0; i < p->ntx * p->nty; ++i)
-        if (tile_used[i]) tile_list.push_back(i);
-    p->n_chk_bm = (int)bm_list.size();
-    p->n_chk_tiles = (int)tile_list.size();
-    const size_t n = rects.size();
-    HIP_CHECK(hipMalloc((void**)&p->chk_bm, std::max<size_t>(1, bm_list.size()) * sizeof(int2)));
-    HIP_CHECK(hipMalloc((void**)&p->chk_tiles, std::max<size_t>(1, tile_list.size()) * sizeof(int)));
-    if (!bm_list.empty())
-        HIP_CHECK(hipMemcpy(p->chk_bm, bm_list.data(), bm_list.size() * sizeof(int2), hipMemcpyHostToDevice));
-    if (!tile_list.empty())
-        HIP_CHECK(hipMemcpy(p->chk_tiles, tile_list.data(), tile_list.size() * sizeof(int),
-                            hipMemcpyHostToDevice));
-    HIP_CHECK(hipMalloc(&p->rects, std::max<size_t>(1, n) * sizeof(CheckRect)));
-    HIP_CHECK(hipMalloc((void**)&p->report, std::max<size_t>(1, n) * 2 * sizeof(float)));
-    HIP_CHECK(hipMalloc((void**)&p->gate, sizeof(int)));
-    HIP_CHECK(hipMalloc((void**)&p->bzp, 3 * p->npx * sizeof(float)));
-    if (n) HIP_CHECK(hipMemcpy(p->rects, rects.data(), n * sizeof(CheckRect), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemset(p->report, 0, std::max<size_t>(1, n) * 2 * sizeof(float)));
-    HIP_CHECK(hipMemset(p->gate, 0, sizeof(int)));
+    if ((rc = build_check_lists(p, o))) return rc;
     p->pilot = net;
+    p->pilot_opts = o;
     p->pilot_thr_db = o.fallback_db;
     p->pilot_full_frac = o.full_fraction;
     return OK;
@@ -1450,7 +1859,12 @@ int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* p, int* n_checked, int* n_fall
             md = std::max(md, rep[2 * i]);
             nf += rep[2 * i + 1] > 0.5f ? 1 : 0;
         }
-        if (nc) HIP_CHECK(hipMemcpy(&full, p->gate, sizeof(int), hipMemcpyDeviceToHost));
+        if (nc) {
+            std::vector<int> g(p->fr.size());
+            HIP_CHECK(hipMemcpy(g.data(), p->gates, sizeof(int) * g.size(), hipMemcpyDeviceToHost));
+            for (size_t f = 0; f < g.size(); ++f)
+                if (p->tab[f].nchk > 0) full += g[f] ? 1 : 0;
+        }
     }
     if (n_checked) *n_checked = nc;
     if (n_fallback) *n_fallback = nf;
@@ -1472,8 +1886,8 @@ int cpipe_bm3d_plan_check(cpipe_bm3d_plan* p) {
 
 int cpipe_bm3d_plan_ref_grid(const cpipe_bm3d_plan* p, int* nx, int* ny) {
     if (!p || !nx || !ny) return fail(BAD_INDEX, "null argument");
-    *nx = p->nx;
-    *ny = p->ny;
+    *nx = p->fr[0].nx;
+    *ny = p->fr[0].ny;
     return OK;
 }
 

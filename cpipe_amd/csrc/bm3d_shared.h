@@ -18,12 +18,58 @@ constexpr int foot_edge(int t) { return (t - 1) * STRIDE + 2 * R + K; }
 constexpr int FOOT = foot_edge(TILE);  // 91
 constexpr int FOOT2 = FOOT * FOOT;     // largest footprint (16 x 16 refs)
 
-// Arguments of the group kernels (one workgroup per (tile, channel)).
-struct GroupParams {
-    const float* noisy[3];
-    const float* basic[3];
-    float sig[3];
-    int w, h, nx, ny, ntx, nty;
+// Per-frame parameters in device memory (one entry per frame of a plan or
+// batch). Geometry and buffers are fixed at creation; sigma, noise level and
+// the caller's input/output planes are rewritten before every run (one small
+// host-to-device copy), so a captured graph replays with new frames and sigmas.
+struct FrameTab {
+    int w, h, c, nx, ny, ntx, nty, tile_y;
+    int dofs;               // den offset in a tile's partial block (FOOT x foot height)
+    int chk0, nchk;         // this frame's spot checks: report entries [chk0, chk0 + nchk)
+    int pad0;
+    size_t tile_stride;     // floats per (channel, tile) partial block
+    float* z[3];            // noisy opponent planes (pitch w); unused channels alias z[0]
+    float* bz[3];           // basic estimate planes
+    float* bzp[3];          // plain step-1 planes on the checked rectangles (learned pilot)
+    uint32_t* pos[2];       // match lists of steps 1 and 2
+    uint8_t* cnt[2];
+    float* partial;         // per (channel, tile) num/den footprints
+    const int* txlo;        // per column: first / last tile whose footprint covers it
+    const int* txhi;
+    const int* tylo;        // per row
+    const int* tyhi;
+    const int* fx0;         // per tile column / row: footprint bounds
+    const int* fx1;
+    const int* fy0;
+    const int* fy1;
+    int* gate;              // learned pilot: 1 = run the plain step 1 on the whole frame
+    // per run
+    float sig[3];           // per-channel sigma (opponent: sigma * |row|)
+    float sigma;
+    const float* in[3];     // caller's input planes
+    float* out[3];          // caller's output planes
+    size_t in_pitch, out_pitch;
+};
+
+// A rectangle of one frame (merge bands, spot-check blocks).
+struct FrameRect {
+    int f, x0, y0, x1, y1, pad0, pad1, pad2;
+};
+
+// Arguments of the group kernels: one workgroup per (item, channel); item =
+// (frame, tile index ty * ntx + tx).
+struct GroupLaunch {
+    const FrameTab* tab;
+    const int2* items;
+    int use_gate;           // run a frame's groups only if its gate is set
+    int* err;               // set to 1 if an ordering wait timed out (never expected)
+};
+
+// The per-frame scalars a group kernel needs, read once per workgroup (the
+// per-channel planes and sigmas are read from the table by channel index).
+struct GroupView {
+    int w, h, nx, ny, ntx, nty, dofs;
+    size_t tile_stride;
     const uint32_t* pos;
     const uint8_t* cnt;
     const int* fx0;
@@ -31,13 +77,27 @@ struct GroupParams {
     const int* fy0;
     const int* fy1;
     float* partial;
-    int* err;  // set to 1 if an ordering wait timed out (never expected)
-    int dofs;               // den offset in a tile's num/den footprint (FOOT x foot height)
-    size_t tile_stride;     // floats per (channel, tile) partial block
-    int tx0, ty0, ntx_sub;  // tile sub-rectangle processed by this launch
-    const int* tiles;       // or an explicit list of tile indices (ty * ntx + tx)
-    const int* gate;        // null, or run only if *gate != 0
 };
+
+__device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
+    GroupView v;
+    v.w = F.w;
+    v.h = F.h;
+    v.nx = F.nx;
+    v.ny = F.ny;
+    v.ntx = F.ntx;
+    v.nty = F.nty;
+    v.dofs = F.dofs;
+    v.tile_stride = F.tile_stride;
+    v.pos = F.pos[step - 1];
+    v.cnt = F.cnt[step - 1];
+    v.fx0 = F.fx0;
+    v.fx1 = F.fx1;
+    v.fy0 = F.fy0;
+    v.fy1 = F.fy1;
+    v.partial = F.partial;
+    return v;
+}
 
 // Adds one filtered group to a tile's LDS num/den footprint (acc[0, dofs) and
 // acc[dofs, 2 dofs)) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
@@ -153,11 +213,11 @@ __device__ __forceinline__ int floor_pow2(int c) {
 // Mixed-precision group kernels (group_mx.hip). group_mx_tables uploads the
 // 8x8 DCT-II matrix C[u][i] and the 8x8 Kaiser window (row-major, FP32) to the
 // current device, once per device; group_mx_launch enqueues the step-1 or
-// step-2 group kernel with FP16 or BF16 MFMA transforms on the GroupParams
-// launch grid (tiles, channels) for tiles of 16 x tile_y refs (tile_y = 8).
+// step-2 group kernel with FP16 or BF16 MFMA transforms on the launch grid
+// (items, channels) for tiles of 16 x tile_y refs (tile_y = 8 or 16).
 // Both return 0 or a hipError_t.
 int group_mx_tables(int device, const float* dct64, const float* kaiser64);
-int group_mx_launch(int step, int precision, int tile_y, const GroupParams& p, dim3 grid,
+int group_mx_launch(int step, int precision, int tile_y, const GroupLaunch& L, dim3 grid,
                     hipStream_t st);
 
 }  // namespace cpipe_internal

@@ -212,16 +212,17 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm,
 // footprint small enough for three workgroups per CU.
 template <int STEP, int PREC, int TY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-void k_group_mx(GroupParams p) {
+void k_group_mx(GroupLaunch L) {
     typedef Mx<PREC> M;
     constexpr int DOFS = FOOT * foot_edge(TY);
     __shared__ float acc[2 * DOFS];
     __shared__ __align__(16) float scratch[4][MX_SCR * 64];
     __shared__ int turn;  // number of groups of this tile already accumulated
-    if (p.gate != nullptr && *p.gate == 0) return;
-    const int tile = p.tiles ? p.tiles[blockIdx.x]
-                             : (p.ty0 + (int)blockIdx.x / p.ntx_sub) * p.ntx + p.tx0 +
-                                   (int)blockIdx.x % p.ntx_sub;
+    const int2 item = L.items[blockIdx.x];
+    const FrameTab& F = L.tab[item.x];
+    if (L.use_gate && *F.gate == 0) return;
+    const GroupView p = group_view(F, STEP);
+    const int tile = item.y;
     const int tx = tile % p.ntx, ty = tile / p.ntx;
     const int c = blockIdx.y;
     if (threadIdx.x == 0) turn = 0;
@@ -248,9 +249,9 @@ void k_group_mx(GroupParams p) {
                                 kmat(16 * nb + 4 * q + 2, 16 * a + r), kmat(16 * nb + 4 * q + 3, 16 * a + r));
         }
     const float kw = c_mx_kaiser[lane];
-    const float* __restrict__ src = p.noisy[c];
-    const float* __restrict__ bsrc = p.basic[c];
-    const float sc = p.sig[c];
+    const float* __restrict__ src = F.z[c];
+    const float* __restrict__ bsrc = F.bz[c];
+    const float sc = F.sig[c];
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
     const int w = p.w;
@@ -276,7 +277,7 @@ void k_group_mx(GroupParams p) {
         const int lo = (li - fy0) * fw + (lj - fx0);
         timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw, DOFS);
     }
-    if (timed_out && lane == 0) atomicOr(p.err, 1);
+    if (timed_out && lane == 0) atomicOr(L.err, 1);
     __syncthreads();
     float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * p.tile_stride;
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
@@ -314,11 +315,8 @@ int group_mx_tables(int device, const float* dct64, const float* kaiser64) {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_mx_haar), haar, sizeof(haar));
 }
 
-int group_mx_launch(int step, int precision, int tile_y, const GroupParams& p, dim3 grid,
-                    hipStream_t st) {
-    if (tile_y != TILE / 2 || p.dofs != FOOT * foot_edge(TILE / 2)) return hipErrorInvalidValue;
-    constexpr int TY = TILE / 2;
-    const bool bf = precision == CPIPE_BM3D_PRECISION_BF16;
+template <int TY>
+void launch_ty(int step, bool bf, const GroupLaunch& p, dim3 grid, hipStream_t st) {
     if (step == 1) {
         if (bf)
             k_group_mx<1, CPIPE_BM3D_PRECISION_BF16, TY><<<grid, 256, 0, st>>>(p);
@@ -330,6 +328,17 @@ int group_mx_launch(int step, int precision, int tile_y, const GroupParams& p, d
         else
             k_group_mx<2, CPIPE_BM3D_PRECISION_FP16, TY><<<grid, 256, 0, st>>>(p);
     }
+}
+
+int group_mx_launch(int step, int precision, int tile_y, const GroupLaunch& p, dim3 grid,
+                    hipStream_t st) {
+    const bool bf = precision == CPIPE_BM3D_PRECISION_BF16;
+    if (tile_y == TILE / 2)
+        launch_ty<TILE / 2>(step, bf, p, grid, st);
+    else if (tile_y == TILE)
+        launch_ty<TILE>(step, bf, p, grid, st);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -31,7 +31,9 @@ EXPORTS = (
     "cpipe_bm3d_pilot_blob_size", "cpipe_bm3d_pilot_param_count", "cpipe_bm3d_pilot_last_error",
     "cpipe_bm3d_plan_set_timing", "cpipe_bm3d_plan_timing", "cpipe_bm3d_compat_rgba16f",
     "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_node_descriptor",
-    "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d",
+    "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d", "cpipe_bm3d_batch_create",
+    "cpipe_bm3d_batch_destroy", "cpipe_bm3d_batch_run", "cpipe_bm3d_batch_tune",
+    "cpipe_bm3d_batch_config",
 )
 
 
@@ -45,6 +47,22 @@ class PlanDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("width", ctypes.c_int32),
                 ("height", ctypes.c_int32), ("channels", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("n_frames", ctypes.c_int32),
+                ("widths", ctypes.POINTER(ctypes.c_int32)), ("heights", ctypes.POINTER(ctypes.c_int32)),
+                ("channels", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
+TUNE_MAX = 16
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("best", ctypes.c_int32),
+                ("tile_y", ctypes.c_int32 * TUNE_MAX), ("order", ctypes.c_int32 * TUNE_MAX),
+                ("band_rows", ctypes.c_int32 * TUNE_MAX), ("ms", ctypes.c_float * TUNE_MAX)]
 
 
 _lib = None
@@ -80,6 +98,12 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_host_planes.argtypes = [P, S, P, S, I, I, I, F, I, I]
     L.cpipe_bm3d_host_rgba16f.argtypes = [P, S, P, S, I, I, F, I, I, I]
     L.cpipe_bm3d_node_descriptor.restype = P
+    L.cpipe_bm3d_batch_create.argtypes = [ctypes.POINTER(BatchDesc), ctypes.POINTER(P)]
+    L.cpipe_bm3d_batch_destroy.argtypes = [P]
+    L.cpipe_bm3d_batch_destroy.restype = None
+    L.cpipe_bm3d_batch_run.argtypes = [P, P, P, P, P, P, P]
+    L.cpipe_bm3d_batch_tune.argtypes = [P, P, P, P, P, P, I, P, ctypes.POINTER(TuneReport)]
+    L.cpipe_bm3d_batch_config.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
     _lib = L
     return L
 
@@ -229,6 +253,69 @@ class Plan:
             raise ValueError("expected float32 CUDA planes with unit stride along W")
         if t.stride(0) < t.stride(1) * self.height:
             raise ValueError("planes overlap")
+
+
+class Batch(Plan):
+    """Frames of uneven sizes in one plan: one launch per stage over all frames,
+    optionally replayed as one captured hipGraph per run."""
+
+    def __init__(self, sizes, channels: int = 3, precision="fp32", device: int = 0,
+                 use_graph: bool = True):
+        if isinstance(precision, str):
+            precision = _PRECISIONS[precision]
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        self.channels, self.precision, self.device = int(channels), int(precision), int(device)
+        n = len(self.sizes)
+        ws = (ctypes.c_int32 * n)(*[w for w, _ in self.sizes])
+        hs = (ctypes.c_int32 * n)(*[h for _, h in self.sizes])
+        d = BatchDesc(ctypes.sizeof(BatchDesc), n, ws, hs, self.channels, self.precision,
+                      self.device, 1 if use_graph else 0)
+        h = ctypes.c_void_p()
+        _check(lib().cpipe_bm3d_batch_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cpipe_bm3d_batch_destroy(self._h)
+            self._h = None
+
+    def _args(self, srcs, dsts, sigmas):
+        n, C = len(self.sizes), self.channels
+        if len(srcs) != n or len(dsts) != n or len(sigmas) != n:
+            raise ValueError(f"expected {n} frames")
+        ins = (ctypes.c_void_p * (n * C))()
+        outs = (ctypes.c_void_p * (n * C))()
+        ip = (ctypes.c_size_t * n)()
+        op = (ctypes.c_size_t * n)()
+        for f, ((w, h), s, d) in enumerate(zip(self.sizes, srcs, dsts)):
+            for t in (s, d):
+                if tuple(t.shape) != (C, h, w) or t.stride(2) != 1 or not t.is_cuda:
+                    raise ValueError(f"frame {f}: expected ({C}, {h}, {w}) float32 CUDA planes")
+            for c in range(C):
+                ins[f * C + c] = s[c].data_ptr()
+                outs[f * C + c] = d[c].data_ptr()
+            ip[f] = s.stride(1)
+            op[f] = d.stride(1)
+        sg = (ctypes.c_float * n)(*[float(v) for v in sigmas])
+        return ins, ip, outs, op, sg
+
+    def run(self, srcs, dsts, sigmas, stream=None) -> None:
+        ins, ip, outs, op, sg = self._args(srcs, dsts, sigmas)
+        _check(lib().cpipe_bm3d_batch_run(self._h, ins, ip, outs, op, sg, _stream_handle(stream)))
+
+    def tune(self, srcs, dsts, sigmas, iters: int = 2, stream=None):
+        ins, ip, outs, op, sg = self._args(srcs, dsts, sigmas)
+        rep = TuneReport()
+        _check(lib().cpipe_bm3d_batch_tune(self._h, ins, ip, outs, op, sg, int(iters),
+                                           _stream_handle(stream), ctypes.byref(rep)))
+        cands = [{"tile_y": rep.tile_y[i], "order": rep.order[i], "band_rows": rep.band_rows[i],
+                  "ms": round(float(rep.ms[i]), 3)} for i in range(rep.n)]
+        return {"candidates": cands, "best": rep.best}
+
+    def config(self):
+        t, o, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().cpipe_bm3d_batch_config(self._h, ctypes.byref(t), ctypes.byref(o), ctypes.byref(b)))
+        return {"tile_y": t.value, "order": o.value, "band_rows": b.value}
 
 
 def compat_rgba16f(src, dst, sigma: float, stage: int = 1, stream=None) -> None:

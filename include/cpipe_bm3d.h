@@ -176,6 +176,60 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* plan, cpipe_bm3d_pilot* net,
 /* Outcome of the last run's spot checks (synchronises the device). */
 int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* plan, int* n_checked, int* n_fallback,
                                  float* max_delta_db, int* full_fallback);
+/* (full_fallback counts the frames whose gate ran the plain step 1 on the whole frame.) */
+
+/* ---------------------------------------------------------------------------
+ * Batches: frames of uneven sizes in one plan. Every stage is one launch over
+ * a work list spanning all frames (block-matching tiles, aggregation tiles,
+ * pixel bands), each frame with its own sigma; with use_graph the whole batch
+ * is captured once as a hipGraph and each run is one graph launch. No
+ * counterpart in the reference (its node processes one buffer per call,
+ * bm3d.cpp:80-104); the per-frame semantics equal cpipe_bm3d_plan_run.
+ * A batch is a plan: cpipe_bm3d_plan_set_pilot, _set_timing, _timing, _check
+ * and _pilot_report apply to it.
+ * ------------------------------------------------------------------------- */
+typedef struct cpipe_bm3d_plan_s cpipe_bm3d_batch;
+
+typedef struct {
+    uint32_t struct_size;    /* sizeof(cpipe_bm3d_batch_desc) */
+    int32_t n_frames;        /* 1 .. 4096 */
+    const int32_t* widths;   /* n_frames entries */
+    const int32_t* heights;
+    int32_t channels;        /* 1 or 3, all frames */
+    int32_t precision;       /* CPIPE_BM3D_PRECISION_* */
+    int32_t device;
+    int32_t use_graph;       /* 1: replay one captured hipGraph per run */
+} cpipe_bm3d_batch_desc;
+
+int cpipe_bm3d_batch_create(const cpipe_bm3d_batch_desc* desc, cpipe_bm3d_batch** out_batch);
+void cpipe_bm3d_batch_destroy(cpipe_bm3d_batch* batch);
+/* in/out: n_frames * channels device plane pointers, frame-major; pitches (elements)
+ * and sigmas per frame. Asynchronous on `stream`. */
+int cpipe_bm3d_batch_run(cpipe_bm3d_batch* batch, const float* const* in_planes,
+                         const size_t* in_pitch, float* const* out_planes, const size_t* out_pitch,
+                         const float* sigmas, void* stream);
+
+/* Autotuner: times every candidate launch configuration on these frames
+ * (`iters` runs each, after one warm-up), keeps the fastest for the batch and
+ * caches it for later batches of the same shape (frame sizes, channels,
+ * precision). Candidates: aggregation tile height (mixed precision only:
+ * 8 or 16 refs; exact FP32 keeps 16, the CPU restatement's tiling), work-list
+ * order (frame raster, or one contiguous run per XCD) and pixel-band height.
+ * Synchronises the stream. */
+#define CPIPE_BM3D_TUNE_MAX 16
+typedef struct {
+    int32_t n, best;
+    int32_t tile_y[CPIPE_BM3D_TUNE_MAX];
+    int32_t order[CPIPE_BM3D_TUNE_MAX];
+    int32_t band_rows[CPIPE_BM3D_TUNE_MAX];
+    float ms[CPIPE_BM3D_TUNE_MAX];
+} cpipe_bm3d_tune_report;
+int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* batch, const float* const* in_planes,
+                          const size_t* in_pitch, float* const* out_planes, const size_t* out_pitch,
+                          const float* sigmas, int iters, void* stream,
+                          cpipe_bm3d_tune_report* report);
+/* Current launch configuration of a plan or batch. */
+int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* batch, int* tile_y, int* order, int* band_rows);
 
 /* Host-memory conveniences used by the node: copy in, run, copy out, sync. */
 int cpipe_bm3d_host_planes(const float* const* in_planes, size_t in_pitch, float* const* out_planes,
