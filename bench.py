@@ -72,6 +72,78 @@ def cpu_baseline(noisy: np.ndarray, sigma: float, sample_w: int, sample_h: int):
                       f"CPU FP32 BM3D (same algorithm and operation order), {dt:.2f} s"}
 
 
+BATCH_SIZES = [(4080, 3072)] * 4 + [(2040, 1536)] * 4 + [(1024, 768)] * 4
+
+
+def batch_bench(torch, steps, warmup, stream, weights=None):
+    """12 frames of three sizes, per-frame Gaussian sigma ~ U(0.01, 0.1), mixed
+    precision (FP16) + learned pilot: one batch (autotuned, one hipGraph launch
+    per run) against the same frames through per-frame plans."""
+    from cpipe_amd import pilot as pilot_mod
+    rng = np.random.default_rng(4)
+    sigmas = [float(v) for v in rng.uniform(0.01, 0.1, size=len(BATCH_SIZES))]
+    t0 = time.perf_counter()
+    cleans, srcs = [], []
+    for i, (w, h) in enumerate(BATCH_SIZES):
+        clean = synth.clean_scene(w, h, 3, seed=100 + i)
+        noisy = synth.add_gaussian(clean, sigmas[i], seed=200 + i)
+        cleans.append(clean)
+        srcs.append(torch.from_numpy(noisy).cuda())
+    gen_s = time.perf_counter() - t0
+    net = pilot_mod.Pilot(seed=None, weights=weights or pilot_mod.DEFAULT_WEIGHTS)
+    mpx = sum(w * h for w, h in BATCH_SIZES) / 1e6
+    outs = [torch.empty_like(s) for s in srcs]
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n * 1e3
+
+    # per-frame launches: one plan per frame, run one after another
+    plans = []
+    for (w, h) in BATCH_SIZES:
+        p = engine.Plan(w, h, 3, "fp16", 0)
+        p.set_pilot(net)
+        plans.append(p)
+
+    def per_frame():
+        for p, s, o, g in zip(plans, srcs, outs, sigmas):
+            p.run(s, o, g, stream)
+    for _ in range(warmup):
+        per_frame()
+    ms_frames = timed(per_frame, steps)
+    ref = [o.cpu().numpy() for o in outs]
+    for p in plans:
+        p.close()
+
+    res = {"frames": "4x4080x3072 + 4x2040x1536 + 4x1024x768 RGB, Gaussian sigma_f ~ U(0.01, 0.1), "
+                     "fp16 + learned pilot", "mpixel": round(mpx, 2), "generation_s": round(gen_s, 1),
+           "per_frame_launches": {"ms": round(ms_frames, 2), "value": round(mpx / (ms_frames / 1e3), 1)}}
+    for graph in (False, True):
+        b = engine.Batch(BATCH_SIZES, 3, "fp16", 0, use_graph=graph)
+        b.set_pilot(net)
+        tune = b.tune(srcs, outs, sigmas, iters=1, stream=stream) if graph else None
+        for _ in range(max(1, warmup)):
+            b.run(srcs, outs, sigmas, stream)
+        ms = timed(lambda: b.run(srcs, outs, sigmas, stream), steps)
+        b.check()
+        got = [o.cpu().numpy() for o in outs]
+        key = "batch_graph_tuned" if graph else "batch_eager"
+        res[key] = {"ms": round(ms, 2), "value": round(mpx / (ms / 1e3), 1),
+                    "speedup_vs_per_frame": round(ms_frames / ms, 3),
+                    "min_psnr_vs_per_frame_db": round(min(synth.psnr(a, r) for a, r in zip(got, ref)), 2),
+                    "mean_psnr_out_db": round(float(np.mean([synth.psnr(a, c) for a, c in zip(got, cleans)])), 3),
+                    "spot_checks": b.pilot_report()}
+        if tune:
+            res[key]["config"] = b.config()
+            res[key]["tune"] = tune["candidates"]
+        b.close()
+    return res
+
+
 def time_form(torch, plan, src, dst, sigma, steps, warmup, stream):
     plan.set_timing(False)
     for _ in range(warmup):
@@ -100,6 +172,7 @@ def main():
                     help="extra forms besides the plain FP32 headline ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
     ap.add_argument("--cpu-sample", default="2048x1536")
     ap.add_argument("--fp64-crop", default="512x384")
     ap.add_argument("--pilot", default=None,
@@ -226,6 +299,12 @@ def main():
                                    "max_abs_vs_fp64": round(synth.max_abs(co, o64), 5)}
         print(f"# fp64 crop: {json.dumps(fp64)}", file=sys.stderr, flush=True)
 
+    batch = None
+    if not args.no_batch and c == 3:
+        print("# batch of 12 frames ...", file=sys.stderr, flush=True)
+        batch = batch_bench(torch, max(2, args.steps // 3), 1, stream, args.pilot)
+        print(f"# batch: {json.dumps(batch)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         sw, sh = (int(v) for v in args.cpu_sample.split("x"))
@@ -246,6 +325,7 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms,
         "fp64_crop": fp64,
+        "batch12": batch,
     }
     if cpu:
         line["gpu_over_cpu"] = round(mpx / cpu["value"], 1)
