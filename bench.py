@@ -168,8 +168,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rgb12mp")
-    ap.add_argument("--forms", default="fp16,bf16,learned_fp16,learned_fp32",
-                    help="extra forms besides the plain FP32 headline ('' for none)")
+    ap.add_argument("--forms", default="fp16,bf16,fp16_mfmabm,learned_fp16,learned_fp32",
+                    help="extra forms besides the plain FP32 headline ('' for none): precision "
+                         "fp16|bf16|fp32, optional learned_ (pilot) and _mfmabm (MFMA block matching)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
@@ -240,14 +241,21 @@ def main():
         from cpipe_amd import pilot as pilot_mod
         blob = args.pilot or pilot_mod.DEFAULT_WEIGHTS
         net = pilot_mod.Pilot(seed=None, weights=blob)
+    def make_plan(name, pw, ph):
+        toks = name.split("_")
+        prec = next(t for t in toks if t in ("fp32", "fp16", "bf16"))
+        pl = engine.Plan(pw, ph, c, prec, 0)
+        if "mfmabm" in toks:
+            pl.set_block_matching("mfma")
+        if "learned" in toks:
+            pl.set_pilot(net)
+        return pl, prec
+
     for name in names:
         learned = name.startswith("learned")
         if learned and net is None:
             continue
-        prec = name.split("_")[-1]
-        fplan = engine.Plan(w, h, c, prec, 0)
-        if learned:
-            fplan.set_pilot(net)
+        fplan, prec = make_plan(name, w, h)
         fdst = torch.empty_like(src)
         fms, fstage = time_form(torch, fplan, src, fdst, sigma, args.steps, args.warmup, stream)
         fout = fdst.cpu().numpy()
@@ -284,10 +292,7 @@ def main():
             learned = name.startswith("learned")
             if learned and net is None:
                 continue
-            prec = name.split("_")[-1]
-            cplan = engine.Plan(cw, ch, c, prec, 0)
-            if learned:
-                cplan.set_pilot(net)
+            cplan, _ = make_plan(name, cw, ch)
             cdst = torch.empty_like(csrc)
             cplan.run(csrc, cdst, sigma)
             torch.cuda.synchronize()

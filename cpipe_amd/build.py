@@ -76,16 +76,18 @@ def build(verbose: bool = False) -> dict:
     dev_o = os.path.join(BUILD, "bm3d_device.o")
     pilot_o = os.path.join(BUILD, "pilot.o")
     mx_o = os.path.join(BUILD, "group_mx.o")
+    bmx_o = os.path.join(BUILD, "bm_mx.o")
     node_o = os.path.join(BUILD, "node.o")
     man_o = os.path.join(BUILD, "manifest_gen.o")
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm3d_device.hip"), "-o", dev_o],
          verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "pilot.hip"), "-o", pilot_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "group_mx.hip"), "-o", mx_o], verbose)
+    _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm_mx.hip"), "-o", bmx_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
     tmp = ENGINE_SO + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, node_o,
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, node_o,
           man_o], verbose)
     os.replace(tmp, ENGINE_SO)
     tmp = MINIHOST_SO + ".tmp"

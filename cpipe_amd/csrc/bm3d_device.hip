@@ -893,6 +893,7 @@ struct cpipe_bm3d_plan_s {
     size_t tile_stride = 2 * (size_t)FOOT2;
     int order = 0;                   // work-list order: 0 frame raster, 1 XCD-contiguous chunks
     int band_rows = 32;              // rows per colour / merge rectangle
+    int bm_mfma = 0;                 // mixed precision: MFMA cross-term block matching (else exact FP32)
     std::vector<Frame> fr;
     std::vector<FrameTab> tab;       // host image of the device frame table
     FrameTab* tab_dev = nullptr;     // followed by one float noise level per frame
@@ -1288,7 +1289,12 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
     int rc;
     if ((rc = rt.mark(sbm, false))) return rc;
     if (L.n_bm > 0) {
-        k_block_match<<<L.n_bm, 256, 0, r.st>>>(p->tab_dev, L.bm, step, tau_ssd(step), use_gate);
+        if (p->bm_mfma) {
+            HIP_CHECK((hipError_t)bm_mx_launch(p->precision, p->tab_dev, L.bm, L.n_bm, step,
+                                               tau_ssd(step), use_gate, r.st));
+        } else {
+            k_block_match<<<L.n_bm, 256, 0, r.st>>>(p->tab_dev, L.bm, step, tau_ssd(step), use_gate);
+        }
         HIP_CHECK(hipGetLastError());
     }
     if ((rc = rt.mark(sbm, true))) return rc;
@@ -1509,7 +1515,7 @@ struct TuneKey {
     }
 };
 struct TuneChoice {
-    int tile_y, order, band_rows;
+    int tile_y, order, band_rows, bm_mfma;
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneChoice> g_tune_cache;
@@ -1527,6 +1533,7 @@ int apply_choice(cpipe_bm3d_plan* p, const TuneChoice& ch) {
     const bool reshape = ch.tile_y != p->tile_y;
     p->order = ch.order;
     p->band_rows = ch.band_rows;
+    p->bm_mfma = ch.bm_mfma;
     drop_graph(p);
     int rc = reshape ? set_tile_shape(p, ch.tile_y) : build_full_lists(p);
     if (rc) return rc;
@@ -1648,11 +1655,15 @@ int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* b, const float* const* in, const siz
     b->use_graph = false;
     b->timing = false;
     std::vector<TuneChoice> cands;
-    std::vector<int> tys = {b->tile_y};
-    if (b->precision != CPIPE_BM3D_PRECISION_FP32) tys = {TILE / 2, TILE};
-    for (int ty : tys)
-        for (int order : {0, 1})
-            for (int rows : {16, 64}) cands.push_back({ty, order, rows});
+    std::vector<int> tys = {b->tile_y}, bms = {0};
+    if (b->precision != CPIPE_BM3D_PRECISION_FP32) {
+        tys = {TILE / 2, TILE};
+        bms = {0, 1};
+    }
+    for (int bm : bms)
+        for (int ty : tys)
+            for (int order : {0, 1})
+                for (int rows : {16, 64}) cands.push_back({ty, order, rows, bm});
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -1677,6 +1688,7 @@ int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* b, const float* const* in, const siz
             report->tile_y[k] = ch.tile_y;
             report->order[k] = ch.order;
             report->band_rows[k] = ch.band_rows;
+            report->bm_mfma[k] = ch.bm_mfma;
             report->ms[k] = ms;
         }
         ++k;
@@ -1697,7 +1709,7 @@ int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* b, const float* const* in, const siz
         report->best = -1;
         for (int i = 0; i < report->n; ++i)
             if (report->tile_y[i] == best.tile_y && report->order[i] == best.order &&
-                report->band_rows[i] == best.band_rows)
+                report->band_rows[i] == best.band_rows && report->bm_mfma[i] == best.bm_mfma)
                 report->best = i;
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1705,11 +1717,26 @@ int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* b, const float* const* in, const siz
     return OK;
 }
 
-int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* b, int* tile_y, int* order, int* band_rows) {
+int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* b, int* tile_y, int* order, int* band_rows,
+                            int* bm_mfma) {
     if (!b) return fail(BAD_INDEX, "null batch");
     if (tile_y) *tile_y = b->tile_y;
     if (order) *order = b->order;
     if (band_rows) *band_rows = b->band_rows;
+    if (bm_mfma) *bm_mfma = b->bm_mfma;
+    return OK;
+}
+
+int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* p, int mfma) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    if (mfma && p->precision == CPIPE_BM3D_PRECISION_FP32)
+        return fail(UNSUPPORTED, "MFMA block matching needs a mixed-precision plan");
+    if ((p->bm_mfma != 0) != (mfma != 0)) {
+        DeviceGuard guard(p->device);
+        HIP_CHECK(hipDeviceSynchronize());
+        drop_graph(p);
+    }
+    p->bm_mfma = mfma != 0;
     return OK;
 }
 

@@ -220,4 +220,11 @@ int group_mx_tables(int device, const float* dct64, const float* kaiser64);
 int group_mx_launch(int step, int precision, int tile_y, const GroupLaunch& L, dim3 grid,
                     hipStream_t st);
 
+// Mixed-precision block matching (bm_mx.hip): FP16 or BF16 MFMA cross terms,
+// exact FP32 re-rank of a 48-candidate shortlist per ref. Same work items as
+// the exact kernel (frame, 16-ref column block, 14-ref row block). Returns 0
+// or a hipError_t.
+int bm_mx_launch(int precision, const FrameTab* tab, const int4* items, int n_items, int step,
+                 float tau, int use_gate, hipStream_t st);
+
 }  // namespace cpipe_internal

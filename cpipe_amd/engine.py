@@ -33,7 +33,7 @@ EXPORTS = (
     "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_node_descriptor",
     "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d", "cpipe_bm3d_batch_create",
     "cpipe_bm3d_batch_destroy", "cpipe_bm3d_batch_run", "cpipe_bm3d_batch_tune",
-    "cpipe_bm3d_batch_config",
+    "cpipe_bm3d_batch_config", "cpipe_bm3d_plan_set_block_matching",
 )
 
 
@@ -56,13 +56,14 @@ class BatchDesc(ctypes.Structure):
                 ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
 
 
-TUNE_MAX = 16
+TUNE_MAX = 32
 
 
 class TuneReport(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("best", ctypes.c_int32),
                 ("tile_y", ctypes.c_int32 * TUNE_MAX), ("order", ctypes.c_int32 * TUNE_MAX),
-                ("band_rows", ctypes.c_int32 * TUNE_MAX), ("ms", ctypes.c_float * TUNE_MAX)]
+                ("band_rows", ctypes.c_int32 * TUNE_MAX), ("bm_mfma", ctypes.c_int32 * TUNE_MAX),
+                ("ms", ctypes.c_float * TUNE_MAX)]
 
 
 _lib = None
@@ -103,7 +104,9 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_batch_destroy.restype = None
     L.cpipe_bm3d_batch_run.argtypes = [P, P, P, P, P, P, P]
     L.cpipe_bm3d_batch_tune.argtypes = [P, P, P, P, P, P, I, P, ctypes.POINTER(TuneReport)]
-    L.cpipe_bm3d_batch_config.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
+    L.cpipe_bm3d_batch_config.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
+                                          ctypes.POINTER(I)]
+    L.cpipe_bm3d_plan_set_block_matching.argtypes = [P, I]
     _lib = L
     return L
 
@@ -224,6 +227,10 @@ class Plan:
                                                ctypes.byref(opts)))
         self._pilot = net  # the plan references the network
 
+    def set_block_matching(self, kind: str) -> None:
+        """Mixed-precision plans: "mfma" (cross term on MFMA + FP32 re-rank) or "exact"."""
+        _check(lib().cpipe_bm3d_plan_set_block_matching(self._h, 1 if kind == "mfma" else 0))
+
     def pilot_report(self):
         nc, nf, full = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         md = ctypes.c_float()
@@ -309,13 +316,16 @@ class Batch(Plan):
         _check(lib().cpipe_bm3d_batch_tune(self._h, ins, ip, outs, op, sg, int(iters),
                                            _stream_handle(stream), ctypes.byref(rep)))
         cands = [{"tile_y": rep.tile_y[i], "order": rep.order[i], "band_rows": rep.band_rows[i],
-                  "ms": round(float(rep.ms[i]), 3)} for i in range(rep.n)]
+                  "bm": "mfma" if rep.bm_mfma[i] else "exact", "ms": round(float(rep.ms[i]), 3)}
+                 for i in range(rep.n)]
         return {"candidates": cands, "best": rep.best}
 
     def config(self):
-        t, o, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(lib().cpipe_bm3d_batch_config(self._h, ctypes.byref(t), ctypes.byref(o), ctypes.byref(b)))
-        return {"tile_y": t.value, "order": o.value, "band_rows": b.value}
+        t, o, b, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().cpipe_bm3d_batch_config(self._h, ctypes.byref(t), ctypes.byref(o), ctypes.byref(b),
+                                             ctypes.byref(m)))
+        return {"tile_y": t.value, "order": o.value, "band_rows": b.value,
+                "bm": "mfma" if m.value else "exact"}
 
 
 def compat_rgba16f(src, dst, sigma: float, stage: int = 1, stream=None) -> None:

@@ -212,16 +212,19 @@ int cpipe_bm3d_batch_run(cpipe_bm3d_batch* batch, const float* const* in_planes,
 /* Autotuner: times every candidate launch configuration on these frames
  * (`iters` runs each, after one warm-up), keeps the fastest for the batch and
  * caches it for later batches of the same shape (frame sizes, channels,
- * precision). Candidates: aggregation tile height (mixed precision only:
- * 8 or 16 refs; exact FP32 keeps 16, the CPU restatement's tiling), work-list
- * order (frame raster, or one contiguous run per XCD) and pixel-band height.
+ * precision). Candidates: aggregation tile height and block-matching kernel
+ * (mixed precision only: 8 or 16 refs, MFMA or exact block matching; exact
+ * FP32 keeps 16-ref tiles, the CPU restatement's tiling, and exact matching),
+ * work-list order (frame raster, or one contiguous run per XCD) and pixel-band
+ * height.
  * Synchronises the stream. */
-#define CPIPE_BM3D_TUNE_MAX 16
+#define CPIPE_BM3D_TUNE_MAX 32
 typedef struct {
     int32_t n, best;
     int32_t tile_y[CPIPE_BM3D_TUNE_MAX];
     int32_t order[CPIPE_BM3D_TUNE_MAX];
     int32_t band_rows[CPIPE_BM3D_TUNE_MAX];
+    int32_t bm_mfma[CPIPE_BM3D_TUNE_MAX];
     float ms[CPIPE_BM3D_TUNE_MAX];
 } cpipe_bm3d_tune_report;
 int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* batch, const float* const* in_planes,
@@ -229,7 +232,15 @@ int cpipe_bm3d_batch_tune(cpipe_bm3d_batch* batch, const float* const* in_planes
                           const float* sigmas, int iters, void* stream,
                           cpipe_bm3d_tune_report* report);
 /* Current launch configuration of a plan or batch. */
-int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* batch, int* tile_y, int* order, int* band_rows);
+int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* batch, int* tile_y, int* order, int* band_rows,
+                            int* bm_mfma);
+
+/* Mixed-precision plans: block matching with the FP16/BF16 MFMA cross term and
+ * an exact FP32 re-rank of a 48-candidate shortlist per ref (mfma = 1), or the
+ * exact FP32 kernel (mfma = 0, the default). Both produce the exact kernel's
+ * match lists whenever the shortlist holds the exact best 15 (always, on the
+ * test images); the autotuner times both. CPIPE_BM3D_UNSUPPORTED for FP32 plans. */
+int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* plan, int mfma);
 
 /* Host-memory conveniences used by the node: copy in, run, copy out, sync. */
 int cpipe_bm3d_host_planes(const float* const* in_planes, size_t in_pitch, float* const* out_planes,

@@ -113,6 +113,7 @@ def test_tuning_keeps_exact_results(cuda):
     rep = b.tune(srcs, outs, sigmas, iters=1)
     print(rep, b.config())
     assert len(rep["candidates"]) == 4 and 0 <= rep["best"] < 4
+    assert all(c["bm"] == "exact" for c in rep["candidates"])
     assert all(c["tile_y"] == 16 for c in rep["candidates"])
     for _ in range(2):
         outs = [torch.empty_like(s) for s in srcs]
@@ -141,6 +142,7 @@ def test_tuning_mixed_tile_shapes(cuda):
     rep = b.tune(srcs, outs, sigmas, iters=1)
     print(rep, b.config())
     assert sorted({c["tile_y"] for c in rep["candidates"]}) == [8, 16]
+    assert sorted({c["bm"] for c in rep["candidates"]}) == ["exact", "mfma"]
     outs = [torch.empty_like(s) for s in srcs]
     b.run(srcs, outs, sigmas)
     torch.cuda.synchronize()

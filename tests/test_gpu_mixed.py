@@ -2,9 +2,10 @@
 
 Bar: the output's PSNR against the clean image within 0.1 dB of the FP64
 textbook oracle's (both steps), PSNR >= 40 dB (FP16) / 34 dB (BF16) against
-the exact FP32 path, match lists of step 1 identical to the FP32 path (block
-matching is exact FP32 in every precision), and bit-identical output on a
-repeated run (fixed aggregation order).
+the exact FP32 path, step-1 match lists equal to the exact FP32 path's for
+>= 99.5 % (FP16) / 98 % (BF16) of the refs (MFMA cross term, exact FP32
+re-rank of the shortlist), and bit-identical output on a repeated run (fixed
+aggregation order).
 """
 import numpy as np
 import pytest
@@ -15,14 +16,17 @@ pytestmark = pytest.mark.gpu
 
 MIN_PSNR_VS_FP32 = {"fp16": 40.0, "bf16": 34.0}
 MAX_DELTA_VS_FP64_DB = 0.1
+MIN_LIST_AGREEMENT = {"fp16": 0.995, "bf16": 0.98}
 
 
-def _run(torch, noisy, sigma, precision):
+def _run(torch, noisy, sigma, precision, bm="mfma"):
     c, h, w = noisy.shape
     src = torch.from_numpy(np.ascontiguousarray(noisy)).to("cuda:0")
     out = torch.empty_like(src)
     basic = torch.empty_like(src)
     plan = engine.Plan(w, h, c, precision, 0)
+    if precision != "fp32":
+        plan.set_block_matching(bm)
     plan.run(src, out, sigma)
     plan.basic(basic)
     p1, c1 = plan.matches(1)
@@ -42,8 +46,9 @@ def test_mixed_close_to_fp32_and_fp64(cuda, oracle_lib, precision, name, seed):
     out, basic, p1, c1 = _run(torch, noisy, sigma, precision)
     ref, rbasic, rp1, rc1 = _run(torch, noisy, sigma, "fp32")
     assert np.all(np.isfinite(out))
-    np.testing.assert_array_equal(c1, rc1)
-    np.testing.assert_array_equal(p1, rp1)
+    same = np.all(p1 == rp1, axis=1) & (c1 == rc1)
+    print(f"step-1 match lists equal to the exact path: {same.mean():.5f}")
+    assert same.mean() >= MIN_LIST_AGREEMENT[precision]
     vs32 = synth.psnr(out, ref)
     vs32_basic = synth.psnr(basic, rbasic)
     o64 = oracle_lib.bm3d_f64(noisy, sigma)
@@ -51,6 +56,20 @@ def test_mixed_close_to_fp32_and_fp64(cuda, oracle_lib, precision, name, seed):
     print(f"psnr vs fp32 {vs32:.2f} dB (basic {vs32_basic:.2f}), delta vs fp64 {d64:+.4f} dB")
     assert vs32 >= MIN_PSNR_VS_FP32[precision]
     assert abs(d64) <= MAX_DELTA_VS_FP64_DB
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("sigma", [0.01, 0.03, 0.1])
+def test_mixed_block_matching_agrees_with_exact(cuda, precision, sigma):
+    """Low and high noise: the shortlist must still hold the exact top 15."""
+    torch = cuda
+    clean = synth.clean_scene(320, 240, 3, seed=11)
+    noisy = synth.add_gaussian(clean, sigma, seed=12)
+    _, _, p1, c1 = _run(torch, noisy, sigma, precision)
+    _, _, rp1, rc1 = _run(torch, noisy, sigma, "fp32")
+    same = np.all(p1 == rp1, axis=1) & (c1 == rc1)
+    print(f"{precision} sigma {sigma}: lists equal for {same.mean():.5f} of {len(same)} refs")
+    assert same.mean() >= MIN_LIST_AGREEMENT[precision]
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
