@@ -27,7 +27,8 @@ typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int TW = 64;  // output tile width (pixels) per workgroup
-constexpr int TH = 4;   // output tile height: one row per wave
+constexpr int TH = 4;   // weight-gradient tile height
+constexpr int CTH = 8;  // convolution output tile height: two rows per wave
 
 // Layer geometry (padded channel counts).
 constexpr int NL = 6;
@@ -68,9 +69,11 @@ __global__ void k_pack_input(const float* __restrict__ y, const float* __restric
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution, stride 1, zero padding, on MFMA 16x16x16 (FP16 -> FP32).
-// Workgroup: 64x4 output pixels, 4 waves (one output row each), 4 segments of
-// 16 pixels per wave. The input tile (66x6 pixels) is staged in LDS with a
-// padded pixel pitch (CIN + 4 halves) to spread banks.
+// Workgroup: 64x8 output pixels, 4 waves (two output rows each), 4 segments
+// of 16 pixels per row. The input tile (66x10 pixels) is staged in LDS with a
+// padded pixel pitch (CIN + 4 halves) to spread banks; weights are loaded as
+// pre-arranged B fragments; outputs leave through a per-wave LDS buffer as
+// 16-byte stores.
 // MFMA operand layouts (wave64, 16x16x16): A[i][k] at lane i + 16*(k/4),
 // element k%4; B[k][j] at lane j + 16*(k/4), element k%4; D[i][j] at lane
 // j + 16*(i/4), element i%4.
@@ -81,7 +84,7 @@ __global__ void k_pack_input(const float* __restrict__ y, const float* __restric
 
 struct ConvArgs {
     const _Float16* in;    // NHWC, channel stride CIN, images stacked
-    const _Float16* wt;    // FP16 weights [tap][cin][cout] (forward layout)
+    const half4* frag;     // weights as B fragments [tap][kb][nb][lane] (see k_frag)
     const float* bias;     // [cout] (EPI 0/1)
     _Float16* out;         // NHWC, channel stride COUT (EPI 0/2)
     const _Float16* mask;  // NHWC, channel stride COUT (EPI 2)
@@ -90,19 +93,53 @@ struct ConvArgs {
     size_t pitch;          // plane row pitch (EPI 1)
     size_t img_stride_planes;
     int w, h;
-    int transposed;  // 1: use weights as the backward (flipped, cin<->cout) operator
 };
+
+// B fragments of a layer's weights, in the order the convolution reads them:
+// frag[((t * KB + kb) * NB + nb) * 64 + lane] = 4 halves, so each fragment is
+// one coalesced 512-byte load per wave. Forward operator (CI -> CO):
+// B[k = cin][j = cout] = W[t][k][j]; backward operator (CO -> CI, flipped):
+// B[k = cout][j = cin] = W[8 - t][j][k].
+template <int CI, int CO>
+__global__ void k_frag(const _Float16* __restrict__ w16, half4* __restrict__ fwd,
+                       half4* __restrict__ bwd) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = idx & 63, lr = lane & 15, lq = lane >> 4;
+    {
+        constexpr int KB = CI / 16, NB = CO / 16;
+        if (idx < 9 * KB * NB * 64) {
+            const int f = idx >> 6, nb = f % NB, kb = (f / NB) % KB, t = f / (NB * KB);
+            half4 v;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                v[e] = w16[((size_t)t * CI + kb * 16 + lq * 4 + e) * CO + nb * 16 + lr];
+            fwd[idx] = v;
+        }
+    }
+    {
+        constexpr int KB = CO / 16, NB = CI / 16;
+        if (idx < 9 * KB * NB * 64) {
+            const int f = idx >> 6, nb = f % NB, kb = (f / NB) % KB, t = f / (NB * KB);
+            half4 v;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                v[e] = w16[((size_t)(8 - t) * CI + nb * 16 + lr) * CO + kb * 16 + lq * 4 + e];
+            bwd[idx] = v;
+        }
+    }
+}
 
 template <int CI, int CO, int EPI>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int PS = CI + 4;  // LDS pixel pitch in halves
-    __shared__ __align__(16) _Float16 tile[(TH + 2) * (TW + 2) * PS];
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, img = blockIdx.z;
+    __shared__ __align__(16) _Float16 tile[(CTH + 2) * (TW + 2) * PS];
+    __shared__ __align__(16) _Float16 ob[4][16 * CO];  // per wave: 16 output pixels x CO
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
     const int w = a.w, h = a.h;
     const _Float16* in = a.in + (size_t)img * w * h * CI;
-    // stage (TW+2) x (TH+2) pixels, CI channels, 8 bytes per thread-step
+    // stage (TW+2) x (CTH+2) pixels, CI channels, 8 bytes per thread-step
     constexpr int QPP = CI / 4;  // quads per pixel
-    for (int idx = threadIdx.x; idx < (TH + 2) * (TW + 2) * QPP; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < (CTH + 2) * (TW + 2) * QPP; idx += blockDim.x) {
         const int q = idx % QPP, pix = idx / QPP;
         const int ty = pix / (TW + 2), tx = pix - ty * (TW + 2);
         const int gx = x0 + tx - 1, gy = y0 + ty - 1;
@@ -111,8 +148,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
             v = *reinterpret_cast<const half4*>(in + ((size_t)gy * w + gx) * CI + q * 4);
         *reinterpret_cast<half4*>(&tile[pix * PS + q * 4]) = v;
     }
-    // weights as B fragments: B[k = cin][j = cout] for tap t (forward), or
-    // B[k = cin'][j = cout'] = W[8 - t][j][k] for the backward operator
+    // weights as B fragments (forward or backward operator, see k_frag)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     constexpr int KB = CI / 16, NB = CO / 16;
@@ -122,60 +158,81 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                half4 f;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int k = kb * 16 + lq * 4 + e, j = nb * 16 + lr;
-                    // forward weights are [tap][CI][CO]; the backward operator reads
-                    // the forward tensor of shape [tap][CO][CI] flipped in the taps
-                    f[e] = a.transposed ? a.wt[((size_t)(8 - t) * CO + j) * CI + k]
-                                        : a.wt[((size_t)t * CI + k) * CO + j];
-                }
-                bw[t][kb][nb] = f;
-            }
+            for (int nb = 0; nb < NB; ++nb) bw[t][kb][nb] = a.frag[((t * KB + kb) * NB + nb) * 64 + lane];
     __syncthreads();
+    _Float16* o16 = ob[wave];
 
-    const int oy = y0 + wave;
 #pragma unroll 1
-    for (int seg = 0; seg < TW / 16; ++seg) {
-        float4v acc[NB];
+    for (int rr = 0; rr < CTH / 4; ++rr) {
+        const int ly = wave + 4 * rr, oy = y0 + ly;  // this wave's output row
+#pragma unroll 1
+        for (int seg = 0; seg < TW / 16; ++seg) {
+            float4v acc[NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4v){0.f, 0.f, 0.f, 0.f};
+            for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4v){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int ky = t / 3, kx = t % 3;
-            const _Float16* row = &tile[((wave + ky) * (TW + 2) + seg * 16 + lr + kx) * PS];
+            for (int t = 0; t < 9; ++t) {
+                const int ky = t / 3, kx = t % 3;
+                const _Float16* row = &tile[((ly + ky) * (TW + 2) + seg * 16 + lr + kx) * PS];
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                const half4 av = *reinterpret_cast<const half4*>(row + kb * 16 + lq * 4);
+                for (int kb = 0; kb < KB; ++kb) {
+                    const half4 av = *reinterpret_cast<const half4*>(row + kb * 16 + lq * 4);
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+                }
             }
-        }
-        if (oy >= h) continue;
+            if (oy >= h) continue;  // uniform per wave
+            if (EPI == 1) {
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int co = nb * 16 + lr;
+                for (int nb = 0; nb < NB; ++nb) {
+                    const int co = nb * 16 + lr;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int ox = x0 + seg * 16 + lq * 4 + e;
-                if (ox >= w) continue;
-                const size_t p = (size_t)img * w * h + (size_t)oy * w + ox;
-                if (EPI == 0) {
-                    const float v = acc[nb][e] + a.bias[co];
-                    a.out[p * CO + co] = (_Float16)(v > 0.0f ? v : 0.0f);
-                } else if (EPI == 1) {
-                    if (co < 3) {
+                    for (int e = 0; e < 4; ++e) {
+                        const int ox = x0 + seg * 16 + lq * 4 + e;
+                        if (ox >= w || co >= 3) continue;
                         const size_t q = img * a.img_stride_planes + (size_t)oy * a.pitch + ox;
                         a.outp[co][q] = a.res[co][q] + (acc[nb][e] + a.bias[co]);
                     }
-                } else {
-                    const float m = (float)a.mask[p * CO + co];
-                    a.out[p * CO + co] = (_Float16)(m > 0.0f ? acc[nb][e] : 0.0f);
+                }
+                continue;
+            }
+            // D[i = pixel][j = channel] (lane lr + 16 lq, element e: pixel 4 lq + e):
+            // through the wave's LDS buffer, then one 16-byte store per lane
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + lr;
+                const float b = EPI == 0 ? a.bias[co] : 0.0f;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float v = acc[nb][e] + b;
+                    if (EPI == 0) v = v > 0.0f ? v : 0.0f;
+                    o16[(lq * 4 + e) * CO + co] = (_Float16)v;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+            constexpr int CH = CO / 8;  // 16-byte chunks per pixel
+#pragma unroll
+            for (int k = lane; k < 16 * CH; k += 64) {
+                const int px = k / CH, ch = k % CH;
+                const int ox = x0 + seg * 16 + px;
+                if (ox < w) {
+                    const size_t p = (size_t)img * w * h + (size_t)oy * w + ox;
+                    half8 v = *reinterpret_cast<const half8*>(o16 + px * CO + ch * 8);
+                    if (EPI == 2) {
+                        const half8 m = *reinterpret_cast<const half8*>(a.mask + p * CO + ch * 8);
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) v[e] = (float)m[e] > 0.0f ? v[e] : (_Float16)0.0f;
+                    }
+                    *reinterpret_cast<half8*>(a.out + p * CO + ch * 8) = v;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }
@@ -382,6 +439,8 @@ struct cpipe_bm3d_pilot_s {
     float* v = nullptr;
     float* g = nullptr;
     _Float16* w16 = nullptr;
+    half4* frag = nullptr;  // per layer: forward then backward B fragments (k_frag)
+    size_t frag_off[NL][2] = {};
     uint8_t* live = nullptr;
     uint64_t step = 0;
     uint64_t seed = 0;
@@ -459,9 +518,23 @@ int ensure_buf(float** p, size_t* cap, size_t n) {
     return 0;
 }
 
+// Rebuilds every layer's B fragments from the FP16 weights (after each change).
+int refresh_frags(cpipe_bm3d_pilot* n, hipStream_t st) {
+    const int blocks[NL] = {(int)((layer_w(0) / 4 + 255) / 256), (int)((layer_w(1) / 4 + 255) / 256),
+                            (int)((layer_w(2) / 4 + 255) / 256), (int)((layer_w(3) / 4 + 255) / 256),
+                            (int)((layer_w(4) / 4 + 255) / 256), (int)((layer_w(5) / 4 + 255) / 256)};
+    auto f = [&](int l) { return n->frag + n->frag_off[l][0]; };
+    auto b = [&](int l) { return n->frag + n->frag_off[l][1]; };
+    k_frag<16, 32><<<blocks[0], 256, 0, st>>>(n->w16 + n->off[0], f(0), b(0));
+    for (int l = 1; l < NL - 1; ++l) k_frag<32, 32><<<blocks[l], 256, 0, st>>>(n->w16 + n->off[l], f(l), b(l));
+    k_frag<32, 16><<<blocks[NL - 1], 256, 0, st>>>(n->w16 + n->off[NL - 1], f(NL - 1), b(NL - 1));
+    PHIP(hipGetLastError());
+    return 0;
+}
+
 template <int CI, int CO, int EPI>
 int launch_conv(const ConvArgs& a, int n_img, hipStream_t st) {
-    const dim3 grid((a.w + TW - 1) / TW, (a.h + TH - 1) / TH, n_img);
+    const dim3 grid((a.w + TW - 1) / TW, (a.h + CTH - 1) / CTH, n_img);
     k_conv3x3<CI, CO, EPI><<<grid, 256, 0, st>>>(a);
     PHIP(hipGetLastError());
     return 0;
@@ -483,7 +556,7 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
     a.img_stride_planes = img_stride;
     // L1
     a.in = n->act[0];
-    a.wt = n->w16 + n->off[0];
+    a.frag = n->frag + n->frag_off[0][0];
     a.bias = n->w32 + n->off[0] + layer_w(0);
     a.out = n->act[1];
     if ((rc = launch_conv<16, 32, 0>(a, n_img, st))) return rc;
@@ -491,14 +564,14 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
     for (int l = 1; l < NL - 1; ++l) {
         const int nxt = train ? l + 1 : (cur == 1 ? 2 : 1);
         a.in = n->act[cur];
-        a.wt = n->w16 + n->off[l];
+        a.frag = n->frag + n->frag_off[l][0];
         a.bias = n->w32 + n->off[l] + layer_w(l);
         a.out = n->act[nxt];
         if ((rc = launch_conv<32, 32, 0>(a, n_img, st))) return rc;
         cur = nxt;
     }
     a.in = n->act[cur];
-    a.wt = n->w16 + n->off[NL - 1];
+    a.frag = n->frag + n->frag_off[NL - 1][0];
     a.bias = n->w32 + n->off[NL - 1] + layer_w(NL - 1);
     for (int c = 0; c < 3; ++c) {
         a.res[c] = in[c];
@@ -547,9 +620,8 @@ int backward(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tg
     ConvArgs a{};
     a.w = w;
     a.h = h;
-    a.transposed = 1;
     a.in = n->dyb[0];
-    a.wt = n->w16 + n->off[NL - 1];
+    a.frag = n->frag + n->frag_off[NL - 1][1];
     a.mask = n->act[NL - 1];
     a.out = n->dyb[1];
     if ((rc = launch_conv<16, 32, 2>(a, n_img, st))) return rc;
@@ -557,7 +629,7 @@ int backward(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tg
     for (int l = NL - 2; l >= 1; --l) {
         if ((rc = wgrad<32, 32>(n, l, n->act[l], n->dyb[cur], w, h, n_img, scale, st))) return rc;
         a.in = n->dyb[cur];
-        a.wt = n->w16 + n->off[l];
+        a.frag = n->frag + n->frag_off[l][1];
         a.mask = n->act[l];
         a.out = n->dyb[1 - cur];
         if ((rc = launch_conv<32, 32, 2>(a, n_img, st))) return rc;
@@ -629,6 +701,15 @@ int cpipe_bm3d_pilot_create(int device, cpipe_bm3d_pilot** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&n->g, o * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&n->w16, o * sizeof(_Float16));
     if (e == hipSuccess) e = hipMalloc((void**)&n->live, o);
+    size_t fo = 0;  // fragments: 9 * CI * CO / 4 half4 per operator
+    for (int l = 0; l < NL; ++l) {
+        n->frag_off[l][0] = fo;
+        fo += layer_w(l) / 4;
+        n->frag_off[l][1] = fo;
+        fo += layer_w(l) / 4;
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&n->frag, fo * sizeof(half4));
+    if (e == hipSuccess) e = hipMemset(n->frag, 0, fo * sizeof(half4));
     if (e == hipSuccess) e = hipMemcpy(n->live, n->live_host.data(), o, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(n->w32, 0, o * sizeof(float));
     if (e == hipSuccess) e = hipMemset(n->m, 0, o * sizeof(float));
@@ -648,7 +729,7 @@ void cpipe_bm3d_pilot_destroy(cpipe_bm3d_pilot* n) {
     (void)hipDeviceSynchronize();
     free_ws(n);
     for (void* p : {(void*)n->w32, (void*)n->m, (void*)n->v, (void*)n->g, (void*)n->w16,
-                    (void*)n->live, (void*)n->partial, (void*)n->sq, (void*)n->levels})
+                    (void*)n->frag, (void*)n->live, (void*)n->partial, (void*)n->sq, (void*)n->levels})
         (void)hipFree(p);
     delete n;
 }
@@ -675,6 +756,7 @@ int cpipe_bm3d_pilot_init(cpipe_bm3d_pilot* n, uint64_t seed) {
     PHIP(hipMemset(n->v, 0, n->nparam * sizeof(float)));
     k_to_half<<<(unsigned)((n->nparam + 255) / 256), 256>>>(n->w32, n->w16, (int)n->nparam);
     PHIP(hipGetLastError());
+    if (int rc = refresh_frags(n, nullptr)) return rc;
     PHIP(hipDeviceSynchronize());
     return 0;
 }
@@ -724,6 +806,7 @@ int cpipe_bm3d_pilot_load(cpipe_bm3d_pilot* n, const void* blob, size_t size) {
     n->seed = h.seed;
     k_to_half<<<(unsigned)((n->nparam + 255) / 256), 256>>>(n->w32, n->w16, (int)n->nparam);
     PHIP(hipGetLastError());
+    if (int rc = refresh_frags(n, nullptr)) return rc;
     PHIP(hipDeviceSynchronize());
     return 0;
 }
@@ -769,6 +852,7 @@ int cpipe_bm3d_pilot_train_step(cpipe_bm3d_pilot* n, const float* const* in, con
                                                                  n->w16, (int)n->nparam, lr, b1, b2,
                                                                  eps, c1, c2);
     PHIP(hipGetLastError());
+    if (int rc2 = refresh_frags(n, st)) return rc2;
     return 0;
 }
 
