@@ -698,18 +698,38 @@ __global__ void k_merge(const FrameTab* __restrict__ tab, const FrameRect* __res
     if (x >= rc.x1 || y >= rc.y1) return;
     const int txl = F.txlo[x], txh = F.txhi[x], tyl = F.tylo[y], tyh = F.tyhi[y];
     const size_t ntiles = (size_t)F.ntx * F.nty;
+    // a pixel lies in at most MT x MT tile footprints (footprint <= 2 tile pitches
+    // + 1 in x and y); the loads of a channel are all issued before the sums,
+    // which still run in (ty, tx) ascending order
+    constexpr int MT = 3;
+    size_t offs[MT * MT];
+    bool live[MT * MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            const int ty = tyl + i, tx = txl + k;
+            live[i * MT + k] = ty <= tyh && tx <= txh;
+            const int tyc = min(ty, tyh), txc = min(tx, txh);
+            const int fw = F.fx1[txc] - F.fx0[txc];
+            offs[i * MT + k] = ((size_t)tyc * F.ntx + txc) * F.tile_stride +
+                               (size_t)((y - F.fy0[tyc]) * fw + (x - F.fx0[txc]));
+        }
     float o[3] = {0.0f, 0.0f, 0.0f};
     for (int c = 0; c < F.c; ++c) {
+        const float* pc = F.partial + (size_t)c * ntiles * F.tile_stride;
+        float vn[MT * MT], vd[MT * MT];
+#pragma unroll
+        for (int i = 0; i < MT * MT; ++i) {
+            vn[i] = live[i] ? pc[offs[i]] : 0.0f;
+            vd[i] = live[i] ? pc[offs[i] + F.dofs] : 0.0f;
+        }
         float num = 0.0f, den = 0.0f;
-        for (int ty = tyl; ty <= tyh; ++ty) {
-            const int ry = y - F.fy0[ty];
-            for (int tx = txl; tx <= txh; ++tx) {
-                const int fw = F.fx1[tx] - F.fx0[tx];
-                const float* pn =
-                    F.partial + ((size_t)c * ntiles + (size_t)ty * F.ntx + tx) * F.tile_stride;
-                const int off = ry * fw + (x - F.fx0[tx]);
-                num = num + pn[off];
-                den = den + pn[F.dofs + off];
+#pragma unroll
+        for (int i = 0; i < MT * MT; ++i) {
+            if (live[i]) {
+                num = num + vn[i];
+                den = den + vd[i];
             }
         }
         o[c] = num / den;
@@ -922,6 +942,8 @@ struct cpipe_bm3d_plan_s {
     std::vector<Check> checks;
     Lists chk;                       // the checked rectangles' work, all checks in one launch per kernel
     float* report = nullptr;         // [n][2]: delta dB, fell back
+    hipStream_t side = nullptr;      // the spot checks' plain step 1 runs here, beside the network
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // one graph per plan: the whole batch as a single submission
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
@@ -983,6 +1005,9 @@ void free_plan(cpipe_bm3d_plan* p) {
     (void)hipFree(p->err);
     (void)hipFree(p->rgb);
     (void)hipFree(p->report);
+    if (p->side) (void)hipStreamDestroy(p->side);
+    if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+    if (p->join_ev) (void)hipEventDestroy(p->join_ev);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
@@ -1079,10 +1104,11 @@ int build_frame(cpipe_bm3d_plan* p, int f) {
             if (tylo[y] < 0) tylo[y] = t;
             tyhi[y] = t;
         }
+    // every pixel lies in 1..3 tile footprints per axis (the merge kernel reads at most 3 x 3)
     for (int x = 0; x < F.w; ++x)
-        if (txlo[x] < 0) return fail(INTERNAL, "tile cover has a gap in x");
+        if (txlo[x] < 0 || txhi[x] - txlo[x] >= 3) return fail(INTERNAL, "tile cover in x out of range");
     for (int y = 0; y < F.h; ++y)
-        if (tylo[y] < 0) return fail(INTERNAL, "tile cover has a gap in y");
+        if (tylo[y] < 0 || tyhi[y] - tylo[y] >= 3) return fail(INTERNAL, "tile cover in y out of range");
     if ((rc = upload(&F.geo_dev, F.geo))) return rc;
     T.nx = F.nx;
     T.ny = F.ny;
@@ -1349,8 +1375,17 @@ int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
     if ((rc = r.mark(CPIPE_BM3D_T_COLOR, true))) return rc;
 
     if (p->pilot) {
-        // learned pilot in place of step 1, then spot checks against the plain step 1
+        // learned pilot in place of step 1; the spot checks' plain step 1 runs on
+        // the side stream at the same time (both only read the noisy planes)
         if ((rc = r.mark(CPIPE_BM3D_T_PILOT, false))) return rc;
+        const int nchk = p->chk.n_rects;
+        if (nchk > 0) {
+            HIP_CHECK(hipEventRecord(p->fork_ev, st));
+            HIP_CHECK(hipStreamWaitEvent(p->side, p->fork_ev, 0));
+            RunCtx rs{p, p->side, false};
+            if ((rc = run_step(rs, 1, p->chk, TO_CHECK, 0, false))) return rc;
+            HIP_CHECK(hipEventRecord(p->join_ev, p->side));
+        }
         for (int f = 0; f < (int)n; ++f) {
             const auto& F = p->fr[f];
             if (F.small) continue;
@@ -1360,9 +1395,8 @@ int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
                                                levels_dev(p) + f, bout, st);
             if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
         }
-        const int nchk = p->chk.n_rects;
         if (nchk > 0) {
-            if ((rc = run_step(r, 1, p->chk, TO_CHECK, 0, false))) return rc;
+            HIP_CHECK(hipStreamWaitEvent(st, p->join_ev, 0));
             k_pilot_check<<<nchk, 256, 0, st>>>(p->tab_dev, p->chk.rects, p->pilot_thr_db, p->report);
             HIP_CHECK(hipGetLastError());
             k_pilot_decide<<<((int)n + 63) / 64, 64, 0, st>>>(p->tab_dev, (int)n, p->report, p->pilot_full_frac);
@@ -1864,6 +1898,11 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
         for (int c = 0; c < 3; ++c) p->tab[f].bzp[c] = F.bzp + c * F.npx;
     }
     if ((rc = build_check_lists(p, o))) return rc;
+    if (!p->side) {
+        HIP_CHECK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
+    }
     p->pilot = net;
     p->pilot_opts = o;
     p->pilot_thr_db = o.fallback_db;
