@@ -84,7 +84,7 @@ __global__ void k_pack_input(const float* __restrict__ y, const float* __restric
 
 struct ConvArgs {
     const _Float16* in;    // NHWC, channel stride CIN, images stacked
-    const half4* frag;     // weights as B fragments [tap][kb][nb][lane] (see k_frag)
+    const _Float16* frag;  // weights as B fragments [tap][kb][nb][lane][e] (see k_frag)
     const float* bias;     // [cout] (EPI 0/1)
     _Float16* out;         // NHWC, channel stride COUT (EPI 0/2)
     const _Float16* mask;  // NHWC, channel stride COUT (EPI 2)
@@ -95,43 +95,41 @@ struct ConvArgs {
     int w, h;
 };
 
-// B fragments of a layer's weights, in the order the convolution reads them:
-// frag[((t * KB + kb) * NB + nb) * 64 + lane] = 4 halves, so each fragment is
-// one coalesced 512-byte load per wave. Forward operator (CI -> CO):
+// B fragments of a layer's weights, in the order the convolution reads them.
+// A layer operator with K = KI input channels uses MFMA 16x16x32 when KI is a
+// multiple of 32 (fragments of 8 halves: k = 32 kb + 8 (lane / 16) + e), else
+// 16x16x16 (4 halves: k = 16 kb + 4 (lane / 16) + e); j = 16 nb + lane % 16.
+// frag[((t * KB + kb) * NB + nb) * 64 + lane] holds one lane's halves, so each
+// fragment is one coalesced load per wave. Forward operator (CI -> CO):
 // B[k = cin][j = cout] = W[t][k][j]; backward operator (CO -> CI, flipped):
 // B[k = cout][j = cin] = W[8 - t][j][k].
-template <int CI, int CO>
-__global__ void k_frag(const _Float16* __restrict__ w16, half4* __restrict__ fwd,
-                       half4* __restrict__ bwd) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int kchunk(int ki) { return ki % 32 == 0 ? 32 : 16; }
+
+template <int KI, int NJ, bool BWD, int CI, int CO>
+__device__ __forceinline__ void frag_one(const _Float16* __restrict__ w16, _Float16* __restrict__ dst, int idx) {
+    constexpr int KC = kchunk(KI), E = KC / 4, KB = KI / KC, NB = NJ / 16;
+    if (idx >= 9 * KB * NB * 64) return;
     const int lane = idx & 63, lr = lane & 15, lq = lane >> 4;
-    {
-        constexpr int KB = CI / 16, NB = CO / 16;
-        if (idx < 9 * KB * NB * 64) {
-            const int f = idx >> 6, nb = f % NB, kb = (f / NB) % KB, t = f / (NB * KB);
-            half4 v;
+    const int f = idx >> 6, nb = f % NB, kb = (f / NB) % KB, t = f / (NB * KB);
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                v[e] = w16[((size_t)t * CI + kb * 16 + lq * 4 + e) * CO + nb * 16 + lr];
-            fwd[idx] = v;
-        }
+    for (int e = 0; e < E; ++e) {
+        const int k = kb * KC + lq * E + e, j = nb * 16 + lr;
+        dst[(size_t)idx * E + e] = BWD ? w16[((size_t)(8 - t) * CI + j) * CO + k]
+                                       : w16[((size_t)t * CI + k) * CO + j];
     }
-    {
-        constexpr int KB = CO / 16, NB = CI / 16;
-        if (idx < 9 * KB * NB * 64) {
-            const int f = idx >> 6, nb = f % NB, kb = (f / NB) % KB, t = f / (NB * KB);
-            half4 v;
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                v[e] = w16[((size_t)(8 - t) * CI + nb * 16 + lr) * CO + kb * 16 + lq * 4 + e];
-            bwd[idx] = v;
-        }
-    }
+}
+
+template <int CI, int CO>
+__global__ void k_frag(const _Float16* __restrict__ w16, _Float16* __restrict__ fwd,
+                       _Float16* __restrict__ bwd) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    frag_one<CI, CO, false, CI, CO>(w16, fwd, idx);
+    frag_one<CO, CI, true, CI, CO>(w16, bwd, idx);
 }
 
 template <int CI, int CO, int EPI>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
-    constexpr int PS = CI + 4;  // LDS pixel pitch in halves
+    constexpr int PS = CI + 4;  // LDS pixel pitch in halves (8-byte aligned pixels)
     __shared__ __align__(16) _Float16 tile[(CTH + 2) * (TW + 2) * PS];
     __shared__ __align__(16) _Float16 ob[4][16 * CO];  // per wave: 16 output pixels x CO
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
@@ -151,14 +149,16 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     // weights as B fragments (forward or backward operator, see k_frag)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    constexpr int KB = CI / 16, NB = CO / 16;
-    half4 bw[9][KB][NB];
+    constexpr int KC = kchunk(CI), E = KC / 4, KB = CI / KC, NB = CO / 16;
+    typedef _Float16 hvec __attribute__((ext_vector_type(E)));
+    hvec bw[9][KB][NB];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) bw[t][kb][nb] = a.frag[((t * KB + kb) * NB + nb) * 64 + lane];
+            for (int nb = 0; nb < NB; ++nb)
+                bw[t][kb][nb] = *reinterpret_cast<const hvec*>(a.frag + (size_t)(((t * KB + kb) * NB + nb) * 64 + lane) * E);
     __syncthreads();
     _Float16* o16 = ob[wave];
 
@@ -176,10 +176,21 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                 const _Float16* row = &tile[((ly + ky) * (TW + 2) + seg * 16 + lr + kx) * PS];
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
-                    const half4 av = *reinterpret_cast<const half4*>(row + kb * 16 + lq * 4);
+                    hvec av;
+                    if constexpr (E == 8) {  // two 8-byte reads (pixels are 8-byte aligned)
+                        const half4 lo = *reinterpret_cast<const half4*>(row + kb * KC + lq * E);
+                        const half4 hi = *reinterpret_cast<const half4*>(row + kb * KC + lq * E + 4);
+                        av = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                    } else {
+                        av = *reinterpret_cast<const hvec*>(row + kb * KC + lq * E);
+                    }
 #pragma unroll
-                    for (int nb = 0; nb < NB; ++nb)
-                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if constexpr (KC == 32)
+                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+                        else
+                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+                    }
                 }
             }
             if (oy >= h) continue;  // uniform per wave
@@ -439,7 +450,7 @@ struct cpipe_bm3d_pilot_s {
     float* v = nullptr;
     float* g = nullptr;
     _Float16* w16 = nullptr;
-    half4* frag = nullptr;  // per layer: forward then backward B fragments (k_frag)
+    _Float16* frag = nullptr;  // per layer: forward then backward B fragments (k_frag)
     size_t frag_off[NL][2] = {};
     uint8_t* live = nullptr;
     uint64_t step = 0;
@@ -520,6 +531,7 @@ int ensure_buf(float** p, size_t* cap, size_t n) {
 
 // Rebuilds every layer's B fragments from the FP16 weights (after each change).
 int refresh_frags(cpipe_bm3d_pilot* n, hipStream_t st) {
+    // one thread per fragment lane: at most 9 * CI * CO / 4 per operator
     const int blocks[NL] = {(int)((layer_w(0) / 4 + 255) / 256), (int)((layer_w(1) / 4 + 255) / 256),
                             (int)((layer_w(2) / 4 + 255) / 256), (int)((layer_w(3) / 4 + 255) / 256),
                             (int)((layer_w(4) / 4 + 255) / 256), (int)((layer_w(5) / 4 + 255) / 256)};
@@ -701,15 +713,15 @@ int cpipe_bm3d_pilot_create(int device, cpipe_bm3d_pilot** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&n->g, o * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&n->w16, o * sizeof(_Float16));
     if (e == hipSuccess) e = hipMalloc((void**)&n->live, o);
-    size_t fo = 0;  // fragments: 9 * CI * CO / 4 half4 per operator
+    size_t fo = 0;  // fragments: 9 * CI * CO halves per operator
     for (int l = 0; l < NL; ++l) {
         n->frag_off[l][0] = fo;
-        fo += layer_w(l) / 4;
+        fo += layer_w(l);
         n->frag_off[l][1] = fo;
-        fo += layer_w(l) / 4;
+        fo += layer_w(l);
     }
-    if (e == hipSuccess) e = hipMalloc((void**)&n->frag, fo * sizeof(half4));
-    if (e == hipSuccess) e = hipMemset(n->frag, 0, fo * sizeof(half4));
+    if (e == hipSuccess) e = hipMalloc((void**)&n->frag, fo * sizeof(_Float16));
+    if (e == hipSuccess) e = hipMemset(n->frag, 0, fo * sizeof(_Float16));
     if (e == hipSuccess) e = hipMemcpy(n->live, n->live_host.data(), o, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(n->w32, 0, o * sizeof(float));
     if (e == hipSuccess) e = hipMemset(n->m, 0, o * sizeof(float));
