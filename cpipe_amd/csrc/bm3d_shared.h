@@ -99,6 +99,58 @@ __device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
     return v;
 }
 
+// One read/add/write round over members [S, S + L) of a group (pairwise
+// disjoint patches): all reads, then all writes.
+template <int S, int L>
+__device__ __forceinline__ void rmw_run(float* acc, const int (&o)[NMAX], const float (&c)[NMAX],
+                                        float wk, int dofs) {
+    float an[L], ad[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        an[m] = acc[o[S + m]];
+        ad[m] = acc[dofs + o[S + m]];
+    }
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        acc[o[S + m]] = an[m] + c[S + m];
+        acc[dofs + o[S + m]] = ad[m] + wk;
+    }
+}
+
+// Members m0 .. m0+3 in rounds of consecutive pairwise-disjoint runs; bit k of
+// `brk` (k = 1..3) starts a new run at member m0 + k.
+template <int M0>
+__device__ __forceinline__ void rmw_block4(float* acc, const int (&o)[NMAX], const float (&c)[NMAX],
+                                           float wk, int dofs, uint32_t brk) {
+    switch (brk & 0xeu) {
+        case 0x0: rmw_run<M0, 4>(acc, o, c, wk, dofs); break;
+        case 0x8: rmw_run<M0, 3>(acc, o, c, wk, dofs); rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs); break;
+        case 0x4: rmw_run<M0, 2>(acc, o, c, wk, dofs); rmw_run<M0 + 2, 2>(acc, o, c, wk, dofs); break;
+        case 0xc:
+            rmw_run<M0, 2>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 2, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs);
+            break;
+        case 0x2: rmw_run<M0, 1>(acc, o, c, wk, dofs); rmw_run<M0 + 1, 3>(acc, o, c, wk, dofs); break;
+        case 0xa:
+            rmw_run<M0, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 1, 2>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs);
+            break;
+        case 0x6:
+            rmw_run<M0, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 1, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 2, 2>(acc, o, c, wk, dofs);
+            break;
+        default:
+            rmw_run<M0, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 1, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 2, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs);
+            break;
+    }
+}
+
 // Adds one filtered group to a tile's LDS num/den footprint (acc[0, dofs) and
 // acc[dofs, 2 dofs)) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
 // member m in v[m]; mp[m] (wave-uniform) is the member's footprint position
@@ -111,10 +163,11 @@ __device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
 //
 // The turns form the kernel's critical path, so everything that does not
 // touch the footprint (offsets, products, the batching plan) is computed
-// before the wait. Members are added in read/add/write rounds: a block of
-// four consecutive members whose patches are pairwise disjoint in one round,
-// else each disjoint pair in one round, else one member per round. A pixel's
-// members are added in member order either way.
+// before the wait. Members are added in read/add/write rounds: each block of
+// four consecutive members is cut greedily into runs of consecutive members
+// whose patches are pairwise disjoint, one round per run. A pixel's members
+// are added in member order either way (members covering one pixel overlap,
+// so they never share a run).
 __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, bool has, int n,
                                                   const uint32_t (&mp)[NMAX],
                                                   const float (&v)[NMAX], float wk, int lo,
@@ -131,18 +184,20 @@ __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, 
         const int dy = (int)(mp[a] >> 16) - (int)(mp[b] >> 16);
         return dx < K && dx > -K && dy < K && dy > -K;
     };
-    // bit b: block b (members 4b .. 4b+3) is one round; bit 4 + k: pair k is one round
+    // run breaks: bit 4b + k set when member 4b + k starts a new run (k = 1..3)
     uint32_t plan = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const int m0 = 4 * b;
-        const bool p0 = !overlap(m0, m0 + 1), p1 = !overlap(m0 + 2, m0 + 3);
-        const bool all = p0 && p1 && !overlap(m0, m0 + 2) && !overlap(m0, m0 + 3) &&
-                         !overlap(m0 + 1, m0 + 2) && !overlap(m0 + 1, m0 + 3);
-        plan |= (all && m0 + 4 <= n ? 1u : 0u) << b;
-        plan |= (p0 && m0 + 2 <= n ? 1u : 0u) << (4 + 2 * b);
-        plan |= (p1 && m0 + 4 <= n ? 1u : 0u) << (5 + 2 * b);
+        const bool b1 = overlap(m0 + 1, m0);
+        const bool b2 = overlap(m0 + 2, m0 + 1) || (!b1 && overlap(m0 + 2, m0));
+        // members of the run that member m0 + 3 would join
+        const bool in0 = !b1 && !b2, in1 = !b2;
+        const bool b3 = overlap(m0 + 3, m0 + 2) || (in1 && overlap(m0 + 3, m0 + 1)) ||
+                        (in0 && overlap(m0 + 3, m0));
+        plan |= ((b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u)) << (4 * b);
     }
+    if (n < 4) plan = n == 2 ? (plan & 2u) | 0xcu : 0xeu;  // n = 1: one member; n = 2: at most 2
     plan = __builtin_amdgcn_readfirstlane(plan);
     // keep the precomputation ahead of the wait
 #pragma unroll
@@ -159,47 +214,24 @@ __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, 
         }
     }
     if (has) {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int m0 = 4 * b;
-            if (m0 < n) {
-                if (plan & (1u << b)) {
-                    float an[4], ad[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        an[j] = acc[o[m0 + j]];
-                        ad[j] = acc[dofs + o[m0 + j]];
-                    }
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        acc[o[m0 + j]] = an[j] + c[m0 + j];
-                        acc[dofs + o[m0 + j]] = ad[j] + wk;
-                    }
-                } else {
-#pragma unroll
-                    for (int pr = 0; pr < 2; ++pr) {
-                        const int q0 = m0 + 2 * pr;
-                        if (q0 < n) {
-                            if (plan & (1u << (4 + 2 * b + pr))) {
-                                const float a0 = acc[o[q0]], a1 = acc[o[q0 + 1]];
-                                const float d0 = acc[dofs + o[q0]], d1 = acc[dofs + o[q0 + 1]];
-                                acc[o[q0]] = a0 + c[q0];
-                                acc[o[q0 + 1]] = a1 + c[q0 + 1];
-                                acc[dofs + o[q0]] = d0 + wk;
-                                acc[dofs + o[q0 + 1]] = d1 + wk;
-                            } else {
-#pragma unroll
-                                for (int j = 0; j < 2; ++j) {
-                                    if (q0 + j < n) {
-                                        acc[o[q0 + j]] = acc[o[q0 + j]] + c[q0 + j];
-                                        acc[dofs + o[q0 + j]] = acc[dofs + o[q0 + j]] + wk;
-                                    }
-                                }
-                            }
-                        }
-                    }
+        if (n >= 4) {
+            rmw_block4<0>(acc, o, c, wk, dofs, plan);
+            if (n >= 8) {
+                rmw_block4<4>(acc, o, c, wk, dofs, plan >> 4);
+                if (n >= 16) {
+                    rmw_block4<8>(acc, o, c, wk, dofs, plan >> 8);
+                    rmw_block4<12>(acc, o, c, wk, dofs, plan >> 12);
                 }
             }
+        } else if (n == 2) {
+            if (plan & 2u) {
+                rmw_run<0, 1>(acc, o, c, wk, dofs);
+                rmw_run<1, 1>(acc, o, c, wk, dofs);
+            } else {
+                rmw_run<0, 2>(acc, o, c, wk, dofs);
+            }
+        } else {
+            rmw_run<0, 1>(acc, o, c, wk, dofs);
         }
     }
     __hip_atomic_store(turn, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
