@@ -69,11 +69,13 @@ __global__ void k_pack_input(const float* __restrict__ y, const float* __restric
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution, stride 1, zero padding, on MFMA 16x16x16 (FP16 -> FP32).
-// Workgroup: 64x8 output pixels, 4 waves (two output rows each), 4 segments
-// of 16 pixels per row. The input tile (66x10 pixels) is staged in LDS with a
-// padded pixel pitch (CIN + 4 halves) to spread banks; weights are loaded as
-// pre-arranged B fragments; outputs leave through a per-wave LDS buffer as
-// 16-byte stores.
+// Workgroup: 64x8 output pixels, 4 waves (two adjacent output rows each, both
+// accumulated at once so that each A fragment read from LDS feeds every row
+// whose window covers it), 4 segments of 16 pixels per row. The input tile
+// (66x10 pixels) is staged in LDS with a padded pixel pitch (CIN + 4 halves)
+// to spread banks, every global load of a thread in flight before its first
+// LDS store; weights are loaded as pre-arranged B fragments; outputs leave
+// through a per-wave LDS buffer as 16-byte stores.
 // MFMA operand layouts (wave64, 16x16x16): A[i][k] at lane i + 16*(k/4),
 // element k%4; B[k][j] at lane j + 16*(k/4), element k%4; D[i][j] at lane
 // j + 16*(i/4), element i%4.
@@ -130,25 +132,17 @@ __global__ void k_frag(const _Float16* __restrict__ w16, _Float16* __restrict__ 
 template <int CI, int CO, int EPI>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int PS = CI + 4;  // LDS pixel pitch in halves (8-byte aligned pixels)
+    constexpr int RPW = CTH / 4;  // adjacent output rows per wave
     __shared__ __align__(16) _Float16 tile[(CTH + 2) * (TW + 2) * PS];
     __shared__ __align__(16) _Float16 ob[4][16 * CO];  // per wave: 16 output pixels x CO
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
     const int w = a.w, h = a.h;
-    const _Float16* in = a.in + (size_t)img * w * h * CI;
-    // stage (TW+2) x (CTH+2) pixels, CI channels, 8 bytes per thread-step
-    constexpr int QPP = CI / 4;  // quads per pixel
-    for (int idx = threadIdx.x; idx < (CTH + 2) * (TW + 2) * QPP; idx += blockDim.x) {
-        const int q = idx % QPP, pix = idx / QPP;
-        const int ty = pix / (TW + 2), tx = pix - ty * (TW + 2);
-        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
-        half4 v = {0, 0, 0, 0};
-        if (gx >= 0 && gx < w && gy >= 0 && gy < h)
-            v = *reinterpret_cast<const half4*>(in + ((size_t)gy * w + gx) * CI + q * 4);
-        *reinterpret_cast<half4*>(&tile[pix * PS + q * 4]) = v;
-    }
-    // weights as B fragments (forward or backward operator, see k_frag)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
+    const _Float16* in = a.in + (size_t)img * w * h * CI;
+    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    // weights as B fragments (forward or backward operator, see k_frag); their
+    // loads are issued first so that they overlap the tile's
     constexpr int KC = kchunk(CI), E = KC / 4, KB = CI / KC, NB = CO / 16;
     typedef _Float16 hvec __attribute__((ext_vector_type(E)));
     hvec bw[9][KB][NB];
@@ -159,21 +153,50 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
                 bw[t][kb][nb] = *reinterpret_cast<const hvec*>(a.frag + (size_t)(((t * KB + kb) * NB + nb) * 64 + lane) * E);
+    // stage (TW+2) x (CTH+2) pixels, CI channels: every 16-byte load of the
+    // thread is in flight before the first LDS store
+    constexpr int OPP = CI / 8;  // 16-byte chunks per pixel
+    constexpr int NCH = (CTH + 2) * (TW + 2) * OPP;
+    constexpr int IT = (NCH + 255) / 256;
+    half8 sv[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int idx = threadIdx.x + 256 * i;
+        const int q = idx % OPP, pix = idx / OPP;
+        const int ty = pix / (TW + 2), tx = pix - ty * (TW + 2);
+        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+        sv[i] = (half8){0, 0, 0, 0, 0, 0, 0, 0};
+        if (idx < NCH && gx >= 0 && gx < w && gy >= 0 && gy < h)
+            sv[i] = *reinterpret_cast<const half8*>(in + ((size_t)gy * w + gx) * CI + q * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int idx = threadIdx.x + 256 * i;
+        if (idx < NCH) {
+            const int q = idx % OPP, pix = idx / OPP;
+            _Float16* d = &tile[pix * PS + q * 8];
+            *reinterpret_cast<half4*>(d) = __builtin_shufflevector(sv[i], sv[i], 0, 1, 2, 3);
+            *reinterpret_cast<half4*>(d + 4) = __builtin_shufflevector(sv[i], sv[i], 4, 5, 6, 7);
+        }
+    }
     __syncthreads();
     _Float16* o16 = ob[wave];
+    const int ly0 = wave * RPW;  // this wave's first output row in the tile
 
 #pragma unroll 1
-    for (int rr = 0; rr < CTH / 4; ++rr) {
-        const int ly = wave + 4 * rr, oy = y0 + ly;  // this wave's output row
-#pragma unroll 1
-        for (int seg = 0; seg < TW / 16; ++seg) {
-            float4v acc[NB];
+    for (int seg = 0; seg < TW / 16; ++seg) {
+        // rows ly0 .. ly0+RPW-1 together: each A fragment (an input row at one
+        // horizontal tap) feeds every output row whose 3x3 window covers it
+        float4v acc[RPW][NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4v){0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < RPW; ++r)
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                const int ky = t / 3, kx = t % 3;
-                const _Float16* row = &tile[((ly + ky) * (TW + 2) + seg * 16 + lr + kx) * PS];
+            for (int nb = 0; nb < NB; ++nb) acc[r][nb] = (float4v){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int iy = 0; iy < RPW + 2; ++iy) {
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const _Float16* row = &tile[((ly0 + iy) * (TW + 2) + seg * 16 + lr + kx) * PS];
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
                     hvec av;
@@ -185,14 +208,23 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                         av = *reinterpret_cast<const hvec*>(row + kb * KC + lq * E);
                     }
 #pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) {
-                        if constexpr (KC == 32)
-                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
-                        else
-                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[t][kb][nb], acc[nb], 0, 0, 0);
+                    for (int r = 0; r < RPW; ++r) {
+                        const int ky = iy - r;
+                        if (ky < 0 || ky > 2) continue;
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) {
+                            if constexpr (KC == 32)
+                                acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bw[ky * 3 + kx][kb][nb], acc[r][nb], 0, 0, 0);
+                            else
+                                acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[ky * 3 + kx][kb][nb], acc[r][nb], 0, 0, 0);
+                        }
                     }
                 }
             }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int oy = y0 + ly0 + r;
             if (oy >= h) continue;  // uniform per wave
             if (EPI == 1) {
 #pragma unroll
@@ -203,7 +235,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                         const int ox = x0 + seg * 16 + lq * 4 + e;
                         if (ox >= w || co >= 3) continue;
                         const size_t q = img * a.img_stride_planes + (size_t)oy * a.pitch + ox;
-                        a.outp[co][q] = a.res[co][q] + (acc[nb][e] + a.bias[co]);
+                        a.outp[co][q] = a.res[co][q] + (acc[r][nb][e] + a.bias[co]);
                     }
                 }
                 continue;
@@ -216,7 +248,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                 const float b = EPI == 0 ? a.bias[co] : 0.0f;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    float v = acc[nb][e] + b;
+                    float v = acc[r][nb][e] + b;
                     if (EPI == 0) v = v > 0.0f ? v : 0.0f;
                     o16[(lq * 4 + e) * CO + co] = (_Float16)v;
                 }
@@ -224,7 +256,6 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            typedef _Float16 half8 __attribute__((ext_vector_type(8)));
             constexpr int CH = CO / 8;  // 16-byte chunks per pixel
 #pragma unroll
             for (int k = lane; k < 16 * CH; k += 64) {
