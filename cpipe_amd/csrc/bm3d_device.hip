@@ -274,12 +274,24 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
     const int bx0 = it.y * BM_TX, by0 = it.z * BM_TY;
     const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
     const int Y0 = ref_pos(by0, h) - R;
-    for (int idx = threadIdx.x; idx < BM_COLS * BM_LH; idx += blockDim.x) {
-        const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
-        const int gx = X0 + xx, gy = Y0 + yy;
-        float v = 0.0f;
-        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = plane[(size_t)gy * w + gx];
-        tile[yy * BM_LW + xx] = v;
+    {
+        // every load of the thread in flight before the first LDS store
+        constexpr int NST = BM_COLS * BM_LH, IT = (NST + 255) / 256;
+        float sv[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
+            const int gx = X0 + xx, gy = Y0 + yy;
+            sv[i] = 0.0f;
+            if (idx < NST && gx >= 0 && gx < w && gy >= 0 && gy < h) sv[i] = plane[(size_t)gy * w + gx];
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
+            if (idx < NST) tile[yy * BM_LW + xx] = sv[i];
+        }
     }
     __syncthreads();
 

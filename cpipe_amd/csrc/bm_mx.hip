@@ -132,16 +132,29 @@ __global__ __launch_bounds__(256) void k_bm_mx(const FrameTab* __restrict__ tab,
     // ---- stage the region; its mean centres the rounded values
     float sum = 0.0f;
     int nin = 0;
-    for (int idx = tid; idx < LH * LW; idx += 256) {
-        const int yy = idx / LW, xx = idx - yy * LW;
-        const int gx = X0 + xx, gy = Y0 + yy;
-        float v = 0.0f;
-        if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-            v = plane[(size_t)gy * w + gx];
-            sum += v;
-            ++nin;
+    {
+        // every load of the thread in flight before the first LDS store
+        constexpr int NST = LH * LW, IT = (NST + 255) / 256;
+        float sv[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int idx = tid + 256 * i;
+            const int yy = idx / LW, xx = idx - yy * LW;
+            const int gx = X0 + xx, gy = Y0 + yy;
+            sv[i] = 0.0f;
+            if (idx < NST && gx >= 0 && gx < w && gy >= 0 && gy < h) sv[i] = plane[(size_t)gy * w + gx];
         }
-        tf[idx] = v;
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int idx = tid + 256 * i;
+            const int yy = idx / LW, xx = idx - yy * LW;
+            const int gx = X0 + xx, gy = Y0 + yy;
+            if (idx < NST && gx >= 0 && gx < w && gy >= 0 && gy < h) {
+                sum += sv[i];
+                ++nin;
+            }
+            if (idx < NST) tf[idx] = sv[i];
+        }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
