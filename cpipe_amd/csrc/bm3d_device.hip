@@ -47,6 +47,7 @@ constexpr int BM_COLS = (BM_TX - 1) * STRIDE + 2 * R + K + 1;  // 92: +1 for the
 constexpr int BM_LH = (BM_TY + 1) * STRIDE + 2 * R + 3 + 1;    // 86: helper rows included
 constexpr int BM_LW = 132;  // LDS row pitch = 4 mod 64: a wave's 16 rows x 4 columns of refs
                             // (stride 3) read 64 distinct banks
+constexpr float BM_OUTSIDE = 1.0e18f;  // staged for pixels outside the image (SSD >= 1e36 > tau)
 constexpr int PEND = 12;    // per-lane queue of candidates awaiting top-16 insertion
 
 enum Status { OK = 0, FAILED = 1, OOM = 3, BAD_INDEX = 5, INTERNAL = 7, UNSUPPORTED = 8 };
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
             const int idx = threadIdx.x + 256 * i;
             const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
             const int gx = X0 + xx, gy = Y0 + yy;
-            sv[i] = 0.0f;
+            sv[i] = BM_OUTSIDE;
             if (idx < NST && gx >= 0 && gx < w && gy >= 0 && gy < h) sv[i] = plane[(size_t)gy * w + gx];
         }
 #pragma unroll
@@ -348,6 +349,19 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
     };
 
     if (regular) {
+        // Candidates whose patch leaves the image read BM_OUTSIDE pixels, so
+        // their SSD (>= 1e36) fails s < tau by itself; what is left to reject
+        // is uniform: dx = R + 1 (the last 4-wide block's overrun), the ref
+        // itself, and helper lanes (threshold -1). The row sums then stay
+        // outside any branch and the DPP row shifts fold into the adds.
+        const float thr_lane = real ? tau : -1.0f;
+        auto offer_in = [&](float s, int dx, int dy, float thr) {
+            if (s < thr && s < bd[NMAX - 1]) {
+                const uint32_t qx = (uint32_t)(rx + dx), qy = (uint32_t)(ry + dy);
+                q[np * 256] = make_float2(s, __uint_as_float((qy << 16) | qx));
+                ++np;
+            }
+        };
         float ref[3][8];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -387,7 +401,9 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
                     sum = sum + r5;
                     sum = sum + r6;
                     sum = sum + r7;
-                    offer(sum, dx0 + k, dy);
+                    const int dx = dx0 + k;
+                    const float thr = (dx > R || (dy == 0 && dx == 0)) ? -1.0f : thr_lane;
+                    offer_in(sum, dx, dy, thr);
                 }
             }
         }
