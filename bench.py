@@ -162,6 +162,39 @@ def time_form(torch, plan, src, dst, sigma, steps, warmup, stream):
     return wall * 1e3 / steps, {k: v / runs for k, v in stage_ms.items() if v > 0}
 
 
+def host_path_bench(noisy, sigma, runs):
+    """The node's path (cpipe_bm3d_host_planes): pageable host planes in, H2D,
+    BM3D, D2H, synchronise; plans are cached across calls, so the first call
+    (plan creation) is reported separately. Not the headline: the headline
+    keeps the frame resident in HBM."""
+    import ctypes
+    c, h, w = noisy.shape
+    L = engine.lib()
+    src = [np.ascontiguousarray(noisy[i]) for i in range(c)]
+    dst = [np.empty((h, w), np.float32) for _ in range(c)]
+    fp = ctypes.POINTER(ctypes.c_float)
+    ins = (fp * c)(*[a.ctypes.data_as(fp) for a in src])
+    outs = (fp * c)(*[a.ctypes.data_as(fp) for a in dst])
+    res = {"frame": f"{w}x{h}x{c} float32 host planes (pageable)"}
+    for name, prec, flags in (("fp32", 0, 0), ("learned_fp16", 2, 1)):
+        if flags and c != 3:
+            continue
+        times = []
+        for _ in range(runs + 1):
+            t0 = time.perf_counter()
+            rc = L.cpipe_bm3d_host_planes(ctypes.cast(ins, ctypes.c_void_p), w,
+                                          ctypes.cast(outs, ctypes.c_void_p), w, w, h, c,
+                                          float(sigma), prec, 0, flags)
+            times.append((time.perf_counter() - t0) * 1e3)
+            if rc != 0:
+                raise engine.EngineError(rc, L.cpipe_bm3d_last_error().decode())
+        ms = float(np.median(times[1:]))
+        res[name] = {"first_call_ms": round(times[0], 2), "ms": round(ms, 2),
+                     "value": round(w * h / (ms / 1e3) / 1e6, 2), "unit": "Mpixel/s"}
+    L.cpipe_bm3d_host_release()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +207,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) leg")
     ap.add_argument("--cpu-sample", default="2048x1536")
     ap.add_argument("--fp64-crop", default="512x384")
     ap.add_argument("--pilot", default=None,
@@ -304,6 +338,12 @@ def main():
                                    "max_abs_vs_fp64": round(synth.max_abs(co, o64), 5)}
         print(f"# fp64 crop: {json.dumps(fp64)}", file=sys.stderr, flush=True)
 
+    # ---- the node's path: host (pageable) planes in and out, PCIe copies included
+    host_path = None
+    if not args.no_host:
+        host_path = host_path_bench(noisy, sigma, 3)
+        print(f"# host path: {json.dumps(host_path)}", file=sys.stderr, flush=True)
+
     batch = None
     if not args.no_batch and c == 3:
         print("# batch of 12 frames ...", file=sys.stderr, flush=True)
@@ -331,6 +371,7 @@ def main():
         "forms": forms,
         "fp64_crop": fp64,
         "batch12": batch,
+        "host_path": host_path,
     }
     if cpu:
         line["gpu_over_cpu"] = round(mpx / cpu["value"], 1)

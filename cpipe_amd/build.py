@@ -67,18 +67,36 @@ def _embed_manifest() -> str:
     return out
 
 
+def _embed_pilot() -> str:
+    """Turn the packaged learned-pilot weights into a C++ byte array linked into the engine."""
+    src = os.path.join(PKG, "weights", "pilot_v1.bin")
+    with open(src, "rb") as f:
+        blob = f.read()
+    out = os.path.join(BUILD, "pilot_blob_gen.cpp")
+    body = ",".join(str(b) for b in blob)
+    with open(out, "w", encoding="utf-8") as f:
+        f.write("// generated from cpipe_amd/weights/pilot_v1.bin\n#include <cstddef>\n")
+        f.write('extern "C" const unsigned char CPIPE_BM3D_PILOT_BLOB[];\n')
+        f.write('extern "C" const size_t CPIPE_BM3D_PILOT_BLOB_SIZE;\n')
+        f.write(f'extern "C" const unsigned char CPIPE_BM3D_PILOT_BLOB[] = {{{body}}};\n')
+        f.write(f'extern "C" const size_t CPIPE_BM3D_PILOT_BLOB_SIZE = {len(blob)};\n')
+    return out
+
+
 def build(verbose: bool = False) -> dict:
     """Compile every native library of the package; returns their paths."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     hipcc = _hipcc()
     manifest_cpp = _embed_manifest()
+    blob_cpp = _embed_pilot()
     dev_o = os.path.join(BUILD, "bm3d_device.o")
     pilot_o = os.path.join(BUILD, "pilot.o")
     mx_o = os.path.join(BUILD, "group_mx.o")
     bmx_o = os.path.join(BUILD, "bm_mx.o")
     node_o = os.path.join(BUILD, "node.o")
     man_o = os.path.join(BUILD, "manifest_gen.o")
+    blob_o = os.path.join(BUILD, "pilot_blob_gen.o")
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm3d_device.hip"), "-o", dev_o],
          verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "pilot.hip"), "-o", pilot_o], verbose)
@@ -86,9 +104,10 @@ def build(verbose: bool = False) -> dict:
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm_mx.hip"), "-o", bmx_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
+    _run(["g++", *CXX_FLAGS, "-c", blob_cpp, "-o", blob_o], verbose)
     tmp = ENGINE_SO + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, node_o,
-          man_o], verbose)
+          man_o, blob_o], verbose)
     os.replace(tmp, ENGINE_SO)
     tmp = MINIHOST_SO + ".tmp"
     _run(["g++", *CXX_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "minihost.cpp"), "-ldl"],

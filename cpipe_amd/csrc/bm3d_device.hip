@@ -28,12 +28,17 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
 #include "bm3d_shared.h"
 #include "cpipe_bm3d.h"
 #include "pilot.h"
+
+// Learned-pilot weights embedded at build time (cpipe_amd/weights/pilot_v1.bin).
+extern "C" const unsigned char CPIPE_BM3D_PILOT_BLOB[];
+extern "C" const size_t CPIPE_BM3D_PILOT_BLOB_SIZE;
 
 using namespace cpipe_internal;
 
@@ -1376,6 +1381,150 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
     return OK;
 }
 
+// ---------------------------------------------------------------------------
+// Process-global cache behind the host-memory conveniences (the node path):
+// the reference host creates and destroys a node instance on every run, so
+// plans, their device buffers and the embedded learned pilot live here and are
+// reused while the image shape stays the same.
+// ---------------------------------------------------------------------------
+enum HostKind { HOST_PLANES = 0, HOST_RGBA = 1, HOST_COMPAT = 2 };
+
+struct HostSlot {
+    int kind = 0, w = 0, h = 0, c = 0, precision = 0, device = -1, flags = 0;
+    cpipe_bm3d_plan* plan = nullptr;  // none for the compat filter
+    void* dev = nullptr;              // device in/out staging
+    void* pin = nullptr;              // pinned host in/out staging, same size
+    size_t dev_bytes = 0;
+    hipEvent_t ev[3] = {};            // per-plane transfer completion
+    uint64_t used = 0;
+};
+
+// Row copy between host buffers on up to 8 threads (the caller's buffers are
+// pageable; the pinned staging lets the DMA engines run at full PCIe rate).
+void par_rows(void* dst, size_t dpitch, const void* src, size_t spitch, size_t row_bytes, int rows) {
+    auto band = [=](int r0, int r1) {
+        if (dpitch == row_bytes && spitch == row_bytes) {
+            std::memcpy(static_cast<char*>(dst) + (size_t)r0 * dpitch,
+                        static_cast<const char*>(src) + (size_t)r0 * spitch, (size_t)(r1 - r0) * row_bytes);
+            return;
+        }
+        for (int r = r0; r < r1; ++r)
+            std::memcpy(static_cast<char*>(dst) + (size_t)r * dpitch,
+                        static_cast<const char*>(src) + (size_t)r * spitch, row_bytes);
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = (size_t)rows * row_bytes < ((size_t)8 << 20) ? 1 : (int)std::min(8u, hw ? hw : 1u);
+    if (nt <= 1) {
+        band(0, rows);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(band, (int)((int64_t)rows * t / nt), (int)((int64_t)rows * (t + 1) / nt));
+    band(0, rows / nt);
+    for (auto& x : th) x.join();
+}
+
+struct HostCache {
+    std::mutex mu;
+    std::vector<HostSlot> slots;
+    std::map<int, cpipe_bm3d_pilot*> nets;  // embedded pilot weights, per device
+    uint64_t clock = 0;
+};
+
+// Never destroyed: HIP must not be called from static destructors at exit.
+HostCache& host_cache() {
+    static HostCache* c = new HostCache();
+    return *c;
+}
+
+void drop_slot(HostSlot& sl) {
+    DeviceGuard guard(sl.device);
+    if (sl.plan) free_plan(sl.plan);
+    (void)hipFree(sl.dev);
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    for (hipEvent_t& e : sl.ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    sl.plan = nullptr;
+    sl.dev = nullptr;
+    sl.pin = nullptr;
+}
+
+int host_net(HostCache& hc, int device, cpipe_bm3d_pilot** out) {
+    auto it = hc.nets.find(device);
+    if (it != hc.nets.end()) {
+        *out = it->second;
+        return OK;
+    }
+    cpipe_bm3d_pilot* net = nullptr;
+    int rc = cpipe_bm3d_pilot_create(device, &net);
+    if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
+    rc = cpipe_bm3d_pilot_load(net, CPIPE_BM3D_PILOT_BLOB, CPIPE_BM3D_PILOT_BLOB_SIZE);
+    if (rc) {
+        cpipe_bm3d_pilot_destroy(net);
+        return fail(rc, std::string("embedded pilot weights: ") + cpipe_bm3d_pilot_last_error());
+    }
+    hc.nets[device] = net;
+    *out = net;
+    return OK;
+}
+
+// Finds or creates the slot for this call's shape (caller holds hc.mu).
+int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int device, int flags,
+              size_t dev_bytes, HostSlot** out) {
+    const int ndev = cpipe_bm3d_device_count();
+    if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
+    if (device < 0 || device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
+    if ((flags & CPIPE_BM3D_HOST_PILOT) && c != 3)
+        return fail(UNSUPPORTED, "the learned pilot is trained for RGB (3-plane) images");
+    if (kind == HOST_COMPAT) precision = 0, flags = 0;
+    for (HostSlot& sl : hc.slots) {
+        if (sl.kind == kind && sl.w == w && sl.h == h && sl.c == c && sl.precision == precision &&
+            sl.device == device && sl.flags == flags) {
+            sl.used = ++hc.clock;
+            *out = &sl;
+            return OK;
+        }
+    }
+    if (hc.slots.size() >= CPIPE_BM3D_HOST_SLOTS) {  // evict the least recently used shape
+        size_t lru = 0;
+        for (size_t i = 1; i < hc.slots.size(); ++i)
+            if (hc.slots[i].used < hc.slots[lru].used) lru = i;
+        drop_slot(hc.slots[lru]);
+        hc.slots.erase(hc.slots.begin() + (long)lru);
+    }
+    HostSlot sl;
+    sl.kind = kind, sl.w = w, sl.h = h, sl.c = c, sl.precision = precision, sl.device = device;
+    sl.flags = flags;
+    int rc = OK;
+    if (kind != HOST_COMPAT) {
+        cpipe_bm3d_plan_desc d{sizeof(cpipe_bm3d_plan_desc), w, h, c, precision, device};
+        if ((rc = cpipe_bm3d_plan_create(&d, &sl.plan))) return rc;
+        if (flags & CPIPE_BM3D_HOST_PILOT) {
+            cpipe_bm3d_pilot* net = nullptr;
+            if (!(rc = host_net(hc, device, &net))) rc = cpipe_bm3d_plan_set_pilot(sl.plan, net, nullptr);
+        }
+    }
+    if (!rc) {
+        DeviceGuard guard(device);
+        hipError_t e = hipMalloc(&sl.dev, dev_bytes);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.pin, dev_bytes, hipHostMallocDefault);
+        for (hipEvent_t& ev : sl.ev)
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) rc = fail(OOM, std::string("staging buffers: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        drop_slot(sl);
+        return rc;
+    }
+    sl.dev_bytes = dev_bytes;
+    sl.used = ++hc.clock;
+    hc.slots.push_back(sl);
+    *out = &hc.slots.back();
+    return OK;
+}
+
 // The whole run: table upload, colour, step 1 (or the learned pilot with its
 // spot checks and device-side fallback), step 2 into the callers' planes.
 int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
@@ -2018,82 +2167,91 @@ int cpipe_bm3d_compat_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, siz
 
 int cpipe_bm3d_host_planes(const float* const* in, size_t in_pitch, float* const* out,
                            size_t out_pitch, int w, int h, int channels, float sigma, int precision,
-                           int device) {
+                           int device, int flags) {
     if (!in || !out) return fail(BAD_INDEX, "null argument");
-    cpipe_bm3d_plan_desc d{sizeof(cpipe_bm3d_plan_desc), w, h, channels, precision, device};
-    cpipe_bm3d_plan* p = nullptr;
-    int rc = cpipe_bm3d_plan_create(&d, &p);
+    if (channels != 1 && channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
+    if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
+    HostCache& hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
+    const size_t npx = (size_t)w * h, pb = npx * sizeof(float), row = (size_t)w * sizeof(float);
+    HostSlot* sl = nullptr;
+    int rc = host_slot(hc, HOST_PLANES, w, h, channels, precision, device, flags, pb * 2 * channels, &sl);
     if (rc) return rc;
     DeviceGuard guard(device);
-    float* dev = nullptr;
-    const size_t npx = (size_t)w * h;
-    hipError_t e = hipMalloc((void**)&dev, sizeof(float) * npx * 2 * channels);
-    if (e != hipSuccess) {
-        free_plan(p);
-        return fail(OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
+    float* dev = static_cast<float*>(sl->dev);
+    float* pin = static_cast<float*>(sl->pin);
     float* din[3];
     float* dout[3];
     for (int c = 0; c < channels; ++c) {
         din[c] = dev + c * npx;
         dout[c] = dev + (channels + c) * npx;
     }
-    rc = OK;
-    for (int c = 0; c < channels && !rc; ++c) {
-        e = hipMemcpy2D(din[c], (size_t)w * sizeof(float), in[c], in_pitch * sizeof(float),
-                        (size_t)w * sizeof(float), h, hipMemcpyHostToDevice);
-        if (e != hipSuccess) rc = fail(INTERNAL, std::string("H2D: ") + hipGetErrorString(e));
+    // plane c's DMA runs while plane c+1 is copied into the pinned staging
+    for (int c = 0; c < channels; ++c) {
+        par_rows(pin + c * npx, row, in[c], in_pitch * sizeof(float), row, h);
+        HIP_CHECK(hipMemcpyAsync(din[c], pin + c * npx, pb, hipMemcpyHostToDevice, nullptr));
     }
-    if (!rc) rc = cpipe_bm3d_plan_run(p, din, (size_t)w, dout, (size_t)w, sigma, nullptr);
-    for (int c = 0; c < channels && !rc; ++c) {
-        e = hipMemcpy2D(out[c], out_pitch * sizeof(float), dout[c], (size_t)w * sizeof(float),
-                        (size_t)w * sizeof(float), h, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(INTERNAL, std::string("D2H: ") + hipGetErrorString(e));
+    if ((rc = cpipe_bm3d_plan_run(sl->plan, din, (size_t)w, dout, (size_t)w, sigma, nullptr))) return rc;
+    float* pout = pin + channels * npx;
+    for (int c = 0; c < channels; ++c) {
+        HIP_CHECK(hipMemcpyAsync(pout + c * npx, dout[c], pb, hipMemcpyDeviceToHost, nullptr));
+        HIP_CHECK(hipEventRecord(sl->ev[c], nullptr));
     }
-    (void)hipFree(dev);
-    free_plan(p);
-    return rc;
+    for (int c = 0; c < channels; ++c) {
+        HIP_CHECK(hipEventSynchronize(sl->ev[c]));
+        par_rows(out[c], out_pitch * sizeof(float), pout + c * npx, row, row, h);
+    }
+    return cpipe_bm3d_plan_check(sl->plan);
 }
 
 int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_t osp, int w, int h,
-                            float sigma, int mode, int precision, int device) {
+                            float sigma, int mode, int precision, int device, int flags) {
     if (!in || !out) return fail(BAD_INDEX, "null argument");
     if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
-    const int ndev = cpipe_bm3d_device_count();
-    if (ndev <= 0) return fail(UNSUPPORTED, "no HIP device visible");
-    if (device < 0 || device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
-    DeviceGuard guard(device);
+    if (mode == 0 && (flags & CPIPE_BM3D_HOST_PILOT))
+        return fail(BAD_INDEX, "the learned pilot needs mode bm3d");
+    HostCache& hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
     const size_t bytes = (size_t)w * h * 8;
-    uint16_t* din = nullptr;
-    uint16_t* dout = nullptr;
-    HIP_CHECK(hipMalloc((void**)&din, bytes));
-    hipError_t e = hipMalloc((void**)&dout, bytes);
-    if (e != hipSuccess) {
-        (void)hipFree(din);
-        return fail(OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HostSlot* sl = nullptr;
+    int rc = host_slot(hc, mode == 0 ? HOST_COMPAT : HOST_RGBA, w, h, 3, precision, device, flags,
+                       2 * bytes, &sl);
+    if (rc) return rc;
+    DeviceGuard guard(device);
+    uint16_t* din = static_cast<uint16_t*>(sl->dev);
+    uint16_t* dout = din + (size_t)w * h * 4;
+    uint16_t* pin = static_cast<uint16_t*>(sl->pin);
+    uint16_t* pout = pin + (size_t)w * h * 4;
+    const size_t row = (size_t)w * 8;
+    par_rows(pin, row, in, isp * 8, row, h);
+    HIP_CHECK(hipMemcpyAsync(din, pin, bytes, hipMemcpyHostToDevice, nullptr));
+    if (mode == 0) {
+        rc = cpipe_bm3d_compat_rgba16f(din, (size_t)w, dout, (size_t)w, w, h, sigma, 1, nullptr);
+    } else {
+        rc = cpipe_bm3d_plan_run_rgba16f(sl->plan, din, (size_t)w, dout, (size_t)w, sigma, nullptr);
     }
-    int rc = OK;
-    e = hipMemcpy2D(din, (size_t)w * 8, in, isp * 8, (size_t)w * 8, h, hipMemcpyHostToDevice);
-    if (e != hipSuccess) rc = fail(INTERNAL, std::string("H2D: ") + hipGetErrorString(e));
-    if (!rc) {
-        if (mode == 0) {
-            rc = cpipe_bm3d_compat_rgba16f(din, (size_t)w, dout, (size_t)w, w, h, sigma, 1, nullptr);
-        } else {
-            cpipe_bm3d_plan_desc d{sizeof(cpipe_bm3d_plan_desc), w, h, 3, precision, device};
-            cpipe_bm3d_plan* p = nullptr;
-            rc = cpipe_bm3d_plan_create(&d, &p);
-            if (!rc) rc = cpipe_bm3d_plan_run_rgba16f(p, din, (size_t)w, dout, (size_t)w, sigma, nullptr);
-            if (!rc && hipDeviceSynchronize() != hipSuccess) rc = fail(INTERNAL, "device sync failed");
-            free_plan(p);
-        }
+    if (rc) return rc;
+    HIP_CHECK(hipMemcpyAsync(pout, dout, bytes, hipMemcpyDeviceToHost, nullptr));
+    HIP_CHECK(hipStreamSynchronize(nullptr));
+    par_rows(out, osp * 8, pout, row, row, h);
+    return mode == 0 ? OK : cpipe_bm3d_plan_check(sl->plan);
+}
+
+void cpipe_bm3d_host_release(void) {
+    HostCache& hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
+    for (HostSlot& sl : hc.slots) drop_slot(sl);
+    hc.slots.clear();
+    for (auto& kv : hc.nets) {
+        DeviceGuard guard(kv.first);
+        cpipe_bm3d_pilot_destroy(kv.second);
     }
-    if (!rc) {
-        e = hipMemcpy2D(out, osp * 8, dout, (size_t)w * 8, (size_t)w * 8, h, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(INTERNAL, std::string("D2H: ") + hipGetErrorString(e));
-    }
-    (void)hipFree(din);
-    (void)hipFree(dout);
-    return rc;
+    hc.nets.clear();
+}
+
+const void* cpipe_bm3d_pilot_embedded(size_t* size) {
+    if (size) *size = CPIPE_BM3D_PILOT_BLOB_SIZE;
+    return CPIPE_BM3D_PILOT_BLOB;
 }
 
 }  // extern "C"

@@ -156,7 +156,11 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
     }
     const std::string mode = enum_param(s, params, "mode", "CPIPE_BM3D_MODE", "compat");
     const std::string prec = enum_param(s, params, "precision", "CPIPE_BM3D_PRECISION", "fp32");
+    const std::string pilot = enum_param(s, params, "pilot", "CPIPE_BM3D_PILOT", "off");
     if (mode != "compat" && mode != "bm3d") return CPIPE_BAD_INDEX;
+    if (pilot != "off" && pilot != "learned") return CPIPE_BAD_INDEX;
+    if (pilot == "learned" && mode != "bm3d") return CPIPE_BAD_INDEX;
+    const int flags = pilot == "learned" ? CPIPE_BM3D_HOST_PILOT : 0;
     int precision = CPIPE_BM3D_PRECISION_FP32;
     if (prec == "bf16") {
         precision = CPIPE_BM3D_PRECISION_BF16;
@@ -200,7 +204,7 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
         if (rgba) {
             rc = cpipe_bm3d_host_rgba16f(static_cast<const uint16_t*>(inp[0]), in[0].row_bytes / 8,
                                          static_cast<uint16_t*>(outp[0]), out[0].row_bytes / 8, w, h,
-                                         sigma, mode == "bm3d" ? 1 : 0, precision, 0);
+                                         sigma, mode == "bm3d" ? 1 : 0, precision, 0, flags);
         } else {
             const float* ip[3] = {};
             float* op[3] = {};
@@ -214,7 +218,7 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
                     rc = CPIPE_UNSUPPORTED;
             if (rc == CPIPE_OK)
                 rc = cpipe_bm3d_host_planes(ip, ipitch, op, opitch, w, h, (int)ctx->n_in, sigma,
-                                            precision, 0);
+                                            precision, 0, flags);
         }
         rc = map_engine_status(rc, host);
     }

@@ -36,8 +36,9 @@ extern "C" {
 /* Arithmetic of the block-matching cross term and group transforms. */
 enum {
     CPIPE_BM3D_PRECISION_FP32 = 0, /* exact FP32 (fma order fixed, matches the CPU FP32 path) */
-    CPIPE_BM3D_PRECISION_BF16 = 1, /* BF16 MFMA cross term + FP32 re-rank */
-    CPIPE_BM3D_PRECISION_FP16 = 2  /* FP16 MFMA cross term + FP32 re-rank */
+    CPIPE_BM3D_PRECISION_BF16 = 1, /* BF16 MFMA group transforms, FP32 shrinkage and aggregation
+                                      (+ optional MFMA block-matching cross term, FP32 re-rank) */
+    CPIPE_BM3D_PRECISION_FP16 = 2  /* the same with FP16 operands */
 };
 
 /* Stage timers reported by cpipe_bm3d_plan_timing (milliseconds, summed over
@@ -242,13 +243,27 @@ int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* batch, int* tile_y, int* ord
  * test images); the autotuner times both. CPIPE_BM3D_UNSUPPORTED for FP32 plans. */
 int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* plan, int mfma);
 
-/* Host-memory conveniences used by the node: copy in, run, copy out, sync. */
+/* Host-memory conveniences used by the node (replace the Halide dispatch of
+ * bm3d_dispatch.cpp:19-41 behind DenoiseBm3d::process, bm3d.cpp:96-100):
+ * copy in, run, copy out, synchronise. The reference host calls a node's
+ * create/process/destroy on every run (Pipeline.cpp:716-772), so the plans,
+ * device buffers and the learned pilot are kept in a process-global,
+ * mutex-protected cache (the last CPIPE_BM3D_HOST_SLOTS shapes, least recently
+ * used evicted) and reused across calls; calls are serialised by the cache's
+ * lock. flags: CPIPE_BM3D_HOST_PILOT runs the learned pilot (weights embedded
+ * in the library) in place of the plain step 1 (3-channel images only). */
+#define CPIPE_BM3D_HOST_PILOT 1
+#define CPIPE_BM3D_HOST_SLOTS 4
 int cpipe_bm3d_host_planes(const float* const* in_planes, size_t in_pitch, float* const* out_planes,
                            size_t out_pitch, int width, int height, int channels, float sigma,
-                           int precision, int device);
+                           int precision, int device, int flags);
 int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
                             size_t out_stride_px, int width, int height, float sigma, int mode,
-                            int precision, int device);
+                            int precision, int device, int flags);
+/* Frees every cached plan, buffer and pilot network of the host conveniences. */
+void cpipe_bm3d_host_release(void);
+/* The learned-pilot weight blob embedded in the library (cpipe_amd/weights/pilot_v1.bin). */
+const void* cpipe_bm3d_pilot_embedded(size_t* size);
 
 #ifdef __cplusplus
 }

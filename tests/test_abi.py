@@ -67,6 +67,20 @@ def test_manifest_contract():
     assert "f16" in m["ports"][0]["caps"]["precision"] and "f32" in m["ports"][0]["caps"]["precision"]
     modes = [p for p in m["params"] if p["name"] == "mode"][0]
     assert modes["enum_values"] == ["compat", "bm3d"] and modes["default"] == "compat"
+    pil = [p for p in m["params"] if p["name"] == "pilot"][0]
+    assert pil["enum_values"] == ["off", "learned"] and pil["default"] == "off"
+
+
+def test_embedded_pilot_weights_match_the_package():
+    """The node's learned-pilot weights are the packaged blob, byte for byte."""
+    import ctypes
+    from cpipe_amd import engine, pilot
+    n = ctypes.c_size_t(0)
+    ptr = engine.lib().cpipe_bm3d_pilot_embedded(ctypes.byref(n))
+    with open(pilot.DEFAULT_WEIGHTS, "rb") as f:
+        want = f.read()
+    assert n.value == len(want)
+    assert ctypes.string_at(ptr, n.value) == want
 
 
 def test_actions_without_work():
