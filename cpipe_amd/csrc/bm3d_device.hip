@@ -73,7 +73,6 @@ int fail(int code, const std::string& msg) {
         }                                                                                \
     } while (0)
 
-__constant__ float c_dct[64];     // C[u][i], row u = frequency
 __constant__ float c_kaiser2[64]; // Kaiser(8, beta=2) outer product
 
 __host__ __device__ inline int ref_pos(int i, int n) {
@@ -479,60 +478,120 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
 // ---------------------------------------------------------------------------
 
 
-// Separable 2D DCT of the N member patches a wave holds (lane p = (i, j) holds
-// pixel p of every member), through a per-wave LDS scratch of 8 patches: each
-// pass writes one value per lane and reads the lane's row (or column, stored
-// transposed) back as two 16-byte reads. Per member the operation order is a
-// row pass then a column pass, each an fma chain in index order.
-constexpr int SCR = 8;  // members per LDS scratch chunk
+// 8-point orthonormal DCT-II and inverse, even/odd butterfly (36 operations
+// each; the CPU FP32 restatement uses the same operations in the same order).
+// Q0 = 1/sqrt(8), Qk = cos(k pi/16)/2, each rounded once to FP32.
+constexpr float DQ0 = 0x1.6a09e6p-2f, DQ1 = 0x1.f6297cp-2f, DQ2 = 0x1.d906bcp-2f,
+                DQ3 = 0x1.a9b662p-2f, DQ5 = 0x1.1c73b4p-2f, DQ6 = 0x1.87de2ap-3f,
+                DQ7 = 0x1.8f8b84p-4f;
 
-__device__ __forceinline__ float row8(const float* p, const float (&c)[8]) {
+__device__ __forceinline__ void fdct8(const float (&x)[8], float (&X)[8]) {
+    const float a0 = x[0] + x[7], a1 = x[1] + x[6], a2 = x[2] + x[5], a3 = x[3] + x[4];
+    const float b0 = x[0] - x[7], b1 = x[1] - x[6], b2 = x[2] - x[5], b3 = x[3] - x[4];
+    const float e0 = a0 + a3, e1 = a1 + a2, f0 = a0 - a3, f1 = a1 - a2;
+    X[0] = (e0 + e1) * DQ0;
+    X[4] = (e0 - e1) * DQ0;
+    X[2] = __builtin_fmaf(DQ2, f0, DQ6 * f1);
+    X[6] = __builtin_fmaf(DQ6, f0, -(DQ2 * f1));
+    X[1] = __builtin_fmaf(DQ7, b3, __builtin_fmaf(DQ5, b2, __builtin_fmaf(DQ3, b1, DQ1 * b0)));
+    X[3] = __builtin_fmaf(-DQ5, b3, __builtin_fmaf(-DQ1, b2, __builtin_fmaf(-DQ7, b1, DQ3 * b0)));
+    X[5] = __builtin_fmaf(DQ3, b3, __builtin_fmaf(DQ7, b2, __builtin_fmaf(-DQ1, b1, DQ5 * b0)));
+    X[7] = __builtin_fmaf(-DQ1, b3, __builtin_fmaf(DQ3, b2, __builtin_fmaf(-DQ5, b1, DQ7 * b0)));
+}
+
+__device__ __forceinline__ void idct8(const float (&X)[8], float (&x)[8]) {
+    const float e0 = (X[0] + X[4]) * DQ0, e1 = (X[0] - X[4]) * DQ0;
+    const float f0 = __builtin_fmaf(DQ2, X[2], DQ6 * X[6]);
+    const float f1 = __builtin_fmaf(DQ6, X[2], -(DQ2 * X[6]));
+    const float a0 = e0 + f0, a3 = e0 - f0, a1 = e1 + f1, a2 = e1 - f1;
+    const float b0 = __builtin_fmaf(DQ7, X[7], __builtin_fmaf(DQ5, X[5], __builtin_fmaf(DQ3, X[3], DQ1 * X[1])));
+    const float b1 = __builtin_fmaf(-DQ5, X[7], __builtin_fmaf(-DQ1, X[5], __builtin_fmaf(-DQ7, X[3], DQ3 * X[1])));
+    const float b2 = __builtin_fmaf(DQ3, X[7], __builtin_fmaf(DQ7, X[5], __builtin_fmaf(-DQ1, X[3], DQ5 * X[1])));
+    const float b3 = __builtin_fmaf(-DQ1, X[7], __builtin_fmaf(DQ3, X[5], __builtin_fmaf(-DQ5, X[3], DQ7 * X[1])));
+    x[0] = a0 + b0;
+    x[7] = a0 - b0;
+    x[1] = a1 + b1;
+    x[6] = a1 - b1;
+    x[2] = a2 + b2;
+    x[5] = a2 - b2;
+    x[3] = a3 + b3;
+    x[4] = a3 - b3;
+}
+
+// Separable 2D DCT of the N member patches a wave holds (lane p = (i, j) holds
+// pixel p of every member), eight members at a time through a per-wave LDS
+// scratch of 8 patches (patch pitch 72 floats: every access below is free of
+// bank conflicts). A pass gives lane l = 8 r + q the whole line q (a row, or a
+// column stored transposed) of patch r as two 16-byte reads and transforms it
+// in registers; the results go back as one scalar write per coefficient.
+// Forward: rows, then columns. Inverse: columns, then rows.
+constexpr int SCR = 8;   // members per LDS scratch chunk
+constexpr int SPP = 72;  // scratch floats per patch
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void load_line(const float* p, float (&x)[8]) {
     const float4 a = *reinterpret_cast<const float4*>(p);
     const float4 b = *reinterpret_cast<const float4*>(p + 4);
-    float r = c[0] * a.x;
-    r = __builtin_fmaf(c[1], a.y, r);
-    r = __builtin_fmaf(c[2], a.z, r);
-    r = __builtin_fmaf(c[3], a.w, r);
-    r = __builtin_fmaf(c[4], b.x, r);
-    r = __builtin_fmaf(c[5], b.y, r);
-    r = __builtin_fmaf(c[6], b.z, r);
-    r = __builtin_fmaf(c[7], b.w, r);
-    return r;
+    x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w;
+    x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
 }
 
 template <int N>
-__device__ __forceinline__ void dct_fwd_n(float (&v)[NMAX], float* sb, int lane, int li, int lj,
-                                          const float (&Cr)[8], const float (&Cc)[8]) {
+__device__ __forceinline__ void dct_fwd_n(float (&v)[NMAX], float* sb, int lane) {
+    const int r = lane >> 3, q = lane & 7;
+    float* line = sb + r * SPP + q * 8;
 #pragma unroll
     for (int c0 = 0; c0 < N; c0 += SCR) {
         constexpr int CH = N < SCR ? N : SCR;
-        float y[CH];
 #pragma unroll
-        for (int m = 0; m < CH; ++m) sb[m * 64 + lane] = v[c0 + m];
+        for (int m = 0; m < CH; ++m) sb[m * SPP + lane] = v[c0 + m];  // [m][i][j]
+        wave_lds_fence();
+        float x[8], X[8];
+        load_line(line, x);  // row q of patch r
+        fdct8(x, X);
 #pragma unroll
-        for (int m = 0; m < CH; ++m) y[m] = row8(sb + m * 64 + li * 8, Cr);  // y[i][v=j]
+        for (int k = 0; k < 8; ++k) sb[r * SPP + k * 8 + q] = X[k];  // stored [v][i]
+        wave_lds_fence();
+        load_line(line, x);  // column q
+        fdct8(x, X);
 #pragma unroll
-        for (int m = 0; m < CH; ++m) sb[m * 64 + lj * 8 + li] = y[m];        // stored [v][i]
+        for (int k = 0; k < 8; ++k) sb[r * SPP + k * 8 + q] = X[k];  // t[u][v = q]
+        wave_lds_fence();
 #pragma unroll
-        for (int m = 0; m < CH; ++m) v[c0 + m] = row8(sb + m * 64 + lj * 8, Cc);  // t[u=i][v=j]
+        for (int m = 0; m < CH; ++m) v[c0 + m] = sb[m * SPP + lane];  // lane = (u, v)
+        wave_lds_fence();
     }
 }
 
 template <int N>
-__device__ __forceinline__ void dct_inv_n(float (&v)[NMAX], float* sb, int lane, int li, int lj,
-                                          const float (&Ci1)[8], const float (&Ci2)[8]) {
+__device__ __forceinline__ void dct_inv_n(float (&v)[NMAX], float* sb, int lane, int li, int lj) {
+    const int r = lane >> 3, q = lane & 7;
+    float* line = sb + r * SPP + q * 8;
 #pragma unroll
     for (int c0 = 0; c0 < N; c0 += SCR) {
         constexpr int CH = N < SCR ? N : SCR;
-        float y[CH];
 #pragma unroll
-        for (int m = 0; m < CH; ++m) sb[m * 64 + lj * 8 + li] = v[c0 + m];   // t stored [v][u]
+        for (int m = 0; m < CH; ++m) sb[m * SPP + lj * 8 + li] = v[c0 + m];  // t stored [v][u]
+        wave_lds_fence();
+        float x[8], X[8];
+        load_line(line, X);  // column q of patch r
+        idct8(X, x);
 #pragma unroll
-        for (int m = 0; m < CH; ++m) y[m] = row8(sb + m * 64 + lj * 8, Ci1);  // y[i][v=j]
+        for (int k = 0; k < 8; ++k) sb[r * SPP + k * 8 + q] = x[k];  // y[i][v = q]
+        wave_lds_fence();
+        load_line(line, X);  // row q
+        idct8(X, x);
+        *reinterpret_cast<float4*>(line) = make_float4(x[0], x[1], x[2], x[3]);
+        *reinterpret_cast<float4*>(line + 4) = make_float4(x[4], x[5], x[6], x[7]);
+        wave_lds_fence();
 #pragma unroll
-        for (int m = 0; m < CH; ++m) sb[m * 64 + lane] = y[m];               // stored [i][v]
-#pragma unroll
-        for (int m = 0; m < CH; ++m) v[c0 + m] = row8(sb + m * 64 + li * 8, Ci2);  // x[i][j]
+        for (int m = 0; m < CH; ++m) v[c0 + m] = sb[m * SPP + lane];  // x[i][j]
+        wave_lds_fence();
     }
 }
 
@@ -571,10 +630,6 @@ __device__ __forceinline__ void haar_inv_n(float (&v)[NMAX], float s) {
 }
 
 
-struct LaneTables {
-    float Cr[8], Cc[8], Ci1[8], Ci2[8];
-};
-
 // Filters one group in place: loads the members' pixels (noisy, and the basic
 // estimate for step 2), 3D transform, shrinkage, inverse; returns the group's
 // aggregation weight (1/N_nonzero for the hard threshold, 1/sum W^2 for Wiener).
@@ -582,8 +637,8 @@ template <int STEP, int N>
 __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (&mp)[NMAX],
                                               const float* __restrict__ src,
                                               const float* __restrict__ bsrc, int w, float* sb,
-                                              int lane, int li, int lj, const LaneTables& T,
-                                              float hs, float thr, float s2) {
+                                              int lane, int li, int lj, float hs, float thr,
+                                              float s2) {
     float b[NMAX];
 #pragma unroll
     for (int m = 0; m < N; ++m) {
@@ -592,7 +647,7 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
         v[m] = src[off];
         if (STEP == 2) b[m] = bsrc[off];
     }
-    dct_fwd_n<N>(v, sb, lane, li, lj, T.Cr, T.Cc);
+    dct_fwd_n<N>(v, sb, lane);
     haar_fwd_n<N>(v, hs);
     float wgt;
     if (STEP == 1) {
@@ -607,7 +662,7 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
         for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
         wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
     } else {
-        dct_fwd_n<N>(b, sb, lane, li, lj, T.Cr, T.Cc);
+        dct_fwd_n<N>(b, sb, lane);
         haar_fwd_n<N>(b, hs);
         float aw = 0.0f;
 #pragma unroll
@@ -623,14 +678,14 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
         wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
     }
     haar_inv_n<N>(v, hs);
-    dct_inv_n<N>(v, sb, lane, li, lj, T.Ci1, T.Ci2);
+    dct_inv_n<N>(v, sb, lane, li, lj);
     return wgt;
 }
 
 template <int STEP>
 __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
     __shared__ float acc[2 * FOOT2];
-    __shared__ __align__(16) float scratch[4][SCR * 64];
+    __shared__ __align__(16) float scratch[4][SCR * SPP];
     __shared__ int turn;  // number of groups of this tile already accumulated
     const int2 item = L.items[blockIdx.x];
     const FrameTab& F = L.tab[item.x];
@@ -649,14 +704,6 @@ __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane >> 3, lj = lane & 7;
     float* sb = scratch[wave];
-    LaneTables T;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        T.Cr[q] = c_dct[lj * 8 + q];
-        T.Cc[q] = c_dct[li * 8 + q];
-        T.Ci1[q] = c_dct[q * 8 + li];
-        T.Ci2[q] = c_dct[q * 8 + lj];
-    }
     const float kw = c_kaiser2[lane];
     const float hs = (float)M_SQRT1_2;
     const float* __restrict__ src = F.z[c];
@@ -689,11 +736,11 @@ __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
                 mp[4 * q + 3] = __builtin_amdgcn_readfirstlane(t.w);
             }
             switch (n) {
-                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
-                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
-                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
-                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
-                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, T, hs, thr, s2); break;
+                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
+                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
+                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
+                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
+                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
             }
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
@@ -884,7 +931,6 @@ int ensure_tables(int device) {
     for (int i = 0; i < 64; ++i) cf[i] = (float)c[i];
     for (int i = 0; i < 8; ++i)
         for (int j = 0; j < 8; ++j) kf[i * 8 + j] = (float)(k1[i] * k1[j]);
-    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dct), cf, sizeof(cf)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_kaiser2), kf, sizeof(kf)));
     HIP_CHECK((hipError_t)group_mx_tables(device, cf, kf));
     g_tables_ready[device] = true;

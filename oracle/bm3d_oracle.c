@@ -298,36 +298,72 @@ void oracle_bm_f32(const float* plane, int width, int height, float tau_ssd, uin
 /* FP32 mirror: 3D transforms                                               */
 /* ------------------------------------------------------------------------ */
 
-static void dct2_fwd_f32(const float* C, const float* x, float* t) {
-    float y[64];
-    for (int i = 0; i < 8; ++i)
-        for (int v = 0; v < 8; ++v) {
-            float acc = C[v * 8] * x[i * 8];
-            for (int j = 1; j < 8; ++j) acc = fmaf(C[v * 8 + j], x[i * 8 + j], acc);
-            y[i * 8 + v] = acc;
-        }
-    for (int u = 0; u < 8; ++u)
-        for (int v = 0; v < 8; ++v) {
-            float acc = C[u * 8] * y[v];
-            for (int i = 1; i < 8; ++i) acc = fmaf(C[u * 8 + i], y[i * 8 + v], acc);
-            t[u * 8 + v] = acc;
-        }
+/* 8-point orthonormal DCT-II and its inverse in FP32, even/odd butterfly
+ * (36 operations each). Constants: Q0 = 1/sqrt(8), Qk = cos(k pi/16)/2, each
+ * rounded once to FP32. Forward: a = x_n + x_{7-n}, b = x_n - x_{7-n};
+ * e = a0+a3, a1+a2, f = a0-a3, a1-a2; X0 = (e0+e1) Q0, X4 = (e0-e1) Q0,
+ * X2 = fma(Q2, f0, Q6 f1), X6 = fma(Q6, f0, -(Q2 f1)); odd outputs are fma
+ * chains over b0..b3 in index order. The inverse is the exact transpose of the
+ * same factorisation. The GPU exact kernels use this order operation for
+ * operation (no contraction besides the written fmaf). */
+static const float kQ0 = 0x1.6a09e6p-2f, kQ1 = 0x1.f6297cp-2f, kQ2 = 0x1.d906bcp-2f,
+                   kQ3 = 0x1.a9b662p-2f, kQ5 = 0x1.1c73b4p-2f, kQ6 = 0x1.87de2ap-3f,
+                   kQ7 = 0x1.8f8b84p-4f;
+
+static void dct8_fwd_f32(const float* x, int sx, float* X, int sX) {
+    const float a0 = x[0] + x[7 * sx], a1 = x[sx] + x[6 * sx], a2 = x[2 * sx] + x[5 * sx],
+                a3 = x[3 * sx] + x[4 * sx];
+    const float b0 = x[0] - x[7 * sx], b1 = x[sx] - x[6 * sx], b2 = x[2 * sx] - x[5 * sx],
+                b3 = x[3 * sx] - x[4 * sx];
+    const float e0 = a0 + a3, e1 = a1 + a2, f0 = a0 - a3, f1 = a1 - a2;
+    X[0] = (e0 + e1) * kQ0;
+    X[4 * sX] = (e0 - e1) * kQ0;
+    X[2 * sX] = fmaf(kQ2, f0, kQ6 * f1);
+    X[6 * sX] = fmaf(kQ6, f0, -(kQ2 * f1));
+    X[sX] = fmaf(kQ7, b3, fmaf(kQ5, b2, fmaf(kQ3, b1, kQ1 * b0)));
+    X[3 * sX] = fmaf(-kQ5, b3, fmaf(-kQ1, b2, fmaf(-kQ7, b1, kQ3 * b0)));
+    X[5 * sX] = fmaf(kQ3, b3, fmaf(kQ7, b2, fmaf(-kQ1, b1, kQ5 * b0)));
+    X[7 * sX] = fmaf(-kQ1, b3, fmaf(kQ3, b2, fmaf(-kQ5, b1, kQ7 * b0)));
 }
 
-static void dct2_inv_f32(const float* C, const float* t, float* x) {
+static void dct8_inv_f32(const float* X, int sX, float* x, int sx) {
+    const float e0 = (X[0] + X[4 * sX]) * kQ0, e1 = (X[0] - X[4 * sX]) * kQ0;
+    const float f0 = fmaf(kQ2, X[2 * sX], kQ6 * X[6 * sX]);
+    const float f1 = fmaf(kQ6, X[2 * sX], -(kQ2 * X[6 * sX]));
+    const float a0 = e0 + f0, a3 = e0 - f0, a1 = e1 + f1, a2 = e1 - f1;
+    const float X1 = X[sX], X3 = X[3 * sX], X5 = X[5 * sX], X7 = X[7 * sX];
+    const float b0 = fmaf(kQ7, X7, fmaf(kQ5, X5, fmaf(kQ3, X3, kQ1 * X1)));
+    const float b1 = fmaf(-kQ5, X7, fmaf(-kQ1, X5, fmaf(-kQ7, X3, kQ3 * X1)));
+    const float b2 = fmaf(kQ3, X7, fmaf(kQ7, X5, fmaf(-kQ1, X3, kQ5 * X1)));
+    const float b3 = fmaf(-kQ1, X7, fmaf(kQ3, X5, fmaf(-kQ5, X3, kQ7 * X1)));
+    x[0] = a0 + b0;
+    x[7 * sx] = a0 - b0;
+    x[sx] = a1 + b1;
+    x[6 * sx] = a1 - b1;
+    x[2 * sx] = a2 + b2;
+    x[5 * sx] = a2 - b2;
+    x[3 * sx] = a3 + b3;
+    x[4 * sx] = a3 - b3;
+}
+
+void oracle_dct8_f32(const float* in, float* out, int inverse) {
+    if (inverse)
+        dct8_inv_f32(in, 1, out, 1);
+    else
+        dct8_fwd_f32(in, 1, out, 1);
+}
+
+/* 2D: forward = rows then columns; inverse = columns then rows. */
+static void dct2_fwd_f32(const float* x, float* t) {
     float y[64];
-    for (int i = 0; i < 8; ++i)
-        for (int v = 0; v < 8; ++v) {
-            float acc = C[i] * t[v];
-            for (int u = 1; u < 8; ++u) acc = fmaf(C[u * 8 + i], t[u * 8 + v], acc);
-            y[i * 8 + v] = acc;
-        }
-    for (int i = 0; i < 8; ++i)
-        for (int j = 0; j < 8; ++j) {
-            float acc = C[j] * y[i * 8];
-            for (int v = 1; v < 8; ++v) acc = fmaf(C[v * 8 + j], y[i * 8 + v], acc);
-            x[i * 8 + j] = acc;
-        }
+    for (int i = 0; i < 8; ++i) dct8_fwd_f32(x + i * 8, 1, y + i * 8, 1);
+    for (int v = 0; v < 8; ++v) dct8_fwd_f32(y + v, 8, t + v, 8);
+}
+
+static void dct2_inv_f32(const float* t, float* x) {
+    float y[64];
+    for (int v = 0; v < 8; ++v) dct8_inv_f32(t + v, 8, y + v, 8);
+    for (int i = 0; i < 8; ++i) dct8_inv_f32(y + i * 8, 1, x + i * 8, 1);
 }
 
 /* Orthonormal multi-level Haar along the stack; output [approx, d_L..d_1]. */
@@ -404,7 +440,6 @@ static void group_job_f32(void* vc, int job) {
     float* num = a->partial + ((size_t)c * a->tx * a->ty + tile) * a->tile_stride;
     float* den = num + FOOT_MAX * FOOT_MAX;
     memset(num, 0, sizeof(float) * 2 * FOOT_MAX * FOOT_MAX);
-    const float* C = a->tb->C;
     const float s = a->tb->haar_s;
     const float sc = a->sig[c];
     const float thr = 2.7f * sc;
@@ -421,12 +456,12 @@ static void group_job_f32(void* vc, int job) {
                 for (int i = 0; i < 8; ++i)
                     for (int j = 0; j < 8; ++j)
                         tmp[i * 8 + j] = a->noisy[c][(size_t)(qy + i) * a->w + qx + j];
-                dct2_fwd_f32(C, tmp, g[m]);
+                dct2_fwd_f32(tmp, g[m]);
                 if (a->step == 2) {
                     for (int i = 0; i < 8; ++i)
                         for (int j = 0; j < 8; ++j)
                             tmp[i * 8 + j] = a->basic[c][(size_t)(qy + i) * a->w + qx + j];
-                    dct2_fwd_f32(C, tmp, gb[m]);
+                    dct2_fwd_f32(tmp, gb[m]);
                 }
             }
             float wgt;
@@ -476,7 +511,7 @@ static void group_job_f32(void* vc, int job) {
             for (int m = 0; m < n; ++m) {
                 const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);
                 float out[64];
-                dct2_inv_f32(C, g[m], out);
+                dct2_inv_f32(g[m], out);
                 for (int i = 0; i < 8; ++i)
                     for (int j = 0; j < 8; ++j) {
                         const int p = i * 8 + j;

@@ -56,6 +56,9 @@ int oracle_ref_pos(int i, int n);
 
 /* 8x8 orthonormal DCT-II matrix (row u = frequency) and 8-tap Kaiser (beta=2). */
 void oracle_dct_matrix_f64(double c[64]);
+/* One 8-point orthonormal DCT-II (inverse = 1: its inverse) in FP32 with the
+ * butterfly order the FP32 BM3D restatement and the GPU exact kernels use. */
+void oracle_dct8_f32(const float* in, float* out, int inverse);
 void oracle_kaiser_f64(double k[8]);
 
 /* Block matching in FP32 on one plane. Writes for every reference position

@@ -55,6 +55,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_ref_pos.argtypes = [I, I]
         L.oracle_ref_pos.restype = I
         L.oracle_dct_matrix_f64.argtypes = [P]
+        L.oracle_dct8_f32.argtypes = [P, P, I]
+        L.oracle_dct8_f32.restype = None
         L.oracle_kaiser_f64.argtypes = [P]
         L.oracle_bm_f32.argtypes = [P, I, I, F, P, P, I]
         L.oracle_bm_f64.argtypes = [P, I, I, D, P, P]
@@ -106,6 +108,15 @@ def compat_rgba16f(rgba_bits: np.ndarray, sigma: float, stage: int = 1) -> np.nd
 def ref_positions(n: int) -> np.ndarray:
     L = lib()
     return np.array([L.oracle_ref_pos(i, n) for i in range(L.oracle_ref_count(n))], dtype=np.int64)
+
+
+def dct8_f32(x, inverse: bool = False) -> np.ndarray:
+    """One 8-point orthonormal DCT-II (or its inverse) with the FP32 butterfly
+    order of the FP32 restatement (bm3d_oracle.c dct8_fwd_f32 / dct8_inv_f32)."""
+    a = np.ascontiguousarray(x, np.float32).reshape(8)
+    out = np.empty(8, np.float32)
+    lib().oracle_dct8_f32(_ptr(a), _ptr(out), 1 if inverse else 0)
+    return out
 
 
 def dct_matrix() -> np.ndarray:

@@ -253,3 +253,29 @@ def test_wiener_with_clean_pilot_beats_hard_threshold(oracle_lib):
     clean, noisy, sigma = synth.workload("gray128x128", seed=0)
     out, basic = oracle_lib.bm3d_f64(noisy.astype(np.float64), sigma, want_basic=True)
     assert synth.psnr(out, clean) > synth.psnr(basic, clean)
+
+
+def test_dct8_f32_butterfly_matches_the_dense_transform(oracle_lib):
+    """The FP32 butterfly (the order the FP32 restatement and the GPU exact
+    kernels share) against the FP64 dense DCT-II matrix: forward, inverse and
+    round trip, to FP32 rounding; a dropped term, a wrong sign or a swapped
+    index in any output is off by O(1)."""
+    c = oracle_lib.dct_matrix()
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        x = rng.uniform(-1, 1, 8).astype(np.float32)
+        X = oracle_lib.dct8_f32(x)
+        np.testing.assert_allclose(X, c @ x.astype(np.float64), atol=2e-6, rtol=0)
+        xi = oracle_lib.dct8_f32(X, inverse=True)
+        np.testing.assert_allclose(xi, c.T @ X.astype(np.float64), atol=2e-6, rtol=0)
+        np.testing.assert_allclose(xi, x, atol=4e-6, rtol=0)
+    # unit vectors: each output column of the matrix separately
+    for n in range(8):
+        e = np.zeros(8, np.float32)
+        e[n] = 1.0
+        np.testing.assert_allclose(oracle_lib.dct8_f32(e), c[:, n], atol=1e-7, rtol=0)
+        np.testing.assert_allclose(oracle_lib.dct8_f32(e, inverse=True), c[n, :], atol=1e-7, rtol=0)
+    # constant input: only the DC survives (exact zeros by the butterfly's symmetry)
+    X = oracle_lib.dct8_f32(np.full(8, 0.25, np.float32))
+    assert X[0] == np.float32(np.float32(2.0) * np.float32(1 / np.sqrt(8)))  # (e0 + e1) Q0, e = 1 + 1
+    assert np.all(X[1:] == 0.0)
