@@ -135,6 +135,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int RPW = CTH / 4;  // adjacent output rows per wave
     __shared__ __align__(16) _Float16 tile[(CTH + 2) * (TW + 2) * PS];
     __shared__ __align__(16) _Float16 ob[4][16 * CO];  // per wave: 16 output pixels x CO
+    // EPI 1: per wave, 16 pixels x 3 output channels (acc + bias) before the residual add
+    __shared__ float ofp[EPI == 1 ? 4 * 3 * 16 : 1];
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
     const int w = a.w, h = a.h;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -227,17 +229,24 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
             const int oy = y0 + ly0 + r;
             if (oy >= h) continue;  // uniform per wave
             if (EPI == 1) {
+                // out = residual + (acc + bias): staged through LDS so that lane
+                // 16 c + p stores pixel p of plane c (coalesced per plane)
+                float* of = ofp + wave * 48;
+                if (lr < 3) {
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    const int co = nb * 16 + lr;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int ox = x0 + seg * 16 + lq * 4 + e;
-                        if (ox >= w || co >= 3) continue;
-                        const size_t q = img * a.img_stride_planes + (size_t)oy * a.pitch + ox;
-                        a.outp[co][q] = a.res[co][q] + (acc[r][nb][e] + a.bias[co]);
-                    }
+                    for (int e = 0; e < 4; ++e) of[lr * 16 + lq * 4 + e] = acc[r][0][e] + a.bias[lr];
                 }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int c = lane >> 4, ox = x0 + seg * 16 + (lane & 15);
+                if (lane < 48 && ox < w) {
+                    const size_t q = img * a.img_stride_planes + (size_t)oy * a.pitch + ox;
+                    a.outp[c][q] = a.res[c][q] + of[lane];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 continue;
             }
             // D[i = pixel][j = channel] (lane lr + 16 lq, element e: pixel 4 lq + e):
@@ -270,6 +279,136 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                         for (int e = 0; e < 8; ++e) v[e] = (float)m[e] > 0.0f ? v[e] : (_Float16)0.0f;
                     }
                     *reinterpret_cast<half8*>(a.out + p * CO + ch * 8) = v;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// First layer for inference, fused with the input packing: reads the three
+// FP32 opponent planes (+ the noise level as channel 3) straight into an LDS
+// tile of 4 FP16 channels per pixel, and packs four taps x four channels into
+// the K = 16 of one v_mfma_f32_16x16x16f16: K index k = 4 t + c for tap
+// t = 3 ky + kx (t >= 9: zero). Three MFMAs per 16 pixels x 16 output
+// channels instead of nine with twelve zero channels. Output as EPI 0
+// (ReLU(acc + bias), FP16 NHWC, 32 channels). Same tile shape as k_conv3x3.
+// ---------------------------------------------------------------------------
+
+constexpr int FIRST_KB = 3;  // ceil(9 taps * 4 channels / 16)
+
+// B fragments of the packed first layer: frag[(kb * NB + nb) * 64 + lane][e]
+// = W[t][c][j] with k = 16 kb + 4 (lane / 16) + e = 4 t + c, j = 16 nb + lane % 16.
+__global__ void k_frag_first(const _Float16* __restrict__ w16, _Float16* __restrict__ dst) {
+    constexpr int CO = COUT[0], CI = CIN[0], NB = CO / 16;
+    const int idx = threadIdx.x;  // (kb * NB + nb) * 64 + lane
+    if (idx >= FIRST_KB * NB * 64) return;
+    const int lane = idx & 63, f = idx >> 6, nb = f % NB, kb = f / NB;
+    const int j = nb * 16 + (lane & 15);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = kb * 16 + (lane >> 4) * 4 + e, t = k >> 2, c = k & 3;
+        dst[(size_t)idx * 4 + e] = t < 9 ? w16[((size_t)t * CI + c) * CO + j] : (_Float16)0.0f;
+    }
+}
+
+struct FirstArgs {
+    const float* in[3];       // opponent planes
+    size_t pitch, img_stride_planes;
+    float level;              // 10 sigma (when levels is null)
+    const float* levels;      // per image
+    const _Float16* frag;     // k_frag_first layout
+    const float* bias;        // [32]
+    _Float16* out;            // NHWC, 32 channels
+    int w, h;
+};
+
+__global__ __launch_bounds__(256) void k_conv_first(FirstArgs a) {
+    constexpr int CO = COUT[0], NB = CO / 16, RPW = CTH / 4;
+    constexpr int TP = (CTH + 2) * (TW + 2);
+    __shared__ __align__(16) half4 tile[TP];
+    __shared__ __align__(16) _Float16 ob[4][16 * CO];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
+    const int w = a.w, h = a.h;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    half4 bw[FIRST_KB][NB];
+#pragma unroll
+    for (int kb = 0; kb < FIRST_KB; ++kb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+            bw[kb][nb] = *reinterpret_cast<const half4*>(a.frag + (size_t)((kb * NB + nb) * 64 + lane) * 4);
+    const float lv = a.levels ? a.levels[img] : a.level;
+    constexpr int IT = (TP + 255) / 256;
+    float sv[IT][3];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int pix = threadIdx.x + 256 * i;
+        const int ty = pix / (TW + 2), tx = pix - ty * (TW + 2);
+        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+        sv[i][0] = sv[i][1] = sv[i][2] = 0.0f;
+        if (pix < TP && gx >= 0 && gx < w && gy >= 0 && gy < h) {
+            const size_t q = img * a.img_stride_planes + (size_t)gy * a.pitch + gx;
+            sv[i][0] = a.in[0][q];
+            sv[i][1] = a.in[1][q];
+            sv[i][2] = a.in[2][q];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int pix = threadIdx.x + 256 * i;
+        const int ty = pix / (TW + 2), tx = pix - ty * (TW + 2);
+        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+        const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+        if (pix < TP)
+            tile[pix] = (half4){(_Float16)sv[i][0], (_Float16)sv[i][1], (_Float16)sv[i][2],
+                                in ? (_Float16)lv : (_Float16)0.0f};
+    }
+    __syncthreads();
+    _Float16* o16 = ob[wave];
+    typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+#pragma unroll 1
+    for (int r = 0; r < RPW; ++r) {
+        const int ly = wave * RPW + r, oy = y0 + ly;
+#pragma unroll 1
+        for (int seg = 0; seg < TW / 16; ++seg) {
+            float4v acc[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4v){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < FIRST_KB; ++kb) {
+                const int t = kb * 4 + lq, ky = t / 3, kx = t - 3 * (t / 3);
+                half4 av = {0, 0, 0, 0};
+                if (t < 9) av = tile[(ly + ky) * (TW + 2) + seg * 16 + lr + kx];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bw[kb][nb], acc[nb], 0, 0, 0);
+            }
+            if (oy >= h) continue;  // uniform per wave
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + lr;
+                const float b = a.bias[co];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = acc[nb][e] + b;
+                    o16[(lq * 4 + e) * CO + co] = (_Float16)(v > 0.0f ? v : 0.0f);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            constexpr int CH = CO / 8;
+#pragma unroll
+            for (int k = lane; k < 16 * CH; k += 64) {
+                const int px = k / CH, ch = k % CH;
+                const int ox = x0 + seg * 16 + px;
+                if (ox < w) {
+                    const size_t p = (size_t)img * w * h + (size_t)oy * w + ox;
+                    *reinterpret_cast<half8*>(a.out + p * CO + ch * 8) = *reinterpret_cast<const half8*>(o16 + px * CO + ch * 8);
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -483,6 +622,7 @@ struct cpipe_bm3d_pilot_s {
     _Float16* w16 = nullptr;
     _Float16* frag = nullptr;  // per layer: forward then backward B fragments (k_frag)
     size_t frag_off[NL][2] = {};
+    size_t frag_first = 0;  // first layer, taps packed into K (k_frag_first)
     uint8_t* live = nullptr;
     uint64_t step = 0;
     uint64_t seed = 0;
@@ -538,7 +678,7 @@ int ensure_ws(cpipe_bm3d_pilot* n, size_t px, bool train) {
     (void)hipDeviceSynchronize();
     free_ws(n);
     const size_t want = px;
-    PHIP(hipMalloc((void**)&n->act[0], want * 16 * sizeof(_Float16)));
+    if (train) PHIP(hipMalloc((void**)&n->act[0], want * 16 * sizeof(_Float16)));  // packed input
     const int nact = train ? NL : 3;
     for (int l = 1; l < nact; ++l) PHIP(hipMalloc((void**)&n->act[l], want * 32 * sizeof(_Float16)));
     if (train) {
@@ -571,6 +711,7 @@ int refresh_frags(cpipe_bm3d_pilot* n, hipStream_t st) {
     k_frag<16, 32><<<blocks[0], 256, 0, st>>>(n->w16 + n->off[0], f(0), b(0));
     for (int l = 1; l < NL - 1; ++l) k_frag<32, 32><<<blocks[l], 256, 0, st>>>(n->w16 + n->off[l], f(l), b(l));
     k_frag<32, 16><<<blocks[NL - 1], 256, 0, st>>>(n->w16 + n->off[NL - 1], f(NL - 1), b(NL - 1));
+    k_frag_first<<<1, FIRST_KB * (COUT[0] / 16) * 64, 0, st>>>(n->w16 + n->off[0], n->frag + n->frag_first);
     PHIP(hipGetLastError());
     return 0;
 }
@@ -589,20 +730,37 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
     const size_t px = (size_t)w * h * n_img;
     int rc = ensure_ws(n, px, train);
     if (rc) return rc;
-    k_pack_input<<<dim3((w + 255) / 256, h, n_img), 256, 0, st>>>(in[0], in[1], in[2], pitch, level,
-                                                                   levels, w, h, img_stride, n->act[0]);
-    PHIP(hipGetLastError());
     ConvArgs a{};
     a.w = w;
     a.h = h;
     a.pitch = pitch;
     a.img_stride_planes = img_stride;
-    // L1
-    a.in = n->act[0];
-    a.frag = n->frag + n->frag_off[0][0];
-    a.bias = n->w32 + n->off[0] + layer_w(0);
-    a.out = n->act[1];
-    if ((rc = launch_conv<16, 32, 0>(a, n_img, st))) return rc;
+    if (train) {
+        // L1 on the packed input, which the weight gradient reads later
+        k_pack_input<<<dim3((w + 255) / 256, h, n_img), 256, 0, st>>>(in[0], in[1], in[2], pitch, level,
+                                                                       levels, w, h, img_stride, n->act[0]);
+        PHIP(hipGetLastError());
+        a.in = n->act[0];
+        a.frag = n->frag + n->frag_off[0][0];
+        a.bias = n->w32 + n->off[0] + layer_w(0);
+        a.out = n->act[1];
+        if ((rc = launch_conv<16, 32, 0>(a, n_img, st))) return rc;
+    } else {
+        // L1 fused with the packing, taps packed into K
+        FirstArgs f{};
+        for (int c = 0; c < 3; ++c) f.in[c] = in[c];
+        f.pitch = pitch;
+        f.img_stride_planes = img_stride;
+        f.level = level;
+        f.levels = levels;
+        f.frag = n->frag + n->frag_first;
+        f.bias = n->w32 + n->off[0] + layer_w(0);
+        f.out = n->act[1];
+        f.w = w;
+        f.h = h;
+        k_conv_first<<<dim3((w + TW - 1) / TW, (h + CTH - 1) / CTH, n_img), 256, 0, st>>>(f);
+        PHIP(hipGetLastError());
+    }
     int cur = 1;
     for (int l = 1; l < NL - 1; ++l) {
         const int nxt = train ? l + 1 : (cur == 1 ? 2 : 1);
@@ -751,6 +909,8 @@ int cpipe_bm3d_pilot_create(int device, cpipe_bm3d_pilot** out) {
         n->frag_off[l][1] = fo;
         fo += layer_w(l);
     }
+    n->frag_first = fo;
+    fo += (size_t)FIRST_KB * (COUT[0] / 16) * 64 * 4;
     if (e == hipSuccess) e = hipMalloc((void**)&n->frag, fo * sizeof(_Float16));
     if (e == hipSuccess) e = hipMemset(n->frag, 0, fo * sizeof(_Float16));
     if (e == hipSuccess) e = hipMemcpy(n->live, n->live_host.data(), o, hipMemcpyHostToDevice);
