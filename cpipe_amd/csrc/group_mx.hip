@@ -55,6 +55,10 @@ struct Mx<CPIPE_BM3D_PRECISION_FP16> {
     static __device__ __forceinline__ f4 mma(vec a, vec b, f4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
     }
+    typedef _Float16 vec8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ f4 mma32(vec8 a, vec8 b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
 };
 
 template <>
@@ -77,11 +81,25 @@ struct Mx<CPIPE_BM3D_PRECISION_BF16> {
     static __device__ __forceinline__ f4 mma(vec a, vec b, f4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
     }
+    typedef __bf16 vec8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ f4 mma32(vec8 a, vec8 b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
 };
 
 template <int PREC>
 __device__ __forceinline__ typename Mx<PREC>::vec pack4(const f4& x) {
     return Mx<PREC>::pack(x[0], x[1], x[2], x[3]);
+}
+
+// Two K = 16 operands of one lane as one K = 32 operand. The 64-long DCT
+// contractions run on 16x16x32 MFMAs with their K index permuted the same way
+// in A and B (lane group g, element e < 4: k = 4 g + e of the first 16-block;
+// e >= 4: k = 4 g + e - 4 of the second), so each D tile of a 16x16 product
+// still feeds the next product in place.
+template <int PREC>
+__device__ __forceinline__ typename Mx<PREC>::vec8 cat8(typename Mx<PREC>::vec a, typename Mx<PREC>::vec b) {
+    return __builtin_bit_cast(typename Mx<PREC>::vec8, __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7));
 }
 
 __device__ __forceinline__ float kmat(int u, int p) {
@@ -99,7 +117,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <int PREC>
 __device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ src, int w,
                                           uint32_t pm, bool live, int q,
-                                          const typename Mx<PREC>::vec (&kf)[4][4],
+                                          const typename Mx<PREC>::vec8 (&kf)[2][4],
                                           const typename Mx<PREC>::vec& ha) {
     typedef Mx<PREC> M;
     const int qx = (int)(pm & 0xffffu), qy = (int)(pm >> 16);
@@ -115,11 +133,11 @@ __device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ 
         xa[kb] = M::pack(a[0], a[1], a[2], a[3]);
     }
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const typename M::vec8 x0 = cat8<PREC>(xa[0], xa[1]), x1 = cat8<PREC>(xa[2], xa[3]);
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
-        f4 y = zero;
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) y = M::mma(xa[kb], kf[kb][nb], y);
+        f4 y = M::mma32(x0, kf[0][nb], zero);
+        y = M::mma32(x1, kf[1][nb], y);
         z[nb] = M::mma(ha, pack4<PREC>(y), zero);
     }
 }
@@ -132,8 +150,8 @@ template <int STEP, int PREC>
 __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm, int r, int q,
                                            int lane, const float* __restrict__ src,
                                            const float* __restrict__ bsrc, int w, float* sb,
-                                           const typename Mx<PREC>::vec (&kf)[4][4],
-                                           const typename Mx<PREC>::vec (&ki)[4][4], float thr,
+                                           const typename Mx<PREC>::vec8 (&kf)[2][4],
+                                           const typename Mx<PREC>::vec8 (&ki)[4][2], float thr,
                                            float s2) {
     typedef Mx<PREC> M;
     const bool live = r < n;
@@ -167,27 +185,23 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm,
             for (int t = 0; t < 4; ++t) {
                 const float bb = zb[nb][t] * zb[nb][t];
                 const float d = bb + s2;
-                float W = d > 0.0f ? bb / d : 1.0f;
+                float W = d > 0.0f ? bb * __builtin_amdgcn_rcpf(d) : 1.0f;  // 1-ulp reciprocal
                 W = (4 * q + t) < n ? W : 0.0f;
                 z[nb][t] = W * z[nb][t];
                 aw = __builtin_fmaf(W, W, aw);
             }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
-        wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
+        wgt = aw > 0.0f ? __builtin_amdgcn_rcpf(aw) : 1.0f;
     }
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     typename M::vec vt[4];
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) vt[nb] = pack4<PREC>(M::mma(pack4<PREC>(z[nb]), hb, zero));
+    const typename M::vec8 v0 = cat8<PREC>(vt[0], vt[1]), v1 = cat8<PREC>(vt[2], vt[3]);
     f4 xt[4];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        f4 x = zero;
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) x = M::mma(ki[mb][nb], vt[nb], x);
-        xt[mb] = x;
-    }
+    for (int mb = 0; mb < 4; ++mb) xt[mb] = M::mma32(ki[mb][1], v1, M::mma32(ki[mb][0], v0, zero));
     // lane (member r, quarter q) holds pixels 16 mb + 4 q + t: transpose through
     // LDS, MX_SCR members at a time
 #pragma unroll
@@ -236,18 +250,29 @@ void k_group_mx(GroupLaunch L) {
     const int r = lane & 15, q = lane >> 4;
     const int li = lane >> 3, lj = lane & 7;
     float* sb = scratch[wave];
-    typename M::vec kf[4][4], ki[4][4];
+    typename M::vec8 kf[2][4], ki[4][2];
+    {
+        typename M::vec f16[4][4], i16[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            // B[k = pixel][j = freq] = K[freq][pixel]
-            kf[a][nb] = M::pack(kmat(16 * nb + r, 16 * a + 4 * q), kmat(16 * nb + r, 16 * a + 4 * q + 1),
-                                kmat(16 * nb + r, 16 * a + 4 * q + 2), kmat(16 * nb + r, 16 * a + 4 * q + 3));
-            // A[i = pixel][k = freq] = K[freq][pixel]
-            ki[a][nb] = M::pack(kmat(16 * nb + 4 * q, 16 * a + r), kmat(16 * nb + 4 * q + 1, 16 * a + r),
-                                kmat(16 * nb + 4 * q + 2, 16 * a + r), kmat(16 * nb + 4 * q + 3, 16 * a + r));
-        }
+            for (int nb = 0; nb < 4; ++nb) {
+                // B[k = pixel][j = freq] = K[freq][pixel]
+                f16[a][nb] = M::pack(kmat(16 * nb + r, 16 * a + 4 * q), kmat(16 * nb + r, 16 * a + 4 * q + 1),
+                                     kmat(16 * nb + r, 16 * a + 4 * q + 2), kmat(16 * nb + r, 16 * a + 4 * q + 3));
+                // A[i = pixel][k = freq] = K[freq][pixel]
+                i16[a][nb] = M::pack(kmat(16 * nb + 4 * q, 16 * a + r), kmat(16 * nb + 4 * q + 1, 16 * a + r),
+                                     kmat(16 * nb + 4 * q + 2, 16 * a + r), kmat(16 * nb + 4 * q + 3, 16 * a + r));
+            }
+        // K = 32 operands: pixel (forward) or frequency (inverse) blocks 2h and 2h + 1
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                kf[h][nb] = cat8<PREC>(f16[2 * h][nb], f16[2 * h + 1][nb]);
+                ki[nb][h] = cat8<PREC>(i16[nb][2 * h], i16[nb][2 * h + 1]);
+            }
+    }
     const float kw = c_mx_kaiser[lane];
     const float* __restrict__ src = F.z[c];
     const float* __restrict__ bsrc = F.bz[c];
