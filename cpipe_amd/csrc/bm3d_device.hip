@@ -684,7 +684,7 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
 
 template <int STEP>
 __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
-    __shared__ float acc[2 * FOOT2];
+    __shared__ float2 acc[FOOT2];  // (num, den) per footprint pixel
     __shared__ __align__(16) float scratch[4][SCR * SPP];
     __shared__ int turn;  // number of groups of this tile already accumulated
     const int2 item = L.items[blockIdx.x];
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
     bool timed_out = false;
     const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
     const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
-    for (int idx = threadIdx.x; idx < 2 * FOOT2; idx += blockDim.x) acc[idx] = 0.0f;
+    for (int idx = threadIdx.x; idx < FOOT2; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -744,14 +744,15 @@ __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
             }
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
-        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw, FOOT2);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
     }
     if (timed_out && lane == 0) atomicOr(L.err, 1);
     __syncthreads();
     float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * p.tile_stride;
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
-        dst[idx] = acc[idx];
-        dst[p.dofs + idx] = acc[FOOT2 + idx];
+        const float2 a = acc[idx];
+        dst[idx] = a.x;
+        dst[p.dofs + idx] = a.y;
     }
 }
 

@@ -102,57 +102,51 @@ __device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
 // One read/add/write round over members [S, S + L) of a group (pairwise
 // disjoint patches): all reads, then all writes.
 template <int S, int L>
-__device__ __forceinline__ void rmw_run(float* acc, const int (&o)[NMAX], const float (&c)[NMAX],
-                                        float wk, int dofs) {
-    float an[L], ad[L];
+__device__ __forceinline__ void rmw_run(float2* acc, const int (&o)[NMAX], const float (&c)[NMAX],
+                                        float wk) {
+    float2 a[L];
 #pragma unroll
-    for (int m = 0; m < L; ++m) {
-        an[m] = acc[o[S + m]];
-        ad[m] = acc[dofs + o[S + m]];
-    }
+    for (int m = 0; m < L; ++m) a[m] = acc[o[S + m]];
 #pragma unroll
-    for (int m = 0; m < L; ++m) {
-        acc[o[S + m]] = an[m] + c[S + m];
-        acc[dofs + o[S + m]] = ad[m] + wk;
-    }
+    for (int m = 0; m < L; ++m) acc[o[S + m]] = make_float2(a[m].x + c[S + m], a[m].y + wk);
 }
 
 // Members m0 .. m0+3 in rounds of consecutive pairwise-disjoint runs; bit k of
 // `brk` (k = 1..3) starts a new run at member m0 + k.
 template <int M0>
-__device__ __forceinline__ void rmw_block4(float* acc, const int (&o)[NMAX], const float (&c)[NMAX],
-                                           float wk, int dofs, uint32_t brk) {
+__device__ __forceinline__ void rmw_block4(float2* acc, const int (&o)[NMAX], const float (&c)[NMAX],
+                                           float wk, uint32_t brk) {
     switch (brk & 0xeu) {
-        case 0x0: rmw_run<M0, 4>(acc, o, c, wk, dofs); break;
-        case 0x8: rmw_run<M0, 3>(acc, o, c, wk, dofs); rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs); break;
-        case 0x4: rmw_run<M0, 2>(acc, o, c, wk, dofs); rmw_run<M0 + 2, 2>(acc, o, c, wk, dofs); break;
+        case 0x0: rmw_run<M0, 4>(acc, o, c, wk); break;
+        case 0x8: rmw_run<M0, 3>(acc, o, c, wk); rmw_run<M0 + 3, 1>(acc, o, c, wk); break;
+        case 0x4: rmw_run<M0, 2>(acc, o, c, wk); rmw_run<M0 + 2, 2>(acc, o, c, wk); break;
         case 0xc:
-            rmw_run<M0, 2>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 2, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0, 2>(acc, o, c, wk);
+            rmw_run<M0 + 2, 1>(acc, o, c, wk);
+            rmw_run<M0 + 3, 1>(acc, o, c, wk);
             break;
-        case 0x2: rmw_run<M0, 1>(acc, o, c, wk, dofs); rmw_run<M0 + 1, 3>(acc, o, c, wk, dofs); break;
+        case 0x2: rmw_run<M0, 1>(acc, o, c, wk); rmw_run<M0 + 1, 3>(acc, o, c, wk); break;
         case 0xa:
-            rmw_run<M0, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 1, 2>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0, 1>(acc, o, c, wk);
+            rmw_run<M0 + 1, 2>(acc, o, c, wk);
+            rmw_run<M0 + 3, 1>(acc, o, c, wk);
             break;
         case 0x6:
-            rmw_run<M0, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 1, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 2, 2>(acc, o, c, wk, dofs);
+            rmw_run<M0, 1>(acc, o, c, wk);
+            rmw_run<M0 + 1, 1>(acc, o, c, wk);
+            rmw_run<M0 + 2, 2>(acc, o, c, wk);
             break;
         default:
-            rmw_run<M0, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 1, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 2, 1>(acc, o, c, wk, dofs);
-            rmw_run<M0 + 3, 1>(acc, o, c, wk, dofs);
+            rmw_run<M0, 1>(acc, o, c, wk);
+            rmw_run<M0 + 1, 1>(acc, o, c, wk);
+            rmw_run<M0 + 2, 1>(acc, o, c, wk);
+            rmw_run<M0 + 3, 1>(acc, o, c, wk);
             break;
     }
 }
 
-// Adds one filtered group to a tile's LDS num/den footprint (acc[0, dofs) and
-// acc[dofs, 2 dofs)) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
+// Adds one filtered group to a tile's LDS footprint of (num, den) pairs (one
+// 8-byte read and write per member pixel) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
 // member m in v[m]; mp[m] (wave-uniform) is the member's footprint position
 // (y << 16 | x); lo = (i - fy0) * fw + (j - fx0). Group g is accumulated right
 // after group g-1: the wave waits for the LDS turn token to reach g, adds
@@ -168,10 +162,10 @@ __device__ __forceinline__ void rmw_block4(float* acc, const int (&o)[NMAX], con
 // whose patches are pairwise disjoint, one round per run. A pixel's members
 // are added in member order either way (members covering one pixel overlap,
 // so they never share a run).
-__device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, bool has, int n,
+__device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g, bool has, int n,
                                                   const uint32_t (&mp)[NMAX],
                                                   const float (&v)[NMAX], float wk, int lo,
-                                                  int fw, int dofs) {
+                                                  int fw) {
     int o[NMAX];
     float c[NMAX];
 #pragma unroll
@@ -215,23 +209,23 @@ __device__ __forceinline__ bool add_group_in_turn(float* acc, int* turn, int g, 
     }
     if (has) {
         if (n >= 4) {
-            rmw_block4<0>(acc, o, c, wk, dofs, plan);
+            rmw_block4<0>(acc, o, c, wk, plan);
             if (n >= 8) {
-                rmw_block4<4>(acc, o, c, wk, dofs, plan >> 4);
+                rmw_block4<4>(acc, o, c, wk, plan >> 4);
                 if (n >= 16) {
-                    rmw_block4<8>(acc, o, c, wk, dofs, plan >> 8);
-                    rmw_block4<12>(acc, o, c, wk, dofs, plan >> 12);
+                    rmw_block4<8>(acc, o, c, wk, plan >> 8);
+                    rmw_block4<12>(acc, o, c, wk, plan >> 12);
                 }
             }
         } else if (n == 2) {
             if (plan & 2u) {
-                rmw_run<0, 1>(acc, o, c, wk, dofs);
-                rmw_run<1, 1>(acc, o, c, wk, dofs);
+                rmw_run<0, 1>(acc, o, c, wk);
+                rmw_run<1, 1>(acc, o, c, wk);
             } else {
-                rmw_run<0, 2>(acc, o, c, wk, dofs);
+                rmw_run<0, 2>(acc, o, c, wk);
             }
         } else {
-            rmw_run<0, 1>(acc, o, c, wk, dofs);
+            rmw_run<0, 1>(acc, o, c, wk);
         }
     }
     __hip_atomic_store(turn, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);

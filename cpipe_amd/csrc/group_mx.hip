@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_group_mx(GroupLaunch L) {
     typedef Mx<PREC> M;
     constexpr int DOFS = FOOT * foot_edge(TY);
-    __shared__ float acc[2 * DOFS];
+    __shared__ float2 acc[DOFS];  // (num, den) per footprint pixel
     __shared__ __align__(16) float scratch[4][MX_SCR * 64];
     __shared__ int turn;  // number of groups of this tile already accumulated
     const int2 item = L.items[blockIdx.x];
@@ -243,7 +243,7 @@ void k_group_mx(GroupLaunch L) {
     bool timed_out = false;
     const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
     const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
-    for (int idx = threadIdx.x; idx < 2 * DOFS; idx += blockDim.x) acc[idx] = 0.0f;
+    for (int idx = threadIdx.x; idx < DOFS; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -300,14 +300,15 @@ void k_group_mx(GroupLaunch L) {
             wgt = filter_mx<STEP, PREC>(v, n, pm, r, q, lane, src, bsrc, w, sb, kf, ki, thr, s2);
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
-        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw, DOFS);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
     }
     if (timed_out && lane == 0) atomicOr(L.err, 1);
     __syncthreads();
     float* dst = p.partial + ((size_t)c * p.ntx * p.nty + tile) * p.tile_stride;
     for (int idx = threadIdx.x; idx < fw * fh; idx += blockDim.x) {
-        dst[idx] = acc[idx];
-        dst[p.dofs + idx] = acc[DOFS + idx];
+        const float2 a = acc[idx];
+        dst[idx] = a.x;
+        dst[p.dofs + idx] = a.y;
     }
 }
 
