@@ -841,18 +841,22 @@ __global__ void k_merge(const FrameTab* __restrict__ tab, const FrameRect* __res
 // step 1.
 // ---------------------------------------------------------------------------
 
-// One block per checked rectangle (report entry blockIdx.x).
-__global__ __launch_bounds__(256) void k_pilot_check(const FrameTab* __restrict__ tab,
-                                                     const FrameRect* __restrict__ rects,
-                                                     float thr_db, float* __restrict__ report) {
+// Three launches over all checked rectangles of all frames: per-row squared
+// residual sums (block (row, rectangle)), per-rectangle totals summed over the
+// rows in a fixed order (one block per rectangle; report entry blockIdx.x),
+// and the fallback copy of the rectangles that fail.
+__global__ __launch_bounds__(256) void k_pilot_check_rows(const FrameTab* __restrict__ tab,
+                                                          const FrameRect* __restrict__ rects,
+                                                          float2* __restrict__ part, int rows) {
     __shared__ float ra[256], rb[256];
-    const FrameRect r = rects[blockIdx.x];
+    const FrameRect r = rects[blockIdx.y];
+    const int y = r.y0 + blockIdx.x;
+    if (y >= r.y1) return;  // uniform per block
     const FrameTab& F = tab[r.f];
-    const int w = F.w;
-    const int rw = r.x1 - r.x0, n = rw * (r.y1 - r.y0);
     float a = 0.0f, b = 0.0f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const size_t o = (size_t)(r.y0 + i / rw) * w + r.x0 + i % rw;
+    for (int x = r.x0 + threadIdx.x; x < r.x1; x += blockDim.x) {
+        const size_t o = (size_t)y * F.w + x;
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float zs = F.z[c][o];
             const float dl = zs - F.bz[c][o], dp = zs - F.bzp[c][o];
@@ -870,19 +874,52 @@ __global__ __launch_bounds__(256) void k_pilot_check(const FrameTab* __restrict_
         }
         __syncthreads();
     }
-    const float sa = ra[0], sb = rb[0];
-    const float delta = (sa > 0.0f && sb > 0.0f) ? fabsf(10.0f * log10f(sa / sb))
-                                                  : (sa == sb ? 0.0f : 1e30f);
-    const bool fall = delta > thr_db;
-    if (fall) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const size_t o = (size_t)(r.y0 + i / rw) * w + r.x0 + i % rw;
-            for (int c = 0; c < 3; ++c) F.bz[c][o] = F.bzp[c][o];
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * rows + blockIdx.x] = make_float2(ra[0], rb[0]);
+}
+
+__global__ __launch_bounds__(256) void k_pilot_check_sum(const FrameRect* __restrict__ rects,
+                                                         const float2* __restrict__ part, int rows,
+                                                         float thr_db, float* __restrict__ report) {
+    __shared__ float ra[256], rb[256];
+    const FrameRect r = rects[blockIdx.x];
+    const int nrow = r.y1 - r.y0;
+    float a = 0.0f, b = 0.0f;
+    for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+        const float2 v = part[(size_t)blockIdx.x * rows + i];
+        a += v.x;
+        b += v.y;
+    }
+    ra[threadIdx.x] = a;
+    rb[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            ra[threadIdx.x] += ra[threadIdx.x + off];
+            rb[threadIdx.x] += rb[threadIdx.x + off];
         }
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
+        const float sa = ra[0], sb = rb[0];
+        const float delta = (sa > 0.0f && sb > 0.0f) ? fabsf(10.0f * log10f(sa / sb))
+                                                      : (sa == sb ? 0.0f : 1e30f);
         report[2 * blockIdx.x] = delta;
-        report[2 * blockIdx.x + 1] = fall ? 1.0f : 0.0f;
+        report[2 * blockIdx.x + 1] = delta > thr_db ? 1.0f : 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pilot_check_apply(const FrameTab* __restrict__ tab,
+                                                           const FrameRect* __restrict__ rects,
+                                                           const float* __restrict__ report) {
+    if (report[2 * blockIdx.y + 1] < 0.5f) return;  // this rectangle keeps the pilot
+    const FrameRect r = rects[blockIdx.y];
+    const int y = r.y0 + blockIdx.x;
+    if (y >= r.y1) return;
+    const FrameTab& F = tab[r.f];
+    for (int x = r.x0 + threadIdx.x; x < r.x1; x += blockDim.x) {
+        const size_t o = (size_t)y * F.w + x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F.bz[c][o] = F.bzp[c][o];
     }
 }
 
@@ -1022,6 +1059,7 @@ struct cpipe_bm3d_plan_s {
     std::vector<Check> checks;
     Lists chk;                       // the checked rectangles' work, all checks in one launch per kernel
     float* report = nullptr;         // [n][2]: delta dB, fell back
+    float2* chk_part = nullptr;      // [n][rect rows]: per-row squared-residual sums of each check
     hipStream_t side = nullptr;      // the spot checks' plain step 1 runs here, beside the network
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // one graph per plan: the whole batch as a single submission
@@ -1085,6 +1123,7 @@ void free_plan(cpipe_bm3d_plan* p) {
     (void)hipFree(p->err);
     (void)hipFree(p->rgb);
     (void)hipFree(p->report);
+    (void)hipFree(p->chk_part);
     if (p->side) (void)hipStreamDestroy(p->side);
     if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
     if (p->join_ev) (void)hipEventDestroy(p->join_ev);
@@ -1621,7 +1660,12 @@ int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
         }
         if (nchk > 0) {
             HIP_CHECK(hipStreamWaitEvent(st, p->join_ev, 0));
-            k_pilot_check<<<nchk, 256, 0, st>>>(p->tab_dev, p->chk.rects, p->pilot_thr_db, p->report);
+            const int rows = p->chk.rect_h;
+            k_pilot_check_rows<<<dim3(rows, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->chk_part, rows);
+            HIP_CHECK(hipGetLastError());
+            k_pilot_check_sum<<<nchk, 256, 0, st>>>(p->chk.rects, p->chk_part, rows, p->pilot_thr_db, p->report);
+            HIP_CHECK(hipGetLastError());
+            k_pilot_check_apply<<<dim3(rows, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->report);
             HIP_CHECK(hipGetLastError());
             k_pilot_decide<<<((int)n + 63) / 64, 64, 0, st>>>(p->tab_dev, (int)n, p->report, p->pilot_full_frac);
             HIP_CHECK(hipGetLastError());
@@ -1695,7 +1739,9 @@ int run_plan(cpipe_bm3d_plan* p, hipStream_t st) {
 int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
     free_lists(p->chk);
     (void)hipFree(p->report);
+    (void)hipFree(p->chk_part);
     p->report = nullptr;
+    p->chk_part = nullptr;
     p->checks.clear();
     std::vector<FrameRect> rects;
     std::vector<int4> bm;
@@ -1756,6 +1802,8 @@ int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
         (rc = dev_alloc((void**)&p->report, std::max<size_t>(1, rects.size()) * 2 * sizeof(float))))
         return rc;
     HIP_CHECK(hipMemset(p->report, 0, std::max<size_t>(1, rects.size()) * 2 * sizeof(float)));
+    if ((rc = dev_alloc((void**)&p->chk_part, std::max<size_t>(1, rects.size() * (size_t)std::max(1, chh)) * sizeof(float2))))
+        return rc;
     L.n_bm = (int)bm.size();
     L.n_grp = (int)grp.size();
     L.n_rects = (int)rects.size();
@@ -2091,7 +2139,9 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     drop_graph(p);
     free_lists(p->chk);
     (void)hipFree(p->report);
+    (void)hipFree(p->chk_part);
     p->report = nullptr;
+    p->chk_part = nullptr;
     for (auto& f : p->fr) {
         (void)hipFree(f.bzp);
         f.bzp = nullptr;
