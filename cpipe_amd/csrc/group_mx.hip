@@ -142,7 +142,8 @@ __device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ 
     }
 }
 
-constexpr int MX_SCR = 4;  // members per LDS transposition chunk
+constexpr int MX_SCR = 4;   // members per LDS transposition chunk
+constexpr int MX_SPP = 68;  // scratch floats per member: the chunk's four 16-byte writes hit distinct banks
 
 // Filters one group: 3D transform, shrinkage, inverse; leaves lane p = pixel p
 // of member m in v[m] (m < n) and returns the group's aggregation weight.
@@ -210,12 +211,12 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm,
             if (r / MX_SCR == hf) {
 #pragma unroll
                 for (int mb = 0; mb < 4; ++mb)
-                    *reinterpret_cast<f4*>(sb + (r % MX_SCR) * 64 + 16 * mb + 4 * q) = xt[mb];
+                    *reinterpret_cast<f4*>(sb + (r % MX_SCR) * MX_SPP + 16 * mb + 4 * q) = xt[mb];
             }
             wave_lds_sync();
 #pragma unroll
             for (int m = 0; m < MX_SCR; ++m)
-                if (hf * MX_SCR + m < n) v[hf * MX_SCR + m] = sb[m * 64 + lane];
+                if (hf * MX_SCR + m < n) v[hf * MX_SCR + m] = sb[m * MX_SPP + lane];
             wave_lds_sync();
         }
     }
@@ -230,7 +231,7 @@ void k_group_mx(GroupLaunch L) {
     typedef Mx<PREC> M;
     constexpr int DOFS = FOOT * foot_edge(TY);
     __shared__ float2 acc[DOFS];  // (num, den) per footprint pixel
-    __shared__ __align__(16) float scratch[4][MX_SCR * 64];
+    __shared__ __align__(16) float scratch[4][MX_SCR * MX_SPP];
     __shared__ int turn;  // number of groups of this tile already accumulated
     const int2 item = L.items[blockIdx.x];
     const FrameTab& F = L.tab[item.x];
