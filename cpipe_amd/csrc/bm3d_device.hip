@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -879,7 +880,8 @@ __global__ __launch_bounds__(256) void k_pilot_check_rows(const FrameTab* __rest
 
 __global__ __launch_bounds__(256) void k_pilot_check_sum(const FrameRect* __restrict__ rects,
                                                          const float2* __restrict__ part, int rows,
-                                                         float thr_db, float* __restrict__ report) {
+                                                         float thr_db, float under_db,
+                                                         float* __restrict__ report) {
     __shared__ float ra[256], rb[256];
     const FrameRect r = rects[blockIdx.x];
     const int nrow = r.y1 - r.y0;
@@ -901,10 +903,10 @@ __global__ __launch_bounds__(256) void k_pilot_check_sum(const FrameRect* __rest
     }
     if (threadIdx.x == 0) {
         const float sa = ra[0], sb = rb[0];
-        const float delta = (sa > 0.0f && sb > 0.0f) ? fabsf(10.0f * log10f(sa / sb))
-                                                      : (sa == sb ? 0.0f : 1e30f);
-        report[2 * blockIdx.x] = delta;
-        report[2 * blockIdx.x + 1] = delta > thr_db ? 1.0f : 0.0f;
+        const float r = (sa > 0.0f && sb > 0.0f) ? 10.0f * log10f(sa / sb)
+                                                  : (sa == sb ? 0.0f : (sa > sb ? 1e30f : -1e30f));
+        report[2 * blockIdx.x] = r;
+        report[2 * blockIdx.x + 1] = (r > thr_db || r < -under_db) ? 1.0f : 0.0f;
     }
 }
 
@@ -1054,6 +1056,7 @@ struct cpipe_bm3d_plan_s {
     // learned pilot (replaces step 1) and its spot checks
     cpipe_bm3d_pilot* pilot = nullptr;
     float pilot_thr_db = 0.5f;
+    float pilot_under_db = 0.25f;
     float pilot_full_frac = 0.5f;
     cpipe_bm3d_pilot_opts pilot_opts{};
     std::vector<Check> checks;
@@ -1663,7 +1666,8 @@ int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
             const int rows = p->chk.rect_h;
             k_pilot_check_rows<<<dim3(rows, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->chk_part, rows);
             HIP_CHECK(hipGetLastError());
-            k_pilot_check_sum<<<nchk, 256, 0, st>>>(p->chk.rects, p->chk_part, rows, p->pilot_thr_db, p->report);
+            k_pilot_check_sum<<<nchk, 256, 0, st>>>(p->chk.rects, p->chk_part, rows, p->pilot_thr_db,
+                                                    p->pilot_under_db, p->report);
             HIP_CHECK(hipGetLastError());
             k_pilot_check_apply<<<dim3(rows, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->report);
             HIP_CHECK(hipGetLastError());
@@ -2157,12 +2161,15 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     bool any = false;
     for (const auto& f : p->fr) any = any || !f.small;
     if (!any) return fail(UNSUPPORTED, "image smaller than one patch");
-    cpipe_bm3d_pilot_opts o{sizeof(cpipe_bm3d_pilot_opts), 0.5f, 256, 16, 0.5f};
+    cpipe_bm3d_pilot_opts o{sizeof(cpipe_bm3d_pilot_opts), 0.5f, 256, 16, 0.5f, 0.25f};
     if (opts) {
-        if (opts->struct_size < sizeof(cpipe_bm3d_pilot_opts)) return fail(BAD_INDEX, "opts.struct_size too small");
-        o = *opts;
+        constexpr size_t v1 = offsetof(cpipe_bm3d_pilot_opts, under_db);  // layout before under_db
+        if (opts->struct_size < v1) return fail(BAD_INDEX, "opts.struct_size too small");
+        std::memcpy(&o, opts, v1);
+        o.under_db = opts->struct_size >= sizeof(cpipe_bm3d_pilot_opts) ? opts->under_db : 0.5f * o.fallback_db;
+        o.struct_size = sizeof(cpipe_bm3d_pilot_opts);
     }
-    if (!(o.fallback_db >= 0.0f) || o.check_block < 16 || o.check_every < 0)
+    if (!(o.fallback_db >= 0.0f) || !(o.under_db >= 0.0f) || o.check_block < 16 || o.check_every < 0)
         return fail(BAD_INDEX, "bad pilot options");
     int rc;
     for (int f = 0; f < (int)p->fr.size(); ++f) {
@@ -2180,6 +2187,7 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     p->pilot = net;
     p->pilot_opts = o;
     p->pilot_thr_db = o.fallback_db;
+    p->pilot_under_db = o.under_db;
     p->pilot_full_frac = o.full_fraction;
     return OK;
 }
@@ -2196,7 +2204,7 @@ int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* p, int* n_checked, int* n_fall
         std::vector<float> rep(2 * std::max(1, nc));
         if (nc) HIP_CHECK(hipMemcpy(rep.data(), p->report, 2 * nc * sizeof(float), hipMemcpyDeviceToHost));
         for (int i = 0; i < nc; ++i) {
-            md = std::max(md, rep[2 * i]);
+            md = std::max(md, std::fabs(rep[2 * i]));
             nf += rep[2 * i + 1] > 0.5f ? 1 : 0;
         }
         if (nc) {

@@ -509,10 +509,12 @@ __global__ void k_reduce(const float* __restrict__ partial, int nwg, int n, floa
 // output and the target): dY (FP16 NHWC, 16 ch) = out - target for channels
 // < 3, and per-block squared-error sums for the loss value.
 __global__ void k_loss_grad(const float* __restrict__ out, const float* __restrict__ tgt,
-                            size_t plane, _Float16* __restrict__ dy, float* __restrict__ sq_partial) {
+                            size_t plane, const float* __restrict__ wimg, _Float16* __restrict__ dy,
+                            float* __restrict__ sq_partial) {
     __shared__ float red[256];
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t img = blockIdx.y;
+    const float wi = wimg ? wimg[img] : 1.0f;
     float s = 0.0f;
     if (p < plane) {
         typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -520,8 +522,8 @@ __global__ void k_loss_grad(const float* __restrict__ out, const float* __restri
         for (int c = 0; c < 3; ++c) {
             const size_t q = (img * 3 + c) * plane + p;
             const float d = out[q] - tgt[q];
-            g[c] = (_Float16)d;
-            s += d * d;
+            g[c] = (_Float16)(wi * d);
+            s += wi * (d * d);
         }
         _Float16* o = dy + (img * plane + p) * 16;
         *reinterpret_cast<half8*>(o) = g;
@@ -639,6 +641,11 @@ struct cpipe_bm3d_pilot_s {
     size_t sq_cap = 0;
     float* levels = nullptr;
     size_t levels_cap = 0;
+    // loss: 0 = 1/2 MSE; 1 = 1/2 MSE with image i weighted by (ref_sigma / sigma_i)^2
+    int loss_mode = 0;
+    float loss_ref_sigma = 0.05f;
+    float* wimg = nullptr;
+    size_t wimg_cap = 0;
 };
 
 namespace {
@@ -812,7 +819,8 @@ int backward(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tg
     const size_t nb_img = (plane + 255) / 256;
     const size_t nblk = nb_img * n_img;
     if ((rc = ensure_buf(&n->sq, &n->sq_cap, nblk))) return rc;
-    k_loss_grad<<<dim3((unsigned)nb_img, n_img), 256, 0, st>>>(n->outp, tgt[0], plane, n->dyb[0], n->sq);
+    k_loss_grad<<<dim3((unsigned)nb_img, n_img), 256, 0, st>>>(n->outp, tgt[0], plane,
+                                                                n->loss_mode ? n->wimg : nullptr, n->dyb[0], n->sq);
     PHIP(hipGetLastError());
     std::vector<float> sqh(nblk);
     const float scale = 1.0f / (float)(px * 3);
@@ -848,9 +856,15 @@ int backward(cpipe_bm3d_pilot* n, const float* const* in, const float* const* tg
 int upload_levels(cpipe_bm3d_pilot* n, const float* sigma, int n_img, hipStream_t st) {
     int rc = ensure_buf(&n->levels, &n->levels_cap, (size_t)n_img);
     if (rc) return rc;
-    std::vector<float> lv(n_img);
-    for (int i = 0; i < n_img; ++i) lv[i] = 10.0f * sigma[i];
+    if ((rc = ensure_buf(&n->wimg, &n->wimg_cap, (size_t)n_img))) return rc;
+    std::vector<float> lv(2 * (size_t)n_img);
+    for (int i = 0; i < n_img; ++i) {
+        lv[i] = 10.0f * sigma[i];
+        const float r = sigma[i] > 0.0f ? n->loss_ref_sigma / sigma[i] : 1.0f;
+        lv[n_img + i] = r * r;
+    }
     PHIP(hipMemcpyAsync(n->levels, lv.data(), n_img * sizeof(float), hipMemcpyHostToDevice, st));
+    PHIP(hipMemcpyAsync(n->wimg, lv.data() + n_img, n_img * sizeof(float), hipMemcpyHostToDevice, st));
     PHIP(hipStreamSynchronize(st));
     return 0;
 }
@@ -932,7 +946,8 @@ void cpipe_bm3d_pilot_destroy(cpipe_bm3d_pilot* n) {
     (void)hipDeviceSynchronize();
     free_ws(n);
     for (void* p : {(void*)n->w32, (void*)n->m, (void*)n->v, (void*)n->g, (void*)n->w16,
-                    (void*)n->frag, (void*)n->live, (void*)n->partial, (void*)n->sq, (void*)n->levels})
+                    (void*)n->frag, (void*)n->live, (void*)n->partial, (void*)n->sq, (void*)n->levels,
+                    (void*)n->wimg})
         (void)hipFree(p);
     delete n;
 }
@@ -1037,6 +1052,14 @@ int cpipe_bm3d_pilot_gradients(cpipe_bm3d_pilot* n, const float* const* in, cons
         PHIP(hipStreamSynchronize(st));
         PHIP(hipMemcpy(grads_host, n->g, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+int cpipe_bm3d_pilot_set_loss(cpipe_bm3d_pilot* n, int mode, float ref_sigma) {
+    if (!n) return perr(5, "null argument");
+    if ((mode != 0 && mode != 1) || !(ref_sigma > 0.0f)) return perr(5, "bad loss mode or reference sigma");
+    n->loss_mode = mode;
+    n->loss_ref_sigma = ref_sigma;
     return 0;
 }
 

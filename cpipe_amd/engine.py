@@ -28,6 +28,7 @@ EXPORTS = (
     "cpipe_bm3d_plan_pilot_report", "cpipe_bm3d_pilot_create", "cpipe_bm3d_pilot_destroy",
     "cpipe_bm3d_pilot_init", "cpipe_bm3d_pilot_save", "cpipe_bm3d_pilot_load",
     "cpipe_bm3d_pilot_forward", "cpipe_bm3d_pilot_gradients", "cpipe_bm3d_pilot_train_step",
+    "cpipe_bm3d_pilot_set_loss",
     "cpipe_bm3d_pilot_blob_size", "cpipe_bm3d_pilot_param_count", "cpipe_bm3d_pilot_last_error",
     "cpipe_bm3d_plan_set_timing", "cpipe_bm3d_plan_timing", "cpipe_bm3d_compat_rgba16f",
     "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_host_release",
@@ -141,7 +142,7 @@ def _stream_handle(stream) -> int:
 class PilotOpts(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("fallback_db", ctypes.c_float),
                 ("check_block", ctypes.c_int32), ("check_every", ctypes.c_int32),
-                ("full_fraction", ctypes.c_float)]
+                ("full_fraction", ctypes.c_float), ("under_db", ctypes.c_float)]
 
 
 class Plan:
@@ -224,10 +225,10 @@ class Plan:
             _stream_handle(stream)))
 
     def set_pilot(self, net, fallback_db: float = 0.5, check_block: int = 256,
-                  check_every: int = 16, full_fraction: float = 0.5) -> None:
+                  check_every: int = 16, full_fraction: float = 0.5, under_db: float = 0.25) -> None:
         """Use a learned pilot (cpipe_amd.pilot.Pilot) in place of step 1; None restores it."""
         opts = PilotOpts(ctypes.sizeof(PilotOpts), fallback_db, check_block, check_every,
-                         full_fraction)
+                         full_fraction, under_db)
         _check(lib().cpipe_bm3d_plan_set_pilot(self._h, net.handle if net is not None else None,
                                                ctypes.byref(opts)))
         self._pilot = net  # the plan references the network

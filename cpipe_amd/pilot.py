@@ -37,6 +37,7 @@ def _lib():
         L.cpipe_bm3d_pilot_forward.argtypes = [P, P, S, S, I, I, I, F, P, P]
         L.cpipe_bm3d_pilot_gradients.argtypes = [P, P, P, I, I, I, P, P, ctypes.POINTER(F)]
         L.cpipe_bm3d_pilot_train_step.argtypes = [P, P, P, I, I, I, P, F, ctypes.POINTER(F)]
+        L.cpipe_bm3d_pilot_set_loss.argtypes = [P, I, F]
         _bound = True
     return L
 
@@ -120,6 +121,12 @@ class Pilot:
         _check(_lib().cpipe_bm3d_pilot_gradients(self._h, _planes3(planes), _planes3(target), w, h, n,
                                                  s.ctypes.data, g.ctypes.data, ctypes.byref(loss)))
         return loss.value, g
+
+    def set_loss(self, mode: str = "mse", ref_sigma: float = 0.05) -> None:
+        """'mse' (default) or 'sigma': each image's squared error scaled by
+        (ref_sigma / sigma_i)^2 (cpipe_bm3d_pilot_set_loss)."""
+        m = {"mse": 0, "sigma": 1}[mode]
+        _check(_lib().cpipe_bm3d_pilot_set_loss(self._h, m, float(ref_sigma)))
 
     def train_step(self, planes, target, sigma, lr: float) -> float:
         n, _, h, w = planes.shape

@@ -109,6 +109,8 @@ def main(argv=None):
     ap.add_argument("--out", default=pilot.DEFAULT_WEIGHTS)
     ap.add_argument("--checkpoint", default=None, help="also write weights + Adam state here")
     ap.add_argument("--resume", default=None, help="start from this blob (weights or checkpoint)")
+    ap.add_argument("--loss", choices=["mse", "sigma"], default="mse",
+                    help="sigma: each crop's squared error scaled by (0.05 / sigma)^2")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
@@ -118,6 +120,7 @@ def main(argv=None):
     X, T = build_pairs(noisy, sigma)
     log(f"step-1 targets in {time.perf_counter() - t0:.1f} s")
     net = pilot.Pilot(seed=args.seed, weights=args.resume)
+    net.set_loss(args.loss, 0.05)
     t0 = time.perf_counter()
     hist = train(net, X, T, sigma, args.epochs, args.batch, args.lr, args.seed, log)
     train_s = time.perf_counter() - t0
@@ -128,7 +131,7 @@ def main(argv=None):
         with open(args.checkpoint, "wb") as f:
             f.write(net.save(True))
     print(json.dumps({"crops": args.crops, "size": args.size, "epochs": args.epochs,
-                      "batch": args.batch, "lr": args.lr, "seed": args.seed,
+                      "batch": args.batch, "lr": args.lr, "seed": args.seed, "loss": args.loss,
                       "loss_per_epoch": hist, "train_seconds": round(train_s, 2),
                       "weights": os.path.abspath(args.out)}), flush=True)
 

@@ -145,6 +145,11 @@ int cpipe_bm3d_pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size
 int cpipe_bm3d_pilot_gradients(cpipe_bm3d_pilot* net, const float* const* in,
                                const float* const* target, int width, int height, int n_img,
                                const float* sigma, float* grads_host, float* loss);
+/* Training loss: mode 0 (default) = 1/2 MSE against the targets; mode 1 =
+ * the same with image i weighted by (ref_sigma / sigma_i)^2, i.e. the error
+ * measured against each crop's noise energy, the quantity the spot checks'
+ * method-noise ratio tests. Applies to _gradients and _train_step. */
+int cpipe_bm3d_pilot_set_loss(cpipe_bm3d_pilot* net, int mode, float ref_sigma);
 int cpipe_bm3d_pilot_train_step(cpipe_bm3d_pilot* net, const float* const* in,
                                 const float* const* target, int width, int height, int n_img,
                                 const float* sigma, float lr, float* loss);
@@ -159,11 +164,15 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* plan, const float* const* in
  * network; blocks of check_block x check_block pixels, one in check_every
  * (block b = by * nbx + bx in raster order is checked when
  * (b + by) % check_every == check_every / 2), are spot-checked against the
- * plain step 1 computed exactly on that block. A block whose method-noise
- * ratio |10 log10(|z - pilot|^2 / |z - plain|^2)| exceeds fallback_db uses the
- * plain estimate; if more than full_fraction of the checked blocks fall back,
- * the plain step 1 runs on the whole frame (decided on the device, no host sync).
- * net = NULL restores the plain path. The plan keeps a reference to net. */
+ * plain step 1 computed exactly on that block. With the method-noise ratio
+ * r = 10 log10(|z - pilot|^2 / |z - plain|^2) (dB), a block uses the plain
+ * estimate when r > fallback_db (the pilot removes more than plain step 1:
+ * over-smoothing) or r < -under_db (it leaves noise in: the Wiener step then
+ * keeps it, so the bound is tighter); if more than full_fraction of the
+ * checked blocks fall back, the plain step 1 runs on the whole frame (decided
+ * on the device, no host sync). net = NULL restores the plain path. The plan
+ * keeps a reference to net. A struct_size that ends before under_db selects
+ * under_db = fallback_db / 2. */
 typedef struct {
     uint32_t struct_size;
     float fallback_db;    /* default 0.5 */
@@ -171,6 +180,7 @@ typedef struct {
     int32_t check_every;  /* default 16; 0 disables spot checks */
     float full_fraction;  /* whole-frame fallback when more than this fraction of the
                              checked blocks fall back; default 0.5 (>= 1 disables it) */
+    float under_db;       /* default 0.25 */
 } cpipe_bm3d_pilot_opts;
 int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* plan, cpipe_bm3d_pilot* net,
                               const cpipe_bm3d_pilot_opts* opts);

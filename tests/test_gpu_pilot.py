@@ -63,6 +63,32 @@ def test_gradients_match_oracle(cuda, oracle_lib):
         assert rel <= 5e-2 and cos >= 0.995, (l, rel, cos)
 
 
+def test_sigma_weighted_loss_gradients_match_oracle(cuda, oracle_lib):
+    """Loss mode 'sigma': image i's squared error scaled by (0.05 / sigma_i)^2.
+    The batch loss is then the mean over images of w_i times the single-image
+    loss, so its gradient is the same weighted mean of the oracle's per-image
+    gradients."""
+    torch = cuda
+    net = pilot.Pilot(seed=13)
+    net.set_loss("sigma", 0.05)
+    x, t, sig, xd, td = _data(torch, 3, 20, 70, 2)
+    print("pilot sigma-weighted gradients 3x20x70", sig, flush=True)
+    loss, g = net.gradients(xd, td, sig)
+    wts = (0.05 / sig.astype(np.float64)) ** 2
+    loss64, g64 = 0.0, 0.0
+    for i in range(3):
+        li, gi = oracle_lib.pilot_grad_f64(net.params(), x[i:i + 1], t[i:i + 1], sig[i:i + 1].astype(np.float64))
+        loss64 = loss64 + wts[i] * li / 3
+        g64 = g64 + wts[i] * gi / 3
+    assert loss == pytest.approx(loss64, rel=2e-2)
+    for l, (off, n) in enumerate(_slices()):
+        a, b = g[off:off + n].astype(np.float64), g64[off:off + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+        print(f"layer {l}: rel {rel:.3e} cos {cos:.6f}")
+        assert rel <= 5e-2 and cos >= 0.995, (l, rel, cos)
+
+
 def test_blob_round_trip_and_checkpoint(cuda):
     torch = cuda
     a = pilot.Pilot(seed=3)

@@ -5,7 +5,9 @@
   to the plain plan;
 - no fallback: the step-2 input is exactly the network's output;
 - partial fallback: checked blocks that fall back carry the plain basic
-  estimate exactly.
+  estimate exactly;
+- an identity pilot (zero network: it leaves all the noise in) falls back on
+  the under-denoising bound, and only on that bound.
 """
 import numpy as np
 import pytest
@@ -42,7 +44,8 @@ def test_full_fallback_is_bit_identical_to_plain(cuda):
     _, _, sigma, src = _frame(torch)
     plain, plain_basic, _ = _run(torch, src, sigma)
     net = pilot.Pilot(seed=1)  # untrained: a poor pilot
-    out, basic, rep = _run(torch, src, sigma, net, fallback_db=0.0, check_block=64, check_every=1)
+    out, basic, rep = _run(torch, src, sigma, net, fallback_db=0.0, under_db=0.0, check_block=64,
+                           check_every=1)
     print(rep)
     assert rep["checked"] == 4 * 3 and rep["fallback"] == rep["checked"] and rep["full_fallback"]
     assert np.array_equal(out.view(np.uint32), plain.view(np.uint32))
@@ -53,7 +56,8 @@ def test_no_fallback_uses_the_network(cuda):
     torch = cuda
     _, noisy, sigma, src = _frame(torch)
     net = pilot.Pilot(seed=2)
-    out, basic, rep = _run(torch, src, sigma, net, fallback_db=1e9, check_block=64, check_every=2)
+    out, basic, rep = _run(torch, src, sigma, net, fallback_db=1e9, under_db=1e9, check_block=64,
+                           check_every=2)
     assert rep["checked"] > 0 and rep["fallback"] == 0 and not rep["full_fallback"]
     # the basic estimate is the network applied to the opponent planes
     plan = engine.Plan(200, 136, 3)
@@ -74,8 +78,8 @@ def test_partial_fallback_blocks_carry_the_plain_estimate(cuda):
     _, _, sigma, src = _frame(torch, "rgb256x192", seed=6)
     _, plain_basic, _ = _run(torch, src, sigma)
     net = pilot.Pilot(seed=3)
-    out, basic, rep = _run(torch, src, sigma, net, fallback_db=0.0, check_block=64, check_every=3,
-                           full_fraction=2.0)
+    out, basic, rep = _run(torch, src, sigma, net, fallback_db=0.0, under_db=0.0, check_block=64,
+                           check_every=3, full_fraction=2.0)
     # every checked block falls back (threshold 0); whole-frame escalation disabled
     assert rep["fallback"] == rep["checked"] == 4 and not rep["full_fallback"]
     nbx = 4
@@ -85,3 +89,21 @@ def test_partial_fallback_blocks_carry_the_plain_estimate(cuda):
         x0, y0 = (b % nbx) * 64, (b // nbx) * 64
         np.testing.assert_array_equal(basic[:, y0:y0 + 64, x0:x0 + 64],
                                       plain_basic[:, y0:y0 + 64, x0:x0 + 64])
+
+
+def test_identity_pilot_falls_back_on_the_under_denoising_bound(cuda):
+    torch = cuda
+    _, _, sigma, src = _frame(torch)
+    plain, _, _ = _run(torch, src, sigma)
+    net = pilot.Pilot(seed=5)
+    blob = bytearray(net.save(False))
+    blob[32:] = bytes(len(blob) - 32)  # zero weights and biases: pilot = noisy input
+    net.load(bytes(blob))
+    out, _, rep = _run(torch, src, sigma, net, check_block=64, check_every=1)
+    print(rep)
+    assert rep["fallback"] == rep["checked"] == 12 and rep["full_fallback"]
+    assert np.array_equal(out.view(np.uint32), plain.view(np.uint32))
+    # the over-smoothing bound alone (under-denoising bound disabled) keeps it
+    _, _, rep = _run(torch, src, sigma, net, fallback_db=0.0, under_db=1e31, check_block=64,
+                     check_every=1)
+    assert rep["fallback"] == 0
