@@ -109,12 +109,14 @@ def main(argv=None):
     ap.add_argument("--out", default=pilot.DEFAULT_WEIGHTS)
     ap.add_argument("--checkpoint", default=None, help="also write weights + Adam state here")
     ap.add_argument("--resume", default=None, help="start from this blob (weights or checkpoint)")
+    ap.add_argument("--poisson-fraction", type=float, default=0.5,
+                    help="share of crops with Poisson-Gaussian noise (the rest Gaussian)")
     ap.add_argument("--loss", choices=["mse", "sigma"], default="mse",
                     help="sigma: each crop's squared error scaled by (0.05 / sigma)^2")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
-    _, noisy, sigma = make_crops(args.crops, args.size, args.seed)
+    _, noisy, sigma = make_crops(args.crops, args.size, args.seed, poisson_fraction=args.poisson_fraction)
     log(f"crops: {args.crops} x {args.size}^2 in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     X, T = build_pairs(noisy, sigma)
@@ -132,6 +134,7 @@ def main(argv=None):
             f.write(net.save(True))
     print(json.dumps({"crops": args.crops, "size": args.size, "epochs": args.epochs,
                       "batch": args.batch, "lr": args.lr, "seed": args.seed, "loss": args.loss,
+                      "poisson_fraction": args.poisson_fraction,
                       "loss_per_epoch": hist, "train_seconds": round(train_s, 2),
                       "weights": os.path.abspath(args.out)}), flush=True)
 
