@@ -6,7 +6,8 @@
 //   pos/cnt   match lists: per reference position 16 packed (y<<16 | x) + raw count
 //   partial   per (channel, tile) num/den planes of the tile footprint (<= 91x91)
 // Reference positions: x in {0, 3, 6, ...} < W-8, plus W-8 (same in y).
-// A tile is 16x16 reference positions; its footprint is every pixel a member
+// A tile is 16x8 reference positions (x by y; mixed-precision plans may use
+// 16x16); its footprint is every pixel a member
 // patch of one of its groups can touch (refs +- 19 px search, + 8 px patch).
 //
 // Arithmetic is FP32 with a fixed operation order (explicit fmaf, -ffp-contract=off):
@@ -688,10 +689,12 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
     return wgt;
 }
 
-template <int STEP>
-__global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
-    __shared__ float2 acc[FOOT2];  // (num, den) per footprint pixel
-    __shared__ __align__(16) float scratch[4][SCR * SPP];
+// Tiles of 16 x TY refs, NW waves per workgroup (one group per wave per round).
+template <int STEP, int TY, int NW>
+__global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
+    constexpr int DOFS = FOOT * foot_edge(TY);
+    __shared__ float2 acc[DOFS];  // (num, den) per footprint pixel
+    __shared__ __align__(16) float scratch[NW][SCR * SPP];
     __shared__ int turn;  // number of groups of this tile already accumulated
     const int2 item = L.items[blockIdx.x];
     const FrameTab& F = L.tab[item.x];
@@ -704,7 +707,7 @@ __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
     bool timed_out = false;
     const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
     const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
-    for (int idx = threadIdx.x; idx < FOOT2; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
+    for (int idx = threadIdx.x; idx < DOFS; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -718,18 +721,18 @@ __global__ __launch_bounds__(256) void k_group(GroupLaunch L) {
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
     const int w = p.w;
-    const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TILE, p.ny - ty * TILE);
+    const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
 
-    for (int rnd = 0; rnd < (G + 3) / 4; ++rnd) {
-        const int g = rnd * 4 + wave;
+    for (int rnd = 0; rnd < (G + NW - 1) / NW; ++rnd) {
+        const int g = rnd * NW + wave;
         const bool has = g < G;
         int n = 0;
         float wgt = 0.0f;
         float v[NMAX];
         uint32_t mp[NMAX];
         if (has) {
-            const int bx = tx * TILE + g % nbx, by = ty * TILE + g / nbx;
+            const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
             const size_t ref = (size_t)by * p.nx + bx;
             n = floor_pow2(p.cnt[ref]);
             const uint4* pp = reinterpret_cast<const uint4*>(p.pos + ref * NMAX);
@@ -1032,7 +1035,7 @@ struct cpipe_bm3d_plan_s {
     };
 
     int c = 0, precision = 0, device = 0;
-    int tile_y = TILE;               // refs per tile column (16 exact FP32, 8 or 16 mixed precision)
+    int tile_y = TILE / 2;           // refs per tile column (8 exact FP32, 8 or 16 mixed precision)
     int dofs = FOOT2;                // den offset in a partial block
     size_t tile_stride = 2 * (size_t)FOOT2;
     int order = 0;                   // work-list order: 0 frame raster, 1 XCD-contiguous chunks
@@ -1308,10 +1311,11 @@ int build_full_lists(cpipe_bm3d_plan* p) {
     return OK;
 }
 
-// Tile shape: exact FP32 plans use 16 x 16-ref tiles (the CPU FP32
-// restatement's tiling); mixed-precision plans default to 16 x 8, whose
-// smaller LDS footprint fits three workgroups per CU (three ordered
-// aggregation chains instead of two).
+// Tile shape: every plan starts with 16 x 8-ref tiles (exact FP32 plans keep
+// them: the CPU FP32 restatement aggregates over the same tiling). Their
+// footprint (91 x 67 px of (num, den)) is small enough for three workgroups
+// of 8 waves per CU, i.e. three ordered aggregation chains per CU instead of
+// two with 16 x 16-ref tiles. Mixed-precision plans may be retuned to 16 x 16.
 int set_tile_shape(cpipe_bm3d_plan* p, int tile_y) {
     p->tile_y = tile_y;
     p->dofs = FOOT * foot_edge(tile_y);
@@ -1369,7 +1373,7 @@ int create_core(cpipe_bm3d_plan** out, const int* widths, const int* heights, in
         free_plan(p);
         return fail(INTERNAL, std::string("init: ") + hipGetErrorString(e));
     }
-    rc = set_tile_shape(p, precision == CPIPE_BM3D_PRECISION_FP32 ? TILE : TILE / 2);
+    rc = set_tile_shape(p, TILE / 2);
     if (rc) {
         free_plan(p);
         return rc;
@@ -1457,10 +1461,12 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
         const dim3 ggrid(L.n_grp, p->c);
         if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
             HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, gl, ggrid, r.st));
+        } else if (p->tile_y != TILE / 2) {
+            return fail(INTERNAL, "exact FP32 plans run 16 x 8-ref tiles only");
         } else if (step == 1) {
-            k_group<1><<<ggrid, 256, 0, r.st>>>(gl);
+            k_group<1, TILE / 2, 8><<<ggrid, 512, 0, r.st>>>(gl);
         } else {
-            k_group<2><<<ggrid, 256, 0, r.st>>>(gl);
+            k_group<2, TILE / 2, 8><<<ggrid, 512, 0, r.st>>>(gl);
         }
         HIP_CHECK(hipGetLastError());
     }

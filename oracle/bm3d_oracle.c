@@ -417,11 +417,11 @@ typedef struct {
     size_t tile_stride;
 } agg_ctx_f32;
 
-#define FOOT_MAX (15 * ORC_STRIDE + 2 * ORC_R + ORC_K) /* 91 */
+#define FOOT_MAX ((ORC_TILE_X - 1) * ORC_STRIDE + 2 * ORC_R + ORC_K) /* 91 */
 
-static void tile_box(int t, int nref, int n, int* f0, int* f1) {
-    const int first = oracle_ref_pos(t * ORC_TILE, n);
-    int lastref = t * ORC_TILE + ORC_TILE - 1;
+static void tile_box(int t, int edge, int nref, int n, int* f0, int* f1) {
+    const int first = oracle_ref_pos(t * edge, n);
+    int lastref = t * edge + edge - 1;
     if (lastref > nref - 1) lastref = nref - 1;
     const int last = oracle_ref_pos(lastref, n);
     *f0 = first - ORC_R < 0 ? 0 : first - ORC_R;
@@ -434,8 +434,8 @@ static void group_job_f32(void* vc, int job) {
     const int tile = job / a->channels;
     const int tx = tile % a->tx, ty = tile / a->tx;
     int fx0, fx1, fy0, fy1;
-    tile_box(tx, a->nx, a->w, &fx0, &fx1);
-    tile_box(ty, a->ny, a->h, &fy0, &fy1);
+    tile_box(tx, ORC_TILE_X, a->nx, a->w, &fx0, &fx1);
+    tile_box(ty, ORC_TILE_Y, a->ny, a->h, &fy0, &fy1);
     const int fw = fx1 - fx0;
     float* num = a->partial + ((size_t)c * a->tx * a->ty + tile) * a->tile_stride;
     float* den = num + FOOT_MAX * FOOT_MAX;
@@ -445,8 +445,8 @@ static void group_job_f32(void* vc, int job) {
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
     float g[ORC_NMAX][64], gb[ORC_NMAX][64];
-    for (int by = ty * ORC_TILE; by < ty * ORC_TILE + ORC_TILE && by < a->ny; ++by) {
-        for (int bx = tx * ORC_TILE; bx < tx * ORC_TILE + ORC_TILE && bx < a->nx; ++bx) {
+    for (int by = ty * ORC_TILE_Y; by < ty * ORC_TILE_Y + ORC_TILE_Y && by < a->ny; ++by) {
+        for (int bx = tx * ORC_TILE_X; bx < tx * ORC_TILE_X + ORC_TILE_X && bx < a->nx; ++bx) {
             const size_t r = (size_t)by * a->nx + bx;
             const uint32_t* pos = a->pos + r * ORC_NMAX;
             const int n = floor_pow2(a->count[r]);
@@ -542,10 +542,10 @@ static void merge_row_job(void* vc, int y) {
             float num = 0.0f, den = 0.0f;
             for (int ty = mc->tylo[y]; ty <= mc->tyhi[y]; ++ty) {
                 int fy0, fy1;
-                tile_box(ty, a->ny, a->h, &fy0, &fy1);
+                tile_box(ty, ORC_TILE_Y, a->ny, a->h, &fy0, &fy1);
                 for (int tx = mc->txlo[x]; tx <= mc->txhi[x]; ++tx) {
                     int fx0, fx1;
-                    tile_box(tx, a->nx, a->w, &fx0, &fx1);
+                    tile_box(tx, ORC_TILE_X, a->nx, a->w, &fx0, &fx1);
                     const int fw = fx1 - fx0;
                     const float* pn =
                         a->partial + ((size_t)c * a->tx * a->ty + (size_t)ty * a->tx + tx) * a->tile_stride;
@@ -559,14 +559,14 @@ static void merge_row_job(void* vc, int y) {
     }
 }
 
-static void tile_cover(int ntile, int nref, int n, int* lo, int* hi) {
+static void tile_cover(int ntile, int edge, int nref, int n, int* lo, int* hi) {
     for (int x = 0; x < n; ++x) {
         lo[x] = -1;
         hi[x] = -2;
     }
     for (int t = 0; t < ntile; ++t) {
         int f0, f1;
-        tile_box(t, nref, n, &f0, &f1);
+        tile_box(t, edge, nref, n, &f0, &f1);
         for (int x = f0; x < f1; ++x) {
             if (lo[x] < 0) lo[x] = t;
             hi[x] = t;
@@ -627,7 +627,7 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
     make_tables_f32(&tb);
     const int nx = oracle_ref_count(width), ny = oracle_ref_count(height);
     const size_t nref = (size_t)nx * ny;
-    const int tx = (nx + ORC_TILE - 1) / ORC_TILE, ty = (ny + ORC_TILE - 1) / ORC_TILE;
+    const int tx = (nx + ORC_TILE_X - 1) / ORC_TILE_X, ty = (ny + ORC_TILE_Y - 1) / ORC_TILE_Y;
 
     float* zbuf = (float*)malloc(sizeof(float) * npx * 3 * 3);
     float* z[3] = {zbuf, zbuf + npx, zbuf + 2 * npx};      /* noisy, opponent space */
@@ -643,8 +643,8 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
         free(zbuf); free(pos); free(cnt); free(partial); free(cover);
         return 3;
     }
-    tile_cover(tx, nx, width, txlo, txhi);
-    tile_cover(ty, ny, height, tylo, tyhi);
+    tile_cover(tx, ORC_TILE_X, nx, width, txlo, txhi);
+    tile_cover(ty, ORC_TILE_Y, ny, height, tylo, tyhi);
 
     if (channels == 3) {
         opp_fwd_f32(in, z, npx);

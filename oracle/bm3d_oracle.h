@@ -39,7 +39,8 @@ extern "C" {
 #define ORC_R 19       /* search radius: 39x39 window */
 #define ORC_STRIDE 3   /* reference-patch stride, last row/col forced */
 #define ORC_NMAX 16    /* maximum group size */
-#define ORC_TILE 16    /* refs per aggregation-tile edge (GPU tiling, mirrored) */
+#define ORC_TILE_X 16  /* refs per aggregation tile along x (GPU tiling, mirrored) */
+#define ORC_TILE_Y 8   /* refs per aggregation tile along y */
 
 /* IEEE binary16 <-> binary32, round to nearest even. */
 uint16_t oracle_f32_to_f16(float v);
