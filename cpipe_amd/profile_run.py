@@ -4,7 +4,7 @@ Runs the plan on the named workload `--runs` times, then a 256 MiB
 device-to-device copy whose counter readings calibrate FETCH_SIZE and
 WRITE_SIZE (the copy reads and writes exactly 268435456 bytes).
 
-Usage: python -m cpipe_amd.profile_run --workload rgb12mp --precision fp32 [--pilot]
+Usage: python -m cpipe_amd.profile_run --workload rgb12mp --precision fp32 [--pilot] [--bm mfma]
 """
 from __future__ import annotations
 
@@ -18,6 +18,8 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--runs", type=int, default=1)
     ap.add_argument("--pilot", action="store_true", help="learned pilot with the packaged weights")
+    ap.add_argument("--bm", choices=["exact", "mfma"], default=None,
+                    help="block-matching kernel of a mixed-precision plan (default: the plan's)")
     args = ap.parse_args(argv)
     import torch
     from . import engine, synth
@@ -26,6 +28,8 @@ def main(argv=None):
     src = torch.from_numpy(noisy).cuda()
     dst = torch.empty_like(src)
     plan = engine.Plan(w, h, c, args.precision, 0)
+    if args.bm is not None:
+        plan.set_block_matching(args.bm)
     if args.pilot:
         from . import pilot
         plan.set_pilot(pilot.Pilot(seed=None, weights=pilot.DEFAULT_WEIGHTS))
