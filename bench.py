@@ -201,9 +201,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rgb12mp")
-    ap.add_argument("--forms", default="fp16,bf16,fp16_mfmabm,learned_fp16,learned_fp32",
+    ap.add_argument("--forms", default="fp16,bf16,fp16_exactbm,learned_fp16,learned_fp32",
                     help="extra forms besides the plain FP32 headline ('' for none): precision "
-                         "fp16|bf16|fp32, optional learned_ (pilot) and _mfmabm (MFMA block matching)")
+                         "fp16|bf16|fp32, optional learned_ (pilot) and _exactbm (exact FP32 block "
+                         "matching in a mixed-precision plan, whose default is the MFMA cross term)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
@@ -279,8 +280,8 @@ def main():
         toks = name.split("_")
         prec = next(t for t in toks if t in ("fp32", "fp16", "bf16"))
         pl = engine.Plan(pw, ph, c, prec, 0)
-        if "mfmabm" in toks:
-            pl.set_block_matching("mfma")
+        if "exactbm" in toks:
+            pl.set_block_matching("exact")
         if "learned" in toks:
             pl.set_pilot(net)
         return pl, prec

@@ -2,9 +2,9 @@
 // MFMA, an exact FP32 re-rank of the shortlisted candidates.
 //
 // Same workgroup tiling as the exact kernel (16 ref columns x 14 ref rows),
-// 8 waves: wave w takes the 4 ref columns 4 (w mod 4) .. +3 of the 4 x 4-ref
-// sub-blocks in ref rows 4 (w / 4) and 4 (w / 4) + 8 (the last sub-block row
-// has 2 real rows). Phases of a workgroup:
+// 16 waves: wave w takes the 4 x 4-ref sub-block at ref column 4 (w mod 4),
+// ref row 4 (w / 4) (the last sub-block row has 2 real rows). Phases of a
+// workgroup:
 //  1. stage the search region in LDS, in FP32 (`tf`) and centred on its mean
 //     mu and rounded to FP16 (or BF16) (`th`), and the squared norm
 //     ||c - mu||^2 of every candidate patch of the rounded tile (`nrm`);
@@ -50,8 +50,8 @@ constexpr int LH = (TY + 1) * STRIDE + 2 * R + 3 + 1;   // 87 rows
 constexpr int NW = 88;                                  // norm columns (candidate x < 84)
 constexpr int NH = (TY - 1) * STRIDE + 2 * R + 1;       // 78 candidate rows
 constexpr int KL = 12;                                  // shortlist per lane
-constexpr int QP = 24;                                  // queue slots per lane (a column group adds <= 16)
-constexpr int NT = 512;                                 // threads per workgroup
+constexpr int QP = 8;                                   // queue slots per lane
+constexpr int NT = 1024;                                // threads per workgroup (4 waves per SIMD)
 // A row block of 16 candidate rows may run past the union's last row (up to
 // tile row 83, patch rows to 90): those rows of the rounded tile and of the
 // norms are padding (zero), read and never kept.
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
                                                int use_gate) {
     typedef Bx<PREC> X;
     __shared__ __align__(16) BmLds L;
-    __shared__ float red[16];
+    __shared__ float red[2 * (NT / 64)];
 
     const int4 it = items[blockIdx.x];
     const FrameTab& F = tab[it.x];
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
         }
         if ((tid & 63) == 0) {
             red[tid >> 6] = sum;
-            red[8 + (tid >> 6)] = __int_as_float(nin);
+            red[NT / 64 + (tid >> 6)] = __int_as_float(nin);
         }
         __syncthreads();
         float tot = 0.0f;
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
 #pragma unroll
         for (int i = 0; i < NT / 64; ++i) {
             tot += red[i];
-            ntot += __float_as_int(red[8 + i]);
+            ntot += __float_as_int(red[NT / 64 + i]);
         }
         const float mu = ntot > 0 ? tot / (float)ntot : 0.0f;
 #pragma unroll
@@ -281,8 +281,8 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
     const int gx = 4 * (wave & 3);
 
     // ---- 2./3. scan a sub-block, then 4. re-rank its shortlists
-    for (int gy = 4 * (wave >> 2); gy < TY; gy += 8) {
-        if (by0 + gy >= ny) break;  // uniform: row groups past the image
+    const int gy = 4 * (wave >> 2);
+    if (by0 + gy < ny) {  // uniform: row groups past the image have no refs
         const SubBlock b = sub_block(gx, gy, j, bx0, by0, X0, Y0, w, h, nx, ny, regular);
         // ref j as B operands (K group q), for column offsets 0..3
         uint4 B0[4], B1[4], B2[4];
@@ -321,11 +321,12 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
         // ref's four lanes (24 candidates are at least that good, so a worse one
         // cannot reach the ref's best 15 with any margin)
         float lim = inf;
+        uint32_t* const pq = L.pend + tid;  // this lane's queue, slot k at pq[k * NT]
         auto flush = [&]() {
 #pragma unroll
             for (int k = 0; k < QP; ++k) {
                 if (!__any(k < np)) break;
-                const float e = k < np ? __uint_as_float(L.pend[k * NT + tid]) : inf;
+                const float e = k < np ? __uint_as_float(pq[k * NT]) : inf;
                 insert_k(bd, e);
             }
             np = 0;
@@ -357,7 +358,6 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
                 // position code: (row - cy0) << 6 | (column - xa0), < 48 x 64
                 const uint32_t prow = (uint32_t)((cyb - b.cy0 + rq) << 6) - (uint32_t)b.xa0;
                 for (int xa = xa_first; xa <= b.xend; xa += 16) {
-                    if (__any(np > QP - 16)) flush();
                     const uint2 a00 = *reinterpret_cast<const uint2*>(ta0 + xa);
                     const uint2 a01 = *reinterpret_cast<const uint2*>(ta0 + xa + 4);
                     const uint2 a10 = *reinterpret_cast<const uint2*>(ta1 + xa);
@@ -381,13 +381,14 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
                     const uint32_t pb = prow + (uint32_t)xa;
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
+                        if (__any(np > QP - 4)) flush();
                         const bool xok = xa + s >= wlo && xa + s <= whi;
 #pragma unroll
                         for (int t = 0; t < 4; ++t) {
                             const float nvv = (xok && yok[t]) ? nv[t][s] : inf;
                             const float sc = __builtin_fmaf(-2.0f, acc[s][t], nvv);
                             if (sc < lim) {
-                                L.pend[np * NT + tid] =
+                                pq[np * NT] =
                                     (__float_as_uint(sc) & ~0xfffu) | (pb + (uint32_t)(((4 * t) << 6) + s));
                                 ++np;
                             }

@@ -248,8 +248,8 @@ int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* batch, int* tile_y, int* ord
                             int* bm_mfma);
 
 /* Mixed-precision plans: block matching with the FP16/BF16 MFMA cross term and
- * an exact FP32 re-rank of a 48-candidate shortlist per ref (mfma = 1), or the
- * exact FP32 kernel (mfma = 0, the default). Both produce the exact kernel's
+ * an exact FP32 re-rank of a 48-candidate shortlist per ref (mfma = 1, the
+ * default), or the exact FP32 kernel (mfma = 0). Both produce the exact kernel's
  * match lists whenever the shortlist holds the exact best 15 (always, on the
  * test images); the autotuner times both. CPIPE_BM3D_UNSUPPORTED for FP32 plans. */
 int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* plan, int mfma);
