@@ -58,12 +58,17 @@ constexpr int NT = 1024;                                // threads per workgroup
 constexpr int LHP = LH + 4;
 constexpr int NHP = NH + 6;
 
+// Members in this order keep every vector access naturally aligned (16-byte
+// norm rows, 8-byte rounded-tile groups); an unaligned LDS access stalls.
 struct BmLds {
+    float nrm[NHP * NW];       // ||c - mu||^2 of the rounded tile per candidate position
     float tf[LH * LW];         // the FP32 tile for the re-rank (0 outside the image)
     uint16_t th[LHP * LWH];    // rounded, centred tile
-    float nrm[NHP * NW];       // ||c - mu||^2 of the rounded tile per candidate position
     uint32_t pend[QP * NT];    // admission queue (packed scores)
 };
+static_assert(sizeof(float) * NHP * NW % 16 == 0 && sizeof(float) * LH * LW % 16 == 0 &&
+                  NW % 4 == 0 && LWH % 4 == 0,
+              "aligned norm rows and rounded-tile groups");
 static_assert(sizeof(BmLds) + 64 <= 160 * 1024, "one workgroup per CU");
 
 typedef float f4 __attribute__((ext_vector_type(4)));
