@@ -50,7 +50,7 @@ constexpr int LH = (TY + 1) * STRIDE + 2 * R + 3 + 1;   // 87 rows
 constexpr int NW = 88;                                  // norm columns (candidate x < 84)
 constexpr int NH = (TY - 1) * STRIDE + 2 * R + 1;       // 78 candidate rows
 constexpr int KL = 12;                                  // shortlist per lane
-constexpr int QP = 8;                                   // queue slots per lane
+constexpr int QP = 16;                                  // queue slots per lane
 constexpr int NT = 1024;                                // threads per workgroup (4 waves per SIMD)
 // A row block of 16 candidate rows may run past the union's last row (up to
 // tile row 83, patch rows to 90): those rows of the rounded tile and of the
@@ -321,26 +321,34 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
         float bd[KL];
 #pragma unroll
         for (int k = 0; k < KL; ++k) bd[k] = inf;
-        int np = 0;
         // admission bound: this lane's 12th best, and the largest 6th best of the
         // ref's four lanes (24 candidates are at least that good, so a worse one
         // cannot reach the ref's best 15 with any margin)
         float lim = inf;
         uint32_t* const pq = L.pend + tid;  // this lane's queue, slot k at pq[k * NT]
+        uint32_t* qp = pq;                  // next free slot
+        uint32_t* const qfull = pq + (QP - 4) * NT;
         auto flush = [&]() {
+            float e[QP];
+#pragma unroll
+            for (int k = 0; k < QP; ++k) e[k] = __uint_as_float(pq[k * NT]);  // all in flight
+            const int np = (int)(qp - pq) / NT;
 #pragma unroll
             for (int k = 0; k < QP; ++k) {
                 if (!__any(k < np)) break;
-                const float e = k < np ? __uint_as_float(pq[k * NT]) : inf;
-                insert_k(bd, e);
+                insert_k(bd, k < np ? e[k] : inf);
             }
-            np = 0;
+            qp = pq;
             float t6 = bd[5];
             t6 = fmaxf(t6, __shfl_xor(t6, 16));
             t6 = fmaxf(t6, __shfl_xor(t6, 32));
             lim = fminf(bd[KL - 1], t6);
         };
-        for (int cyb = b.cy0; cyb <= b.cy1; cyb += 16) {
+        // row blocks: the one holding the refs' own rows first (near candidates
+        // tighten the admission bound early), then the ones above and below
+        for (int bi = 0; bi < 3; ++bi) {
+            const int cyb = b.cy0 + 16 * (bi == 0 ? 1 : (bi == 1 ? 0 : 2));
+            if (cyb > b.cy1) continue;
             // column groups in 4 passes of equal rotation (xa / 4 mod 4), so every
             // row offset below is fixed within a pass
 #pragma unroll
@@ -386,16 +394,15 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
                     const uint32_t pb = prow + (uint32_t)xa;
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
-                        if (__any(np > QP - 4)) flush();
+                        if (__any(qp > qfull)) flush();
                         const bool xok = xa + s >= wlo && xa + s <= whi;
 #pragma unroll
                         for (int t = 0; t < 4; ++t) {
                             const float nvv = (xok && yok[t]) ? nv[t][s] : inf;
                             const float sc = __builtin_fmaf(-2.0f, acc[s][t], nvv);
                             if (sc < lim) {
-                                pq[np * NT] =
-                                    (__float_as_uint(sc) & ~0xfffu) | (pb + (uint32_t)(((4 * t) << 6) + s));
-                                ++np;
+                                *qp = (__float_as_uint(sc) & ~0xfffu) | (pb + (uint32_t)(((4 * t) << 6) + s));
+                                qp += NT;
                             }
                         }
                     }
@@ -407,12 +414,17 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
         // ---- 4. exact FP32 re-rank (row sums as fma chains, rows added in order)
         uint32_t bp[KL];  // (y << 16) | x
         float es[KL];
+        int cb[KL];       // FP32-tile offset of a live entry; 0 (one shared address) otherwise
+        uint32_t lm = 0;  // live entries, bit k
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             const uint32_t rel = __float_as_uint(bd[k]) & 0xfffu;
             const int qy = b.cy0 + (int)(rel >> 6), qx = X0 + b.xa0 + (int)(rel & 63u);
             bp[k] = ((uint32_t)qy << 16) | (uint32_t)qx;
             es[k] = 0.0f;
+            const bool live = b.real && bd[k] < inf && !(qx == b.rx && qy == b.ry);
+            lm |= live ? 1u << k : 0u;
+            cb[k] = live ? (qy - Y0) * LW + (qx - X0) : 0;
         }
 #pragma unroll 1
         for (int i = 0; i < 8; ++i) {
@@ -420,18 +432,27 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
             float rr[8];
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) rr[jj] = ri[jj];
+            // three candidate rows in flight at a time
 #pragma unroll
-            for (int k = 0; k < KL; ++k) {
-                const int qx = (int)(bp[k] & 0xffffu), qy = (int)(bp[k] >> 16);
-                const float* ci = L.tf + (qy - Y0 + i) * LW + (qx - X0);
-                float e = rr[0] - ci[0];
-                float r = e * e;
+            for (int k0 = 0; k0 < KL; k0 += 3) {
+                float c[3][8];
 #pragma unroll
-                for (int jj = 1; jj < 8; ++jj) {
-                    e = rr[jj] - ci[jj];
-                    r = __builtin_fmaf(e, e, r);
+                for (int kk = 0; kk < 3; ++kk) {
+                    const float* ci = L.tf + cb[k0 + kk] + i * LW;
+#pragma unroll
+                    for (int jj = 0; jj < 8; ++jj) c[kk][jj] = ci[jj];
                 }
-                es[k] = i == 0 ? r : es[k] + r;
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    float e = rr[0] - c[kk][0];
+                    float r = e * e;
+#pragma unroll
+                    for (int jj = 1; jj < 8; ++jj) {
+                        e = rr[jj] - c[kk][jj];
+                        r = __builtin_fmaf(e, e, r);
+                    }
+                    es[k0 + kk] = i == 0 ? r : es[k0 + kk] + r;
+                }
             }
         }
         // order key (ssd bits, scan order (dy + R) * 64 + (dx + R)); the ref
@@ -441,7 +462,7 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             const int qx = (int)(bp[k] & 0xffffu), qy = (int)(bp[k] >> 16);
-            const bool ok = b.real && bd[k] < inf && !(qx == b.rx && qy == b.ry) && es[k] < tau;
+            const bool ok = ((lm >> k) & 1u) && es[k] < tau;
             key[k] = ok ? ((uint64_t)__float_as_uint(es[k]) << 32) | (uint64_t)((qy - b.ry + R) * 64 + (qx - b.rx + R))
                         : none;
         }
@@ -457,11 +478,15 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
 #pragma unroll 1
         for (int o = 1; o < 4; ++o) {
             const int src = (lane + 16 * o) & 63;
-#pragma unroll 1
+            uint32_t lo[KL], hi[KL];  // the other lane's keys, all fetches in flight
+#pragma unroll
             for (int m = 0; m < KL; ++m) {
-                const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)key[m], src);
-                const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(key[m] >> 32), src);
-                const uint64_t ok = ((uint64_t)hi << 32) | lo;
+                lo[m] = (uint32_t)__shfl((int)(uint32_t)key[m], src);
+                hi[m] = (uint32_t)__shfl((int)(uint32_t)(key[m] >> 32), src);
+            }
+#pragma unroll
+            for (int m = 0; m < KL; ++m) {
+                const uint64_t ok = ((uint64_t)hi[m] << 32) | lo[m];
 #pragma unroll
                 for (int k = 0; k < KL; ++k) rank[k] += ok < key[k] ? 1 : 0;
             }
