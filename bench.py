@@ -31,7 +31,9 @@ K, R, STRIDE = 8, 19, 3
 PEAK_FP32_TFLOPS = 157.3
 PEAK_F16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "group_wiener_fp32_rgb12mp_traffic.json")
+# HBM bytes per launch of each exact FP32 kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, rgb12mp)
+TRAFFIC_FILES = {k: os.path.join(ROOT, "profiles", f"{k}_fp32_rgb12mp_traffic.json")
+                 for k in ("block_match", "group_ht", "group_wiener")}
 
 
 def ref_positions(n: int) -> np.ndarray:
@@ -259,8 +261,9 @@ def main():
     achieved = flop / (ms / 1e3) / 1e12
     total_flop = 2 * bm_flop + g1_flop + g2_flop
     traffic = None
-    if dominant == "group_wiener" and os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
+    tfile = TRAFFIC_FILES.get(dominant)
+    if args.workload == "rgb12mp" and tfile and os.path.exists(tfile):
+        with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "mfma", "kernel": dominant, "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
