@@ -288,130 +288,141 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
     // ---- 2./3. scan a sub-block, then 4. re-rank its shortlists
     const int gy = 4 * (wave >> 2);
     if (by0 + gy < ny) {  // uniform: row groups past the image have no refs
-        const SubBlock b = sub_block(gx, gy, j, bx0, by0, X0, Y0, w, h, nx, ny, regular);
-        // ref j as B operands (K group q), for column offsets 0..3
-        uint4 B0[4], B1[4], B2[4];
-        {
-            uint16_t r0[8], r1[8], r2[8], r3[8];
-            const uint16_t* rp = L.th + b.lry * LWH + b.lrx;
+        float bd[KL];  // this lane's shortlist, ascending packed scores
+        {  // scan
+            const SubBlock b = sub_block(gx, gy, j, bx0, by0, X0, Y0, w, h, nx, ny, regular);
+            // ref j as B operands (K group q), for column offsets 0..3
+            uint4 B0[4], B1[4], B2[4];
+            {
+                uint16_t r0[8], r1[8], r2[8], r3[8];
+                const uint16_t* rp = L.th + b.lry * LWH + b.lrx;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                r0[e] = rp[q * LWH + e];
-                r1[e] = rp[(4 + q) * LWH + e];
-                r2[e] = rp[(2 * q) * LWH + e];
-                r3[e] = rp[(2 * q + 1) * LWH + e];
-            }
-            B0[0] = b_head<0>(r0);
-            B0[1] = b_head<1>(r0);
-            B0[2] = b_head<2>(r0);
-            B0[3] = b_head<3>(r0);
-            B1[0] = b_head<0>(r1);
-            B1[1] = b_head<1>(r1);
-            B1[2] = b_head<2>(r1);
-            B1[3] = b_head<3>(r1);
-            B2[0] = make_uint4(0u, 0u, 0u, 0u);
-            B2[1] = b_tail<1>(r2, r3);
-            B2[2] = b_tail<2>(r2, r3);
-            B2[3] = b_tail<3>(r2, r3);
-        }
-        // this lane's window (tile x, absolute y); empty if not real. The ref
-        // itself is scanned too (it is the best match, and is dropped at the re-rank).
-        const int wlo = b.real ? max(b.lrx - R, -X0) : 1, whi = b.real ? min(b.lrx + R, w - K - X0) : 0;
-        const int ylo = b.real ? max(b.ry - R, 0) : 1, yhi = b.real ? min(b.ry + R, h - K) : 0;
-        float bd[KL];
-#pragma unroll
-        for (int k = 0; k < KL; ++k) bd[k] = inf;
-        // admission bound: this lane's 12th best, and the largest 6th best of the
-        // ref's four lanes (24 candidates are at least that good, so a worse one
-        // cannot reach the ref's best 15 with any margin)
-        float lim = inf;
-        uint32_t* const pq = L.pend + tid;  // this lane's queue, slot k at pq[k * NT]
-        uint32_t* qp = pq;                  // next free slot
-        uint32_t* const qfull = pq + (QP - 4) * NT;
-        auto flush = [&]() {
-            float e[QP];
-#pragma unroll
-            for (int k = 0; k < QP; ++k) e[k] = __uint_as_float(pq[k * NT]);  // all in flight
-            const int np = (int)(qp - pq) / NT;
-#pragma unroll
-            for (int k = 0; k < QP; ++k) {
-                if (!__any(k < np)) break;
-                insert_k(bd, k < np ? e[k] : inf);
-            }
-            qp = pq;
-            float t6 = bd[5];
-            t6 = fmaxf(t6, __shfl_xor(t6, 16));
-            t6 = fmaxf(t6, __shfl_xor(t6, 32));
-            lim = fminf(bd[KL - 1], t6);
-        };
-        // row blocks: the one holding the refs' own rows first (near candidates
-        // tighten the admission bound early), then the ones above and below
-        for (int bi = 0; bi < 3; ++bi) {
-            const int cyb = b.cy0 + 16 * (bi == 0 ? 1 : (bi == 1 ? 0 : 2));
-            if (cyb > b.cy1) continue;
-            // column groups in 4 passes of equal rotation (xa / 4 mod 4), so every
-            // row offset below is fixed within a pass
-#pragma unroll
-            for (int rot = 0; rot < 4; ++rot) {
-                const int xa_first = b.xa0 + 4 * ((rot - (b.xa0 >> 2)) & 3);
-                // A: matrix row j = 4 q' + t' is candidate row cyb + 4 t' + ((q' + rot) & 3)
-                const int rowa = cyb + 4 * (j & 3) + (((j >> 2) + rot) & 3) - Y0;
-                const uint16_t* ta0 = L.th + (rowa + q) * LWH;
-                const uint16_t* ta1 = L.th + (rowa + 4 + q) * LWH;
-                const uint16_t* ta2 = L.th + (rowa + 2 * q) * LWH + 8;
-                // this lane's D rows: element t is candidate row cyb + 4 t + rq
-                const int rq = (q + rot) & 3;
-                const float* tn = L.nrm + (cyb + rq - Y0) * NW;
-                bool yok[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int cy = cyb + 4 * t + rq;
-                    yok[t] = cy >= ylo && cy <= yhi;
+                for (int e = 0; e < 8; ++e) {
+                    r0[e] = rp[q * LWH + e];
+                    r1[e] = rp[(4 + q) * LWH + e];
+                    r2[e] = rp[(2 * q) * LWH + e];
+                    r3[e] = rp[(2 * q + 1) * LWH + e];
                 }
-                // position code: (row - cy0) << 6 | (column - xa0), < 48 x 64
-                const uint32_t prow = (uint32_t)((cyb - b.cy0 + rq) << 6) - (uint32_t)b.xa0;
-                for (int xa = xa_first; xa <= b.xend; xa += 16) {
-                    const uint2 a00 = *reinterpret_cast<const uint2*>(ta0 + xa);
-                    const uint2 a01 = *reinterpret_cast<const uint2*>(ta0 + xa + 4);
-                    const uint2 a10 = *reinterpret_cast<const uint2*>(ta1 + xa);
-                    const uint2 a11 = *reinterpret_cast<const uint2*>(ta1 + xa + 4);
-                    const uint2 a20 = *reinterpret_cast<const uint2*>(ta2 + xa);
-                    const uint2 a21 = *reinterpret_cast<const uint2*>(ta2 + LWH + xa);
-                    const uint4 A0 = make_uint4(a00.x, a00.y, a01.x, a01.y);
-                    const uint4 A1 = make_uint4(a10.x, a10.y, a11.x, a11.y);
-                    const uint4 A2 = make_uint4(a20.x, a20.y, a21.x, a21.y);
-                    f4 nv[4];
+                B0[0] = b_head<0>(r0);
+                B0[1] = b_head<1>(r0);
+                B0[2] = b_head<2>(r0);
+                B0[3] = b_head<3>(r0);
+                B1[0] = b_head<0>(r1);
+                B1[1] = b_head<1>(r1);
+                B1[2] = b_head<2>(r1);
+                B1[3] = b_head<3>(r1);
+                B2[0] = make_uint4(0u, 0u, 0u, 0u);
+                B2[1] = b_tail<1>(r2, r3);
+                B2[2] = b_tail<2>(r2, r3);
+                B2[3] = b_tail<3>(r2, r3);
+            }
+            // this lane's window (tile x, absolute y); empty if not real. The ref
+            // itself is scanned too (it is the best match, and is dropped at the re-rank).
+            const int wlo = b.real ? max(b.lrx - R, -X0) : 1, whi = b.real ? min(b.lrx + R, w - K - X0) : 0;
+            const int ylo = b.real ? max(b.ry - R, 0) : 1, yhi = b.real ? min(b.ry + R, h - K) : 0;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) nv[t] = *reinterpret_cast<const f4*>(tn + 4 * t * NW + xa);
-                    f4 acc[4];
+            for (int k = 0; k < KL; ++k) bd[k] = inf;
+            // admission bound: this lane's 12th best, and the largest 6th best of the
+            // ref's four lanes (24 candidates are at least that good, so a worse one
+            // cannot reach the ref's best 15 with any margin)
+            float lim = inf;
+            uint32_t* const pq = L.pend + tid;  // this lane's queue, slot k at pq[k * NT]
+            uint32_t* qp = pq;                  // next free slot
+            uint32_t* const qfull = pq + (QP - 4) * NT;
+            auto flush = [&]() {
+                const int np = (int)(qp - pq) / NT;
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-                        acc[s] = X::mma(A0, B0[s], z);
-                        acc[s] = X::mma(A1, B1[s], acc[s]);
-                        if (s > 0) acc[s] = X::mma(A2, B2[s], acc[s]);
+                for (int k0 = 0; k0 < QP; k0 += 8) {
+                    if (!__any(k0 < np)) break;
+                    float e[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) e[k] = __uint_as_float(pq[(k0 + k) * NT]);  // in flight together
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        if (!__any(k0 + k < np)) break;
+                        insert_k(bd, k0 + k < np ? e[k] : inf);
                     }
-                    const uint32_t pb = prow + (uint32_t)xa;
+                }
+                qp = pq;
+                float t6 = bd[5];
+                t6 = fmaxf(t6, __shfl_xor(t6, 16));
+                t6 = fmaxf(t6, __shfl_xor(t6, 32));
+                lim = fminf(bd[KL - 1], t6);
+            };
+            // row blocks: the one holding the refs' own rows first (near candidates
+            // tighten the admission bound early), then the ones above and below
+            for (int bi = 0; bi < 3; ++bi) {
+                const int cyb = b.cy0 + 16 * (bi == 0 ? 1 : (bi == 1 ? 0 : 2));
+                if (cyb > b.cy1) continue;
+                // column groups in 4 passes of equal rotation (xa / 4 mod 4), so every
+                // row offset below is fixed within a pass
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        if (__any(qp > qfull)) flush();
-                        const bool xok = xa + s >= wlo && xa + s <= whi;
+                for (int rot = 0; rot < 4; ++rot) {
+                    const int xa_first = b.xa0 + 4 * ((rot - (b.xa0 >> 2)) & 3);
+                    // A: matrix row j = 4 q' + t' is candidate row cyb + 4 t' + ((q' + rot) & 3)
+                    const int rowa = cyb + 4 * (j & 3) + (((j >> 2) + rot) & 3) - Y0;
+                    const uint16_t* ta0 = L.th + (rowa + q) * LWH;
+                    const uint16_t* ta1 = L.th + (rowa + 4 + q) * LWH;
+                    const uint16_t* ta2 = L.th + (rowa + 2 * q) * LWH + 8;
+                    // this lane's D rows: element t is candidate row cyb + 4 t + rq
+                    const int rq = (q + rot) & 3;
+                    const float* tn = L.nrm + (cyb + rq - Y0) * NW;
+                    bool yok[4];
 #pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const float nvv = (xok && yok[t]) ? nv[t][s] : inf;
-                            const float sc = __builtin_fmaf(-2.0f, acc[s][t], nvv);
-                            if (sc < lim) {
-                                *qp = (__float_as_uint(sc) & ~0xfffu) | (pb + (uint32_t)(((4 * t) << 6) + s));
-                                qp += NT;
+                    for (int t = 0; t < 4; ++t) {
+                        const int cy = cyb + 4 * t + rq;
+                        yok[t] = cy >= ylo && cy <= yhi;
+                    }
+                    // position code: (row - cy0) << 6 | (column - xa0), < 48 x 64
+                    const uint32_t prow = (uint32_t)((cyb - b.cy0 + rq) << 6) - (uint32_t)b.xa0;
+                    for (int xa = xa_first; xa <= b.xend; xa += 16) {
+                        const uint2 a00 = *reinterpret_cast<const uint2*>(ta0 + xa);
+                        const uint2 a01 = *reinterpret_cast<const uint2*>(ta0 + xa + 4);
+                        const uint2 a10 = *reinterpret_cast<const uint2*>(ta1 + xa);
+                        const uint2 a11 = *reinterpret_cast<const uint2*>(ta1 + xa + 4);
+                        const uint2 a20 = *reinterpret_cast<const uint2*>(ta2 + xa);
+                        const uint2 a21 = *reinterpret_cast<const uint2*>(ta2 + LWH + xa);
+                        const uint4 A0 = make_uint4(a00.x, a00.y, a01.x, a01.y);
+                        const uint4 A1 = make_uint4(a10.x, a10.y, a11.x, a11.y);
+                        const uint4 A2 = make_uint4(a20.x, a20.y, a21.x, a21.y);
+                        f4 nv[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) nv[t] = *reinterpret_cast<const f4*>(tn + 4 * t * NW + xa);
+                        f4 acc[4];
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+                            acc[s] = X::mma(A0, B0[s], z);
+                            acc[s] = X::mma(A1, B1[s], acc[s]);
+                            if (s > 0) acc[s] = X::mma(A2, B2[s], acc[s]);
+                        }
+                        const uint32_t pb = prow + (uint32_t)xa;
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            if (__any(qp > qfull)) flush();
+                            const bool xok = xa + s >= wlo && xa + s <= whi;
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                const float nvv = (xok && yok[t]) ? nv[t][s] : inf;
+                                const float sc = __builtin_fmaf(-2.0f, acc[s][t], nvv);
+                                if (sc < lim) {
+                                    *qp = (__float_as_uint(sc) & ~0xfffu) | (pb + (uint32_t)(((4 * t) << 6) + s));
+                                    qp += NT;
+                                }
                             }
                         }
                     }
                 }
             }
+            flush();
         }
-        flush();
 
-        // ---- 4. exact FP32 re-rank (row sums as fma chains, rows added in order)
+        // ---- 4. exact FP32 re-rank (row sums as fma chains, rows added in order).
+        // The sub-block geometry is recomputed here rather than kept live through
+        // the scan (it would be spilled to scratch).
+        int jr = j;
+        asm volatile("" : "+v"(jr));
+        const SubBlock b = sub_block(gx, gy, jr, bx0, by0, X0, Y0, w, h, nx, ny, regular);
         uint32_t bp[KL];  // (y << 16) | x
         float es[KL];
         int cb[KL];       // FP32-tile offset of a live entry; 0 (one shared address) otherwise
