@@ -197,6 +197,60 @@ def host_path_bench(noisy, sigma, runs):
     return res
 
 
+def wavelet_bench(torch, noisy, stream, runs, want_cpu):
+    """The downstream node com.cpipe.denoise.wavelet_bayes_shrink on the same frame as
+    RGBA16F (alpha 1), resident in HBM: one kernel launch per run, timed with HIP
+    events on the launch stream. HBM-bound: 16 algorithmic bytes per pixel (8 read,
+    8 written). CPU baseline: the oracle (single thread) on a 1024x768 crop, whose
+    output the GPU's must equal bit for bit."""
+    c, h, w = noisy.shape
+    if c != 3:
+        return None
+    rgba = np.empty((h, w, 4), np.float32)
+    rgba[..., :3] = np.moveaxis(noisy, 0, -1)
+    rgba[..., 3] = 1.0
+    bits = rgba.astype(np.float16).view(np.uint16)
+    src = torch.from_numpy(bits.view(np.int16)).cuda()
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        engine.wavelet_rgba16f(src, dst, 1.0, stream)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(runs):
+        engine.wavelet_rgba16f(src, dst, 1.0, stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / runs
+    gbs = 16.0 * w * h / (ms / 1e3) / 1e9
+    res = {"node": "com.cpipe.denoise.wavelet_bayes_shrink", "frame": f"{w}x{h} RGBA16F (alpha 1), chroma_strength 1",
+           "value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpixel/s", "ms": round(ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                        "bytes_per_launch": 16 * w * h}}
+    tfile = os.path.join(ROOT, "profiles", "wavelet_rgba16f_rgb12mp_traffic.json")
+    if (w, h) == (4080, 3072) and os.path.exists(tfile):
+        with open(tfile) as f:
+            res["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    if want_cpu:
+        from oracle import oracle
+        cw, ch = min(1024, w), min(768, h)
+        crop = np.ascontiguousarray(bits[:ch, :cw])
+        t = time.perf_counter()
+        want = oracle.wavelet_rgba16f(crop, 1.0)
+        dt = time.perf_counter() - t
+        csrc = torch.from_numpy(crop.view(np.int16)).cuda()
+        cdst = torch.empty_like(csrc)
+        engine.wavelet_rgba16f(csrc, cdst, 1.0, stream)
+        torch.cuda.synchronize()
+        res["bit_exact_vs_oracle_crop"] = bool(np.array_equal(cdst.cpu().numpy().view(np.uint16), want))
+        res["cpu_baseline"] = {"value": round(cw * ch / dt / 1e6, 2), "unit": "Mpixel/s", "cores": 1,
+                               "kind": "port", "sample": f"{cw}x{ch} top-left crop, oracle_wavelet_rgba16f, "
+                                                         f"{dt:.3f} s"}
+        res["gpu_over_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +265,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) leg")
+    ap.add_argument("--no-wavelet", action="store_true",
+                    help="skip the downstream wavelet BayesShrink node leg")
     ap.add_argument("--cpu-sample", default="2048x1536")
     ap.add_argument("--fp64-crop", default="512x384")
     ap.add_argument("--pilot", default=None,
@@ -348,6 +404,11 @@ def main():
         host_path = host_path_bench(noisy, sigma, 3)
         print(f"# host path: {json.dumps(host_path)}", file=sys.stderr, flush=True)
 
+    wavelet = None
+    if not args.no_wavelet:
+        wavelet = wavelet_bench(torch, noisy, stream, 20, not args.no_cpu)
+        print(f"# wavelet node: {json.dumps(wavelet)}", file=sys.stderr, flush=True)
+
     batch = None
     if not args.no_batch and c == 3:
         print("# batch of 12 frames ...", file=sys.stderr, flush=True)
@@ -376,6 +437,7 @@ def main():
         "fp64_crop": fp64,
         "batch12": batch,
         "host_path": host_path,
+        "wavelet": wavelet,
     }
     if cpu:
         line["gpu_over_cpu"] = round(mpx / cpu["value"], 1)

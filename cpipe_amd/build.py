@@ -1,8 +1,8 @@
 """In-tree build of the native libraries (hipcc for gfx950, g++ for host-only code).
 
 Outputs (git-ignored, shipped with the tree):
-  cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h) + the node
-                                      descriptor in the `cpipe_registry` section
+  cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h, include/cpipe_wavelet.h)
+                                      + the nodes' descriptors in the `cpipe_registry` section
   cpipe_amd/lib/libcpipe_minihost.so  minimal cpipe host (suites, registry walk, one-node runner)
 """
 from __future__ import annotations
@@ -52,18 +52,24 @@ def _run(cmd, verbose):
     return r
 
 
+MANIFESTS = {"CPIPE_BM3D_MANIFEST_JSON": "bm3d.json",
+             "CPIPE_WAVELET_MANIFEST_JSON": "wavelet_bayes_shrink.json"}
+
+
 def _embed_manifest() -> str:
-    """Turn csrc/bm3d.json into a C++ string constant (like the reference's EmbedJson.cmake)."""
-    src = os.path.join(CSRC, "bm3d.json")
-    with open(src, encoding="utf-8") as f:
-        text = f.read()
-    json.loads(text)  # fail early on a malformed manifest
+    """Turn the nodes' csrc/*.json manifests into C++ string constants (like the
+    reference's EmbedJson.cmake)."""
     out = os.path.join(BUILD, "manifest_gen.cpp")
-    body = ", ".join(str(b) for b in text.encode("utf-8")) + ", 0"
+    lines = ["// generated from " + ", ".join(f"cpipe_amd/csrc/{f}" for f in MANIFESTS.values())]
+    for sym, name in MANIFESTS.items():
+        with open(os.path.join(CSRC, name), encoding="utf-8") as f:
+            text = f.read()
+        json.loads(text)  # fail early on a malformed manifest
+        body = ", ".join(str(b) for b in text.encode("utf-8")) + ", 0"
+        lines.append(f'extern "C" const char {sym}[];')
+        lines.append(f'extern "C" const char {sym}[] = {{{body}}};')
     with open(out, "w", encoding="utf-8") as f:
-        f.write("// generated from cpipe_amd/csrc/bm3d.json\n")
-        f.write('extern "C" const char CPIPE_BM3D_MANIFEST_JSON[];\n')
-        f.write(f'extern "C" const char CPIPE_BM3D_MANIFEST_JSON[] = {{{body}}};\n')
+        f.write("\n".join(lines) + "\n")
     return out
 
 
@@ -94,6 +100,7 @@ def build(verbose: bool = False) -> dict:
     pilot_o = os.path.join(BUILD, "pilot.o")
     mx_o = os.path.join(BUILD, "group_mx.o")
     bmx_o = os.path.join(BUILD, "bm_mx.o")
+    wav_o = os.path.join(BUILD, "wavelet.o")
     node_o = os.path.join(BUILD, "node.o")
     man_o = os.path.join(BUILD, "manifest_gen.o")
     blob_o = os.path.join(BUILD, "pilot_blob_gen.o")
@@ -102,11 +109,12 @@ def build(verbose: bool = False) -> dict:
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "pilot.hip"), "-o", pilot_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "group_mx.hip"), "-o", mx_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm_mx.hip"), "-o", bmx_o], verbose)
+    _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "wavelet.hip"), "-o", wav_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", blob_cpp, "-o", blob_o], verbose)
     tmp = ENGINE_SO + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, node_o,
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, wav_o, node_o,
           man_o, blob_o], verbose)
     os.replace(tmp, ENGINE_SO)
     tmp = MINIHOST_SO + ".tmp"

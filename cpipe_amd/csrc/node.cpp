@@ -1,4 +1,6 @@
-// com.cpipe.denoise.bm3d -- the cpipe node (plugin ABI 0.4) driving the MI355X engine.
+// com.cpipe.denoise.bm3d -- the cpipe node (plugin ABI 0.4) driving the MI355X engine,
+// and com.cpipe.denoise.wavelet_bayes_shrink, the next node of the reference's classic
+// pipeline (examples/pipelines/full-classic-pipeline.cpipe.json:127-129), further down.
 //
 // Replaces src/cpipe/nodes/denoise/bm3d.cpp (DenoiseBm3d) and its registration
 // CPIPE_REGISTER_NODE (bm3d.cpp:111, include/cpipe/sdk/registry.hpp:14-32).
@@ -26,8 +28,10 @@
 
 #include "cpipe_bm3d.h"
 #include "cpipe_node_abi.h"
+#include "cpipe_wavelet.h"
 
 extern "C" const char CPIPE_BM3D_MANIFEST_JSON[];
+extern "C" const char CPIPE_WAVELET_MANIFEST_JSON[];
 
 namespace {
 
@@ -263,6 +267,110 @@ __attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_
     CPIPE_ABI_MAJOR, CPIPE_ABI_MINOR, kNodeId, kNodeVersion, CPIPE_BM3D_MANIFEST_JSON,
     &node_main_entry};
 
+// ---------------------------------------------------------------------------
+// com.cpipe.denoise.wavelet_bayes_shrink: replaces DenoiseWaveletBayesShrink
+// (src/cpipe/nodes/denoise/wavelet_bayes_shrink.cpp:27-47) and its
+// CPIPE_REGISTER_NODE (:54-55). Behaviour kept: one input, one output and an
+// output metadata builder, else CPIPE_BAD_INDEX (:31-35); chroma_strength read
+// as a number, 1 when absent, clamped to [0, 2] (:37-38 with
+// src/cpipe/nodes/ParamUtils.hpp:13-29); add_applied_step
+// ("denoise.wavelet_bayes_shrink") on success (:46). The Halide submit becomes
+// cpipe_wavelet_host_rgba16f. Added: a format/size gate (RGBA16F in and out of
+// one size, else CPIPE_BAD_PRECISION / CPIPE_BAD_INDEX) and CPIPE_UNSUPPORTED
+// (logged) when no HIP device is visible.
+// ---------------------------------------------------------------------------
+
+const char kWaveletId[] = "com.cpipe.denoise.wavelet_bayes_shrink";
+const char kWaveletVersion[] = "2.0.0";
+
+struct WaveletNode {
+    int unused = 0;
+};
+
+float clamped_param_float_or(const Suites& s, const cpipe_props_t* params, const char* key,
+                             float fallback, float lo, float hi) {
+    double v = fallback;
+    if (s.param != nullptr && s.param->get_double != nullptr && params != nullptr) {
+        double d = 0.0;
+        if (s.param->get_double(params, key, &d) == CPIPE_OK) v = d;
+    }
+    return std::clamp(static_cast<float>(v), lo, hi);
+}
+
+int process_wavelet(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process_ctx* ctx) {
+    const Suites s = fetch_suites(host);
+    if (ctx == nullptr || ctx->n_in != 1 || ctx->n_out != 1 || ctx->inputs == nullptr ||
+        ctx->outputs == nullptr || ctx->inputs[0] == nullptr || ctx->outputs[0] == nullptr ||
+        ctx->out_metadata == nullptr || ctx->out_metadata[0] == nullptr) {
+        host_log(host, 4, "denoise.wavelet_bayes_shrink missing buffers");
+        return CPIPE_BAD_INDEX;
+    }
+    if (s.buffer == nullptr || s.buffer->lock_cpu == nullptr || s.buffer->unlock_cpu == nullptr)
+        return CPIPE_BAD_INDEX;
+    Image in, out;
+    int rc = describe_buffer(s, ctx->inputs[0], &in);
+    if (rc != CPIPE_OK) return rc;
+    rc = describe_buffer(s, ctx->outputs[0], &out);
+    if (rc != CPIPE_OK) return rc;
+    if (in.format != CPIPE_FORMAT_R16G16B16A16_SFLOAT || out.format != CPIPE_FORMAT_R16G16B16A16_SFLOAT)
+        return CPIPE_BAD_PRECISION;
+    if (in.w != out.w || in.h != out.h || in.w == 0 || in.h == 0) return CPIPE_BAD_INDEX;
+    const float strength = clamped_param_float_or(s, params, "chroma_strength", 1.0F, 0.0F, 2.0F);
+    if (cpipe_bm3d_device_count() <= 0) {
+        host_log(host, 3, "denoise.wavelet_bayes_shrink: no HIP device visible; the node runs on the GPU only");
+        return CPIPE_UNSUPPORTED;
+    }
+    void* ip = nullptr;
+    void* op = nullptr;
+    rc = s.buffer->lock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[0]), CPIPE_CPU_ACCESS_READ, &ip);
+    if (rc != CPIPE_OK) return rc;
+    rc = s.buffer->lock_cpu(ctx->outputs[0], CPIPE_CPU_ACCESS_WRITE, &op);
+    if (rc != CPIPE_OK) {
+        s.buffer->unlock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[0]));
+        return rc;
+    }
+    rc = cpipe_wavelet_host_rgba16f(static_cast<const uint16_t*>(ip), in.row_bytes / 8,
+                                    static_cast<uint16_t*>(op), out.row_bytes / 8, (int)in.w, (int)in.h,
+                                    strength);
+    if (rc != CPIPE_OK)
+        host_log(host, 3, std::string("denoise.wavelet_bayes_shrink: ") + cpipe_wavelet_last_error());
+    s.buffer->unlock_cpu(ctx->outputs[0]);
+    if (rc == CPIPE_OK && s.buffer->flush_cpu_writes != nullptr) s.buffer->flush_cpu_writes(ctx->outputs[0]);
+    s.buffer->unlock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[0]));
+    if (rc != CPIPE_OK) return rc;
+    if (s.builder == nullptr || s.builder->add_applied_step == nullptr) return CPIPE_BAD_INDEX;
+    return s.builder->add_applied_step(ctx->out_metadata[0], "denoise.wavelet_bayes_shrink");
+}
+
+int wavelet_main_entry(const char* action, cpipe_host_t* host, cpipe_node_t* node,
+                       cpipe_props_t* params, void* in_ctx, void* out_ctx) {
+    try {
+        if (action == nullptr) return CPIPE_REPLY_DEFAULT;
+        if (std::strcmp(action, CPIPE_ACTION_DESCRIBE) == 0) return CPIPE_REPLY_DEFAULT;
+        if (std::strcmp(action, CPIPE_ACTION_CREATE) == 0) {
+            if (out_ctx == nullptr) return CPIPE_BAD_INDEX;
+            *static_cast<void**>(out_ctx) = new WaveletNode();
+            return CPIPE_OK;
+        }
+        if (std::strcmp(action, CPIPE_ACTION_DESTROY) == 0) {
+            delete static_cast<WaveletNode*>(in_ctx);
+            return CPIPE_OK;
+        }
+        if (std::strcmp(action, CPIPE_ACTION_PROCESS) == 0) {
+            if (node == nullptr) return CPIPE_BAD_INDEX;
+            return process_wavelet(host, params, static_cast<const cpipe_process_ctx*>(in_ctx));
+        }
+    } catch (...) {
+        host_log(host, 4, "plugin dispatch threw an exception");
+        return CPIPE_INTERNAL_ERROR;
+    }
+    return CPIPE_REPLY_DEFAULT;
+}
+
+__attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_desc_t kWaveletDescriptor = {
+    CPIPE_ABI_MAJOR, CPIPE_ABI_MINOR, kWaveletId, kWaveletVersion, CPIPE_WAVELET_MANIFEST_JSON,
+    &wavelet_main_entry};
+
 }  // namespace
 
 // Section bounds, defined by the linker for the `cpipe_registry` section.
@@ -270,6 +378,7 @@ extern "C" const cpipe_plugin_desc_t __start_cpipe_registry[];
 extern "C" const cpipe_plugin_desc_t __stop_cpipe_registry[];
 
 extern "C" const cpipe_plugin_desc_t* cpipe_bm3d_node_descriptor(void) { return &kBm3dDescriptor; }
+extern "C" const cpipe_plugin_desc_t* cpipe_wavelet_node_descriptor(void) { return &kWaveletDescriptor; }
 
 // Descriptors of this library's `cpipe_registry` section, for hosts that dlopen
 // the node instead of linking it (the reference host walks the section of its
@@ -279,6 +388,8 @@ extern "C" size_t cpipe_registry_entries(const cpipe_plugin_desc_t** first) {
     return (size_t)(__stop_cpipe_registry - __start_cpipe_registry);
 }
 
-// Link anchor with the reference node's name (src/cpipe/nodes/denoise/bm3d.cpp:115-117),
-// so a host's builtin_link.cpp pulls this object in from the static archive.
+// Link anchors with the reference nodes' names (src/cpipe/nodes/denoise/bm3d.cpp:115-117,
+// wavelet_bayes_shrink.cpp:59-61), so a host's builtin_link.cpp pulls this object in
+// from the static archive.
 extern "C" void cpipe_link_builtin_denoise_bm3d(void) {}
+extern "C" void cpipe_link_builtin_denoise_wavelet_bayes_shrink(void) {}

@@ -36,6 +36,9 @@ EXPORTS = (
     "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d", "cpipe_bm3d_batch_create",
     "cpipe_bm3d_batch_destroy", "cpipe_bm3d_batch_run", "cpipe_bm3d_batch_tune",
     "cpipe_bm3d_batch_config", "cpipe_bm3d_plan_set_block_matching",
+    # include/cpipe_wavelet.h and the second node of the registry section
+    "cpipe_wavelet_rgba16f", "cpipe_wavelet_host_rgba16f", "cpipe_wavelet_last_error",
+    "cpipe_wavelet_node_descriptor", "cpipe_link_builtin_denoise_wavelet_bayes_shrink",
 )
 
 
@@ -113,6 +116,10 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_batch_config.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
                                           ctypes.POINTER(I)]
     L.cpipe_bm3d_plan_set_block_matching.argtypes = [P, I]
+    L.cpipe_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F, P]
+    L.cpipe_wavelet_host_rgba16f.argtypes = [P, S, P, S, I, I, F]
+    L.cpipe_wavelet_last_error.restype = ctypes.c_char_p
+    L.cpipe_wavelet_node_descriptor.restype = P
     _lib = L
     return L
 
@@ -342,6 +349,38 @@ def compat_rgba16f(src, dst, sigma: float, stage: int = 1, stream=None) -> None:
     _check(lib().cpipe_bm3d_compat_rgba16f(src.data_ptr(), src.stride(0) // 4, dst.data_ptr(),
                                            dst.stride(0) // 4, w, h, float(sigma), int(stage),
                                            _stream_handle(stream)))
+
+
+def _wavelet_check(rc: int) -> None:
+    if rc != 0:
+        raise EngineError(rc, lib().cpipe_wavelet_last_error().decode(errors="replace"))
+
+
+def wavelet_rgba16f(src, dst, chroma_strength: float = 1.0, stream=None) -> None:
+    """Wavelet BayesShrink chroma denoise (com.cpipe.denoise.wavelet_bayes_shrink) on
+    (H, W, 4) RGBA16F CUDA tensors (float16, or int16/uint16 holding binary16 bits)."""
+    h, w, c = src.shape
+    if c != 4 or tuple(dst.shape) != (h, w, 4):
+        raise ValueError("expected (H, W, 4) RGBA16F tensors")
+    for t in (src, dst):
+        if t.element_size() != 2 or t.stride(2) != 1 or t.stride(1) != 4 or not t.is_cuda:
+            raise ValueError("expected 2-byte CUDA tensors with interleaved pixels")
+    _wavelet_check(lib().cpipe_wavelet_rgba16f(src.data_ptr(), src.stride(0) // 4, dst.data_ptr(),
+                                               dst.stride(0) // 4, w, h, float(chroma_strength),
+                                               _stream_handle(stream)))
+
+
+def wavelet_host_rgba16f(src, chroma_strength: float = 1.0):
+    """The same on a host (H, W, 4) uint16 numpy array (binary16 bits); returns the output."""
+    import numpy as np
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    h, w, c = src.shape
+    if c != 4:
+        raise ValueError("expected (H, W, 4) RGBA16F bits")
+    out = np.empty_like(src)
+    _wavelet_check(lib().cpipe_wavelet_host_rgba16f(src.ctypes.data, w, out.ctypes.data, w, w, h,
+                                                    float(chroma_strength)))
+    return out
 
 
 def denoise(planes, sigma: float, precision="fp32", stream=None):

@@ -234,6 +234,12 @@ size_t cpipe_registry_entries(const cpipe_plugin_desc_t** first);
  * src/cpipe/nodes/denoise/bm3d.cpp:115-117 (called by builtin_link.cpp:32-60). */
 void cpipe_link_builtin_denoise_bm3d(void);
 
+/* The wavelet BayesShrink node's descriptor, in the same section (replaces
+ * CPIPE_REGISTER_NODE, src/cpipe/nodes/denoise/wavelet_bayes_shrink.cpp:54-55),
+ * and its link anchor (wavelet_bayes_shrink.cpp:59-61). */
+const cpipe_plugin_desc_t* cpipe_wavelet_node_descriptor(void);
+void cpipe_link_builtin_denoise_wavelet_bayes_shrink(void);
+
 #if defined(__cplusplus) && defined(__x86_64__)
 static_assert(sizeof(cpipe_plugin_desc_t) == 40, "cpipe_plugin_desc_t layout");
 static_assert(offsetof(cpipe_plugin_desc_t, node_id) == 8, "node_id offset");

@@ -5,7 +5,7 @@
  * load this library, and only as the checker (or the timed CPU baseline).
  * The product path (cpipe_amd/) never links or calls it.
  *
- * Three restatements live here:
+ * Restatements (the BM3D node's three, the wavelet node's one):
  *   1. oracle_compat_rgba16f  -- the reference node's actual arithmetic:
  *      src/cpipe/nodes/denoise/bm3d_step1_generator.cpp:9-16,42-56 (step 1,
  *      2x2 box-mean blend) and bm3d_step2_generator.cpp:33-42 (identity).
@@ -18,6 +18,9 @@
  *      operation order of the GPU FP32 path (fma chains, tie rules,
  *      aggregation tiles and merge order), multithreaded. It is both the
  *      parity partner of the GPU path and the CPU baseline.
+ * and, for the downstream node com.cpipe.denoise.wavelet_bayes_shrink,
+ *   4. oracle_wavelet_rgba16f -- its Halide generator's arithmetic
+ *      (wavelet_oracle.c).
  *
  * Parity pins (see tests/test_oracle_*.py): compat known answers from the
  * reference unit tests; DCT/Haar orthonormality; Kaiser from the I0 series;
@@ -82,6 +85,12 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
 /* Full BM3D, FP64 textbook. Same I/O conventions (double planes). */
 int oracle_bm3d_f64(const double* const* in, double* const* out, double* const* basic,
                     int width, int height, int channels, double sigma);
+
+/* Wavelet BayesShrink chroma denoise of interleaved RGBA16F (wavelet_oracle.c):
+ * the reference Halide generator denoise_wavelet_bayes_shrink
+ * (src/cpipe/nodes/denoise/wavelet_bayes_shrink_generator.cpp:57-86). Strides in pixels. */
+void oracle_wavelet_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
+                            size_t out_stride_px, int width, int height, float chroma_strength);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
         L.oracle_f16_to_f32.restype = F
         L.oracle_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I]
+        L.oracle_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F]
         L.oracle_compat_rgba16f.restype = None
         L.oracle_ref_count.argtypes = [I]
         L.oracle_ref_count.restype = I
@@ -102,6 +103,16 @@ def compat_rgba16f(rgba_bits: np.ndarray, sigma: float, stage: int = 1) -> np.nd
     assert c == 4
     out = np.empty_like(src)
     lib().oracle_compat_rgba16f(_ptr(src), w, _ptr(out), w, w, h, float(sigma), int(stage))
+    return out
+
+
+def wavelet_rgba16f(rgba_bits: np.ndarray, chroma_strength: float = 1.0) -> np.ndarray:
+    """Restated wavelet BayesShrink chroma denoise on (H, W, 4) uint16 (binary16 bits)."""
+    src = np.ascontiguousarray(rgba_bits, dtype=np.uint16)
+    h, w, c = src.shape
+    assert c == 4
+    out = np.empty_like(src)
+    lib().oracle_wavelet_rgba16f(_ptr(src), w, _ptr(out), w, w, h, float(chroma_strength))
     return out
 
 

@@ -26,8 +26,10 @@ def _declared_functions(header):
 
 def test_library_exports_every_declared_symbol():
     lib = engine.lib()
-    declared = _declared_functions("cpipe_bm3d.h") + _declared_functions("cpipe_node_abi.h")
+    declared = (_declared_functions("cpipe_bm3d.h") + _declared_functions("cpipe_node_abi.h") +
+                _declared_functions("cpipe_wavelet.h"))
     assert "cpipe_bm3d_plan_run" in declared and "cpipe_bm3d_node_descriptor" in declared
+    assert "cpipe_wavelet_rgba16f" in declared and "cpipe_wavelet_node_descriptor" in declared
     for name in declared:
         assert hasattr(lib, name), f"missing export {name}"
     for name in engine.EXPORTS:
@@ -36,21 +38,69 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_header_layout_static_asserts(tmp_path):
     src = tmp_path / "abi.cpp"
-    src.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\nint main(){return 0;}\n')
+    src.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\n#include "cpipe_wavelet.h"\n'
+                   'int main(){return 0;}\n')
     subprocess.run(["g++", "-std=c++17", f"-I{INCLUDE}", "-fsyntax-only", str(src)], check=True)
     csrc = tmp_path / "abi.c"
-    csrc.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\nint main(void){return 0;}\n')
+    csrc.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\n#include "cpipe_wavelet.h"\n'
+                    'int main(void){return 0;}\n')
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{INCLUDE}", "-fsyntax-only", str(csrc)],
                    check=True)
 
 
-def test_registry_section_holds_the_node():
-    assert host.mh().mh_registry_count(engine.ENGINE_SO.encode()) == 1
+def test_registry_section_holds_the_nodes():
+    assert host.mh().mh_registry_count(engine.ENGINE_SO.encode()) == 2
     d = host.find_node()
     assert d.contents.abi_major == 0 and d.contents.abi_minor == 4
     assert d.contents.node_id == b"com.cpipe.denoise.bm3d"
     direct = engine.lib().cpipe_bm3d_node_descriptor()
     assert direct == ctypes.addressof(d.contents)
+    wd = host.find_node(WAVELET_ID)
+    assert wd.contents.abi_major == 0 and wd.contents.abi_minor == 4
+    assert engine.lib().cpipe_wavelet_node_descriptor() == ctypes.addressof(wd.contents)
+
+
+WAVELET_ID = "com.cpipe.denoise.wavelet_bayes_shrink"
+
+
+def test_wavelet_manifest_contract():
+    """Assertions of test_node_denoise_wavelet_bayes_shrink.cpp:46-52 (ports, applied
+    step) and the manifest's params (wavelet_bayes_shrink.json:37-46), with the
+    engine changed to HIP."""
+    m = json.loads(host.find_node(WAVELET_ID).contents.manifest_json.decode())
+    assert m["id"] == WAVELET_ID
+    assert m["ports"][1]["metadata"]["sets_steps_applied"] == ["denoise.wavelet_bayes_shrink"]
+    assert m["ports"][0]["metadata"]["requires_steps_applied"] == ["color_matrix"]
+    assert m["ports"][0]["caps"]["precision"] == ["f16"] and m["ports"][0]["caps"]["channels"] == ["rgba"]
+    p = m["params"][0]
+    assert p["name"] == "chroma_strength" and p["range"] == {"min": 0, "max": 2} and p["default"] == 1
+    assert m["compute"]["device"] == "GPU" and m["compute"]["engine"] == "HIP"
+
+
+def test_wavelet_status_contract():
+    """Arity -> BAD_INDEX (wavelet_bayes_shrink.cpp:31-35), format gate -> BAD_PRECISION,
+    and without a device CPIPE_UNSUPPORTED, never a CPU fallback."""
+    d = host.find_node(WAVELET_ID)
+    a, b = _rgba_buffers()
+    c = host.Buffer(host.FORMAT_RGBA16F, 4, 4)
+    assert host.run_node(d, [a, c], [b]) == host.CPIPE_BAD_INDEX
+    f1 = host.Buffer(host.FORMAT_R32F, 4, 4)
+    assert host.run_node(d, [f1], [b]) == host.CPIPE_BAD_PRECISION
+    big = host.Buffer(host.FORMAT_RGBA16F, 8, 4)
+    assert host.run_node(d, [a], [big]) == host.CPIPE_BAD_INDEX
+    assert host.mh().mh_call_action(d, b"describe") == host.CPIPE_REPLY_DEFAULT
+    L = engine.lib()
+    assert L.cpipe_wavelet_rgba16f(None, 4, None, 4, 4, 4, ctypes.c_float(1.0), None) == host.CPIPE_BAD_INDEX
+    assert b"null" in L.cpipe_wavelet_last_error()
+    import torch
+    if torch.cuda.is_available():
+        return
+    assert host.run_node(d, [a], [b]) == host.CPIPE_UNSUPPORTED
+    assert "no HIP device" in host.host_log()
+    import numpy as np
+    with pytest.raises(engine.EngineError) as e:
+        engine.wavelet_host_rgba16f(np.zeros((4, 4, 4), np.uint16))
+    assert e.value.status == host.CPIPE_UNSUPPORTED
 
 
 def test_manifest_contract():

@@ -1,0 +1,128 @@
+"""Pins for the wavelet BayesShrink oracle (runs without a GPU).
+
+The oracle restates the reference Halide generator
+src/cpipe/nodes/denoise/wavelet_bayes_shrink_generator.cpp:9-31,57-86. Each
+check is one that a plausible slip (a wrong band sign, hard instead of soft
+thresholding, a wrong threshold constant, a different edge rule) fails: the
+reference's own unit test, algebraic identities of the YCoCg-style transform
+and the Haar pair, and the repeat-edge rule.
+"""
+import numpy as np
+
+F16_EPS = 2.0 ** -10  # binary16 relative spacing
+
+
+def _f16(x):
+    return np.asarray(x, np.float32).astype(np.float16).view(np.uint16)
+
+
+def _from_f16(b):
+    return np.asarray(b, np.uint16).view(np.float16).astype(np.float64)
+
+
+def _luma(p):
+    return (p[..., 0] + 2.0 * p[..., 1] + p[..., 2]) * 0.25
+
+
+def _random_rgba(rng, h, w, lo=0.05, hi=0.9):
+    img = rng.uniform(lo, hi, size=(h, w, 4)).astype(np.float32)
+    img[..., 3] = rng.uniform(0.0, 1.0, size=(h, w))
+    return _f16(img)
+
+
+def test_reference_unit_test(oracle_lib):
+    """tests/unit/test_node_denoise_wavelet_bayes_shrink.cpp:38-87: a 2x2 image
+    loses chroma span, keeps luma within 0.002 and alpha within 0.001."""
+    px = np.array([[[0.38, 0.30, 0.22, 1.0], [0.22, 0.30, 0.38, 0.8]],
+                   [[0.37, 0.30, 0.23, 0.6], [0.23, 0.30, 0.37, 0.4]]], np.float32)
+    inp = _f16(px)
+    out = oracle_lib.wavelet_rgba16f(inp, 1.0)
+    a, b = _from_f16(inp), _from_f16(out)
+    span = lambda p: float(np.ptp(p[..., 0] - p[..., 2]))
+    assert span(b) < span(a)
+    assert np.all(np.abs(_luma(b) - _luma(a)) <= 0.002)
+    assert np.all(np.abs(b[..., 3] - a[..., 3]) <= 0.001)
+
+
+def test_constant_image_is_reproduced_bit_for_bit(oracle_lib):
+    """No detail bands: the transform pair returns every input value (its FP32
+    rounding error is far below half a binary16 step)."""
+    for rgb in ([0.2, 0.5, 0.7], [0.9, 0.1, 0.4], [0.0, 0.25, 1.0]):
+        img = np.empty((6, 10, 4), np.float32)
+        img[..., :3] = rgb
+        img[..., 3] = 0.5
+        inp = _f16(img)
+        for strength in (0.0, 1.0, 2.0):
+            assert np.array_equal(oracle_lib.wavelet_rgba16f(inp, strength), inp)
+
+
+def test_strength_zero_is_the_identity(oracle_lib):
+    """Threshold 0 keeps every band: analysis + synthesis is the identity."""
+    rng = np.random.default_rng(3)
+    inp = _random_rgba(rng, 17, 23)
+    out = oracle_lib.wavelet_rgba16f(inp, 0.0)
+    a, b = _from_f16(inp), _from_f16(out)
+    assert np.all(np.abs(b - a) <= np.maximum(np.abs(a) * F16_EPS, 2.0 ** -24))
+
+
+def test_luma_and_alpha_are_preserved(oracle_lib):
+    """Only chroma is shrunk: r + 2g + b is invariant algebraically
+    ((t + c/2) + 2 (t + g') + (t - c/2) = 4 t + 2 g'), alpha passes through."""
+    rng = np.random.default_rng(5)
+    inp = _random_rgba(rng, 32, 48, lo=0.3, hi=0.7)  # far from the max(0, .) clamp
+    out = oracle_lib.wavelet_rgba16f(inp, 2.0)
+    a, b = _from_f16(inp), _from_f16(out)
+    assert np.all(np.abs(_luma(b) - _luma(a)) <= 2.0 * F16_EPS)
+    assert np.array_equal(out[..., 3], inp[..., 3])
+
+
+def _block_bands(co):
+    """Haar bands of each 2x2 block of a plane (FP64)."""
+    p00, p10 = co[0::2, 0::2], co[0::2, 1::2]
+    p01, p11 = co[1::2, 0::2], co[1::2, 1::2]
+    return (0.25 * (p00 + p10 + p01 + p11), 0.25 * (p00 - p10 + p01 - p11),
+            0.25 * (p00 + p10 - p01 - p11), 0.25 * (p00 - p10 - p01 + p11))
+
+
+def test_soft_threshold_constant_and_signs(oracle_lib):
+    """A horizontal co detail of amplitude A in every block comes out as A - thr
+    with thr = 0.035 * strength (soft, not hard, thresholding); detail below thr
+    vanishes. Pins the threshold constant and the h-band sign convention."""
+    h, w = 8, 8
+    for strength, amp in ((1.0, 0.06), (2.0, 0.1), (1.0, 0.02)):
+        img = np.full((h, w, 4), 0.5, np.float32)
+        sign = np.where(np.arange(w) % 2 == 0, 1.0, -1.0)[None, :]  # + at even x: h band
+        img[..., 0] = 0.5 + 0.5 * amp * sign  # co = r - b = amp * sign
+        img[..., 2] = 0.5 - 0.5 * amp * sign
+        inp = _f16(img)
+        out = _from_f16(oracle_lib.wavelet_rgba16f(inp, strength))
+        a = _from_f16(inp)
+        _, hb_in, vb_in, db_in = _block_bands(a[..., 0] - a[..., 2])
+        ll, hb, vb, db = _block_bands(out[..., 0] - out[..., 2])
+        thr = 0.035 * strength
+        want = np.sign(hb_in) * np.maximum(np.abs(hb_in) - thr, 0.0)
+        assert np.all(np.abs(hb - want) <= 4e-3), (strength, amp)
+        assert np.all(np.abs(vb) <= 2e-3) and np.all(np.abs(db) <= 2e-3)
+
+
+def test_repeat_edge_rule(oracle_lib):
+    """Odd sizes read past the edge as the last row / column (Halide
+    repeat_edge): the output equals the first W x H of the output of the image
+    padded by replicating its last column and row."""
+    rng = np.random.default_rng(7)
+    for h, w in ((5, 7), (9, 4), (1, 1), (3, 2)):
+        inp = _random_rgba(rng, h, w)
+        pad = np.concatenate([inp, inp[:, -1:]], axis=1) if w % 2 else inp
+        pad = np.concatenate([pad, pad[-1:]], axis=0) if h % 2 else pad
+        full = oracle_lib.wavelet_rgba16f(pad, 1.0)
+        assert np.array_equal(oracle_lib.wavelet_rgba16f(inp, 1.0), full[:h, :w]), (h, w)
+
+
+def test_negative_results_clamp_to_positive_zero(oracle_lib):
+    """max(0, .) on every channel; the result is +0, never -0."""
+    img = np.zeros((2, 2, 4), np.float32)
+    img[0, 0, :3] = [0.0, 0.0, 0.3]  # strong co detail pushes r and b below zero
+    img[1, 1, :3] = [0.3, 0.0, 0.0]
+    out = oracle_lib.wavelet_rgba16f(_f16(img), 0.0)
+    assert np.all(out != 0x8000)
+    assert np.all(_from_f16(out) >= 0.0)
