@@ -251,6 +251,40 @@ def wavelet_bench(torch, noisy, stream, runs, want_cpu):
     return res
 
 
+def convert_bench(torch, noisy, stream, runs):
+    """The precision-convert node's kernels on the same frame (resident in HBM), one
+    launch per run timed with HIP events: R16G16B16A16_SFLOAT -> R8G8B8A8_UNORM
+    (12 bytes per pixel) and R32_SFLOAT -> R16G16B16A16_SFLOAT (12 bytes per pixel),
+    both HBM-bound."""
+    c, h, w = noisy.shape
+    rgba = np.ones((h, w, 4), np.float32)
+    rgba[..., :c] = np.moveaxis(noisy, 0, -1)
+    f16 = torch.from_numpy(rgba.astype(np.float16).view(np.int16)).cuda()
+    grey = torch.from_numpy(np.ascontiguousarray(noisy[0])).cuda()
+    cases = {"rgba16f_to_rgba8": (f16, "R16G16B16A16_SFLOAT", torch.empty((h, w, 4), dtype=torch.uint8, device="cuda"),
+                                  "R8G8B8A8_UNORM"),
+             "f32_to_rgba16f": (grey, "R32_SFLOAT", torch.empty((h, w, 4), dtype=torch.int16, device="cuda"),
+                                "R16G16B16A16_SFLOAT")}
+    res = {"node": "com.cpipe.precision_convert", "frame": f"{w}x{h}"}
+    for name, (src, sf, dst, df) in cases.items():
+        for _ in range(3):
+            engine.convert(src, sf, dst, df, stream)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for _ in range(runs):
+            engine.convert(src, sf, dst, df, stream)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / runs
+        gbs = 12.0 * w * h / (ms / 1e3) / 1e9
+        res[name] = {"value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpixel/s", "ms": round(ms, 4),
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                                  "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                                  "bytes_per_launch": 12 * w * h}}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,7 +300,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) leg")
     ap.add_argument("--no-wavelet", action="store_true",
-                    help="skip the downstream wavelet BayesShrink node leg")
+                    help="skip the downstream wavelet BayesShrink and precision-convert node legs")
     ap.add_argument("--cpu-sample", default="2048x1536")
     ap.add_argument("--fp64-crop", default="512x384")
     ap.add_argument("--pilot", default=None,
@@ -404,10 +438,12 @@ def main():
         host_path = host_path_bench(noisy, sigma, 3)
         print(f"# host path: {json.dumps(host_path)}", file=sys.stderr, flush=True)
 
-    wavelet = None
+    wavelet = convert = None
     if not args.no_wavelet:
         wavelet = wavelet_bench(torch, noisy, stream, 20, not args.no_cpu)
         print(f"# wavelet node: {json.dumps(wavelet)}", file=sys.stderr, flush=True)
+        convert = convert_bench(torch, noisy, stream, 20)
+        print(f"# precision_convert node: {json.dumps(convert)}", file=sys.stderr, flush=True)
 
     batch = None
     if not args.no_batch and c == 3:
@@ -438,6 +474,7 @@ def main():
         "batch12": batch,
         "host_path": host_path,
         "wavelet": wavelet,
+        "precision_convert": convert,
     }
     if cpu:
         line["gpu_over_cpu"] = round(mpx / cpu["value"], 1)

@@ -1,7 +1,8 @@
 """In-tree build of the native libraries (hipcc for gfx950, g++ for host-only code).
 
 Outputs (git-ignored, shipped with the tree):
-  cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h, include/cpipe_wavelet.h)
+  cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h, include/cpipe_wavelet.h,
+                                      include/cpipe_convert.h)
                                       + the nodes' descriptors in the `cpipe_registry` section
   cpipe_amd/lib/libcpipe_minihost.so  minimal cpipe host (suites, registry walk, one-node runner)
 """
@@ -53,7 +54,8 @@ def _run(cmd, verbose):
 
 
 MANIFESTS = {"CPIPE_BM3D_MANIFEST_JSON": "bm3d.json",
-             "CPIPE_WAVELET_MANIFEST_JSON": "wavelet_bayes_shrink.json"}
+             "CPIPE_WAVELET_MANIFEST_JSON": "wavelet_bayes_shrink.json",
+             "CPIPE_CONVERT_MANIFEST_JSON": "precision_convert.json"}
 
 
 def _embed_manifest() -> str:
@@ -101,6 +103,7 @@ def build(verbose: bool = False) -> dict:
     mx_o = os.path.join(BUILD, "group_mx.o")
     bmx_o = os.path.join(BUILD, "bm_mx.o")
     wav_o = os.path.join(BUILD, "wavelet.o")
+    cvt_o = os.path.join(BUILD, "convert.o")
     node_o = os.path.join(BUILD, "node.o")
     man_o = os.path.join(BUILD, "manifest_gen.o")
     blob_o = os.path.join(BUILD, "pilot_blob_gen.o")
@@ -110,11 +113,12 @@ def build(verbose: bool = False) -> dict:
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "group_mx.hip"), "-o", mx_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm_mx.hip"), "-o", bmx_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "wavelet.hip"), "-o", wav_o], verbose)
+    _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "convert.hip"), "-o", cvt_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", blob_cpp, "-o", blob_o], verbose)
     tmp = ENGINE_SO + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, wav_o, node_o,
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, wav_o, cvt_o, node_o,
           man_o, blob_o], verbose)
     os.replace(tmp, ENGINE_SO)
     tmp = MINIHOST_SO + ".tmp"

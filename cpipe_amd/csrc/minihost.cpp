@@ -58,6 +58,9 @@ int bpp_of(int format) {
         case CPIPE_FORMAT_R16G16B16A16_SFLOAT: return 8;
         case CPIPE_FORMAT_R32_SFLOAT: return 4;
         case 8: return 16;  // R32G32B32A32_SFLOAT
+        case 1: return 2;   // R16_UINT
+        case 3: return 4;   // R8G8B8A8_UNORM
+        case 16: return 8;  // R16G16B16A16_UNORM
         default: return 0;
     }
 }

@@ -1,6 +1,7 @@
 // com.cpipe.denoise.bm3d -- the cpipe node (plugin ABI 0.4) driving the MI355X engine,
-// and com.cpipe.denoise.wavelet_bayes_shrink, the next node of the reference's classic
-// pipeline (examples/pipelines/full-classic-pipeline.cpipe.json:127-129), further down.
+// and, further down, com.cpipe.denoise.wavelet_bayes_shrink, the next node of the
+// reference's classic pipeline (examples/pipelines/full-classic-pipeline.cpipe.json:127-129),
+// and com.cpipe.precision_convert, the conversion node its precision planner inserts.
 //
 // Replaces src/cpipe/nodes/denoise/bm3d.cpp (DenoiseBm3d) and its registration
 // CPIPE_REGISTER_NODE (bm3d.cpp:111, include/cpipe/sdk/registry.hpp:14-32).
@@ -27,11 +28,13 @@
 #include <vector>
 
 #include "cpipe_bm3d.h"
+#include "cpipe_convert.h"
 #include "cpipe_node_abi.h"
 #include "cpipe_wavelet.h"
 
 extern "C" const char CPIPE_BM3D_MANIFEST_JSON[];
 extern "C" const char CPIPE_WAVELET_MANIFEST_JSON[];
+extern "C" const char CPIPE_CONVERT_MANIFEST_JSON[];
 
 namespace {
 
@@ -371,6 +374,152 @@ __attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_
     CPIPE_ABI_MAJOR, CPIPE_ABI_MINOR, kWaveletId, kWaveletVersion, CPIPE_WAVELET_MANIFEST_JSON,
     &wavelet_main_entry};
 
+// ---------------------------------------------------------------------------
+// com.cpipe.precision_convert: replaces PrecisionConvert
+// (src/cpipe/nodes/precision/precision_convert.cpp:40-80) and its
+// CPIPE_REGISTER_NODE (:86). Behaviour kept: one input, one output and an output
+// metadata builder, else CPIPE_BAD_INDEX (:49-52); the string param
+// target_format, its status when absent (:54-57, ParamView::string in
+// include/cpipe/sdk/sdk.hpp:557-569); the output format must be the target
+// (:30-36,63-66) and the pair one of the four conversions (:13-28,68-72), else
+// CPIPE_BAD_PRECISION; add_applied_step("precision_convert") (:78). The Halide
+// submit becomes cpipe_convert_host. Added: CPIPE_UNSUPPORTED (logged) when no
+// HIP device is visible, CPIPE_BAD_INDEX on a size mismatch.
+// ---------------------------------------------------------------------------
+
+const char kConvertId[] = "com.cpipe.precision_convert";
+const char kConvertVersion[] = "2.0.0";
+
+struct ConvertNode {
+    int unused = 0;
+};
+
+int format_of_target(const std::string& t) {
+    if (t == "R32_SFLOAT") return CPIPE_CONVERT_R32_SFLOAT;
+    if (t == "R16G16B16A16_SFLOAT") return CPIPE_CONVERT_R16G16B16A16_SFLOAT;
+    if (t == "R8G8B8A8_UNORM") return CPIPE_CONVERT_R8G8B8A8_UNORM;
+    if (t == "R16G16B16A16_UNORM") return CPIPE_CONVERT_R16G16B16A16_UNORM;
+    return -1;
+}
+
+size_t pixel_bytes(int format) {
+    switch (format) {
+        case CPIPE_CONVERT_R16_UINT: return 2;
+        case CPIPE_CONVERT_R8G8B8A8_UNORM: return 4;
+        case CPIPE_CONVERT_R32_SFLOAT: return 4;
+        case CPIPE_CONVERT_R16G16B16A16_SFLOAT:
+        case CPIPE_CONVERT_R16G16B16A16_UNORM: return 8;
+        default: return 0;
+    }
+}
+
+// Format, size and row bytes of a buffer of any of the node's formats.
+int describe_any(const Suites& s, const cpipe_buffer_t* b, Image* out) {
+    if (s.buffer == nullptr || s.buffer->get_format == nullptr || s.buffer->get_dims == nullptr)
+        return CPIPE_BAD_INDEX;
+    int rc = s.buffer->get_format(b, &out->format);
+    if (rc != CPIPE_OK) return rc;
+    uint8_t ndim = 0;
+    uint32_t dims[8] = {};
+    rc = s.buffer->get_dims(b, &ndim, dims);
+    if (rc != CPIPE_OK) return rc;
+    if (ndim < 2) return CPIPE_BAD_INDEX;
+    out->w = dims[0];
+    out->h = dims[1];
+    const size_t bpp = pixel_bytes(out->format);
+    out->row_bytes = (size_t)out->w * (bpp ? bpp : 1);
+    if (bpp != 0 && s.buffer->get_stride != nullptr) {
+        uint64_t st[8] = {};
+        if (s.buffer->get_stride(b, st) == CPIPE_OK) {
+            if (st[0] != 0 && st[0] != bpp) return CPIPE_UNSUPPORTED;
+            if (st[1] != 0) out->row_bytes = st[1];
+        }
+    }
+    return CPIPE_OK;
+}
+
+int process_convert(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process_ctx* ctx) {
+    const Suites s = fetch_suites(host);
+    if (ctx == nullptr || ctx->n_in != 1 || ctx->n_out != 1 || ctx->inputs == nullptr ||
+        ctx->outputs == nullptr || ctx->inputs[0] == nullptr || ctx->outputs[0] == nullptr ||
+        ctx->out_metadata == nullptr || ctx->out_metadata[0] == nullptr) {
+        host_log(host, 4, "precision_convert missing buffers");
+        return CPIPE_BAD_INDEX;
+    }
+    if (s.param == nullptr || s.param->get_enum == nullptr || params == nullptr) return CPIPE_NEED_PARAM;
+    const char* target = nullptr;
+    int rc = s.param->get_enum(params, "target_format", &target);
+    if (rc != CPIPE_OK) return rc;
+    if (target == nullptr) return CPIPE_NEED_PARAM;
+    if (s.buffer == nullptr || s.buffer->lock_cpu == nullptr || s.buffer->unlock_cpu == nullptr)
+        return CPIPE_BAD_INDEX;
+    Image in, out;
+    rc = describe_any(s, ctx->inputs[0], &in);
+    if (rc != CPIPE_OK) return rc;
+    rc = describe_any(s, ctx->outputs[0], &out);
+    if (rc != CPIPE_OK) return rc;
+    const int tf = format_of_target(target);
+    if (tf < 0 || out.format != tf) {
+        host_log(host, 4, "precision_convert target/output mismatch");
+        return CPIPE_BAD_PRECISION;
+    }
+    if (!cpipe_convert_supported(in.format, tf)) {
+        host_log(host, 4, "precision_convert unsupported conversion");
+        return CPIPE_BAD_PRECISION;
+    }
+    if (in.w != out.w || in.h != out.h || in.w == 0 || in.h == 0) return CPIPE_BAD_INDEX;
+    if (cpipe_bm3d_device_count() <= 0) {
+        host_log(host, 3, "precision_convert: no HIP device visible; the node runs on the GPU only");
+        return CPIPE_UNSUPPORTED;
+    }
+    void* ip = nullptr;
+    void* op = nullptr;
+    rc = s.buffer->lock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[0]), CPIPE_CPU_ACCESS_READ, &ip);
+    if (rc != CPIPE_OK) return rc;
+    rc = s.buffer->lock_cpu(ctx->outputs[0], CPIPE_CPU_ACCESS_WRITE, &op);
+    if (rc != CPIPE_OK) {
+        s.buffer->unlock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[0]));
+        return rc;
+    }
+    rc = cpipe_convert_host(in.format, ip, in.row_bytes, tf, op, out.row_bytes, (int)in.w, (int)in.h);
+    if (rc != CPIPE_OK) host_log(host, 3, std::string("precision_convert: ") + cpipe_convert_last_error());
+    s.buffer->unlock_cpu(ctx->outputs[0]);
+    if (rc == CPIPE_OK && s.buffer->flush_cpu_writes != nullptr) s.buffer->flush_cpu_writes(ctx->outputs[0]);
+    s.buffer->unlock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[0]));
+    if (rc != CPIPE_OK) return rc;
+    if (s.builder == nullptr || s.builder->add_applied_step == nullptr) return CPIPE_BAD_INDEX;
+    return s.builder->add_applied_step(ctx->out_metadata[0], "precision_convert");
+}
+
+int convert_main_entry(const char* action, cpipe_host_t* host, cpipe_node_t* node,
+                       cpipe_props_t* params, void* in_ctx, void* out_ctx) {
+    try {
+        if (action == nullptr) return CPIPE_REPLY_DEFAULT;
+        if (std::strcmp(action, CPIPE_ACTION_DESCRIBE) == 0) return CPIPE_REPLY_DEFAULT;
+        if (std::strcmp(action, CPIPE_ACTION_CREATE) == 0) {
+            if (out_ctx == nullptr) return CPIPE_BAD_INDEX;
+            *static_cast<void**>(out_ctx) = new ConvertNode();
+            return CPIPE_OK;
+        }
+        if (std::strcmp(action, CPIPE_ACTION_DESTROY) == 0) {
+            delete static_cast<ConvertNode*>(in_ctx);
+            return CPIPE_OK;
+        }
+        if (std::strcmp(action, CPIPE_ACTION_PROCESS) == 0) {
+            if (node == nullptr) return CPIPE_BAD_INDEX;
+            return process_convert(host, params, static_cast<const cpipe_process_ctx*>(in_ctx));
+        }
+    } catch (...) {
+        host_log(host, 4, "plugin dispatch threw an exception");
+        return CPIPE_INTERNAL_ERROR;
+    }
+    return CPIPE_REPLY_DEFAULT;
+}
+
+__attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_desc_t kConvertDescriptor = {
+    CPIPE_ABI_MAJOR, CPIPE_ABI_MINOR, kConvertId, kConvertVersion, CPIPE_CONVERT_MANIFEST_JSON,
+    &convert_main_entry};
+
 }  // namespace
 
 // Section bounds, defined by the linker for the `cpipe_registry` section.
@@ -379,6 +528,7 @@ extern "C" const cpipe_plugin_desc_t __stop_cpipe_registry[];
 
 extern "C" const cpipe_plugin_desc_t* cpipe_bm3d_node_descriptor(void) { return &kBm3dDescriptor; }
 extern "C" const cpipe_plugin_desc_t* cpipe_wavelet_node_descriptor(void) { return &kWaveletDescriptor; }
+extern "C" const cpipe_plugin_desc_t* cpipe_convert_node_descriptor(void) { return &kConvertDescriptor; }
 
 // Descriptors of this library's `cpipe_registry` section, for hosts that dlopen
 // the node instead of linking it (the reference host walks the section of its
@@ -389,7 +539,8 @@ extern "C" size_t cpipe_registry_entries(const cpipe_plugin_desc_t** first) {
 }
 
 // Link anchors with the reference nodes' names (src/cpipe/nodes/denoise/bm3d.cpp:115-117,
-// wavelet_bayes_shrink.cpp:59-61), so a host's builtin_link.cpp pulls this object in
-// from the static archive.
+// wavelet_bayes_shrink.cpp:59-61, src/cpipe/nodes/precision/precision_convert.cpp:90-92),
+// so a host's builtin_link.cpp pulls this object in from the static archive.
 extern "C" void cpipe_link_builtin_denoise_bm3d(void) {}
 extern "C" void cpipe_link_builtin_denoise_wavelet_bayes_shrink(void) {}
+extern "C" void cpipe_link_builtin_precision_convert(void) {}

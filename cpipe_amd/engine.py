@@ -39,6 +39,9 @@ EXPORTS = (
     # include/cpipe_wavelet.h and the second node of the registry section
     "cpipe_wavelet_rgba16f", "cpipe_wavelet_host_rgba16f", "cpipe_wavelet_last_error",
     "cpipe_wavelet_node_descriptor", "cpipe_link_builtin_denoise_wavelet_bayes_shrink",
+    # include/cpipe_convert.h and the third node
+    "cpipe_convert", "cpipe_convert_host", "cpipe_convert_supported", "cpipe_convert_last_error",
+    "cpipe_convert_node_descriptor", "cpipe_link_builtin_precision_convert",
 )
 
 
@@ -120,6 +123,11 @@ def lib() -> ctypes.CDLL:
     L.cpipe_wavelet_host_rgba16f.argtypes = [P, S, P, S, I, I, F]
     L.cpipe_wavelet_last_error.restype = ctypes.c_char_p
     L.cpipe_wavelet_node_descriptor.restype = P
+    L.cpipe_convert.argtypes = [I, P, S, I, P, S, I, I, P]
+    L.cpipe_convert_host.argtypes = [I, P, S, I, P, S, I, I]
+    L.cpipe_convert_supported.argtypes = [I, I]
+    L.cpipe_convert_last_error.restype = ctypes.c_char_p
+    L.cpipe_convert_node_descriptor.restype = P
     _lib = L
     return L
 
@@ -381,6 +389,31 @@ def wavelet_host_rgba16f(src, chroma_strength: float = 1.0):
     _wavelet_check(lib().cpipe_wavelet_host_rgba16f(src.ctypes.data, w, out.ctypes.data, w, w, h,
                                                     float(chroma_strength)))
     return out
+
+
+# pixel formats, numbered as the reference's PixelFormat (include/cpipe_convert.h)
+CONVERT_FORMATS = {"R16_UINT": 1, "R8G8B8A8_UNORM": 3, "R16G16B16A16_SFLOAT": 6, "R32_SFLOAT": 10,
+                   "R16G16B16A16_UNORM": 16}
+
+
+def _convert_check(rc: int) -> None:
+    if rc != 0:
+        raise EngineError(rc, lib().cpipe_convert_last_error().decode(errors="replace"))
+
+
+def convert(src, src_format: str, dst, dst_format: str, stream=None) -> None:
+    """Precision conversion (com.cpipe.precision_convert) between CUDA tensors laid out
+    as (H, W) or (H, W, 4) with unit stride along the pixel's channels."""
+    h, w = src.shape[:2]
+    if tuple(dst.shape[:2]) != (h, w):
+        raise ValueError("source and destination sizes differ")
+    for t in (src, dst):
+        if not t.is_cuda or t.stride(-1) != 1 or (t.dim() == 3 and t.stride(1) != t.shape[2]):
+            raise ValueError("expected CUDA tensors with interleaved pixels")
+    _convert_check(lib().cpipe_convert(CONVERT_FORMATS[src_format], src.data_ptr(),
+                                       src.stride(0) * src.element_size(), CONVERT_FORMATS[dst_format],
+                                       dst.data_ptr(), dst.stride(0) * dst.element_size(), w, h,
+                                       _stream_handle(stream)))
 
 
 def denoise(planes, sigma: float, precision="fp32", stream=None):

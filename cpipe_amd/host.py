@@ -16,8 +16,15 @@ from . import engine
 MINIHOST_SO = os.path.join(engine.PKG, "lib", "libcpipe_minihost.so")
 NODE_ID = "com.cpipe.denoise.bm3d"
 
+FORMAT_R16U = 1
+FORMAT_RGBA8 = 3
 FORMAT_RGBA16F = 6
 FORMAT_R32F = 10
+FORMAT_RGBA16U = 16
+
+# element type and channels per pixel of each format
+_LAYOUT = {FORMAT_R16U: (np.uint16, 1), FORMAT_RGBA8: (np.uint8, 4), FORMAT_RGBA16F: (np.uint16, 4),
+           FORMAT_R32F: (np.float32, 1), FORMAT_RGBA16U: (np.uint16, 4)}
 
 CPIPE_OK = 0
 CPIPE_REPLY_DEFAULT = 2
@@ -96,10 +103,10 @@ class Buffer:
             pass
 
     def _view(self) -> np.ndarray:
-        dtype, per_px = (np.uint16, 4) if self.fmt == FORMAT_RGBA16F else (np.float32, 1)
+        dtype, per_px = _LAYOUT[self.fmt]
         n = self.row_bytes * self.height // np.dtype(dtype).itemsize
-        base = np.ctypeslib.as_array(ctypes.cast(mh().mh_buffer_data(self.h),
-                                                 ctypes.POINTER(ctypes.c_uint16 if dtype == np.uint16 else ctypes.c_float)),
+        ctype = np.ctypeslib.as_ctypes_type(dtype)
+        base = np.ctypeslib.as_array(ctypes.cast(mh().mh_buffer_data(self.h), ctypes.POINTER(ctype)),
                                      shape=(n,))
         rows = base.reshape(self.height, -1)
         return rows[:, : self.width * per_px]
@@ -110,7 +117,7 @@ class Buffer:
 
     def read(self) -> np.ndarray:
         v = self._view().copy()
-        return v.reshape(self.height, self.width, 4) if self.fmt == FORMAT_RGBA16F else v
+        return v.reshape(self.height, self.width, 4) if _LAYOUT[self.fmt][1] == 4 else v
 
     def set_noise_profile(self, pairs) -> None:
         beta = np.array([p[0] for p in pairs], np.float32)

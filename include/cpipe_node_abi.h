@@ -240,6 +240,12 @@ void cpipe_link_builtin_denoise_bm3d(void);
 const cpipe_plugin_desc_t* cpipe_wavelet_node_descriptor(void);
 void cpipe_link_builtin_denoise_wavelet_bayes_shrink(void);
 
+/* The precision-convert node's descriptor, in the same section (replaces
+ * CPIPE_REGISTER_NODE, src/cpipe/nodes/precision/precision_convert.cpp:86), and its
+ * link anchor (precision_convert.cpp:90-92). */
+const cpipe_plugin_desc_t* cpipe_convert_node_descriptor(void);
+void cpipe_link_builtin_precision_convert(void);
+
 #if defined(__cplusplus) && defined(__x86_64__)
 static_assert(sizeof(cpipe_plugin_desc_t) == 40, "cpipe_plugin_desc_t layout");
 static_assert(offsetof(cpipe_plugin_desc_t, node_id) == 8, "node_id offset");

@@ -20,7 +20,8 @@
  *      parity partner of the GPU path and the CPU baseline.
  * and, for the downstream node com.cpipe.denoise.wavelet_bayes_shrink,
  *   4. oracle_wavelet_rgba16f -- its Halide generator's arithmetic
- *      (wavelet_oracle.c).
+ *      (wavelet_oracle.c);
+ * and for com.cpipe.precision_convert, oracle_convert_* (convert_oracle.c).
  *
  * Parity pins (see tests/test_oracle_*.py): compat known answers from the
  * reference unit tests; DCT/Haar orthonormality; Kaiser from the I0 series;
@@ -91,6 +92,16 @@ int oracle_bm3d_f64(const double* const* in, double* const* out, double* const* 
  * (src/cpipe/nodes/denoise/wavelet_bayes_shrink_generator.cpp:57-86). Strides in pixels. */
 void oracle_wavelet_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
                             size_t out_stride_px, int width, int height, float chroma_strength);
+
+/* Precision conversions of the precision-convert node (convert_oracle.c): the
+ * reference Halide generators src/cpipe/nodes/precision/precision_convert_generator.cpp.
+ * Strides in bytes; bits = 8 (R8G8B8A8_UNORM) or 16 (R16G16B16A16_UNORM). */
+void oracle_convert_r16u_to_f32(const uint8_t* in, size_t in_stride, uint8_t* out, size_t out_stride,
+                                int width, int height);
+void oracle_convert_f32_to_rgba16f(const uint8_t* in, size_t in_stride, uint8_t* out,
+                                   size_t out_stride, int width, int height);
+void oracle_convert_rgba16f_to_unorm(const uint8_t* in, size_t in_stride, uint8_t* out,
+                                     size_t out_stride, int width, int height, int bits);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_f16_to_f32.restype = F
         L.oracle_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I]
         L.oracle_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F]
+        L.oracle_convert_r16u_to_f32.argtypes = [P, S, P, S, I, I]
+        L.oracle_convert_f32_to_rgba16f.argtypes = [P, S, P, S, I, I]
+        L.oracle_convert_rgba16f_to_unorm.argtypes = [P, S, P, S, I, I, I]
         L.oracle_compat_rgba16f.restype = None
         L.oracle_ref_count.argtypes = [I]
         L.oracle_ref_count.restype = I
@@ -113,6 +116,29 @@ def wavelet_rgba16f(rgba_bits: np.ndarray, chroma_strength: float = 1.0) -> np.n
     assert c == 4
     out = np.empty_like(src)
     lib().oracle_wavelet_rgba16f(_ptr(src), w, _ptr(out), w, w, h, float(chroma_strength))
+    return out
+
+
+def convert(src: np.ndarray, target: str) -> np.ndarray:
+    """Restated precision conversions: (H, W) uint16 -> "R32_SFLOAT" (H, W) float32;
+    (H, W) float32 -> "R16G16B16A16_SFLOAT" (H, W, 4) uint16 bits; (H, W, 4) uint16
+    binary16 bits -> "R8G8B8A8_UNORM" (H, W, 4) uint8 or "R16G16B16A16_UNORM" uint16."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape[:2]
+    L = lib()
+    if target == "R32_SFLOAT":
+        assert src.dtype == np.uint16 and src.ndim == 2
+        out = np.empty((h, w), np.float32)
+        L.oracle_convert_r16u_to_f32(_ptr(src), src.strides[0], _ptr(out), out.strides[0], w, h)
+    elif target == "R16G16B16A16_SFLOAT":
+        assert src.dtype == np.float32 and src.ndim == 2
+        out = np.empty((h, w, 4), np.uint16)
+        L.oracle_convert_f32_to_rgba16f(_ptr(src), src.strides[0], _ptr(out), out.strides[0], w, h)
+    else:
+        assert src.dtype == np.uint16 and src.shape[2] == 4
+        bits = 8 if target == "R8G8B8A8_UNORM" else 16
+        out = np.empty((h, w, 4), np.uint8 if bits == 8 else np.uint16)
+        L.oracle_convert_rgba16f_to_unorm(_ptr(src), src.strides[0], _ptr(out), out.strides[0], w, h, bits)
     return out
 
 

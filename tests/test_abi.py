@@ -27,7 +27,8 @@ def _declared_functions(header):
 def test_library_exports_every_declared_symbol():
     lib = engine.lib()
     declared = (_declared_functions("cpipe_bm3d.h") + _declared_functions("cpipe_node_abi.h") +
-                _declared_functions("cpipe_wavelet.h"))
+                _declared_functions("cpipe_wavelet.h") + _declared_functions("cpipe_convert.h"))
+    assert "cpipe_convert" in declared and "cpipe_convert_node_descriptor" in declared
     assert "cpipe_bm3d_plan_run" in declared and "cpipe_bm3d_node_descriptor" in declared
     assert "cpipe_wavelet_rgba16f" in declared and "cpipe_wavelet_node_descriptor" in declared
     for name in declared:
@@ -39,17 +40,17 @@ def test_library_exports_every_declared_symbol():
 def test_abi_header_layout_static_asserts(tmp_path):
     src = tmp_path / "abi.cpp"
     src.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\n#include "cpipe_wavelet.h"\n'
-                   'int main(){return 0;}\n')
+                   '#include "cpipe_convert.h"\nint main(){return 0;}\n')
     subprocess.run(["g++", "-std=c++17", f"-I{INCLUDE}", "-fsyntax-only", str(src)], check=True)
     csrc = tmp_path / "abi.c"
     csrc.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\n#include "cpipe_wavelet.h"\n'
-                    'int main(void){return 0;}\n')
+                    '#include "cpipe_convert.h"\nint main(void){return 0;}\n')
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{INCLUDE}", "-fsyntax-only", str(csrc)],
                    check=True)
 
 
 def test_registry_section_holds_the_nodes():
-    assert host.mh().mh_registry_count(engine.ENGINE_SO.encode()) == 2
+    assert host.mh().mh_registry_count(engine.ENGINE_SO.encode()) == 3
     d = host.find_node()
     assert d.contents.abi_major == 0 and d.contents.abi_minor == 4
     assert d.contents.node_id == b"com.cpipe.denoise.bm3d"
@@ -58,6 +59,39 @@ def test_registry_section_holds_the_nodes():
     wd = host.find_node(WAVELET_ID)
     assert wd.contents.abi_major == 0 and wd.contents.abi_minor == 4
     assert engine.lib().cpipe_wavelet_node_descriptor() == ctypes.addressof(wd.contents)
+    cd = host.find_node(CONVERT_ID)
+    assert engine.lib().cpipe_convert_node_descriptor() == ctypes.addressof(cd.contents)
+
+
+CONVERT_ID = "com.cpipe.precision_convert"
+
+
+def test_convert_manifest_and_status_contract():
+    """Manifest ports and applied step (precision_convert.json), and the node's statuses
+    (precision_convert.cpp:49-72): arity -> BAD_INDEX, no target_format -> its param
+    status, target/output mismatch and unsupported pairs -> BAD_PRECISION; without a
+    device CPIPE_UNSUPPORTED, never a CPU fallback."""
+    d = host.find_node(CONVERT_ID)
+    m = json.loads(d.contents.manifest_json.decode())
+    assert m["id"] == CONVERT_ID and m["params"] == []
+    assert m["ports"][1]["metadata"]["sets_steps_applied"] == ["precision_convert"]
+    assert m["compute"]["device"] == "GPU" and m["compute"]["engine"] == "HIP"
+    a = host.Buffer(host.FORMAT_RGBA16F, 2, 1)
+    b8 = host.Buffer(host.FORMAT_RGBA8, 2, 1)
+    c = host.Buffer(host.FORMAT_RGBA16F, 2, 1)
+    assert host.run_node(d, [a, c], [b8], {"target_format": "R8G8B8A8_UNORM"}) == host.CPIPE_BAD_INDEX
+    assert host.run_node(d, [a], [b8], {}) == 6  # CPIPE_NEED_PARAM
+    assert host.run_node(d, [a], [b8], {"target_format": "R16G16B16A16_UNORM"}) == host.CPIPE_BAD_PRECISION
+    f = host.Buffer(host.FORMAT_R32F, 2, 1)
+    assert host.run_node(d, [f], [b8], {"target_format": "R8G8B8A8_UNORM"}) == host.CPIPE_BAD_PRECISION
+    L = engine.lib()
+    assert L.cpipe_convert_supported(6, 3) == 1 and L.cpipe_convert_supported(3, 6) == 0
+    assert L.cpipe_convert(6, None, 16, 3, None, 8, 2, 1, None) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_convert(3, None, 16, 6, None, 8, 2, 1, None) == host.CPIPE_BAD_PRECISION
+    import torch
+    if torch.cuda.is_available():
+        return
+    assert host.run_node(d, [a], [b8], {"target_format": "R8G8B8A8_UNORM"}) == host.CPIPE_UNSUPPORTED
 
 
 WAVELET_ID = "com.cpipe.denoise.wavelet_bayes_shrink"
