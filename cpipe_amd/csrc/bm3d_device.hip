@@ -710,7 +710,9 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     for (int idx = threadIdx.x; idx < DOFS; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
     __syncthreads();
 
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index as a scalar: group indices, the round loop and the turn
+    // wait below stay wave-uniform (scalar branches, no exec-mask bookkeeping)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane >> 3, lj = lane & 7;
     float* sb = scratch[wave];
     const float kw = c_kaiser2[lane];

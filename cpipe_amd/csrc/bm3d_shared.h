@@ -200,7 +200,9 @@ __device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g,
 
     bool timed_out = false;
     int spins = 0;
-    while (__hip_atomic_load(turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != g) {
+    // every lane reads the same token: compare it as a scalar
+    while (__builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) != g) {
         __builtin_amdgcn_s_sleep(1);
         if (++spins > (1 << 24)) {
             timed_out = true;
