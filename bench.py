@@ -197,6 +197,56 @@ def host_path_bench(noisy, sigma, runs):
     return res
 
 
+def node_chain_bench(noisy, sigma, runs):
+    """The classic pipeline's chain bm3d -> wavelet_bayes_shrink -> precision_convert
+    (to RGBA8) through the plugin ABI with the minimal host, on one RGBA16F frame:
+    host buffers (each node copies in and out over PCIe) against device-resident
+    buffers (the "hip_buffer" suite: device addresses, no host copy). The first
+    chain of each kind (plan creation) is left out of the median."""
+    from cpipe_amd import host
+    c, h, w = noisy.shape
+    if c != 3:
+        return None
+    rgba = np.ones((h, w, 4), np.float32)
+    rgba[..., :3] = np.moveaxis(noisy, 0, -1)
+    bits = rgba.astype(np.float16).view(np.uint16)
+    nodes = (host.find_node(), host.find_node("com.cpipe.denoise.wavelet_bayes_shrink"),
+             host.find_node("com.cpipe.precision_convert"))
+    res = {"frame": f"{w}x{h} RGBA16F -> bm3d -> wavelet_bayes_shrink -> precision_convert RGBA8"}
+    for form, params in (("fp32", {"mode": "bm3d", "sigma": float(sigma)}),
+                         ("learned_fp16", {"mode": "bm3d", "sigma": float(sigma), "precision": "fp16",
+                                           "pilot": "learned"})):
+        for kind, dev in (("host_buffers", None), ("device_buffers", 0)):
+            a = host.Buffer(host.FORMAT_RGBA16F, w, h, device=dev)
+            a.write(bits)
+            mid = host.Buffer(host.FORMAT_RGBA16F, w, h, device=dev)
+            wav = host.Buffer(host.FORMAT_RGBA16F, w, h, device=dev)
+            out = host.Buffer(host.FORMAT_RGBA8, w, h, device=dev)
+            chain, per = [], []
+            for _ in range(runs + 1):
+                t = [time.perf_counter()]
+                for d, i, o, prm in ((nodes[0], a, mid, params), (nodes[1], mid, wav, {}),
+                                     (nodes[2], wav, out, {"target_format": "R8G8B8A8_UNORM"})):
+                    rc = host.run_node(d, [i], [o], prm)
+                    if rc != host.CPIPE_OK:
+                        raise RuntimeError(f"node chain: status {rc}: {host.host_log()}")
+                    t.append(time.perf_counter())
+                chain.append((t[-1] - t[0]) * 1e3)
+                per.append([(t[k + 1] - t[k]) * 1e3 for k in range(3)])
+            ms = float(np.median(chain[1:]))
+            nodes_ms = np.median(np.array(per[1:]), axis=0)
+            res.setdefault(form, {})[kind] = {
+                "first_chain_ms": round(chain[0], 2), "ms": round(ms, 2),
+                "node_ms": {"bm3d": round(float(nodes_ms[0]), 2), "wavelet": round(float(nodes_ms[1]), 3),
+                            "convert": round(float(nodes_ms[2]), 3)},
+                "value": round(w * h / (ms / 1e3) / 1e6, 2), "unit": "Mpixel/s"}
+            del a, mid, wav, out
+        f = res[form]
+        f["device_over_host"] = round(f["host_buffers"]["ms"] / f["device_buffers"]["ms"], 3)
+    engine.lib().cpipe_bm3d_host_release()
+    return res
+
+
 def wavelet_bench(torch, noisy, stream, runs, want_cpu):
     """The downstream node com.cpipe.denoise.wavelet_bayes_shrink on the same frame as
     RGBA16F (alpha 1), resident in HBM: one kernel launch per run, timed with HIP
@@ -298,7 +348,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
-    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) and node-chain legs")
     ap.add_argument("--no-wavelet", action="store_true",
                     help="skip the downstream wavelet BayesShrink and precision-convert node legs")
     ap.add_argument("--cpu-sample", default="2048x1536")
@@ -433,10 +483,12 @@ def main():
         print(f"# fp64 crop: {json.dumps(fp64)}", file=sys.stderr, flush=True)
 
     # ---- the node's path: host (pageable) planes in and out, PCIe copies included
-    host_path = None
+    host_path = node_chain = None
     if not args.no_host:
         host_path = host_path_bench(noisy, sigma, 3)
         print(f"# host path: {json.dumps(host_path)}", file=sys.stderr, flush=True)
+        node_chain = node_chain_bench(noisy, sigma, 3)
+        print(f"# node chain: {json.dumps(node_chain)}", file=sys.stderr, flush=True)
 
     wavelet = convert = None
     if not args.no_wavelet:
@@ -473,6 +525,7 @@ def main():
         "fp64_crop": fp64,
         "batch12": batch,
         "host_path": host_path,
+        "node_chain": node_chain,
         "wavelet": wavelet,
         "precision_convert": convert,
     }

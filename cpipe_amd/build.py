@@ -4,7 +4,8 @@ Outputs (git-ignored, shipped with the tree):
   cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h, include/cpipe_wavelet.h,
                                       include/cpipe_convert.h)
                                       + the nodes' descriptors in the `cpipe_registry` section
-  cpipe_amd/lib/libcpipe_minihost.so  minimal cpipe host (suites, registry walk, one-node runner)
+  cpipe_amd/lib/libcpipe_minihost.so  minimal cpipe host (suites, registry walk, one-node runner,
+                                      host and HIP device-resident buffers)
 """
 from __future__ import annotations
 
@@ -34,6 +35,10 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
              f"-I{INCLUDE}"]
+# Host C++ that calls the HIP runtime API (device selection, streams, copies).
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIP_HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
+HIP_HOST_LIBS = [f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"]
 
 
 def _hipcc() -> str:
@@ -114,7 +119,7 @@ def build(verbose: bool = False) -> dict:
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm_mx.hip"), "-o", bmx_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "wavelet.hip"), "-o", wav_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "convert.hip"), "-o", cvt_o], verbose)
-    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
+    _run(["g++", *CXX_FLAGS, *HIP_HOST_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", blob_cpp, "-o", blob_o], verbose)
     tmp = ENGINE_SO + ".tmp"
@@ -122,8 +127,8 @@ def build(verbose: bool = False) -> dict:
           man_o, blob_o], verbose)
     os.replace(tmp, ENGINE_SO)
     tmp = MINIHOST_SO + ".tmp"
-    _run(["g++", *CXX_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "minihost.cpp"), "-ldl"],
-         verbose)
+    _run(["g++", *CXX_FLAGS, *HIP_HOST_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "minihost.cpp"), "-ldl",
+          *HIP_HOST_LIBS], verbose)
     os.replace(tmp, MINIHOST_SO)
     return {"engine": ENGINE_SO, "minihost": MINIHOST_SO}
 

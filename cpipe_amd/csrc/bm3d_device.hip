@@ -1573,6 +1573,22 @@ int host_net(HostCache& hc, int device, cpipe_bm3d_pilot** out) {
     return OK;
 }
 
+// Grows a slot's device and pinned staging to `bytes` each (none for the
+// device-resident path, which passes 0).
+int slot_staging(HostSlot& sl, size_t bytes) {
+    if (bytes <= sl.dev_bytes) return OK;
+    DeviceGuard guard(sl.device);
+    (void)hipFree(sl.dev);
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    sl.dev = sl.pin = nullptr;
+    sl.dev_bytes = 0;
+    hipError_t e = hipMalloc(&sl.dev, bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.pin, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(OOM, std::string("staging buffers: ") + hipGetErrorString(e));
+    sl.dev_bytes = bytes;
+    return OK;
+}
+
 // Finds or creates the slot for this call's shape (caller holds hc.mu).
 int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int device, int flags,
               size_t dev_bytes, HostSlot** out) {
@@ -1585,6 +1601,8 @@ int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int d
     for (HostSlot& sl : hc.slots) {
         if (sl.kind == kind && sl.w == w && sl.h == h && sl.c == c && sl.precision == precision &&
             sl.device == device && sl.flags == flags) {
+            int rc = slot_staging(sl, dev_bytes);
+            if (rc) return rc;
             sl.used = ++hc.clock;
             *out = &sl;
             return OK;
@@ -1611,17 +1629,16 @@ int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int d
     }
     if (!rc) {
         DeviceGuard guard(device);
-        hipError_t e = hipMalloc(&sl.dev, dev_bytes);
-        if (e == hipSuccess) e = hipHostMalloc(&sl.pin, dev_bytes, hipHostMallocDefault);
+        hipError_t e = hipSuccess;
         for (hipEvent_t& ev : sl.ev)
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-        if (e != hipSuccess) rc = fail(OOM, std::string("staging buffers: ") + hipGetErrorString(e));
+        if (e != hipSuccess) rc = fail(INTERNAL, std::string("slot events: ") + hipGetErrorString(e));
     }
+    if (!rc) rc = slot_staging(sl, dev_bytes);
     if (rc) {
         drop_slot(sl);
         return rc;
     }
-    sl.dev_bytes = dev_bytes;
     sl.used = ++hc.clock;
     hc.slots.push_back(sl);
     *out = &hc.slots.back();
@@ -2353,6 +2370,42 @@ int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_
     HIP_CHECK(hipMemcpyAsync(pout, dout, bytes, hipMemcpyDeviceToHost, nullptr));
     HIP_CHECK(hipStreamSynchronize(nullptr));
     par_rows(out, osp * 8, pout, row, row, h);
+    return mode == 0 ? OK : cpipe_bm3d_plan_check(sl->plan);
+}
+
+int cpipe_bm3d_device_planes(const float* const* in, size_t in_pitch, float* const* out,
+                             size_t out_pitch, int w, int h, int channels, float sigma, int precision,
+                             int device, int flags, void* stream) {
+    if (!in || !out) return fail(BAD_INDEX, "null argument");
+    if (channels != 1 && channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
+    if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
+    HostCache& hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
+    HostSlot* sl = nullptr;
+    int rc = host_slot(hc, HOST_PLANES, w, h, channels, precision, device, flags, 0, &sl);
+    if (rc) return rc;
+    DeviceGuard guard(device);
+    if ((rc = cpipe_bm3d_plan_run(sl->plan, in, in_pitch, out, out_pitch, sigma, stream))) return rc;
+    HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return cpipe_bm3d_plan_check(sl->plan);
+}
+
+int cpipe_bm3d_device_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_t osp, int w, int h,
+                              float sigma, int mode, int precision, int device, int flags, void* stream) {
+    if (!in || !out) return fail(BAD_INDEX, "null argument");
+    if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
+    if (mode == 0 && (flags & CPIPE_BM3D_HOST_PILOT))
+        return fail(BAD_INDEX, "the learned pilot needs mode bm3d");
+    HostCache& hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
+    HostSlot* sl = nullptr;
+    int rc = host_slot(hc, mode == 0 ? HOST_COMPAT : HOST_RGBA, w, h, 3, precision, device, flags, 0, &sl);
+    if (rc) return rc;
+    DeviceGuard guard(device);
+    rc = mode == 0 ? cpipe_bm3d_compat_rgba16f(in, isp, out, osp, w, h, sigma, 1, stream)
+                   : cpipe_bm3d_plan_run_rgba16f(sl->plan, in, isp, out, osp, sigma, stream);
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
     return mode == 0 ? OK : cpipe_bm3d_plan_check(sl->plan);
 }
 

@@ -9,8 +9,14 @@
 // create -> process -> freeze metadata -> destroy. It is not the reference's
 // Pipeline/Scheduler; it drives one node on host buffers.
 //
+// Buffers are host memory (CpuBuffer analogue) or HIP device memory (the
+// device-resident kind, offered to nodes through the "hip_buffer" suite; its
+// lock_cpu stages the pixels through a host copy, as a mapped VulkanBuffer
+// gives the host a view of device memory, VulkanBuffer.hpp:38-40).
+//
 // Exposed as a small C API (mh_*) so Python tests and tools can drive nodes.
 #include <dlfcn.h>
+#include <hip/hip_runtime_api.h>
 
 #include <cstdint>
 #include <cstdlib>
@@ -35,8 +41,11 @@ struct cpipe_buffer_s {
     uint64_t row_bytes = 0;
     uint64_t pixel_bytes = 0;
     std::vector<uint8_t> storage;
-    uint8_t* data = nullptr;  // 64-byte aligned view into storage
+    uint8_t* data = nullptr;  // 64-byte aligned view into storage (the host copy of a device buffer)
+    void* dev = nullptr;      // device-resident pixels (row_bytes * h), or null for a host buffer
+    int device = -1;
     int locks = 0;
+    int lock_access = 0;
     cpipe_metadata_s meta;
 };
 
@@ -94,22 +103,50 @@ int b_get_color_role(const cpipe_buffer_t* b, const char** r) {
     *r = b->meta.cs_role.c_str();
     return CPIPE_OK;
 }
-int b_lock(cpipe_buffer_t* b, int, void** p) {
+// Copies between a device buffer and its host copy on the buffer's device.
+bool dev_copy(cpipe_buffer_t* b, bool to_device) {
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(b->device) != hipSuccess) return false;
+    const size_t n = (size_t)b->row_bytes * b->h;
+    const hipError_t e = to_device ? hipMemcpy(b->dev, b->data, n, hipMemcpyHostToDevice)
+                                   : hipMemcpy(b->data, b->dev, n, hipMemcpyDeviceToHost);
+    (void)hipSetDevice(prev);
+    return e == hipSuccess;
+}
+int b_lock(cpipe_buffer_t* b, int access, void** p) {
     if (!b || !p) return CPIPE_BAD_INDEX;
     if (b->locks != 0) return CPIPE_FAILED;  // single outstanding lock, like CpuBuffer
+    if (b->dev && access != CPIPE_CPU_ACCESS_WRITE && !dev_copy(b, false)) return CPIPE_FAILED;
     b->locks = 1;
+    b->lock_access = access;
     *p = b->data;
     return CPIPE_OK;
 }
 int b_unlock(cpipe_buffer_t* b) {
     if (!b) return CPIPE_BAD_INDEX;
+    const bool wrote = b->locks != 0 && b->lock_access != CPIPE_CPU_ACCESS_READ;
     b->locks = 0;
+    if (b->dev && wrote && !dev_copy(b, true)) return CPIPE_FAILED;
     return CPIPE_OK;
 }
 int b_flush(cpipe_buffer_t* b) { return b ? CPIPE_OK : CPIPE_BAD_INDEX; }
 int b_get_metadata(const cpipe_buffer_t* b, const cpipe_metadata_t** out) {
     if (!b || !out) return CPIPE_BAD_INDEX;
     *out = &b->meta;
+    return CPIPE_OK;
+}
+
+// ---- hip_buffer suite ----------------------------------------------------------
+int hb_get_device_ptr(const cpipe_buffer_t* b, void** p, int* device) {
+    if (!b || !p || !device) return CPIPE_BAD_INDEX;
+    if (!b->dev) return CPIPE_UNSUPPORTED;
+    *p = b->dev;
+    *device = b->device;
+    return CPIPE_OK;
+}
+int hb_get_stream(cpipe_compute_t*, void** stream) {
+    if (!stream) return CPIPE_BAD_INDEX;
+    *stream = nullptr;  // the null stream: ordered with the host copies of lock_cpu
     return CPIPE_OK;
 }
 
@@ -190,6 +227,8 @@ struct Host {
     cpipe_param_suite_v1 param{};
     cpipe_compute_suite_v1 compute{};
     cpipe_inference_suite_v1 inference{};
+    cpipe_hip_buffer_suite_v1 hip{};
+    bool offer_hip = true;  // false: behave as a host without the suite
 };
 
 const void* h_get_suite(cpipe_host_t* self, const char* name, int version);
@@ -216,6 +255,8 @@ Host& host_singleton() {
         x.builder.set_cs_role = &mb_set_cs_role;
         x.param.get_double = &p_get_double;
         x.param.get_enum = &p_get_enum;
+        x.hip.get_device_ptr = &hb_get_device_ptr;
+        x.hip.get_stream = &hb_get_stream;
         return x;
     }();
     return h;
@@ -230,6 +271,7 @@ const void* h_get_suite(cpipe_host_t*, const char* name, int version) {
     if (!std::strcmp(name, "param")) return &h.param;
     if (!std::strcmp(name, "compute")) return &h.compute;
     if (!std::strcmp(name, "inference")) return &h.inference;
+    if (!std::strcmp(name, "hip_buffer") && h.offer_hip) return &h.hip;
     return nullptr;
 }
 
@@ -302,7 +344,49 @@ cpipe_buffer_t* mh_buffer_new(int format, uint32_t w, uint32_t h, uint64_t row_b
     b->data = reinterpret_cast<uint8_t*>((base + 63) & ~uintptr_t(63));
     return b;
 }
-void mh_buffer_free(cpipe_buffer_t* b) { delete b; }
+// A device-resident buffer on HIP device `device` (zero-filled), or null when
+// the layout is bad or the device allocation fails (no device, out of memory).
+cpipe_buffer_t* mh_buffer_new_device(int format, uint32_t w, uint32_t h, uint64_t row_bytes, int device) {
+    cpipe_buffer_t* b = mh_buffer_new(format, w, h, row_bytes);
+    if (!b) return nullptr;
+    int prev = 0;
+    if (device < 0 || hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) {
+        g_log += "device buffer: no such HIP device\n";
+        delete b;
+        return nullptr;
+    }
+    const size_t n = (size_t)b->row_bytes * h;
+    hipError_t e = hipMalloc(&b->dev, n);
+    if (e == hipSuccess) e = hipMemset(b->dev, 0, n);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) {
+        g_log += std::string("device buffer: ") + hipGetErrorString(e) + "\n";
+        if (b->dev) (void)hipFree(b->dev);
+        delete b;
+        return nullptr;
+    }
+    b->device = device;
+    return b;
+}
+void mh_buffer_free(cpipe_buffer_t* b) {
+    if (b && b->dev) {
+        int prev = 0;
+        if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(b->device) == hipSuccess) {
+            (void)hipFree(b->dev);
+            (void)hipSetDevice(prev);
+        }
+    }
+    delete b;
+}
+int mh_buffer_is_device(const cpipe_buffer_t* b) { return b && b->dev ? 1 : 0; }
+// lock_cpu / unlock_cpu of the buffer suite, for tools filling and reading buffers.
+void* mh_buffer_lock(cpipe_buffer_t* b, int access) {
+    void* p = nullptr;
+    return b_lock(b, access, &p) == CPIPE_OK ? p : nullptr;
+}
+int mh_buffer_unlock(cpipe_buffer_t* b) { return b_unlock(b); }
+// Whether get_suite offers "hip_buffer" (1, the default) or not (0).
+void mh_offer_hip_buffer(int on) { host_singleton().offer_hip = on != 0; }
 void* mh_buffer_data(cpipe_buffer_t* b) { return b ? b->data : nullptr; }
 uint64_t mh_buffer_row_bytes(const cpipe_buffer_t* b) { return b ? b->row_bytes : 0; }
 int mh_buffer_set_noise_profile(cpipe_buffer_t* b, const float* beta, const float* lambda, size_t n) {

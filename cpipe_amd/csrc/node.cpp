@@ -17,8 +17,13 @@
 //     params clamped to [0, 0.2] (bm3d.cpp:21-55);
 //   * add_applied_step("denoise.bm3d") on success (bm3d.cpp:103).
 // Added: `mode` (compat | bm3d) and `precision` params, FP32 plane I/O
-// (1 x R32_SFLOAT grey or 3 x R32_SFLOAT RGB planes), and a defined status
-// when no HIP device is visible (CPIPE_UNSUPPORTED, logged).
+// (1 x R32_SFLOAT grey or 3 x R32_SFLOAT RGB planes), a defined status
+// when no HIP device is visible (CPIPE_UNSUPPORTED, logged), and, for hosts
+// that offer the "hip_buffer" suite, a device-resident path: when every input
+// and output is in HIP device memory on one device, the three nodes run on the
+// device addresses on the host's stream, with no lock_cpu and no host copy.
+#include <hip/hip_runtime_api.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -46,6 +51,7 @@ struct Suites {
     const cpipe_param_suite_v1* param = nullptr;
     const cpipe_metadata_suite_v1* metadata = nullptr;
     const cpipe_metadata_builder_suite_v1* builder = nullptr;
+    const cpipe_hip_buffer_suite_v1* hip = nullptr;  // optional: device-resident buffers
 };
 
 Suites fetch_suites(cpipe_host_t* host) {
@@ -56,7 +62,50 @@ Suites fetch_suites(cpipe_host_t* host) {
     s.metadata = static_cast<const cpipe_metadata_suite_v1*>(host->get_suite(host, "metadata", 1));
     s.builder = static_cast<const cpipe_metadata_builder_suite_v1*>(
         host->get_suite(host, "metadata_builder", 1));
+    s.hip = static_cast<const cpipe_hip_buffer_suite_v1*>(host->get_suite(host, "hip_buffer", 1));
     return s;
+}
+
+// Device addresses of every input and output when the host keeps them all in
+// HIP device memory on one device; false when the host has no "hip_buffer"
+// suite or any buffer is host memory (the node then goes through lock_cpu).
+bool device_resident(const Suites& s, const cpipe_process_ctx* ctx, std::vector<void*>* in,
+                     std::vector<void*>* out, int* device) {
+    if (s.hip == nullptr || s.hip->get_device_ptr == nullptr) return false;
+    int dev = -1;
+    auto one = [&](const cpipe_buffer_t* b, void** p) {
+        int d = -1;
+        if (s.hip->get_device_ptr(b, p, &d) != CPIPE_OK || *p == nullptr || d < 0) return false;
+        if (dev < 0) dev = d;
+        return d == dev;
+    };
+    in->assign(ctx->n_in, nullptr);
+    out->assign(ctx->n_out, nullptr);
+    for (size_t i = 0; i < ctx->n_in; ++i)
+        if (!one(ctx->inputs[i], &(*in)[i])) return false;
+    for (size_t i = 0; i < ctx->n_out; ++i)
+        if (!one(ctx->outputs[i], &(*out)[i])) return false;
+    *device = dev;
+    return true;
+}
+
+// The stream the host orders this node's device work on (NULL: the null stream).
+int device_stream(const Suites& s, const cpipe_process_ctx* ctx, void** stream) {
+    *stream = nullptr;
+    if (s.hip == nullptr || s.hip->get_stream == nullptr) return CPIPE_OK;
+    return s.hip->get_stream(ctx->compute, stream);
+}
+
+// Launches an entry point's kernels on `device`, then waits for `stream`.
+template <class F>
+int on_device(int device, void* stream, F&& launch) {
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return CPIPE_BAD_INDEX;
+    int rc = launch();
+    if (rc == CPIPE_OK && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+        rc = CPIPE_INTERNAL_ERROR;
+    (void)hipSetDevice(prev);
+    return rc;
 }
 
 void host_log(cpipe_host_t* host, int level, const std::string& msg) {
@@ -194,9 +243,49 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
         return CPIPE_UNSUPPORTED;
     }
 
-    std::vector<void*> inp(ctx->n_in, nullptr), outp(ctx->n_out, nullptr);
-    size_t locked_in = 0, locked_out = 0;
+    // One engine call on host (locked) or device addresses.
+    auto run = [&](const std::vector<void*>& ip, const std::vector<void*>& op, bool dev_path, int device,
+                   void* stream) -> int {
+        const int w = (int)in[0].w, h = (int)in[0].h;
+        if (rgba) {
+            const auto* i16 = static_cast<const uint16_t*>(ip[0]);
+            auto* o16 = static_cast<uint16_t*>(op[0]);
+            const int m = mode == "bm3d" ? 1 : 0;
+            return dev_path ? cpipe_bm3d_device_rgba16f(i16, in[0].row_bytes / 8, o16, out[0].row_bytes / 8, w,
+                                                        h, sigma, m, precision, device, flags, stream)
+                            : cpipe_bm3d_host_rgba16f(i16, in[0].row_bytes / 8, o16, out[0].row_bytes / 8, w, h,
+                                                      sigma, m, precision, 0, flags);
+        }
+        const float* fi[3] = {};
+        float* fo[3] = {};
+        for (size_t c = 0; c < ctx->n_in; ++c) {
+            fi[c] = static_cast<const float*>(ip[c]);
+            fo[c] = static_cast<float*>(op[c]);
+        }
+        for (size_t c = 1; c < ctx->n_in; ++c)
+            if (in[c].row_bytes != in[0].row_bytes || out[c].row_bytes != out[0].row_bytes)
+                return CPIPE_UNSUPPORTED;
+        const size_t ipitch = in[0].row_bytes / 4, opitch = out[0].row_bytes / 4;
+        return dev_path ? cpipe_bm3d_device_planes(fi, ipitch, fo, opitch, w, h, (int)ctx->n_in, sigma,
+                                                   precision, device, flags, stream)
+                        : cpipe_bm3d_host_planes(fi, ipitch, fo, opitch, w, h, (int)ctx->n_in, sigma,
+                                                 precision, 0, flags);
+    };
+
+    std::vector<void*> inp, outp;
+    int device = 0;
     int rc = CPIPE_OK;
+    if (device_resident(s, ctx, &inp, &outp, &device)) {
+        void* stream = nullptr;
+        rc = device_stream(s, ctx, &stream);
+        if (rc == CPIPE_OK) rc = map_engine_status(run(inp, outp, true, device, stream), host);
+        if (rc != CPIPE_OK) return rc;
+        if (s.builder == nullptr || s.builder->add_applied_step == nullptr) return CPIPE_BAD_INDEX;
+        return s.builder->add_applied_step(ctx->out_metadata[0], "denoise.bm3d");
+    }
+    inp.assign(ctx->n_in, nullptr);
+    outp.assign(ctx->n_out, nullptr);
+    size_t locked_in = 0, locked_out = 0;
     for (size_t i = 0; i < ctx->n_in && rc == CPIPE_OK; ++i) {
         rc = s.buffer->lock_cpu(const_cast<cpipe_buffer_t*>(ctx->inputs[i]), CPIPE_CPU_ACCESS_READ,
                                 &inp[i]);
@@ -206,29 +295,7 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
         rc = s.buffer->lock_cpu(ctx->outputs[i], CPIPE_CPU_ACCESS_WRITE, &outp[i]);
         if (rc == CPIPE_OK) ++locked_out;
     }
-    if (rc == CPIPE_OK) {
-        const int w = (int)in[0].w, h = (int)in[0].h;
-        if (rgba) {
-            rc = cpipe_bm3d_host_rgba16f(static_cast<const uint16_t*>(inp[0]), in[0].row_bytes / 8,
-                                         static_cast<uint16_t*>(outp[0]), out[0].row_bytes / 8, w, h,
-                                         sigma, mode == "bm3d" ? 1 : 0, precision, 0, flags);
-        } else {
-            const float* ip[3] = {};
-            float* op[3] = {};
-            for (size_t c = 0; c < ctx->n_in; ++c) {
-                ip[c] = static_cast<const float*>(inp[c]);
-                op[c] = static_cast<float*>(outp[c]);
-            }
-            size_t ipitch = in[0].row_bytes / 4, opitch = out[0].row_bytes / 4;
-            for (size_t c = 1; c < ctx->n_in; ++c)
-                if (in[c].row_bytes != in[0].row_bytes || out[c].row_bytes != out[0].row_bytes)
-                    rc = CPIPE_UNSUPPORTED;
-            if (rc == CPIPE_OK)
-                rc = cpipe_bm3d_host_planes(ip, ipitch, op, opitch, w, h, (int)ctx->n_in, sigma,
-                                            precision, 0, flags);
-        }
-        rc = map_engine_status(rc, host);
-    }
+    if (rc == CPIPE_OK) rc = map_engine_status(run(inp, outp, false, 0, nullptr), host);
     for (size_t i = 0; i < locked_out; ++i) {
         s.buffer->unlock_cpu(ctx->outputs[i]);
         if (rc == CPIPE_OK && s.buffer->flush_cpu_writes != nullptr)
@@ -278,7 +345,8 @@ __attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_
 // as a number, 1 when absent, clamped to [0, 2] (:37-38 with
 // src/cpipe/nodes/ParamUtils.hpp:13-29); add_applied_step
 // ("denoise.wavelet_bayes_shrink") on success (:46). The Halide submit becomes
-// cpipe_wavelet_host_rgba16f. Added: a format/size gate (RGBA16F in and out of
+// cpipe_wavelet_host_rgba16f (host buffers) or cpipe_wavelet_rgba16f
+// (device-resident buffers). Added: a format/size gate (RGBA16F in and out of
 // one size, else CPIPE_BAD_PRECISION / CPIPE_BAD_INDEX) and CPIPE_UNSUPPORTED
 // (logged) when no HIP device is visible.
 // ---------------------------------------------------------------------------
@@ -322,6 +390,24 @@ int process_wavelet(cpipe_host_t* host, const cpipe_props_t* params, const cpipe
     if (cpipe_bm3d_device_count() <= 0) {
         host_log(host, 3, "denoise.wavelet_bayes_shrink: no HIP device visible; the node runs on the GPU only");
         return CPIPE_UNSUPPORTED;
+    }
+    std::vector<void*> dip, dop;
+    int device = 0;
+    if (device_resident(s, ctx, &dip, &dop, &device)) {
+        void* stream = nullptr;
+        rc = device_stream(s, ctx, &stream);
+        if (rc == CPIPE_OK)
+            rc = on_device(device, stream, [&] {
+                return cpipe_wavelet_rgba16f(static_cast<const uint16_t*>(dip[0]), in.row_bytes / 8,
+                                             static_cast<uint16_t*>(dop[0]), out.row_bytes / 8, (int)in.w,
+                                             (int)in.h, strength, stream);
+            });
+        if (rc != CPIPE_OK) {
+            host_log(host, 3, std::string("denoise.wavelet_bayes_shrink: ") + cpipe_wavelet_last_error());
+            return rc;
+        }
+        if (s.builder == nullptr || s.builder->add_applied_step == nullptr) return CPIPE_BAD_INDEX;
+        return s.builder->add_applied_step(ctx->out_metadata[0], "denoise.wavelet_bayes_shrink");
     }
     void* ip = nullptr;
     void* op = nullptr;
@@ -383,7 +469,8 @@ __attribute__((used, section("cpipe_registry"), aligned(8))) const cpipe_plugin_
 // include/cpipe/sdk/sdk.hpp:557-569); the output format must be the target
 // (:30-36,63-66) and the pair one of the four conversions (:13-28,68-72), else
 // CPIPE_BAD_PRECISION; add_applied_step("precision_convert") (:78). The Halide
-// submit becomes cpipe_convert_host. Added: CPIPE_UNSUPPORTED (logged) when no
+// submit becomes cpipe_convert_host (host buffers) or cpipe_convert
+// (device-resident buffers). Added: CPIPE_UNSUPPORTED (logged) when no
 // HIP device is visible, CPIPE_BAD_INDEX on a size mismatch.
 // ---------------------------------------------------------------------------
 
@@ -471,6 +558,23 @@ int process_convert(cpipe_host_t* host, const cpipe_props_t* params, const cpipe
     if (cpipe_bm3d_device_count() <= 0) {
         host_log(host, 3, "precision_convert: no HIP device visible; the node runs on the GPU only");
         return CPIPE_UNSUPPORTED;
+    }
+    std::vector<void*> dip, dop;
+    int device = 0;
+    if (device_resident(s, ctx, &dip, &dop, &device)) {
+        void* stream = nullptr;
+        rc = device_stream(s, ctx, &stream);
+        if (rc == CPIPE_OK)
+            rc = on_device(device, stream, [&] {
+                return cpipe_convert(in.format, dip[0], in.row_bytes, tf, dop[0], out.row_bytes, (int)in.w,
+                                     (int)in.h, stream);
+            });
+        if (rc != CPIPE_OK) {
+            host_log(host, 3, std::string("precision_convert: ") + cpipe_convert_last_error());
+            return rc;
+        }
+        if (s.builder == nullptr || s.builder->add_applied_step == nullptr) return CPIPE_BAD_INDEX;
+        return s.builder->add_applied_step(ctx->out_metadata[0], "precision_convert");
     }
     void* ip = nullptr;
     void* op = nullptr;

@@ -2,7 +2,9 @@
 
 Mirrors the reference unit-test driver process_bm3d
 (tests/unit/test_node_denoise_bm3d.cpp:89-125): registry lookup by node id,
-create -> process -> freeze metadata -> destroy, host buffers in and out.
+create -> process -> freeze metadata -> destroy, host buffers in and out, or
+device-resident buffers (HIP device memory, offered to the nodes through the
+host's "hip_buffer" suite) that chained nodes read and write with no host copy.
 """
 from __future__ import annotations
 
@@ -25,6 +27,8 @@ FORMAT_RGBA16U = 16
 # element type and channels per pixel of each format
 _LAYOUT = {FORMAT_R16U: (np.uint16, 1), FORMAT_RGBA8: (np.uint8, 4), FORMAT_RGBA16F: (np.uint16, 4),
            FORMAT_R32F: (np.float32, 1), FORMAT_RGBA16U: (np.uint16, 4)}
+
+CPU_READ, CPU_WRITE = 0, 1  # cpipe_cpu_access_t
 
 CPIPE_OK = 0
 CPIPE_REPLY_DEFAULT = 2
@@ -56,6 +60,13 @@ def mh() -> ctypes.CDLL:
         L.mh_registry_count.argtypes = [ctypes.c_char_p]
         L.mh_buffer_new.argtypes = [I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
         L.mh_buffer_new.restype = P
+        L.mh_buffer_new_device.argtypes = [I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, I]
+        L.mh_buffer_new_device.restype = P
+        L.mh_buffer_is_device.argtypes = [P]
+        L.mh_buffer_lock.argtypes = [P, I]
+        L.mh_buffer_lock.restype = P
+        L.mh_buffer_unlock.argtypes = [P]
+        L.mh_offer_hip_buffer.argtypes = [I]
         L.mh_buffer_free.argtypes = [P]
         L.mh_buffer_data.argtypes = [P]
         L.mh_buffer_data.restype = P
@@ -86,15 +97,30 @@ def find_node(node_id: str = NODE_ID, library: str = engine.ENGINE_SO):
     return d
 
 
-class Buffer:
-    """A host image buffer with cpipe metadata (CpuBuffer analogue)."""
+def offer_hip_buffer(on: bool) -> None:
+    """Whether the host offers the "hip_buffer" suite (default on); off, the nodes
+    see a host without it and reach every buffer through lock_cpu."""
+    mh().mh_offer_hip_buffer(1 if on else 0)
 
-    def __init__(self, fmt: int, width: int, height: int, row_bytes: int = 0):
+
+class Buffer:
+    """An image buffer with cpipe metadata: host memory (CpuBuffer analogue) or,
+    with `device` set, HIP device memory on that device ordinal (zero-filled;
+    write/read go through lock_cpu, which stages a host copy)."""
+
+    def __init__(self, fmt: int, width: int, height: int, row_bytes: int = 0, device: int | None = None):
         self.fmt, self.width, self.height = fmt, width, height
-        self.h = mh().mh_buffer_new(fmt, width, height, row_bytes)
+        if device is None:
+            self.h = mh().mh_buffer_new(fmt, width, height, row_bytes)
+        else:
+            self.h = mh().mh_buffer_new_device(fmt, width, height, row_bytes, device)
         if not self.h:
-            raise ValueError("bad buffer layout")
+            raise ValueError(f"bad buffer layout or device allocation failed: {host_log()}")
         self.row_bytes = int(mh().mh_buffer_row_bytes(self.h))
+
+    @property
+    def on_device(self) -> bool:
+        return bool(mh().mh_buffer_is_device(self.h))
 
     def __del__(self):  # pragma: no cover
         try:
@@ -102,21 +128,33 @@ class Buffer:
         except Exception:
             pass
 
-    def _view(self) -> np.ndarray:
+    def _view(self, ptr) -> np.ndarray:
         dtype, per_px = _LAYOUT[self.fmt]
         n = self.row_bytes * self.height // np.dtype(dtype).itemsize
         ctype = np.ctypeslib.as_ctypes_type(dtype)
-        base = np.ctypeslib.as_array(ctypes.cast(mh().mh_buffer_data(self.h), ctypes.POINTER(ctype)),
-                                     shape=(n,))
+        base = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n,))
         rows = base.reshape(self.height, -1)
         return rows[:, : self.width * per_px]
 
+    def _locked(self, access: int):
+        ptr = mh().mh_buffer_lock(self.h, access)
+        if not ptr:
+            raise RuntimeError(f"lock_cpu failed: {host_log()}")
+        return ptr
+
     def write(self, arr: np.ndarray) -> None:
-        v = self._view()
-        v[...] = np.asarray(arr).reshape(v.shape)
+        v = self._view(self._locked(CPU_WRITE))
+        try:
+            v[...] = np.asarray(arr).reshape(v.shape)
+        finally:
+            if mh().mh_buffer_unlock(self.h) != CPIPE_OK:
+                raise RuntimeError(f"unlock_cpu failed: {host_log()}")
 
     def read(self) -> np.ndarray:
-        v = self._view().copy()
+        try:
+            v = self._view(self._locked(CPU_READ)).copy()
+        finally:
+            mh().mh_buffer_unlock(self.h)
         return v.reshape(self.height, self.width, 4) if _LAYOUT[self.fmt][1] == 4 else v
 
     def set_noise_profile(self, pairs) -> None:

@@ -271,6 +271,20 @@ int cpipe_bm3d_host_planes(const float* const* in_planes, size_t in_pitch, float
 int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
                             size_t out_stride_px, int width, int height, float sigma, int mode,
                             int precision, int device, int flags);
+/* The same on device-resident images (the node's path when the host keeps its
+ * buffers in HIP device memory; replaces the same dispatch with the copies and
+ * the lock_cpu round trip, HalideBufferAdapter.cpp:41, taken out). Pointers are
+ * device addresses on `device`, pitches/strides as above; work is ordered on
+ * `stream` (hipStream_t as void*, NULL = null stream) and the call returns once
+ * the output is written and the plan's consistency flag is read. Same plan
+ * cache and lock as the host conveniences; no staging buffers are allocated. */
+int cpipe_bm3d_device_planes(const float* const* in_planes, size_t in_pitch,
+                             float* const* out_planes, size_t out_pitch, int width, int height,
+                             int channels, float sigma, int precision, int device, int flags,
+                             void* stream);
+int cpipe_bm3d_device_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
+                              size_t out_stride_px, int width, int height, float sigma, int mode,
+                              int precision, int device, int flags, void* stream);
 /* Frees every cached plan, buffer and pilot network of the host conveniences. */
 void cpipe_bm3d_host_release(void);
 /* The learned-pilot weight blob embedded in the library (cpipe_amd/weights/pilot_v1.bin). */

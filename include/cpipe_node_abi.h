@@ -187,6 +187,25 @@ typedef struct {
     int (*get_color)(const cpipe_props_t*, const char* key, float rgba[4]);
 } cpipe_param_suite_v1;
 
+/* Device-resident buffers (suite name "hip_buffer", version 1). No counterpart in
+ * the reference's ABI: its device plane keeps images in VulkanBuffer objects
+ * (include/cpipe/runtime/VulkanBuffer.hpp:20-52) whose device handle,
+ * vk_buffer() (:36), only the host sees, and its nodes reach pixels through
+ * lock_cpu (a host mapping; HalideBufferAdapter.cpp:41). A host that keeps
+ * images in HIP device memory offers this suite; a node that finds every
+ * input and output device-resident runs on the device addresses with no
+ * host copy, and falls back to lock_cpu otherwise. Optional: get_suite
+ * returns NULL on hosts without it, and the nodes then behave as before.
+ *   get_device_ptr: CPIPE_OK with the pixel address (row stride from
+ *     get_stride) and the HIP device ordinal when the buffer lives in device
+ *     memory; CPIPE_UNSUPPORTED when it is host memory.
+ *   get_stream: the hipStream_t (as void*, NULL = the null stream) the host
+ *     orders the node's device work on; `compute` is the process context's. */
+typedef struct {
+    int (*get_device_ptr)(const cpipe_buffer_t*, void** out_ptr, int* out_device);
+    int (*get_stream)(cpipe_compute_t* compute, void** out_stream);
+} cpipe_hip_buffer_suite_v1;
+
 struct cpipe_host_s {
     uint32_t abi_major;
     uint32_t abi_minor;
@@ -272,6 +291,7 @@ static_assert(offsetof(cpipe_metadata_builder_suite_v1, add_applied_step) == 48,
 static_assert(sizeof(cpipe_calibration_view) == 232, "calibration view layout");
 static_assert(offsetof(cpipe_calibration_view, get_noise_profile) == 224, "noise profile offset");
 static_assert(sizeof(cpipe_inference_suite_v1) == 8, "inference suite layout");
+static_assert(sizeof(cpipe_hip_buffer_suite_v1) == 16, "hip buffer suite layout");
 #endif
 
 #ifdef __cplusplus

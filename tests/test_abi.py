@@ -227,3 +227,41 @@ def test_engine_argument_checks_without_device():
     assert L.cpipe_bm3d_plan_create(None, ctypes.byref(h)) == host.CPIPE_BAD_INDEX
     assert L.cpipe_bm3d_compat_rgba16f(None, 4, None, 4, 4, 4, ctypes.c_float(0.03), 1, None) == host.CPIPE_BAD_INDEX
     assert b"null" in L.cpipe_bm3d_last_error()
+
+
+def test_device_buffer_path_without_device():
+    """The device-resident path: the engine's device entry points refuse with
+    defined statuses, and the minimal host refuses a device buffer, when no HIP
+    device is visible; host buffers keep the lock_cpu path whether or not the
+    host offers the "hip_buffer" suite."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("this checks the no-device status")
+    L = engine.lib()
+    F = ctypes.c_float
+    assert L.cpipe_bm3d_device_rgba16f(None, 4, None, 4, 4, 4, F(0.03), 1, 0, 0, 0, None) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_device_planes(None, 4, None, 4, 4, 4, 3, F(0.03), 0, 0, 0, None) == host.CPIPE_BAD_INDEX
+    dummy = (ctypes.c_uint16 * 64)()
+    assert L.cpipe_bm3d_device_rgba16f(dummy, 4, dummy, 4, 4, 4, F(0.03), 1, 0, 0, 0, None) == host.CPIPE_UNSUPPORTED
+    with pytest.raises(ValueError):
+        host.Buffer(host.FORMAT_RGBA16F, 4, 4, device=0)
+    d = host.find_node()
+    for offer in (True, False):
+        host.offer_hip_buffer(offer)
+        a, b = _rgba_buffers()
+        assert host.run_node(d, [a], [b]) == host.CPIPE_UNSUPPORTED
+    host.offer_hip_buffer(True)
+
+
+def test_host_buffer_lock_round_trip():
+    """write/read go through lock_cpu/unlock_cpu; a second lock while one is held fails."""
+    import numpy as np
+    b = host.Buffer(host.FORMAT_R32F, 5, 3, row_bytes=32)
+    v = np.arange(15, dtype=np.float32).reshape(3, 5)
+    b.write(v)
+    assert np.array_equal(b.read(), v)
+    assert not b.on_device
+    p = host.mh().mh_buffer_lock(b.h, host.CPU_READ)
+    assert p
+    assert not host.mh().mh_buffer_lock(b.h, host.CPU_READ)
+    assert host.mh().mh_buffer_unlock(b.h) == host.CPIPE_OK
