@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "bm3d_shared.h"
+#include "host_staging.h"
 #include "cpipe_bm3d.h"
 #include "pilot.h"
 
@@ -1501,31 +1502,6 @@ struct HostSlot {
     hipEvent_t ev[3] = {};            // per-plane transfer completion
     uint64_t used = 0;
 };
-
-// Row copy between host buffers on up to 8 threads (the caller's buffers are
-// pageable; the pinned staging lets the DMA engines run at full PCIe rate).
-void par_rows(void* dst, size_t dpitch, const void* src, size_t spitch, size_t row_bytes, int rows) {
-    auto band = [=](int r0, int r1) {
-        if (dpitch == row_bytes && spitch == row_bytes) {
-            std::memcpy(static_cast<char*>(dst) + (size_t)r0 * dpitch,
-                        static_cast<const char*>(src) + (size_t)r0 * spitch, (size_t)(r1 - r0) * row_bytes);
-            return;
-        }
-        for (int r = r0; r < r1; ++r)
-            std::memcpy(static_cast<char*>(dst) + (size_t)r * dpitch,
-                        static_cast<const char*>(src) + (size_t)r * spitch, row_bytes);
-    };
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int nt = (size_t)rows * row_bytes < ((size_t)8 << 20) ? 1 : (int)std::min(8u, hw ? hw : 1u);
-    if (nt <= 1) {
-        band(0, rows);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) th.emplace_back(band, (int)((int64_t)rows * t / nt), (int)((int64_t)rows * (t + 1) / nt));
-    band(0, rows / nt);
-    for (auto& x : th) x.join();
-}
 
 struct HostCache {
     std::mutex mu;

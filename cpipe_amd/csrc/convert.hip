@@ -17,6 +17,7 @@
 #include <string>
 
 #include "cpipe_convert.h"
+#include "host_staging.h"
 
 namespace {
 
@@ -93,11 +94,10 @@ int bytes_per_pixel(int format) {
     }
 }
 
+// Device and pinned staging of the host-memory entry point (grow-only, process-wide).
 struct HostBuffers {
     std::mutex mu;
-    uint8_t* in = nullptr;
-    uint8_t* out = nullptr;
-    size_t in_bytes = 0, out_bytes = 0;
+    cpipe_internal::StagedBuffer in, out;
 };
 
 HostBuffers& host_buffers() {
@@ -158,24 +158,15 @@ extern "C" int cpipe_convert_host(int in_format, const void* in, size_t in_strid
     HostBuffers& hb = host_buffers();
     std::lock_guard<std::mutex> lock(hb.mu);
     const size_t ib = irow * (size_t)height, ob = orow * (size_t)height;
-    if (ib > hb.in_bytes) {
-        if (hb.in) CHIP_CHECK(hipFree(hb.in));
-        hb.in = nullptr;
-        hb.in_bytes = 0;
-        CHIP_CHECK(hipMalloc(&hb.in, ib));
-        hb.in_bytes = ib;
-    }
-    if (ob > hb.out_bytes) {
-        if (hb.out) CHIP_CHECK(hipFree(hb.out));
-        hb.out = nullptr;
-        hb.out_bytes = 0;
-        CHIP_CHECK(hipMalloc(&hb.out, ob));
-        hb.out_bytes = ob;
-    }
-    CHIP_CHECK(hipMemcpy2D(hb.in, irow, in, in_stride_bytes, irow, (size_t)height, hipMemcpyHostToDevice));
-    const int rc = cpipe_convert(in_format, hb.in, irow, out_format, hb.out, orow, width, height, nullptr);
+    CHIP_CHECK(hb.in.reserve(ib));
+    CHIP_CHECK(hb.out.reserve(ob));
+    cpipe_internal::par_rows(hb.in.pin, irow, in, in_stride_bytes, irow, height);
+    CHIP_CHECK(hipMemcpyAsync(hb.in.dev, hb.in.pin, ib, hipMemcpyHostToDevice, nullptr));
+    const int rc = cpipe_convert(in_format, hb.in.dev, irow, out_format, hb.out.dev, orow, width, height, nullptr);
     if (rc != OK) return rc;
-    CHIP_CHECK(hipMemcpy2D(out, out_stride_bytes, hb.out, orow, orow, (size_t)height, hipMemcpyDeviceToHost));
+    CHIP_CHECK(hipMemcpyAsync(hb.out.pin, hb.out.dev, ob, hipMemcpyDeviceToHost, nullptr));
+    CHIP_CHECK(hipStreamSynchronize(nullptr));
+    cpipe_internal::par_rows(out, out_stride_bytes, hb.out.pin, orow, orow, height);
     return OK;
 }
 

@@ -25,6 +25,7 @@
 #include <string>
 
 #include "cpipe_wavelet.h"
+#include "host_staging.h"
 
 namespace {
 
@@ -115,12 +116,10 @@ __global__ __launch_bounds__(WX * WY) void k_wavelet(const uint16_t* __restrict_
     }
 }
 
-// Device buffers of the host-memory entry point (grow-only, process-wide).
+// Device and pinned staging of the host-memory entry point (grow-only, process-wide).
 struct HostBuffers {
     std::mutex mu;
-    uint16_t* in = nullptr;
-    uint16_t* out = nullptr;
-    size_t bytes = 0;
+    cpipe_internal::StagedBuffer in, out;
 };
 
 HostBuffers& host_buffers() {
@@ -161,20 +160,17 @@ extern "C" int cpipe_wavelet_host_rgba16f(const uint16_t* in, size_t in_stride_p
     HostBuffers& hb = host_buffers();
     std::lock_guard<std::mutex> lock(hb.mu);
     const size_t row = (size_t)width * 8, bytes = row * (size_t)height;
-    if (bytes > hb.bytes) {
-        if (hb.in) WHIP_CHECK(hipFree(hb.in));
-        if (hb.out) WHIP_CHECK(hipFree(hb.out));
-        hb.in = hb.out = nullptr;
-        hb.bytes = 0;
-        WHIP_CHECK(hipMalloc(&hb.in, bytes));
-        WHIP_CHECK(hipMalloc(&hb.out, bytes));
-        hb.bytes = bytes;
-    }
-    WHIP_CHECK(hipMemcpy2D(hb.in, row, in, in_stride_px * 8, row, (size_t)height, hipMemcpyHostToDevice));
-    int rc = cpipe_wavelet_rgba16f(hb.in, (size_t)width, hb.out, (size_t)width, width, height,
+    WHIP_CHECK(hb.in.reserve(bytes));
+    WHIP_CHECK(hb.out.reserve(bytes));
+    cpipe_internal::par_rows(hb.in.pin, row, in, in_stride_px * 8, row, height);
+    WHIP_CHECK(hipMemcpyAsync(hb.in.dev, hb.in.pin, bytes, hipMemcpyHostToDevice, nullptr));
+    int rc = cpipe_wavelet_rgba16f(static_cast<const uint16_t*>(hb.in.dev), (size_t)width,
+                                   static_cast<uint16_t*>(hb.out.dev), (size_t)width, width, height,
                                    chroma_strength, nullptr);
     if (rc != OK) return rc;
-    WHIP_CHECK(hipMemcpy2D(out, out_stride_px * 8, hb.out, row, row, (size_t)height, hipMemcpyDeviceToHost));
+    WHIP_CHECK(hipMemcpyAsync(hb.out.pin, hb.out.dev, bytes, hipMemcpyDeviceToHost, nullptr));
+    WHIP_CHECK(hipStreamSynchronize(nullptr));
+    cpipe_internal::par_rows(out, out_stride_px * 8, hb.out.pin, row, row, height);
     return OK;
 }
 

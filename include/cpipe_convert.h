@@ -50,7 +50,9 @@ int cpipe_convert(int in_format, const void* in, size_t in_stride_bytes, int out
                   size_t out_stride_bytes, int width, int height, void* stream);
 
 /* The same on host memory (HIP device 0): copies in, runs, copies out;
- * synchronous. Device buffers are kept between calls (grow-only, process-wide). */
+ * synchronous. Device and pinned host staging buffers are kept between calls
+ * (grow-only, process-wide); rows are copied into the pinned staging on up to
+ * 8 threads, so PCIe runs at DMA rate. */
 int cpipe_convert_host(int in_format, const void* in, size_t in_stride_bytes, int out_format,
                        void* out, size_t out_stride_bytes, int width, int height);
 

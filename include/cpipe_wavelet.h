@@ -41,7 +41,9 @@ int cpipe_wavelet_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out
                           void* stream);
 
 /* The same on host memory (HIP device 0): copies in, runs, copies out;
- * synchronous. Device buffers are kept between calls (grow-only, process-wide). */
+ * synchronous. Device and pinned host staging buffers are kept between calls
+ * (grow-only, process-wide); rows are copied into the pinned staging on up to
+ * 8 threads, so PCIe runs at DMA rate. */
 int cpipe_wavelet_host_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
                                size_t out_stride_px, int width, int height,
                                float chroma_strength);
