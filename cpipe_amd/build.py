@@ -130,7 +130,8 @@ def build(verbose: bool = False) -> dict:
     _run(["g++", *CXX_FLAGS, *HIP_HOST_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "minihost.cpp"), "-ldl",
           *HIP_HOST_LIBS], verbose)
     os.replace(tmp, MINIHOST_SO)
-    return {"engine": ENGINE_SO, "minihost": MINIHOST_SO}
+    from . import schema  # the pipeline-schema node blocks follow the manifests
+    return {"engine": ENGINE_SO, "minihost": MINIHOST_SO, "schema": schema.write()}
 
 
 if __name__ == "__main__":

@@ -1,17 +1,28 @@
-"""Regenerates tests/golden/denoise_bm3d_golden.json.
+"""Regenerates the golden pairs tests/golden/denoise.bm3d/{in,out}.exr and
+tests/golden/denoise.wavelet_bayes_shrink/{in,out}.exr.
 
-The reference's golden pair tests/golden/denoise.bm3d/{in,out}.exr is stored as
-Git LFS pointer stubs, so its pixels are rebuilt here from the formulas of the
-reference's generator tests/fixtures/gen_golden_isp_nodes.cpp:
-  denoise_input()  :619-634  (8x8 ramp + chroma checkerboard + luma pattern)
-  bm3d_mean2x2()   :717-724  (2x2-aligned mean of half-rounded samples)
-  bm3d_output()    :726-740  (RGB = half(max(0, mean)), alpha = half(alpha))
-All arithmetic is float32 like the C++ generator; halves are IEEE RNE.
+The reference's pairs of the same names are stored as Git LFS pointer stubs, so
+their pixels are rebuilt here from the formulas of the reference's generator
+tests/fixtures/gen_golden_isp_nodes.cpp:
+  denoise_sample()               :556-564  (edge-clamped, half-rounded sample)
+  denoise_input()                :619-634  (8x8 ramp + chroma checkerboard + luma pattern)
+  ycocg_at() / soft_threshold()  :656-673  (luma, co, cg; threshold 0.035)
+  wavelet_chroma()               :675-697  (2x2 Haar bands, soft-thresholded, rebuilt)
+  wavelet_bayes_shrink_output()  :699-715  (RGB = half(max(0, .)), alpha = half(alpha))
+  bm3d_mean2x2() / bm3d_output() :717-740  (RGB = half(max(0, 2x2 mean)), alpha = half(alpha))
+and written the way write_pair (:1100-1120) writes them: FLOAT pixels, 4
+channels, in.exr holding the un-rounded float32 input. All arithmetic is float32
+like the C++ generator; halves are IEEE round to nearest even.
+
+Run from the repository root: python tests/golden/make_golden.py
 """
-import json
 import os
+import sys
 
 import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+from cpipe_amd import exr  # noqa: E402
 
 f = np.float32
 
@@ -29,6 +40,51 @@ def denoise_input():
             luma = f(((x * 3 + y * 5) % 5) - 2) * f(0.006)
             img[y, x] = [base + luma + chroma, base + f(0.5) * luma, base + luma - chroma, f(1.0)]
     return img
+
+
+def _sampler(img):
+    h, w = img.shape[:2]
+    return lambda x, y, c: half(img[min(max(y, 0), h - 1), min(max(x, 0), w - 1), c])
+
+
+def _soft(v):
+    t = f(0.035)
+    return f(v - t) if v > t else (f(v + t) if v < -t else f(0.0))
+
+
+def wavelet_output(img):
+    h, w = img.shape[:2]
+    sample = _sampler(img)
+
+    def ycocg(x, y):
+        r, g, b = sample(x, y, 0), sample(x, y, 1), sample(x, y, 2)
+        return [f(f(f(r + f(f(2.0) * g)) + b) * f(0.25)), f(r - b), f(g - f(f(r + b) * f(0.5)))]
+
+    def chroma(x, y, k):
+        bx, by = (x // 2) * 2, (y // 2) * 2
+        p00, p10 = ycocg(bx, by)[k], ycocg(bx + 1, by)[k]
+        p01, p11 = ycocg(bx, by + 1)[k], ycocg(bx + 1, by + 1)[k]
+        ll = f(f(0.25) * f(f(f(p00 + p10) + p01) + p11))
+        hh = _soft(f(f(0.25) * f(f(f(p00 - p10) + p01) - p11)))
+        vv = _soft(f(f(0.25) * f(f(f(p00 + p10) - p01) - p11)))
+        dd = _soft(f(f(0.25) * f(f(f(p00 - p10) - p01) + p11)))
+        if x % 2 == 0 and y % 2 == 0:
+            return f(f(f(ll + hh) + vv) + dd)
+        if x % 2 == 1 and y % 2 == 0:
+            return f(f(f(ll - hh) + vv) - dd)
+        if x % 2 == 0:
+            return f(f(f(ll + hh) - vv) - dd)
+        return f(f(f(ll - hh) - vv) + dd)
+
+    out = np.zeros_like(img)
+    for y in range(h):
+        for x in range(w):
+            luma = ycocg(x, y)[0]
+            co, cg = chroma(x, y, 1), chroma(x, y, 2)
+            t = f(luma - f(f(0.5) * cg))
+            out[y, x] = [half(max(f(0.0), f(t + f(f(0.5) * co)))), half(max(f(0.0), f(t + cg))),
+                         half(max(f(0.0), f(t - f(f(0.5) * co)))), half(sample(x, y, 3))]
+    return out
 
 
 def bm3d_output(img):
@@ -50,16 +106,13 @@ def bm3d_output(img):
 
 def main():
     inp = denoise_input()
-    in_bits = inp.astype(np.float16).view(np.uint16)
-    out = bm3d_output(inp.astype(np.float16).astype(np.float32))
-    out_bits = out.astype(np.float16).view(np.uint16)
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "denoise_bm3d_golden.json")
-    with open(path, "w") as fh:
-        json.dump({"source": "tests/fixtures/gen_golden_isp_nodes.cpp:619-634,717-740",
-                   "width": 8, "height": 8, "format": "R16G16B16A16_SFLOAT",
-                   "in_f16_bits": in_bits.ravel().tolist(),
-                   "out_f16_bits": out_bits.ravel().tolist()}, fh)
-    print("wrote", path, "pixel(0,0) =", out[0, 0])
+    root = os.path.dirname(os.path.abspath(__file__))
+    for name, out in (("denoise.bm3d", bm3d_output(inp)), ("denoise.wavelet_bayes_shrink", wavelet_output(inp))):
+        d = os.path.join(root, name)
+        os.makedirs(d, exist_ok=True)
+        exr.write(os.path.join(d, "in.exr"), inp, "float")
+        exr.write(os.path.join(d, "out.exr"), out, "float")
+        print("wrote", d, "out pixel(0,0) =", out[0, 0])
 
 
 if __name__ == "__main__":

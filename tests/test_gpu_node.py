@@ -5,13 +5,12 @@ tests/unit/test_node_denoise_bm3d_params.cpp:13-32, plus the golden gate of
 tests/unit/test_golden_isp_nodes.cpp:628-647, with the minimal host
 (cpipe_amd/csrc/minihost.cpp) in place of the reference HostContext.
 """
-import json
 import os
 
 import numpy as np
 import pytest
 
-from cpipe_amd import host, synth
+from cpipe_amd import exr, host, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -89,16 +88,17 @@ def test_reference_params_test(cuda):
 
 
 def test_golden_gate(cuda, oracle_lib):
-    with open(os.path.join(GOLDEN, "denoise_bm3d_golden.json")) as f:
-        g = json.load(f)
+    """assert_denoise_golden (test_golden_isp_nodes.cpp:628-647,788) on the
+    regenerated pair tests/golden/denoise.bm3d/{in,out}.exr."""
     d = host.find_node()
     inp = host.Buffer(host.FORMAT_RGBA16F, 8, 8)
-    inp.write(np.array(g["in_f16_bits"], np.uint16).reshape(8, 8, 4))
+    inp.write(_f16(exr.read(os.path.join(GOLDEN, "denoise.bm3d", "in.exr"))))
     inp.set_cs_role("scene_linear_rec2020")
+    inp.add_step("color_matrix")
     out = host.Buffer(host.FORMAT_RGBA16F, 8, 8)
     assert host.run_node(d, [inp], [out]) == host.CPIPE_OK
     got = out.read()
-    want = np.array(g["out_f16_bits"], np.uint16).reshape(8, 8, 4)
+    want = _f16(exr.read(os.path.join(GOLDEN, "denoise.bm3d", "out.exr")))
     np.testing.assert_array_equal(got, want)
     assert synth.psnr(_from_f16(got)[..., :3], _from_f16(want)[..., :3]) >= 40.0
 

@@ -9,7 +9,9 @@ ABI with the minimal host; the parameter clamp of wavelet_bayes_shrink.cpp:37-38
 import numpy as np
 import pytest
 
-from cpipe_amd import engine, host, synth
+import os
+
+from cpipe_amd import engine, exr, host, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +118,19 @@ def test_node_param_clamp_and_row_padding(cuda, oracle_lib):
         want = oracle_lib.wavelet_rgba16f(bits, eff)
         assert np.array_equal(out.read(), want), params
         assert out.applied_steps == ["color_matrix", "denoise.wavelet_bayes_shrink"]
+
+
+def test_golden_gate(cuda, oracle_lib):
+    """assert_denoise_golden for this node (test_golden_isp_nodes.cpp:628-647,795-796)
+    on the regenerated pair tests/golden/denoise.wavelet_bayes_shrink/{in,out}.exr:
+    PSNR >= 40 dB is the reference's bar; the output here is bit-identical."""
+    gdir = os.path.join(os.path.dirname(__file__), "golden", "denoise.wavelet_bayes_shrink")
+    inp = _node_input(exr.read(os.path.join(gdir, "in.exr")), steps=("denoise.bm3d",))
+    out = host.Buffer(host.FORMAT_RGBA16F, 8, 8)
+    assert host.run_node(host.find_node(NODE), [inp], [out], {}) == host.CPIPE_OK, host.host_log()
+    want = _f16(exr.read(os.path.join(gdir, "out.exr")))
+    np.testing.assert_array_equal(out.read(), want)
+    assert synth.psnr(_from_f16(out.read())[..., :3], _from_f16(want)[..., :3]) >= 40.0
 
 
 def test_host_entry_point(cuda, oracle_lib):

@@ -5,13 +5,12 @@ reference's own tests, library routines that compute the same thing
 independently (numpy's Kaiser window and float16 casts), closed forms
 (orthonormality), brute force on tiny inputs, and special cases.
 """
-import json
 import os
 
 import numpy as np
 import pytest
 
-from cpipe_amd import synth
+from cpipe_amd import exr, synth
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -91,11 +90,10 @@ def test_compat_params_known_answer(oracle_lib):
 
 
 def test_compat_golden_fixture(oracle_lib):
-    """Regenerated golden pair of tests/golden/denoise.bm3d (the reference's EXRs are LFS stubs)."""
-    with open(os.path.join(GOLDEN, "denoise_bm3d_golden.json")) as f:
-        g = json.load(f)
-    inp = np.array(g["in_f16_bits"], np.uint16).reshape(8, 8, 4)
-    want = np.array(g["out_f16_bits"], np.uint16).reshape(8, 8, 4)
+    """Regenerated golden pair tests/golden/denoise.bm3d/{in,out}.exr (the reference's
+    EXRs are LFS stubs; FLOAT pixels, written into an RGBA16F buffer as the gate does)."""
+    inp = _f16(exr.read(os.path.join(GOLDEN, "denoise.bm3d", "in.exr")))
+    want = _f16(exr.read(os.path.join(GOLDEN, "denoise.bm3d", "out.exr")))
     got = oracle_lib.compat_rgba16f(inp, 0.03)
     np.testing.assert_array_equal(got, want)
     px = _from_f16(got[0, 0])

@@ -5,9 +5,18 @@ src/cpipe/nodes/denoise/wavelet_bayes_shrink_generator.cpp:9-31,57-86. Each
 check is one that a plausible slip (a wrong band sign, hard instead of soft
 thresholding, a wrong threshold constant, a different edge rule) fails: the
 reference's own unit test, algebraic identities of the YCoCg-style transform
-and the Haar pair, and the repeat-edge rule.
+and the Haar pair, the repeat-edge rule, and the reference's golden pair as its
+fixture generator computes it (tests/fixtures/gen_golden_isp_nodes.cpp:656-715,
+an independent scalar restatement of the same node, regenerated in
+tests/golden/denoise.wavelet_bayes_shrink).
 """
+import os
+
 import numpy as np
+
+from cpipe_amd import exr
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "denoise.wavelet_bayes_shrink")
 
 F16_EPS = 2.0 ** -10  # binary16 relative spacing
 
@@ -126,3 +135,16 @@ def test_negative_results_clamp_to_positive_zero(oracle_lib):
     out = oracle_lib.wavelet_rgba16f(_f16(img), 0.0)
     assert np.all(out != 0x8000)
     assert np.all(_from_f16(out) >= 0.0)
+
+
+def test_golden_pair_bit_exact(oracle_lib):
+    """The oracle on the golden input equals the golden generator's output bit
+    for bit (the reference's gate asks PSNR >= 40 dB,
+    tests/unit/test_golden_isp_nodes.cpp:72,795-796)."""
+    inp = _f16(exr.read(os.path.join(GOLDEN, "in.exr")))
+    want = _f16(exr.read(os.path.join(GOLDEN, "out.exr")))
+    got = oracle_lib.wavelet_rgba16f(inp, 1.0)
+    np.testing.assert_array_equal(got, want)
+    # the generator's chroma pattern is attenuated, luma kept
+    a, b = _from_f16(inp), _from_f16(got)
+    assert np.ptp(b[..., 0] - b[..., 2]) < np.ptp(a[..., 0] - a[..., 2])
