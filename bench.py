@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle crop check")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-frame batch leg")
+    ap.add_argument("--no-noise-map", action="store_true",
+                    help="skip the noise-profile sigma map leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) and node-chain legs")
     ap.add_argument("--no-wavelet", action="store_true",
                     help="skip the downstream wavelet BayesShrink and precision-convert node legs")
@@ -451,6 +453,43 @@ def main():
         forms[name] = entry
         print(f"# {name}: {json.dumps(entry)}", file=sys.stderr, flush=True)
 
+    # ---- noise-profile sigma map: the workload's own profile (read 0.002, gain 0.01)
+    noise_map = None
+    if not args.no_noise_map and args.workload.startswith(("rgb", "gray")):
+        beta, lam = 0.002 ** 2, 0.01
+        if args.workload.startswith("gray") or args.workload == "gray512":
+            beta, lam = 0.05 ** 2, 0.0  # Gaussian workloads: a signal-independent profile
+        noise_map = {"profile": {"beta": beta, "lambda": lam}}
+        for prec in ("fp32", "fp16"):
+            mplan = engine.Plan(w, h, c, prec, 0)
+            mplan.set_noise_profile(beta, lam)
+            mdst = torch.empty_like(src)
+            mms, mstage = time_form(torch, mplan, src, mdst, sigma, args.steps, args.warmup, stream)
+            mout = mdst.cpu().numpy()
+            scalar_psnr = psnr_out if prec == "fp32" else forms.get("fp16", {}).get("psnr_out_db")
+            entry = {"value": round(w * h / (mms / 1e3) / 1e6, 3), "unit": "Mpixel/s",
+                     "ms_per_step": round(mms, 3), "psnr_out_db": round(synth.psnr(mout, clean), 3),
+                     "psnr_scalar_sigma_db": None if scalar_psnr is None else round(scalar_psnr, 3),
+                     "stage_ms": {k: round(v, 4) for k, v in mstage.items()}}
+            if prec == "fp32" and not args.no_cpu:
+                from oracle import oracle
+                cw, ch = min(512, w), min(384, h)
+                cn = np.ascontiguousarray(noisy[:, :ch, :cw])
+                csrc = torch.from_numpy(cn).to(dev)
+                cplan = engine.Plan(cw, ch, c, "fp32", 0)
+                cplan.set_noise_profile(beta, lam)
+                cdst = torch.empty_like(csrc)
+                cplan.run(csrc, cdst, sigma)
+                torch.cuda.synchronize()
+                cplan.check()
+                want = oracle.bm3d_f32_profile(cn, beta, lam, nthreads=min(16, os.cpu_count() or 1))
+                entry["crop_bit_exact_vs_oracle"] = bool(np.array_equal(cdst.cpu().numpy().view(np.uint32),
+                                                                        want.view(np.uint32)))
+                cplan.close()
+            mplan.close()
+            noise_map[prec] = entry
+        print(f"# noise map: {json.dumps(noise_map)}", file=sys.stderr, flush=True)
+
     # ---- FP64 oracle on a crop: delta PSNR of every form
     fp64 = None
     if not args.no_fp64:
@@ -526,6 +565,7 @@ def main():
         "batch12": batch,
         "host_path": host_path,
         "node_chain": node_chain,
+        "noise_map": noise_map,
         "wavelet": wavelet,
         "precision_convert": convert,
     }

@@ -132,6 +132,35 @@ __global__ void k_colour_in(const FrameTab* __restrict__ tab, const FrameRect* _
     }
 }
 
+// Noise-profile sigma map of frame 0 (one thread per reference position):
+// x = mean of the reference's 8x8 input patch per channel (row-major FP32 sum
+// times 1/64), v = max(0, beta + lambda x); opponent channel k gets
+// sqrt(sum_c A_kc^2 v_c) for the opponent rows A = (1/3, 1/3, 1/3),
+// (1/2, 0, -1/2), (1/4, -1/2, 1/4); grey planes sqrt(v).
+__global__ void k_sigma_map(const FrameTab* __restrict__ tab, float* __restrict__ smap) {
+    const FrameTab& F = tab[0];
+    const int bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y;
+    if (bx >= F.nx || by >= F.ny) return;
+    const int x0 = ref_pos(bx, F.w), y0 = ref_pos(by, F.h);
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < F.c; ++c) {
+        const float* __restrict__ src = F.in[c] + (size_t)y0 * F.in_pitch + x0;
+        float sum = 0.0f;
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j < K; ++j) sum = sum + src[(size_t)i * F.in_pitch + j];
+        const float t = F.nbeta[c] + F.nlambda[c] * (sum * 0.015625f);
+        v[c] = t > 0.0f ? t : 0.0f;
+    }
+    const size_t nref = (size_t)F.nx * F.ny, r = (size_t)by * F.nx + bx;
+    if (F.c == 1) {
+        smap[r] = sqrtf(v[0]);
+    } else {
+        smap[r] = sqrtf(((v[0] + v[1]) + v[2]) * (float)(1.0 / 9.0));
+        smap[nref + r] = sqrtf((v[0] + v[2]) * 0.25f);
+        smap[2 * nref + r] = sqrtf(((v[0] + v[2]) * 0.0625f) + (v[1] * 0.25f));
+    }
+}
+
 // Frames smaller than one patch: output = input (blockIdx.z = frame rectangle).
 __global__ void k_copy_through(const FrameTab* __restrict__ tab, const FrameRect* __restrict__ rects) {
     const FrameRect rc = rects[blockIdx.z];
@@ -691,7 +720,7 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
 }
 
 // Tiles of 16 x TY refs, NW waves per workgroup (one group per wave per round).
-template <int STEP, int TY, int NW>
+template <int STEP, int TY, int NW, bool MAP>
 __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     constexpr int DOFS = FOOT * foot_edge(TY);
     __shared__ float2 acc[DOFS];  // (num, den) per footprint pixel
@@ -723,6 +752,9 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     const float sc = F.sig[c];
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
+    // noise-profile map (MAP): each group shrinks with its reference patch's sigma
+    // and weighs its aggregation by 1/sigma^2 (which a single sigma cancels)
+    const float* __restrict__ smap = MAP ? F.smap + (size_t)c * p.nx * p.ny : nullptr;
     const int w = p.w;
     const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
@@ -747,13 +779,20 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
                 mp[4 * q + 2] = __builtin_amdgcn_readfirstlane(t.z);
                 mp[4 * q + 3] = __builtin_amdgcn_readfirstlane(t.w);
             }
-            switch (n) {
-                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
-                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
-                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
-                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
-                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, thr, s2); break;
+            float gthr = thr, gs2 = s2;
+            if (MAP) {
+                const float sg = smap[ref];
+                gthr = 2.7f * sg;
+                gs2 = sg * sg;
             }
+            switch (n) {
+                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
+                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
+                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
+                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
+                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
+            }
+            if (MAP) wgt = wgt * (1.0f / (gs2 > 1e-30f ? gs2 : 1.0f));
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
         timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
@@ -1057,6 +1096,10 @@ struct cpipe_bm3d_plan_s {
     FrameRect* small_rects = nullptr;  // frames smaller than a patch (copied through)
     int n_small = 0, small_w = 0, small_h = 0;
     float* rgb = nullptr;            // 6 planes for the RGBA16F path (single-frame plans)
+    // noise-profile sigma map (single-frame plans): per (channel, ref) sigma
+    bool profile = false;
+    float pbeta[3] = {0.0f, 0.0f, 0.0f}, plambda[3] = {0.0f, 0.0f, 0.0f};
+    float* smap = nullptr;
     bool ran = false;
     // timing (eager runs)
     bool timing = false;
@@ -1136,6 +1179,7 @@ void free_plan(cpipe_bm3d_plan* p) {
     (void)hipFree(p->gates);
     (void)hipFree(p->err);
     (void)hipFree(p->rgb);
+    (void)hipFree(p->smap);
     (void)hipFree(p->report);
     (void)hipFree(p->chk_part);
     if (p->side) (void)hipStreamDestroy(p->side);
@@ -1409,6 +1453,11 @@ void set_run(cpipe_bm3d_plan* p, int f, const float* const* in, size_t in_pitch,
     }
     T.in_pitch = in_pitch;
     T.out_pitch = out_pitch;
+    T.smap = p->profile ? p->smap : nullptr;
+    for (int ch = 0; ch < 3; ++ch) {
+        T.nbeta[ch] = p->pbeta[ch < p->c ? ch : 0];
+        T.nlambda[ch] = p->plambda[ch < p->c ? ch : 0];
+    }
 }
 
 // Copies the frame table into the pinned staging buffer once the previous
@@ -1463,14 +1512,21 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
     if (L.n_grp > 0) {
         const GroupLaunch gl{p->tab_dev, L.grp, use_gate, p->err};
         const dim3 ggrid(L.n_grp, p->c);
+        const bool map = p->profile && p->smap;  // kernels built with the per-ref sigma read
         if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
-            HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, gl, ggrid, r.st));
+            HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, map, gl, ggrid, r.st));
         } else if (p->tile_y != TILE / 2) {
             return fail(INTERNAL, "exact FP32 plans run 16 x 8-ref tiles only");
         } else if (step == 1) {
-            k_group<1, TILE / 2, 8><<<ggrid, 512, 0, r.st>>>(gl);
+            if (map)
+                k_group<1, TILE / 2, 8, true><<<ggrid, 512, 0, r.st>>>(gl);
+            else
+                k_group<1, TILE / 2, 8, false><<<ggrid, 512, 0, r.st>>>(gl);
         } else {
-            k_group<2, TILE / 2, 8><<<ggrid, 512, 0, r.st>>>(gl);
+            if (map)
+                k_group<2, TILE / 2, 8, true><<<ggrid, 512, 0, r.st>>>(gl);
+            else
+                k_group<2, TILE / 2, 8, false><<<ggrid, 512, 0, r.st>>>(gl);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -1565,6 +1621,24 @@ int slot_staging(HostSlot& sl, size_t bytes) {
     return OK;
 }
 
+// The calling thread's noise profile for CPIPE_BM3D_HOST_PROFILE calls.
+thread_local float g_prof_beta[3] = {0.0f, 0.0f, 0.0f}, g_prof_lambda[3] = {0.0f, 0.0f, 0.0f};
+thread_local int g_prof_n = 0;
+
+// Puts the thread's noise profile on the slot's plan (one pair per channel, or
+// one pair for every channel), or leaves the plan on the scalar sigma.
+int apply_profile(HostSlot* sl, int flags) {
+    if (!(flags & CPIPE_BM3D_HOST_PROFILE)) return OK;
+    if (g_prof_n != 1 && g_prof_n != sl->c)
+        return fail(BAD_INDEX, "CPIPE_BM3D_HOST_PROFILE: set one (beta, lambda) pair or one per channel first");
+    float b[3], l[3];
+    for (int c = 0; c < sl->c; ++c) {
+        b[c] = g_prof_beta[g_prof_n == 1 ? 0 : c];
+        l[c] = g_prof_lambda[g_prof_n == 1 ? 0 : c];
+    }
+    return cpipe_bm3d_plan_set_noise_profile(sl->plan, b, l, sl->c);
+}
+
 // Finds or creates the slot for this call's shape (caller holds hc.mu).
 int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int device, int flags,
               size_t dev_bytes, HostSlot** out) {
@@ -1573,6 +1647,8 @@ int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int d
     if (device < 0 || device >= ndev) return fail(BAD_INDEX, "device ordinal out of range");
     if ((flags & CPIPE_BM3D_HOST_PILOT) && c != 3)
         return fail(UNSUPPORTED, "the learned pilot is trained for RGB (3-plane) images");
+    if ((flags & CPIPE_BM3D_HOST_PILOT) && (flags & CPIPE_BM3D_HOST_PROFILE))
+        return fail(UNSUPPORTED, "the learned pilot takes a scalar noise level");
     if (kind == HOST_COMPAT) precision = 0, flags = 0;
     for (HostSlot& sl : hc.slots) {
         if (sl.kind == kind && sl.w == w && sl.h == h && sl.c == c && sl.precision == precision &&
@@ -1643,6 +1719,11 @@ int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
     {
         const dim3 g((p->full.rect_w + 255) / 256, p->full.rect_h, p->full.n_rects);
         k_colour_in<<<g, 256, 0, st>>>(p->tab_dev, p->full.rects);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (p->profile && p->smap) {  // single-frame plans only
+        const auto& F0 = p->fr[0];
+        k_sigma_map<<<dim3((unsigned)((F0.nx + 63) / 64), (unsigned)F0.ny), 64, 0, st>>>(p->tab_dev, p->smap);
         HIP_CHECK(hipGetLastError());
     }
     if ((rc = r.mark(CPIPE_BM3D_T_COLOR, true))) return rc;
@@ -2044,6 +2125,36 @@ int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* b, int* tile_y, int* order, 
     return OK;
 }
 
+int cpipe_bm3d_plan_set_noise_profile(cpipe_bm3d_plan* p, const float* beta, const float* lambda, int n) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    if (n != 0 && n != p->c) return fail(BAD_INDEX, "a noise profile needs one (beta, lambda) pair per channel");
+    if (n != 0 && (!beta || !lambda)) return fail(BAD_INDEX, "null noise profile");
+    for (int c = 0; c < n; ++c)
+        if (!std::isfinite(beta[c]) || !std::isfinite(lambda[c]))
+            return fail(BAD_INDEX, "noise profile values must be finite");
+    if (n != 0 && p->fr.size() != 1) return fail(UNSUPPORTED, "noise-profile maps run on single-frame plans");
+    if (n != 0 && p->pilot) return fail(UNSUPPORTED, "the learned pilot takes a scalar noise level");
+    DeviceGuard guard(p->device);
+    if ((n != 0) != p->profile) {
+        HIP_CHECK(hipDeviceSynchronize());
+        drop_graph(p);
+    }
+    const auto& F0 = p->fr[0];
+    if (n != 0 && !p->smap && !F0.small) {
+        const size_t bytes = sizeof(float) * (size_t)p->c * F0.nx * F0.ny;
+        if (hipMalloc(&p->smap, bytes) != hipSuccess) {
+            p->smap = nullptr;
+            return fail(OOM, "noise-profile map");
+        }
+    }
+    p->profile = n != 0;
+    for (int c = 0; c < 3; ++c) {
+        p->pbeta[c] = c < n ? beta[c] : 0.0f;
+        p->plambda[c] = c < n ? lambda[c] : 0.0f;
+    }
+    return OK;
+}
+
 int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* p, int mfma) {
     if (!p) return fail(BAD_INDEX, "null plan");
     if (mfma && p->precision == CPIPE_BM3D_PRECISION_FP32)
@@ -2145,6 +2256,7 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* p, const float* const* in, s
 int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
                               const cpipe_bm3d_pilot_opts* opts) {
     if (!p) return fail(BAD_INDEX, "null plan");
+    if (net && p->profile) return fail(UNSUPPORTED, "the learned pilot takes a scalar noise level");
     DeviceGuard guard(p->device);
     HIP_CHECK(hipDeviceSynchronize());
     drop_graph(p);
@@ -2288,7 +2400,7 @@ int cpipe_bm3d_host_planes(const float* const* in, size_t in_pitch, float* const
     const size_t npx = (size_t)w * h, pb = npx * sizeof(float), row = (size_t)w * sizeof(float);
     HostSlot* sl = nullptr;
     int rc = host_slot(hc, HOST_PLANES, w, h, channels, precision, device, flags, pb * 2 * channels, &sl);
-    if (rc) return rc;
+    if (rc || (rc = apply_profile(sl, flags))) return rc;
     DeviceGuard guard(device);
     float* dev = static_cast<float*>(sl->dev);
     float* pin = static_cast<float*>(sl->pin);
@@ -2320,15 +2432,15 @@ int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_
                             float sigma, int mode, int precision, int device, int flags) {
     if (!in || !out) return fail(BAD_INDEX, "null argument");
     if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
-    if (mode == 0 && (flags & CPIPE_BM3D_HOST_PILOT))
-        return fail(BAD_INDEX, "the learned pilot needs mode bm3d");
+    if (mode == 0 && (flags & (CPIPE_BM3D_HOST_PILOT | CPIPE_BM3D_HOST_PROFILE)))
+        return fail(BAD_INDEX, "the learned pilot and the noise-profile map need mode bm3d");
     HostCache& hc = host_cache();
     std::lock_guard<std::mutex> lock(hc.mu);
     const size_t bytes = (size_t)w * h * 8;
     HostSlot* sl = nullptr;
     int rc = host_slot(hc, mode == 0 ? HOST_COMPAT : HOST_RGBA, w, h, 3, precision, device, flags,
                        2 * bytes, &sl);
-    if (rc) return rc;
+    if (rc || (rc = apply_profile(sl, flags))) return rc;
     DeviceGuard guard(device);
     uint16_t* din = static_cast<uint16_t*>(sl->dev);
     uint16_t* dout = din + (size_t)w * h * 4;
@@ -2359,7 +2471,7 @@ int cpipe_bm3d_device_planes(const float* const* in, size_t in_pitch, float* con
     std::lock_guard<std::mutex> lock(hc.mu);
     HostSlot* sl = nullptr;
     int rc = host_slot(hc, HOST_PLANES, w, h, channels, precision, device, flags, 0, &sl);
-    if (rc) return rc;
+    if (rc || (rc = apply_profile(sl, flags))) return rc;
     DeviceGuard guard(device);
     if ((rc = cpipe_bm3d_plan_run(sl->plan, in, in_pitch, out, out_pitch, sigma, stream))) return rc;
     HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
@@ -2370,19 +2482,32 @@ int cpipe_bm3d_device_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, siz
                               float sigma, int mode, int precision, int device, int flags, void* stream) {
     if (!in || !out) return fail(BAD_INDEX, "null argument");
     if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
-    if (mode == 0 && (flags & CPIPE_BM3D_HOST_PILOT))
-        return fail(BAD_INDEX, "the learned pilot needs mode bm3d");
+    if (mode == 0 && (flags & (CPIPE_BM3D_HOST_PILOT | CPIPE_BM3D_HOST_PROFILE)))
+        return fail(BAD_INDEX, "the learned pilot and the noise-profile map need mode bm3d");
     HostCache& hc = host_cache();
     std::lock_guard<std::mutex> lock(hc.mu);
     HostSlot* sl = nullptr;
     int rc = host_slot(hc, mode == 0 ? HOST_COMPAT : HOST_RGBA, w, h, 3, precision, device, flags, 0, &sl);
-    if (rc) return rc;
+    if (rc || (rc = apply_profile(sl, flags))) return rc;
     DeviceGuard guard(device);
     rc = mode == 0 ? cpipe_bm3d_compat_rgba16f(in, isp, out, osp, w, h, sigma, 1, stream)
                    : cpipe_bm3d_plan_run_rgba16f(sl->plan, in, isp, out, osp, sigma, stream);
     if (rc) return rc;
     HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
     return mode == 0 ? OK : cpipe_bm3d_plan_check(sl->plan);
+}
+
+int cpipe_bm3d_host_noise_profile(const float* beta, const float* lambda, int n) {
+    if (n < 0 || n > 3) return fail(BAD_INDEX, "noise profile: 0 to 3 (beta, lambda) pairs");
+    if (n > 0 && (!beta || !lambda)) return fail(BAD_INDEX, "null noise profile");
+    for (int c = 0; c < n; ++c) {
+        if (!std::isfinite(beta[c]) || !std::isfinite(lambda[c]))
+            return fail(BAD_INDEX, "noise profile values must be finite");
+        g_prof_beta[c] = beta[c];
+        g_prof_lambda[c] = lambda[c];
+    }
+    g_prof_n = n;
+    return OK;
 }
 
 void cpipe_bm3d_host_release(void) {

@@ -46,6 +46,9 @@ struct FrameTab {
     // per run
     float sig[3];           // per-channel sigma (opponent: sigma * |row|)
     float sigma;
+    const float* smap;      // noise-profile map: per (channel, ref) sigma, or null (scalar sig[])
+    float nbeta[3];         // noise profile of the input channels: variance nbeta + nlambda * x
+    float nlambda[3];
     const float* in[3];     // caller's input planes
     float* out[3];          // caller's output planes
     size_t in_pitch, out_pitch;
@@ -242,10 +245,11 @@ __device__ __forceinline__ int floor_pow2(int c) {
 // 8x8 DCT-II matrix C[u][i] and the 8x8 Kaiser window (row-major, FP32) to the
 // current device, once per device; group_mx_launch enqueues the step-1 or
 // step-2 group kernel with FP16 or BF16 MFMA transforms on the launch grid
-// (items, channels) for tiles of 16 x tile_y refs (tile_y = 8 or 16).
+// (items, channels) for tiles of 16 x tile_y refs (tile_y = 8 or 16), with the
+// per-ref sigma of the frame's noise-profile map when `map` is set.
 // Both return 0 or a hipError_t.
 int group_mx_tables(int device, const float* dct64, const float* kaiser64);
-int group_mx_launch(int step, int precision, int tile_y, const GroupLaunch& L, dim3 grid,
+int group_mx_launch(int step, int precision, int tile_y, bool map, const GroupLaunch& L, dim3 grid,
                     hipStream_t st);
 
 // Mixed-precision block matching (bm_mx.hip): FP16 or BF16 MFMA cross terms,

@@ -225,7 +225,7 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm,
 
 // One workgroup per (tile of 16 x TY refs, channel); 16 x 8 tiles keep the LDS
 // footprint small enough for three workgroups per CU.
-template <int STEP, int PREC, int TY>
+template <int STEP, int PREC, int TY, bool MAP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_group_mx(GroupLaunch L) {
     typedef Mx<PREC> M;
@@ -282,6 +282,9 @@ void k_group_mx(GroupLaunch L) {
     const float sc = F.sig[c];
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
+    // noise-profile map (MAP): each group shrinks with its reference patch's sigma
+    // and weighs its aggregation by 1/sigma^2 (which a single sigma cancels)
+    const float* __restrict__ smap = MAP ? F.smap + (size_t)c * p.nx * p.ny : nullptr;
     const int w = p.w;
     const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
@@ -300,7 +303,14 @@ void k_group_mx(GroupLaunch L) {
             const uint32_t pm = p.pos[ref * NMAX + r];  // this lane's member (rows of X)
 #pragma unroll
             for (int m = 0; m < NMAX; ++m) mp[m] = __builtin_amdgcn_readlane(pm, m);
-            wgt = filter_mx<STEP, PREC>(v, n, pm, r, q, lane, src, bsrc, w, sb, kf, ki, thr, s2);
+            float gthr = thr, gs2 = s2;
+            if (MAP) {
+                const float sg = smap[ref];
+                gthr = 2.7f * sg;
+                gs2 = sg * sg;
+            }
+            wgt = filter_mx<STEP, PREC>(v, n, pm, r, q, lane, src, bsrc, w, sb, kf, ki, gthr, gs2);
+            if (MAP) wgt = wgt * (1.0f / (gs2 > 1e-30f ? gs2 : 1.0f));
         }
         const int lo = (li - fy0) * fw + (lj - fx0);
         timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
@@ -344,28 +354,32 @@ int group_mx_tables(int device, const float* dct64, const float* kaiser64) {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_mx_haar), haar, sizeof(haar));
 }
 
-template <int TY>
+template <int TY, bool MAP>
 void launch_ty(int step, bool bf, const GroupLaunch& p, dim3 grid, hipStream_t st) {
     if (step == 1) {
         if (bf)
-            k_group_mx<1, CPIPE_BM3D_PRECISION_BF16, TY><<<grid, 256, 0, st>>>(p);
+            k_group_mx<1, CPIPE_BM3D_PRECISION_BF16, TY, MAP><<<grid, 256, 0, st>>>(p);
         else
-            k_group_mx<1, CPIPE_BM3D_PRECISION_FP16, TY><<<grid, 256, 0, st>>>(p);
+            k_group_mx<1, CPIPE_BM3D_PRECISION_FP16, TY, MAP><<<grid, 256, 0, st>>>(p);
     } else {
         if (bf)
-            k_group_mx<2, CPIPE_BM3D_PRECISION_BF16, TY><<<grid, 256, 0, st>>>(p);
+            k_group_mx<2, CPIPE_BM3D_PRECISION_BF16, TY, MAP><<<grid, 256, 0, st>>>(p);
         else
-            k_group_mx<2, CPIPE_BM3D_PRECISION_FP16, TY><<<grid, 256, 0, st>>>(p);
+            k_group_mx<2, CPIPE_BM3D_PRECISION_FP16, TY, MAP><<<grid, 256, 0, st>>>(p);
     }
 }
 
-int group_mx_launch(int step, int precision, int tile_y, const GroupLaunch& p, dim3 grid,
+int group_mx_launch(int step, int precision, int tile_y, bool map, const GroupLaunch& p, dim3 grid,
                     hipStream_t st) {
     const bool bf = precision == CPIPE_BM3D_PRECISION_BF16;
-    if (tile_y == TILE / 2)
-        launch_ty<TILE / 2>(step, bf, p, grid, st);
+    if (tile_y == TILE / 2 && map)
+        launch_ty<TILE / 2, true>(step, bf, p, grid, st);
+    else if (tile_y == TILE / 2)
+        launch_ty<TILE / 2, false>(step, bf, p, grid, st);
+    else if (tile_y == TILE && map)
+        launch_ty<TILE, true>(step, bf, p, grid, st);
     else if (tile_y == TILE)
-        launch_ty<TILE>(step, bf, p, grid, st);
+        launch_ty<TILE, false>(step, bf, p, grid, st);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

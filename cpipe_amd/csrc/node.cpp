@@ -116,6 +116,31 @@ struct Bm3dNode {
     int unused = 0;
 };
 
+// The input's noise-profile pairs (variance a + b x per channel), false when
+// it carries none (the metadata path of bm3d.cpp:29-52).
+bool noise_profile(const Suites& s, const cpipe_buffer_t* input, std::vector<float>* a, std::vector<float>* b) {
+    if (s.buffer == nullptr || s.buffer->get_metadata == nullptr || s.metadata == nullptr ||
+        s.metadata->get_calibration == nullptr)
+        return false;
+    const cpipe_metadata_t* meta = nullptr;
+    if (s.buffer->get_metadata(input, &meta) != CPIPE_OK || meta == nullptr) return false;
+    cpipe_calibration_view view{};
+    if (s.metadata->get_calibration(meta, &view) != CPIPE_OK || view.get_noise_profile == nullptr) return false;
+    size_t total = 0;
+    if (view.get_noise_profile(meta, 0, &total, nullptr, nullptr) != CPIPE_OK || total == 0) return false;
+    a->assign(total, 0.0F);
+    b->assign(total, 0.0F);
+    return view.get_noise_profile(meta, total, &total, a->data(), b->data()) == CPIPE_OK && total > 0;
+}
+
+// Whether the params name sigma or sigma_override (which take precedence over metadata).
+bool explicit_sigma(const Suites& s, const cpipe_props_t* params) {
+    double v = 0.0;
+    return s.param != nullptr && s.param->get_double != nullptr && params != nullptr &&
+           (s.param->get_double(params, "sigma", &v) == CPIPE_OK ||
+            s.param->get_double(params, "sigma_override", &v) == CPIPE_OK);
+}
+
 // bm3d.cpp:21-55
 float resolve_sigma(const Suites& s, const cpipe_props_t* params, const cpipe_buffer_t* input) {
     if (s.param != nullptr && s.param->get_double != nullptr && params != nullptr) {
@@ -213,10 +238,13 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
     const std::string mode = enum_param(s, params, "mode", "CPIPE_BM3D_MODE", "compat");
     const std::string prec = enum_param(s, params, "precision", "CPIPE_BM3D_PRECISION", "fp32");
     const std::string pilot = enum_param(s, params, "pilot", "CPIPE_BM3D_PILOT", "off");
+    const std::string nmap = enum_param(s, params, "noise_map", "CPIPE_BM3D_NOISE_MAP", "off");
     if (mode != "compat" && mode != "bm3d") return CPIPE_BAD_INDEX;
     if (pilot != "off" && pilot != "learned") return CPIPE_BAD_INDEX;
     if (pilot == "learned" && mode != "bm3d") return CPIPE_BAD_INDEX;
-    const int flags = pilot == "learned" ? CPIPE_BM3D_HOST_PILOT : 0;
+    if (nmap != "off" && nmap != "profile") return CPIPE_BAD_INDEX;
+    if (nmap == "profile" && (mode != "bm3d" || pilot == "learned")) return CPIPE_BAD_INDEX;
+    int flags = pilot == "learned" ? CPIPE_BM3D_HOST_PILOT : 0;
     int precision = CPIPE_BM3D_PRECISION_FP32;
     if (prec == "bf16") {
         precision = CPIPE_BM3D_PRECISION_BF16;
@@ -241,6 +269,33 @@ int process(cpipe_host_t* host, const cpipe_props_t* params, const cpipe_process
     if (cpipe_bm3d_device_count() <= 0) {
         host_log(host, 3, "denoise.bm3d: no HIP device visible; the node runs on the GPU only");
         return CPIPE_UNSUPPORTED;
+    }
+    // noise_map=profile: a sigma per reference patch from the noise profile
+    // (one pair per channel, else the pairs' mean for every channel); an explicit
+    // sigma param still wins, and without a profile the scalar sigma is used
+    if (nmap == "profile" && !explicit_sigma(s, params)) {
+        std::vector<float> a, b;
+        if (noise_profile(s, ctx->inputs[0], &a, &b)) {
+            const size_t nc = rgba ? 3 : ctx->n_in;
+            float pb[3] = {0.0F, 0.0F, 0.0F}, pl[3] = {0.0F, 0.0F, 0.0F};
+            for (size_t c = 0; c < nc; ++c) {
+                if (a.size() == nc) {
+                    pb[c] = a[c];
+                    pl[c] = b[c];
+                } else {
+                    for (size_t i = 0; i < a.size(); ++i) {
+                        pb[c] += a[i];
+                        pl[c] += b[i];
+                    }
+                    pb[c] /= static_cast<float>(a.size());
+                    pl[c] /= static_cast<float>(a.size());
+                }
+            }
+            if (cpipe_bm3d_host_noise_profile(pb, pl, (int)nc) != CPIPE_OK) return CPIPE_BAD_INDEX;
+            flags |= CPIPE_BM3D_HOST_PROFILE;
+        } else {
+            host_log(host, 2, "denoise.bm3d: noise_map=profile without a noise profile; using one sigma");
+        }
     }
 
     // One engine call on host (locked) or device addresses.

@@ -37,6 +37,7 @@ EXPORTS = (
     "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d", "cpipe_bm3d_batch_create",
     "cpipe_bm3d_batch_destroy", "cpipe_bm3d_batch_run", "cpipe_bm3d_batch_tune",
     "cpipe_bm3d_batch_config", "cpipe_bm3d_plan_set_block_matching",
+    "cpipe_bm3d_plan_set_noise_profile", "cpipe_bm3d_host_noise_profile",
     # include/cpipe_wavelet.h and the second node of the registry section
     "cpipe_wavelet_rgba16f", "cpipe_wavelet_host_rgba16f", "cpipe_wavelet_last_error",
     "cpipe_wavelet_node_descriptor", "cpipe_link_builtin_denoise_wavelet_bayes_shrink",
@@ -122,6 +123,8 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_batch_config.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
                                           ctypes.POINTER(I)]
     L.cpipe_bm3d_plan_set_block_matching.argtypes = [P, I]
+    L.cpipe_bm3d_plan_set_noise_profile.argtypes = [P, P, P, I]
+    L.cpipe_bm3d_host_noise_profile.argtypes = [P, P, I]
     L.cpipe_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F, P]
     L.cpipe_wavelet_host_rgba16f.argtypes = [P, S, P, S, I, I, F]
     L.cpipe_wavelet_last_error.restype = ctypes.c_char_p
@@ -254,6 +257,24 @@ class Plan:
     def set_block_matching(self, kind: str) -> None:
         """Mixed-precision plans: "mfma" (cross term on MFMA + FP32 re-rank) or "exact"."""
         _check(lib().cpipe_bm3d_plan_set_block_matching(self._h, 1 if kind == "mfma" else 0))
+
+    def set_noise_profile(self, beta=None, lam=None) -> None:
+        """Per-reference sigma from a noise profile (variance beta + lam * x per
+        channel; scalars apply to every channel); None returns to the scalar sigma."""
+        import numpy as np
+        if beta is None:
+            _check(lib().cpipe_bm3d_plan_set_noise_profile(self._h, None, None, 0))
+            return
+        b = np.atleast_1d(np.asarray(beta, np.float32)).ravel()
+        l_ = np.atleast_1d(np.asarray(lam, np.float32)).ravel()
+        if b.size == 1:
+            b = np.repeat(b, self.channels)
+        if l_.size == 1:
+            l_ = np.repeat(l_, self.channels)
+        if b.size != l_.size:
+            raise ValueError("beta and lambda need the same number of entries")
+        b, l_ = np.ascontiguousarray(b), np.ascontiguousarray(l_)
+        _check(lib().cpipe_bm3d_plan_set_noise_profile(self._h, b.ctypes.data, l_.ctypes.data, int(b.size)))
 
     def pilot_report(self):
         nc, nf, full = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

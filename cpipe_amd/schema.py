@@ -4,10 +4,11 @@ The reference's pipeline schema constrains each node's params with an
 `allOf` list of `if type == <node id> then params {...}` blocks; for
 com.cpipe.denoise.bm3d it allows `sigma` (0..0.2) and `sigma_override` (>= 0)
 only (schemas/pipeline-v0.4.json:101-117), so a pipeline file cannot select the
-`mode`, `precision` or `pilot` this node adds. This module derives the
+`mode`, `precision`, `pilot` or `noise_map` this node adds. This module derives the
 successor blocks for the three nodes of the library from the manifests the
 library embeds (cpipe_amd/csrc/*.json), adds the rules a manifest cannot state
-(pilot=learned needs mode=bm3d; precision_convert's target_format, which the
+(pilot=learned needs mode=bm3d; noise_map=profile needs mode=bm3d and no pilot;
+precision_convert's target_format, which the
 reference's precision planner sets, precision_convert.cpp:54-57), and writes
 them to cpipe_amd/schemas/pipeline-v0.5-nodes.json, a fragment a host merges
 into its pipeline schema's node `allOf`. The build regenerates it.
@@ -33,7 +34,9 @@ HOST_SET = {"com.cpipe.precision_convert": {
 # Cross-parameter rules the node enforces (CPIPE_BAD_INDEX otherwise).
 RULES = {"com.cpipe.denoise.bm3d": [
     {"if": {"properties": {"pilot": {"const": "learned"}}, "required": ["pilot"]},
-     "then": {"properties": {"mode": {"const": "bm3d"}}, "required": ["mode"]}}]}
+     "then": {"properties": {"mode": {"const": "bm3d"}}, "required": ["mode"]}},
+    {"if": {"properties": {"noise_map": {"const": "profile"}}, "required": ["noise_map"]},
+     "then": {"properties": {"mode": {"const": "bm3d"}, "pilot": {"const": "off"}}, "required": ["mode"]}}]}
 
 
 def _param_schema(p: dict) -> dict:
@@ -73,7 +76,7 @@ def overlay() -> dict:
                  "com.cpipe.precision_convert (pipeline schema 0.5 node blocks)",
         "description": "Generated from the nodes' manifests by cpipe_amd/schema.py; merge into the "
                        "pipeline schema's node items allOf. Successor of the bm3d block of pipeline "
-                       "schema 0.4, adding mode, precision and pilot.",
+                       "schema 0.4, adding mode, precision, pilot and noise_map.",
         "allOf": blocks,
     }
 

@@ -254,6 +254,20 @@ int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* batch, int* tile_y, int* ord
  * test images); the autotuner times both. CPIPE_BM3D_UNSUPPORTED for FP32 plans. */
 int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* plan, int mfma);
 
+/* Noise-profile sigma map (no counterpart in the reference, whose node reduces
+ * the DNG NoiseProfile pairs of CalibrationBlock::noise_profile to one sigma,
+ * src/cpipe/nodes/denoise/bm3d.cpp:21-55). With n = channels pairs, later runs
+ * of a single-frame plan give each reference position and (opponent) channel
+ * its own sigma: variance v_c = max(0, beta[c] + lambda[c] * x_c) at the mean
+ * x_c of the reference's 8x8 input patch, opponent channel k sigma^2 =
+ * sum_c A_kc^2 v_c (A the opponent rows (1/3, 1/3, 1/3), (1/2, 0, -1/2),
+ * (1/4, -1/2, 1/4)); each group shrinks with its reference's sigma and its
+ * aggregation weight is divided by sigma^2. The runs' sigma argument is then
+ * unused. n = 0 returns to the scalar sigma. CPIPE_BM3D_UNSUPPORTED for batch
+ * plans and together with the learned pilot. */
+int cpipe_bm3d_plan_set_noise_profile(cpipe_bm3d_plan* plan, const float* beta, const float* lambda,
+                                      int n);
+
 /* Host-memory conveniences used by the node (replace the Halide dispatch of
  * bm3d_dispatch.cpp:19-41 behind DenoiseBm3d::process, bm3d.cpp:96-100):
  * copy in, run, copy out, synchronise. The reference host calls a node's
@@ -264,7 +278,12 @@ int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* plan, int mfma);
  * lock. flags: CPIPE_BM3D_HOST_PILOT runs the learned pilot (weights embedded
  * in the library) in place of the plain step 1 (3-channel images only). */
 #define CPIPE_BM3D_HOST_PILOT 1
+/* flags: CPIPE_BM3D_HOST_PROFILE runs with the noise-profile sigma map of the
+ * pairs last given to cpipe_bm3d_host_noise_profile on the calling thread (one
+ * pair for every channel, or one per channel); mode bm3d, not with the pilot. */
+#define CPIPE_BM3D_HOST_PROFILE 2
 #define CPIPE_BM3D_HOST_SLOTS 4
+int cpipe_bm3d_host_noise_profile(const float* beta, const float* lambda, int n);
 int cpipe_bm3d_host_planes(const float* const* in_planes, size_t in_pitch, float* const* out_planes,
                            size_t out_pitch, int width, int height, int channels, float sigma,
                            int precision, int device, int flags);

@@ -409,6 +409,7 @@ typedef struct {
     const float* noisy[3];
     const float* basic[3]; /* step 2 only */
     float sig[3];          /* per-channel sigma */
+    const float* smap;     /* per-ref sigma [c][nref] (noise-profile map) or NULL */
     int channels;
     int step;
     const uint32_t* pos;
@@ -441,13 +442,14 @@ static void group_job_f32(void* vc, int job) {
     float* den = num + FOOT_MAX * FOOT_MAX;
     memset(num, 0, sizeof(float) * 2 * FOOT_MAX * FOOT_MAX);
     const float s = a->tb->haar_s;
-    const float sc = a->sig[c];
-    const float thr = 2.7f * sc;
-    const float s2 = sc * sc;
     float g[ORC_NMAX][64], gb[ORC_NMAX][64];
     for (int by = ty * ORC_TILE_Y; by < ty * ORC_TILE_Y + ORC_TILE_Y && by < a->ny; ++by) {
         for (int bx = tx * ORC_TILE_X; bx < tx * ORC_TILE_X + ORC_TILE_X && bx < a->nx; ++bx) {
             const size_t r = (size_t)by * a->nx + bx;
+            /* the group's sigma: the channel's, or its reference patch's from the map */
+            const float sc = a->smap ? a->smap[(size_t)c * a->nx * a->ny + r] : a->sig[c];
+            const float thr = 2.7f * sc;
+            const float s2 = sc * sc;
             const uint32_t* pos = a->pos + r * ORC_NMAX;
             const int n = floor_pow2(a->count[r]);
             for (int m = 0; m < n; ++m) {
@@ -508,6 +510,8 @@ static void group_job_f32(void* vc, int job) {
                     for (int p = 0; p < off; ++p) accw[p] = accw[p] + accw[p + off];
                 wgt = accw[0] > 0.0f ? 1.0f / accw[0] : 1.0f;
             }
+            /* with a map the textbook 1/sigma^2 factor no longer cancels */
+            if (a->smap) wgt = wgt * (1.0f / (s2 > 1e-30f ? s2 : 1.0f));
             for (int m = 0; m < n; ++m) {
                 const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);
                 float out[64];
@@ -611,9 +615,66 @@ static void aggregate_step_f32(agg_ctx_f32* a, float* const* outp, int nthreads,
     run_jobs(merge_row_job, &mc, a->h, nthreads);
 }
 
+/* Per-ref sigma of a noise profile (variance beta[c] + lambda[c] * x per input
+ * channel): x = the mean of the reference's 8x8 noisy patch (row-major FP32
+ * sum, times 1/64), v = max(0, beta + lambda * x); opponent channel k gets
+ * sqrt(sum_c A_kc^2 v_c) with A the rows (1/3, 1/3, 1/3), (1/2, 0, -1/2),
+ * (1/4, -1/2, 1/4); grey planes sqrt(v). For v the same everywhere this is
+ * the scalar path's sigma * |A_k| per channel. */
+int oracle_sigma_map_f32(const float* const* in, size_t pitch, int width, int height, int channels,
+                         const float* beta, const float* lambda, float* smap) {
+    if ((channels != 1 && channels != 3) || width < ORC_K || height < ORC_K) return 1;
+    const int nx = oracle_ref_count(width), ny = oracle_ref_count(height);
+    const size_t nref = (size_t)nx * ny;
+    const float c19 = (float)(1.0 / 9.0);
+    for (int by = 0; by < ny; ++by) {
+        for (int bx = 0; bx < nx; ++bx) {
+            const int x0 = oracle_ref_pos(bx, width), y0 = oracle_ref_pos(by, height);
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            for (int c = 0; c < channels; ++c) {
+                float sum = 0.0f;
+                for (int i = 0; i < ORC_K; ++i)
+                    for (int j = 0; j < ORC_K; ++j) sum = sum + in[c][(size_t)(y0 + i) * pitch + x0 + j];
+                const float t = beta[c] + lambda[c] * (sum * 0.015625f);
+                v[c] = t > 0.0f ? t : 0.0f;
+            }
+            const size_t r = (size_t)by * nx + bx;
+            if (channels == 1) {
+                smap[r] = sqrtf(v[0]);
+            } else {
+                smap[r] = sqrtf(((v[0] + v[1]) + v[2]) * c19);
+                smap[nref + r] = sqrtf((v[0] + v[2]) * 0.25f);
+                smap[2 * nref + r] = sqrtf(((v[0] + v[2]) * 0.0625f) + (v[1] * 0.25f));
+            }
+        }
+    }
+    return 0;
+}
+
+static int bm3d_f32(const float* const* in, float* const* out, float* const* basic, int width,
+                    int height, int channels, float sigma, const float* beta, const float* lambda,
+                    int nthreads, uint32_t* matches1, uint8_t* counts1, uint32_t* matches2,
+                    uint8_t* counts2);
+
 int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* basic, int width,
                     int height, int channels, float sigma, int nthreads, uint32_t* matches1,
                     uint8_t* counts1, uint32_t* matches2, uint8_t* counts2) {
+    return bm3d_f32(in, out, basic, width, height, channels, sigma, NULL, NULL, nthreads, matches1,
+                    counts1, matches2, counts2);
+}
+
+int oracle_bm3d_f32_profile(const float* const* in, float* const* out, float* const* basic,
+                            int width, int height, int channels, const float* beta,
+                            const float* lambda, int nthreads) {
+    if (!beta || !lambda) return 1;
+    return bm3d_f32(in, out, basic, width, height, channels, 0.0f, beta, lambda, nthreads, NULL, NULL,
+                    NULL, NULL);
+}
+
+static int bm3d_f32(const float* const* in, float* const* out, float* const* basic, int width,
+                    int height, int channels, float sigma, const float* beta, const float* lambda,
+                    int nthreads, uint32_t* matches1, uint8_t* counts1, uint32_t* matches2,
+                    uint8_t* counts2) {
     if ((channels != 1 && channels != 3) || width <= 0 || height <= 0) return 1;
     const size_t npx = (size_t)width * height;
     if (width < ORC_K || height < ORC_K) {
@@ -639,10 +700,12 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
     float* partial = (float*)malloc(sizeof(float) * tile_stride * (size_t)tx * ty * channels);
     int* cover = (int*)malloc(sizeof(int) * 2 * (size_t)(width + height));
     int *txlo = cover, *txhi = cover + width, *tylo = cover + 2 * width, *tyhi = tylo + height;
-    if (!zbuf || !pos || !cnt || !partial || !cover) {
-        free(zbuf); free(pos); free(cnt); free(partial); free(cover);
+    float* smap = beta ? (float*)malloc(sizeof(float) * nref * channels) : NULL;
+    if (!zbuf || !pos || !cnt || !partial || !cover || (beta && !smap)) {
+        free(zbuf); free(pos); free(cnt); free(partial); free(cover); free(smap);
         return 3;
     }
+    if (smap) oracle_sigma_map_f32(in, (size_t)width, width, height, channels, beta, lambda, smap);
     tile_cover(tx, ORC_TILE_X, nx, width, txlo, txhi);
     tile_cover(ty, ORC_TILE_Y, ny, height, tylo, tyhi);
 
@@ -659,7 +722,7 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
     if (matches1) memcpy(matches1, pos, sizeof(uint32_t) * nref * ORC_NMAX);
     if (counts1) memcpy(counts1, cnt, nref);
     agg_ctx_f32 a = {width, height, nx, ny, tx, ty, &tb, {z[0], z[1], z[2]}, {0, 0, 0},
-                     {sig[0], sig[1], sig[2]}, channels, 1, pos, cnt, partial, tile_stride};
+                     {sig[0], sig[1], sig[2]}, smap, channels, 1, pos, cnt, partial, tile_stride};
     aggregate_step_f32(&a, bz, nthreads, txlo, txhi, tylo, tyhi);
     if (basic) {
         if (channels == 3) {
@@ -681,7 +744,7 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
     } else {
         memcpy(out[0], fz[0], npx * sizeof(float));
     }
-    free(zbuf); free(pos); free(cnt); free(partial); free(cover);
+    free(zbuf); free(pos); free(cnt); free(partial); free(cover); free(smap);
     return 0;
 }
 
@@ -776,8 +839,8 @@ static void dct2_f64(const double* C, const double* x, double* t, int inverse) {
 }
 
 static void step_f64(const double* const* z, const double* const* bz, double* const* outp,
-                     int w, int h, int channels, const double* sig, int step, const uint32_t* posall,
-                     const uint8_t* cntall) {
+                     int w, int h, int channels, const double* sig, const double* smap, int step,
+                     const uint32_t* posall, const uint8_t* cntall) {
     double C[64], k1[8];
     oracle_dct_matrix_f64(C);
     oracle_kaiser_f64(k1);
@@ -788,8 +851,8 @@ static void step_f64(const double* const* z, const double* const* bz, double* co
     double(*gb)[64] = malloc(sizeof(double) * 64 * ORC_NMAX);
     for (int c = 0; c < channels; ++c) {
         memset(acc, 0, 2 * npx * sizeof(double));
-        const double sc = sig[c];
         for (size_t r = 0; r < (size_t)nx * ny; ++r) {
+            const double sc = smap ? smap[(size_t)c * nx * ny + r] : sig[c];
             const uint32_t* pos = posall + r * ORC_NMAX;
             const int n = floor_pow2(cntall[r]);
             for (int m = 0; m < n; ++m) {
@@ -862,8 +925,49 @@ static void step_f64(const double* const* z, const double* const* bz, double* co
     free(gb);
 }
 
+/* The noise-profile map of oracle_sigma_map_f32 in FP64 (no rounding order). */
+static void sigma_map_f64(const double* const* in, int width, int height, int channels,
+                          const double* beta, const double* lambda, double* smap) {
+    const int nx = oracle_ref_count(width), ny = oracle_ref_count(height);
+    const size_t nref = (size_t)nx * ny;
+    for (int by = 0; by < ny; ++by)
+        for (int bx = 0; bx < nx; ++bx) {
+            const int x0 = oracle_ref_pos(bx, width), y0 = oracle_ref_pos(by, height);
+            double v[3] = {0.0, 0.0, 0.0};
+            for (int c = 0; c < channels; ++c) {
+                double sum = 0.0;
+                for (int i = 0; i < ORC_K; ++i)
+                    for (int j = 0; j < ORC_K; ++j) sum += in[c][(size_t)(y0 + i) * width + x0 + j];
+                v[c] = fmax(0.0, beta[c] + lambda[c] * sum / 64.0);
+            }
+            const size_t r = (size_t)by * nx + bx;
+            if (channels == 1) {
+                smap[r] = sqrt(v[0]);
+            } else {
+                smap[r] = sqrt((v[0] + v[1] + v[2]) / 9.0);
+                smap[nref + r] = sqrt((v[0] + v[2]) / 4.0);
+                smap[2 * nref + r] = sqrt((v[0] + v[2]) / 16.0 + v[1] / 4.0);
+            }
+        }
+}
+
+static int bm3d_f64(const double* const* in, double* const* out, double* const* basic, int width,
+                    int height, int channels, double sigma, const double* beta, const double* lambda);
+
 int oracle_bm3d_f64(const double* const* in, double* const* out, double* const* basic,
                     int width, int height, int channels, double sigma) {
+    return bm3d_f64(in, out, basic, width, height, channels, sigma, NULL, NULL);
+}
+
+int oracle_bm3d_f64_profile(const double* const* in, double* const* out, double* const* basic,
+                            int width, int height, int channels, const double* beta,
+                            const double* lambda) {
+    if (!beta || !lambda) return 1;
+    return bm3d_f64(in, out, basic, width, height, channels, 0.0, beta, lambda);
+}
+
+static int bm3d_f64(const double* const* in, double* const* out, double* const* basic, int width,
+                    int height, int channels, double sigma, const double* beta, const double* lambda) {
     if ((channels != 1 && channels != 3) || width <= 0 || height <= 0) return 1;
     const size_t npx = (size_t)width * height;
     if (width < ORC_K || height < ORC_K) {
@@ -895,11 +999,17 @@ int oracle_bm3d_f64(const double* const* in, double* const* out, double* const* 
         memcpy(z[0], in[0], npx * sizeof(double));
         sig[0] = sigma;
     }
+    double* smap = NULL;
+    if (beta) {
+        smap = (double*)malloc(sizeof(double) * (size_t)nx * ny * channels);
+        sigma_map_f64(in, width, height, channels, beta, lambda, smap);
+    }
     oracle_bm_f64(z[0], width, height, 64.0 * kTauHt, pos, cnt);
-    step_f64((const double* const*)z, NULL, bz, width, height, channels, sig, 1, pos, cnt);
+    step_f64((const double* const*)z, NULL, bz, width, height, channels, sig, smap, 1, pos, cnt);
     oracle_bm_f64(bz[0], width, height, 64.0 * kTauWie, pos, cnt);
-    step_f64((const double* const*)z, (const double* const*)bz, fz, width, height, channels, sig, 2,
-             pos, cnt);
+    step_f64((const double* const*)z, (const double* const*)bz, fz, width, height, channels, sig, smap,
+             2, pos, cnt);
+    free(smap);
     double* const* outs[2] = {basic, out};
     double* const* srcs[2] = {bz, fz};
     for (int k = 0; k < 2; ++k) {

@@ -17,7 +17,9 @@
  *   3. oracle_bm3d_f32        -- the same algorithm in FP32 with exactly the
  *      operation order of the GPU FP32 path (fma chains, tie rules,
  *      aggregation tiles and merge order), multithreaded. It is both the
- *      parity partner of the GPU path and the CPU baseline.
+ *      parity partner of the GPU path and the CPU baseline. Variants
+ *      with a per-reference sigma from a noise profile (oracle_sigma_map_f32,
+ *      oracle_bm3d_f32_profile / _f64_profile).
  * and, for the downstream node com.cpipe.denoise.wavelet_bayes_shrink,
  *   4. oracle_wavelet_rgba16f -- its Halide generator's arithmetic
  *      (wavelet_oracle.c);
@@ -86,6 +88,21 @@ int oracle_bm3d_f32(const float* const* in, float* const* out, float* const* bas
 /* Full BM3D, FP64 textbook. Same I/O conventions (double planes). */
 int oracle_bm3d_f64(const double* const* in, double* const* out, double* const* basic,
                     int width, int height, int channels, double sigma);
+
+/* Noise-profile sigma map: per reference position and (opponent) channel, the
+ * sigma of variance beta[c] + lambda[c] * x at the reference patch's mean x
+ * (smap: channels x nref floats, channel-major; `in` planes with row pitch
+ * `pitch`). The BM3D variants below use it in place of the scalar sigma in the
+ * group shrinkage and multiply each group's aggregation weight by 1/sigma^2
+ * (the textbook weights 1/(sigma^2 N) and 1/(sigma^2 sum W^2)). */
+int oracle_sigma_map_f32(const float* const* in, size_t pitch, int width, int height, int channels,
+                         const float* beta, const float* lambda, float* smap);
+int oracle_bm3d_f32_profile(const float* const* in, float* const* out, float* const* basic,
+                            int width, int height, int channels, const float* beta,
+                            const float* lambda, int nthreads);
+int oracle_bm3d_f64_profile(const double* const* in, double* const* out, double* const* basic,
+                            int width, int height, int channels, const double* beta,
+                            const double* lambda);
 
 /* Wavelet BayesShrink chroma denoise of interleaved RGBA16F (wavelet_oracle.c):
  * the reference Halide generator denoise_wavelet_bayes_shrink

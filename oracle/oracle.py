@@ -68,6 +68,12 @@ def lib() -> ctypes.CDLL:
         L.oracle_bm3d_f32.restype = I
         L.oracle_bm3d_f64.argtypes = [P, P, P, I, I, I, D]
         L.oracle_bm3d_f64.restype = I
+        L.oracle_sigma_map_f32.argtypes = [P, S, I, I, I, P, P, P]
+        L.oracle_sigma_map_f32.restype = I
+        L.oracle_bm3d_f32_profile.argtypes = [P, P, P, I, I, I, P, P, I]
+        L.oracle_bm3d_f32_profile.restype = I
+        L.oracle_bm3d_f64_profile.argtypes = [P, P, P, I, I, I, P, P]
+        L.oracle_bm3d_f64_profile.restype = I
         L.oracle_pilot_param_count.restype = S
         L.oracle_pilot_forward_f64.argtypes = [P, P, P, I, I, I, P]
         L.oracle_pilot_loss_f64.argtypes = [P, P, P, P, I, I, I]
@@ -229,6 +235,53 @@ def bm3d_f64(planes: np.ndarray, sigma: float, want_basic: bool = False):
     if rc != 0:
         raise RuntimeError(f"oracle_bm3d_f64 failed: {rc}")
     return (out, basic) if want_basic else out
+
+
+def _profile(beta, lam, c, dtype):
+    b = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, dtype), (c,)))
+    l_ = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, dtype), (c,)))
+    return b, l_
+
+
+def sigma_map_f32(planes: np.ndarray, beta, lam) -> np.ndarray:
+    """Per-ref sigma of the noise profile variance beta + lam * x: (C, ny, nx) float32."""
+    src = np.ascontiguousarray(planes, dtype=np.float32)
+    c, h, w = src.shape
+    L = lib()
+    nx, ny = L.oracle_ref_count(w), L.oracle_ref_count(h)
+    out = np.zeros((c, ny, nx), np.float32)
+    b, l_ = _profile(beta, lam, c, np.float32)
+    rc = L.oracle_sigma_map_f32(_plane_ptrs(list(src)), w, w, h, c, _ptr(b), _ptr(l_), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle_sigma_map_f32 failed: {rc}")
+    return out
+
+
+def bm3d_f32_profile(planes: np.ndarray, beta, lam, nthreads: int = 8, want_basic: bool = False):
+    """BM3D with the per-ref sigma map of a noise profile (per channel beta, lam)."""
+    src = np.ascontiguousarray(planes, dtype=np.float32)
+    c, h, w = src.shape
+    out = np.empty_like(src)
+    basic = np.empty_like(src) if want_basic else None
+    b, l_ = _profile(beta, lam, c, np.float32)
+    rc = lib().oracle_bm3d_f32_profile(_plane_ptrs(list(src)), _plane_ptrs(list(out)),
+                                       _plane_ptrs(list(basic)) if want_basic else None, w, h, c,
+                                       _ptr(b), _ptr(l_), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_bm3d_f32_profile failed: {rc}")
+    return (out, basic) if want_basic else out
+
+
+def bm3d_f64_profile(planes: np.ndarray, beta, lam) -> np.ndarray:
+    src = np.ascontiguousarray(planes, dtype=np.float64)
+    c, h, w = src.shape
+    out = np.empty_like(src)
+    b, l_ = _profile(beta, lam, c, np.float64)
+    rc = lib().oracle_bm3d_f64_profile(_plane_ptrs(list(src)), _plane_ptrs(list(out)), None, w, h, c,
+                                       _ptr(b), _ptr(l_))
+    if rc != 0:
+        raise RuntimeError(f"oracle_bm3d_f64_profile failed: {rc}")
+    return out
 
 
 # ---------------------------------------------------------------- learned pilot

@@ -46,6 +46,9 @@ def test_pipeline_node_entries_validate(kind):
     ("com.cpipe.denoise.bm3d", {"mode": "fast"}),
     ("com.cpipe.denoise.bm3d", {"pilot": "learned"}),
     ("com.cpipe.denoise.bm3d", {"pilot": "learned", "mode": "compat"}),
+    ("com.cpipe.denoise.bm3d", {"noise_map": "profile"}),
+    ("com.cpipe.denoise.bm3d", {"noise_map": "profile", "mode": "bm3d", "pilot": "learned"}),
+    ("com.cpipe.denoise.bm3d", {"noise_map": "on", "mode": "bm3d"}),
     ("com.cpipe.denoise.bm3d", {"strength": 1.0}),
     ("com.cpipe.denoise.bm3d", {"sigma": "0.05"}),
     ("com.cpipe.denoise.wavelet_bayes_shrink", {"chroma_strength": 2.5}),
@@ -60,7 +63,7 @@ def test_reference_bm3d_block_is_a_subset():
     bm3d = _blocks()["com.cpipe.denoise.bm3d"]["properties"]
     assert bm3d["sigma"] == {"type": "number", "minimum": 0, "maximum": 0.2}
     assert bm3d["sigma_override"] == {"type": "number", "minimum": 0}
-    assert set(bm3d) == {"sigma", "sigma_override", "mode", "precision", "pilot"}
+    assert set(bm3d) == {"sigma", "sigma_override", "mode", "precision", "pilot", "noise_map"}
 
 
 def test_node_parses_exactly_the_schema_enums():
@@ -74,6 +77,7 @@ def test_node_parses_exactly_the_schema_enums():
 
     def status(params):
         a = host.Buffer(host.FORMAT_RGBA16F, 4, 4)
+        a.set_noise_profile([(1e-4, 0.01)] * 3)
         b = host.Buffer(host.FORMAT_RGBA16F, 4, 4)
         return host.run_node(d, [a], [b], params)
 
@@ -81,8 +85,10 @@ def test_node_parses_exactly_the_schema_enums():
     for mode in props["mode"]["enum"]:
         for prec in props["precision"]["enum"]:
             for pilot in props["pilot"]["enum"]:
-                params = {"mode": mode, "precision": prec, "pilot": pilot}
-                valid = not schema.validate({"type": "com.cpipe.denoise.bm3d", "params": params}, s)
-                assert status(params) == (host.CPIPE_UNSUPPORTED if valid else host.CPIPE_BAD_INDEX), params
-    for key, bad in (("mode", "fast"), ("precision", "int4"), ("pilot", "on")):
+                for nmap in props["noise_map"]["enum"]:
+                    params = {"mode": mode, "precision": prec, "pilot": pilot, "noise_map": nmap}
+                    valid = not schema.validate({"type": "com.cpipe.denoise.bm3d", "params": params}, s)
+                    want = host.CPIPE_UNSUPPORTED if valid else host.CPIPE_BAD_INDEX
+                    assert status(params) == want, params
+    for key, bad in (("mode", "fast"), ("precision", "int4"), ("pilot", "on"), ("noise_map", "on")):
         assert status({key: bad}) == host.CPIPE_BAD_INDEX
