@@ -132,15 +132,16 @@ __global__ void k_colour_in(const FrameTab* __restrict__ tab, const FrameRect* _
     }
 }
 
-// Noise-profile sigma map of frame 0 (one thread per reference position):
+// Noise-profile sigma map of frame blockIdx.z (one thread per reference position):
 // x = mean of the reference's 8x8 input patch per channel (row-major FP32 sum
 // times 1/64), v = max(0, beta + lambda x); opponent channel k gets
 // sqrt(sum_c A_kc^2 v_c) for the opponent rows A = (1/3, 1/3, 1/3),
 // (1/2, 0, -1/2), (1/4, -1/2, 1/4); grey planes sqrt(v).
-__global__ void k_sigma_map(const FrameTab* __restrict__ tab, float* __restrict__ smap) {
-    const FrameTab& F = tab[0];
+__global__ void k_sigma_map(const FrameTab* __restrict__ tab) {
+    const FrameTab& F = tab[blockIdx.z];
     const int bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y;
-    if (bx >= F.nx || by >= F.ny) return;
+    if (F.smap == nullptr || bx >= F.nx || by >= F.ny) return;
+    float* __restrict__ smap = F.smap;
     const int x0 = ref_pos(bx, F.w), y0 = ref_pos(by, F.h);
     float v[3] = {0.0f, 0.0f, 0.0f};
     for (int c = 0; c < F.c; ++c) {
@@ -1096,10 +1097,12 @@ struct cpipe_bm3d_plan_s {
     FrameRect* small_rects = nullptr;  // frames smaller than a patch (copied through)
     int n_small = 0, small_w = 0, small_h = 0;
     float* rgb = nullptr;            // 6 planes for the RGBA16F path (single-frame plans)
-    // noise-profile sigma map (single-frame plans): per (channel, ref) sigma
+    // noise-profile sigma map: per frame, per (channel, ref) sigma
     bool profile = false;
-    float pbeta[3] = {0.0f, 0.0f, 0.0f}, plambda[3] = {0.0f, 0.0f, 0.0f};
+    std::vector<float> pbeta, plambda;  // [frame][3]
+    std::vector<size_t> smap_off;       // per frame (floats into smap)
     float* smap = nullptr;
+    int smap_nx = 0, smap_ny = 0;       // largest reference grid (launch shape)
     bool ran = false;
     // timing (eager runs)
     bool timing = false;
@@ -1453,10 +1456,11 @@ void set_run(cpipe_bm3d_plan* p, int f, const float* const* in, size_t in_pitch,
     }
     T.in_pitch = in_pitch;
     T.out_pitch = out_pitch;
-    T.smap = p->profile ? p->smap : nullptr;
+    const bool map = p->profile && p->smap && !p->fr[f].small;
+    T.smap = map ? p->smap + p->smap_off[f] : nullptr;
     for (int ch = 0; ch < 3; ++ch) {
-        T.nbeta[ch] = p->pbeta[ch < p->c ? ch : 0];
-        T.nlambda[ch] = p->plambda[ch < p->c ? ch : 0];
+        T.nbeta[ch] = map ? p->pbeta[3 * f + (ch < p->c ? ch : 0)] : 0.0f;
+        T.nlambda[ch] = map ? p->plambda[3 * f + (ch < p->c ? ch : 0)] : 0.0f;
     }
 }
 
@@ -1721,9 +1725,9 @@ int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
         k_colour_in<<<g, 256, 0, st>>>(p->tab_dev, p->full.rects);
         HIP_CHECK(hipGetLastError());
     }
-    if (p->profile && p->smap) {  // single-frame plans only
-        const auto& F0 = p->fr[0];
-        k_sigma_map<<<dim3((unsigned)((F0.nx + 63) / 64), (unsigned)F0.ny), 64, 0, st>>>(p->tab_dev, p->smap);
+    if (p->profile && p->smap) {
+        const dim3 g((unsigned)((p->smap_nx + 63) / 64), (unsigned)p->smap_ny, (unsigned)n);
+        k_sigma_map<<<g, 64, 0, st>>>(p->tab_dev);
         HIP_CHECK(hipGetLastError());
     }
     if ((rc = r.mark(CPIPE_BM3D_T_COLOR, true))) return rc;
@@ -2125,34 +2129,66 @@ int cpipe_bm3d_batch_config(const cpipe_bm3d_batch* b, int* tile_y, int* order, 
     return OK;
 }
 
-int cpipe_bm3d_plan_set_noise_profile(cpipe_bm3d_plan* p, const float* beta, const float* lambda, int n) {
-    if (!p) return fail(BAD_INDEX, "null plan");
-    if (n != 0 && n != p->c) return fail(BAD_INDEX, "a noise profile needs one (beta, lambda) pair per channel");
-    if (n != 0 && (!beta || !lambda)) return fail(BAD_INDEX, "null noise profile");
-    for (int c = 0; c < n; ++c)
-        if (!std::isfinite(beta[c]) || !std::isfinite(lambda[c]))
-            return fail(BAD_INDEX, "noise profile values must be finite");
-    if (n != 0 && p->fr.size() != 1) return fail(UNSUPPORTED, "noise-profile maps run on single-frame plans");
-    if (n != 0 && p->pilot) return fail(UNSUPPORTED, "the learned pilot takes a scalar noise level");
+// Per-frame noise profiles (beta/lambda: [frame][channel], `stride` floats per
+// frame; stride 0 = one set of `p->c` pairs for every frame), or none.
+static int set_profiles(cpipe_bm3d_plan* p, const float* beta, const float* lambda, size_t stride, bool on) {
+    const int nf = (int)p->fr.size();
+    if (on && (!beta || !lambda)) return fail(BAD_INDEX, "null noise profile");
+    if (on) {
+        for (int f = 0; f < nf; ++f)
+            for (int c = 0; c < p->c; ++c)
+                if (!std::isfinite(beta[f * stride + c]) || !std::isfinite(lambda[f * stride + c]))
+                    return fail(BAD_INDEX, "noise profile values must be finite");
+        if (p->pilot) return fail(UNSUPPORTED, "the learned pilot takes a scalar noise level");
+    }
     DeviceGuard guard(p->device);
-    if ((n != 0) != p->profile) {
+    if (on != p->profile) {
         HIP_CHECK(hipDeviceSynchronize());
         drop_graph(p);
     }
-    const auto& F0 = p->fr[0];
-    if (n != 0 && !p->smap && !F0.small) {
-        const size_t bytes = sizeof(float) * (size_t)p->c * F0.nx * F0.ny;
-        if (hipMalloc(&p->smap, bytes) != hipSuccess) {
+    if (on && !p->smap) {
+        std::vector<size_t> off(nf, 0);
+        size_t total = 0;
+        int mx = 0, my = 0;
+        for (int f = 0; f < nf; ++f) {
+            const auto& F = p->fr[f];
+            off[f] = total;
+            if (F.small) continue;
+            total += (size_t)p->c * F.nx * F.ny;
+            mx = std::max(mx, F.nx);
+            my = std::max(my, F.ny);
+        }
+        if (total > 0 && hipMalloc(&p->smap, sizeof(float) * total) != hipSuccess) {
             p->smap = nullptr;
             return fail(OOM, "noise-profile map");
         }
+        p->smap_off = off;
+        p->smap_nx = mx;
+        p->smap_ny = my;
     }
-    p->profile = n != 0;
-    for (int c = 0; c < 3; ++c) {
-        p->pbeta[c] = c < n ? beta[c] : 0.0f;
-        p->plambda[c] = c < n ? lambda[c] : 0.0f;
-    }
+    p->profile = on;
+    p->pbeta.assign(3 * (size_t)nf, 0.0f);
+    p->plambda.assign(3 * (size_t)nf, 0.0f);
+    for (int f = 0; on && f < nf; ++f)
+        for (int c = 0; c < p->c; ++c) {
+            p->pbeta[3 * f + c] = beta[f * stride + c];
+            p->plambda[3 * f + c] = lambda[f * stride + c];
+        }
     return OK;
+}
+
+int cpipe_bm3d_plan_set_noise_profile(cpipe_bm3d_plan* p, const float* beta, const float* lambda, int n) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    if (n != 0 && n != p->c) return fail(BAD_INDEX, "a noise profile needs one (beta, lambda) pair per channel");
+    return set_profiles(p, beta, lambda, 0, n != 0);
+}
+
+int cpipe_bm3d_batch_set_noise_profiles(cpipe_bm3d_batch* b, const float* beta, const float* lambda,
+                                        int pairs_per_frame) {
+    if (!b) return fail(BAD_INDEX, "null batch");
+    if (pairs_per_frame != 0 && pairs_per_frame != b->c)
+        return fail(BAD_INDEX, "a noise profile needs one (beta, lambda) pair per channel");
+    return set_profiles(b, beta, lambda, (size_t)b->c, pairs_per_frame != 0);
 }
 
 int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* p, int mfma) {

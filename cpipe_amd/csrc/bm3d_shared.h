@@ -46,7 +46,7 @@ struct FrameTab {
     // per run
     float sig[3];           // per-channel sigma (opponent: sigma * |row|)
     float sigma;
-    const float* smap;      // noise-profile map: per (channel, ref) sigma, or null (scalar sig[])
+    float* smap;            // noise-profile map: per (channel, ref) sigma, or null (scalar sig[])
     float nbeta[3];         // noise profile of the input channels: variance nbeta + nlambda * x
     float nlambda[3];
     const float* in[3];     // caller's input planes

@@ -38,6 +38,7 @@ EXPORTS = (
     "cpipe_bm3d_batch_destroy", "cpipe_bm3d_batch_run", "cpipe_bm3d_batch_tune",
     "cpipe_bm3d_batch_config", "cpipe_bm3d_plan_set_block_matching",
     "cpipe_bm3d_plan_set_noise_profile", "cpipe_bm3d_host_noise_profile",
+    "cpipe_bm3d_batch_set_noise_profiles",
     # include/cpipe_wavelet.h and the second node of the registry section
     "cpipe_wavelet_rgba16f", "cpipe_wavelet_host_rgba16f", "cpipe_wavelet_last_error",
     "cpipe_wavelet_node_descriptor", "cpipe_link_builtin_denoise_wavelet_bayes_shrink",
@@ -125,6 +126,7 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_plan_set_block_matching.argtypes = [P, I]
     L.cpipe_bm3d_plan_set_noise_profile.argtypes = [P, P, P, I]
     L.cpipe_bm3d_host_noise_profile.argtypes = [P, P, I]
+    L.cpipe_bm3d_batch_set_noise_profiles.argtypes = [P, P, P, I]
     L.cpipe_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F, P]
     L.cpipe_wavelet_host_rgba16f.argtypes = [P, S, P, S, I, I, F]
     L.cpipe_wavelet_last_error.restype = ctypes.c_char_p
@@ -330,6 +332,18 @@ class Batch(Plan):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().cpipe_bm3d_batch_destroy(self._h)
             self._h = None
+
+    def set_noise_profiles(self, betas=None, lams=None) -> None:
+        """Per-frame noise profiles: (n_frames, channels) arrays (a row per frame,
+        scalars broadcast); None returns every frame to its scalar sigma."""
+        import numpy as np
+        n, C = len(self.sizes), self.channels
+        if betas is None:
+            _check(lib().cpipe_bm3d_batch_set_noise_profiles(self._h, None, None, 0))
+            return
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, np.float32), (n, C)))
+        l_ = np.ascontiguousarray(np.broadcast_to(np.asarray(lams, np.float32), (n, C)))
+        _check(lib().cpipe_bm3d_batch_set_noise_profiles(self._h, b.ctypes.data, l_.ctypes.data, C))
 
     def _args(self, srcs, dsts, sigmas):
         n, C = len(self.sizes), self.channels

@@ -263,10 +263,14 @@ int cpipe_bm3d_plan_set_block_matching(cpipe_bm3d_plan* plan, int mfma);
  * sum_c A_kc^2 v_c (A the opponent rows (1/3, 1/3, 1/3), (1/2, 0, -1/2),
  * (1/4, -1/2, 1/4)); each group shrinks with its reference's sigma and its
  * aggregation weight is divided by sigma^2. The runs' sigma argument is then
- * unused. n = 0 returns to the scalar sigma. CPIPE_BM3D_UNSUPPORTED for batch
- * plans and together with the learned pilot. */
+ * unused. n = 0 returns to the scalar sigma. On a batch the pairs apply to
+ * every frame. CPIPE_BM3D_UNSUPPORTED together with the learned pilot. */
 int cpipe_bm3d_plan_set_noise_profile(cpipe_bm3d_plan* plan, const float* beta, const float* lambda,
                                       int n);
+/* The same per frame of a batch: beta/lambda hold n_frames x channels values
+ * (frame-major); pairs_per_frame = channels, or 0 to return to the sigmas. */
+int cpipe_bm3d_batch_set_noise_profiles(cpipe_bm3d_batch* batch, const float* beta,
+                                        const float* lambda, int pairs_per_frame);
 
 /* Host-memory conveniences used by the node (replace the Halide dispatch of
  * bm3d_dispatch.cpp:19-41 behind DenoiseBm3d::process, bm3d.cpp:96-100):

@@ -3,7 +3,7 @@ the node's noise_map=profile).
 
 Bar: exact FP32 plans bit-identical to the CPU FP32 restatement
 (oracle_bm3d_f32_profile) for grey and RGB, per-channel and shared profiles,
-odd sizes and the clamp; mixed-precision plans within the mixed-precision bar
+odd sizes and the clamp, and per frame of a batch (eager and graph replays); mixed-precision plans within the mixed-precision bar
 of the FP64 oracle with the same map; switching the map off returns the
 scalar path bit for bit; the node reads the profile from the input's
 calibration metadata (the pairs of bm3d.cpp:29-52) on host and device buffers.
@@ -78,15 +78,49 @@ def test_mixed_precision_against_fp64(cuda, oracle_lib, precision):
     assert abs(synth.psnr(got, clean) - synth.psnr(o32, clean)) <= 0.1
 
 
-def test_batch_plans_and_pilot_refuse_the_map(cuda):
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_per_frame_profiles(cuda, oracle_lib, use_graph):
+    """Uneven frames (one smaller than a patch) with their own profiles: every
+    frame bit-identical to the CPU restatement with its profile, eager and over
+    graph replays with new profiles; then back to the scalar sigmas."""
+    torch = cuda
+    sizes = [(64, 48), (40, 36), (6, 7), (37, 29)]
+    frames = [synth.workload(f"rgb{w}x{h}", seed=20 + i)[1] if min(w, h) >= 8 else
+              np.random.default_rng(i).uniform(0, 1, (3, h, w)).astype(np.float32)
+              for i, (w, h) in enumerate(sizes)]
+    srcs = [torch.from_numpy(f).cuda() for f in frames]
+    b = engine.Batch(sizes, 3, "fp32", 0, use_graph=use_graph)
+    for rep in range(3):
+        betas = np.array([[1e-5 * (1 + f + rep)] * 3 for f in range(len(sizes))], np.float32)
+        lams = np.array([[0.01, 0.012, 0.008]] * len(sizes), np.float32) * (1 + 0.5 * rep)
+        b.set_noise_profiles(betas, lams)
+        outs = [torch.empty_like(t) for t in srcs]
+        print(f"batch profiles rep {rep} graph {use_graph}", flush=True)
+        b.run(srcs, outs, [0.05] * len(sizes))
+        torch.cuda.synchronize()
+        b.check()
+        for f, (o, fr) in enumerate(zip(outs, frames)):
+            want = oracle_lib.bm3d_f32_profile(fr, betas[f], lams[f])
+            assert np.array_equal(o.cpu().numpy().view(np.uint32), want.view(np.uint32)), (rep, f)
+        _ = (srcs[0] * 2.0).sum().item()  # eager work between replays
+    b.set_noise_profiles(None)
+    outs = [torch.empty_like(t) for t in srcs]
+    b.run(srcs, outs, [0.05] * len(sizes))
+    torch.cuda.synchronize()
+    for o, fr in zip(outs, frames):
+        want = oracle_lib.bm3d_f32(fr, np.float32(0.05))
+        assert np.array_equal(o.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    b.close()
+
+
+def test_pilot_and_bad_profiles_are_refused(cuda):
     from cpipe_amd import pilot
-    b = engine.Batch([(64, 48), (40, 36)], channels=3, use_graph=False)
-    with pytest.raises(engine.EngineError) as e:
-        b.set_noise_profile(1e-4, 0.01)
-    assert e.value.status == host.CPIPE_UNSUPPORTED
     plan = engine.Plan(64, 48, 3)
     with pytest.raises(engine.EngineError) as e:
         plan.set_noise_profile([1e-4, 1e-4], [0.01, 0.01])  # two pairs for three channels
+    assert e.value.status == host.CPIPE_BAD_INDEX
+    with pytest.raises(engine.EngineError) as e:
+        plan.set_noise_profile(float("nan"), 0.01)
     assert e.value.status == host.CPIPE_BAD_INDEX
     net = pilot.Pilot(seed=1)
     plan.set_noise_profile(1e-4, 0.01)
