@@ -265,3 +265,20 @@ def test_host_buffer_lock_round_trip():
     assert p
     assert not host.mh().mh_buffer_lock(b.h, host.CPU_READ)
     assert host.mh().mh_buffer_unlock(b.h) == host.CPIPE_OK
+
+
+def test_noise_profile_argument_checks_without_device():
+    """The noise-profile entry points check their arguments before any device work."""
+    L = engine.lib()
+    F3 = ctypes.c_float * 3
+    b, lam = F3(1e-4, 1e-4, 1e-4), F3(0.01, 0.01, 0.01)
+    assert L.cpipe_bm3d_host_noise_profile(b, lam, 4) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_host_noise_profile(None, None, 2) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_host_noise_profile(F3(float("nan"), 0, 0), lam, 1) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_host_noise_profile(b, lam, 3) == host.CPIPE_OK
+    assert L.cpipe_bm3d_host_noise_profile(None, None, 0) == host.CPIPE_OK
+    assert L.cpipe_bm3d_plan_set_noise_profile(None, b, lam, 3) == host.CPIPE_BAD_INDEX
+    assert L.cpipe_bm3d_batch_set_noise_profiles(None, b, lam, 3) == host.CPIPE_BAD_INDEX
+    # the profile flag needs mode bm3d (checked before the device)
+    px = (ctypes.c_uint16 * 64)()
+    assert L.cpipe_bm3d_host_rgba16f(px, 4, px, 4, 4, 4, ctypes.c_float(0.03), 0, 0, 0, 2) == host.CPIPE_BAD_INDEX
