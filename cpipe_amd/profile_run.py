@@ -19,6 +19,8 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--runs", type=int, default=1)
     ap.add_argument("--pilot", action="store_true", help="learned pilot with the packaged weights")
+    ap.add_argument("--noise-profile", default=None, metavar="BETA,LAMBDA",
+                    help="per-reference sigma map of this noise profile (every channel)")
     ap.add_argument("--bm", choices=["exact", "mfma"], default=None,
                     help="block-matching kernel of a mixed-precision plan (default: the plan's)")
     ap.add_argument("--wavelet", action="store_true",
@@ -43,6 +45,9 @@ def main(argv=None):
     src = torch.from_numpy(noisy).cuda()
     dst = torch.empty_like(src)
     plan = engine.Plan(w, h, c, args.precision, 0)
+    if args.noise_profile:
+        beta, lam = (float(v) for v in args.noise_profile.split(","))
+        plan.set_noise_profile(beta, lam)
     if args.bm is not None:
         plan.set_block_matching(args.bm)
     if args.pilot:
