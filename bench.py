@@ -359,7 +359,7 @@ def main():
                     help="learned-pilot weight blob (default: the packaged weights)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+        print(f"bench.py measures one GPU: the multi-GPU path is not implemented (asked for "
               f"--gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 

@@ -2,7 +2,7 @@
  * CPU oracle for the BM3D denoise node -- TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, and only as the checker (or the timed CPU baseline).
+ * load this library, and only as the reference result (or the timed CPU baseline).
  * The product path (cpipe_amd/) never links or calls it.
  *
  * Restatements (the BM3D node's three, the wavelet node's one):

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
-import this module, and only as the checker (or the timed CPU baseline). The
+import this module, and only as the reference result (or the timed CPU baseline). The
 product package ``cpipe_amd`` never imports it.
 
 What each function restates is documented in ``oracle/bm3d_oracle.h``.

@@ -1,6 +1,6 @@
 /*
  * CPU FP64 oracle of the learned pilot network -- TEST INFRASTRUCTURE ONLY.
- * Only tests/ and __graft_entry__.smoke() may load it, as the checker.
+ * Only tests/ and __graft_entry__.smoke() may load it, as the reference result.
  *
  * Network (the node's design; no reference counterpart): input channels
  * [Y, U, V, 10*sigma] of the noisy opponent planes; L1 conv3x3 4->32 + ReLU;

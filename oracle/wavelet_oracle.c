@@ -2,7 +2,7 @@
  * CPU oracle for the wavelet BayesShrink chroma denoise node
  * (com.cpipe.denoise.wavelet_bayes_shrink) -- TEST INFRASTRUCTURE ONLY.
  *
- * Only tests/ and bench.py's cpu_baseline leg may load it, as the checker.
+ * Only tests/ and bench.py's cpu_baseline leg may load it, as the reference result.
  *
  * Restates the reference Halide generator
  * src/cpipe/nodes/denoise/wavelet_bayes_shrink_generator.cpp:9-31,57-86 in
