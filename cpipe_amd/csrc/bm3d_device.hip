@@ -721,6 +721,11 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
 }
 
 // Tiles of 16 x TY refs, NW waves per workgroup (one group per wave per round).
+// Exact FP32 plans aggregate over 16 x 4-ref tiles with 4-wave workgroups:
+// each workgroup's ordered aggregation is one chain, and three workgroups
+// (40 KB footprint + 9 KB scratch each) keep three chains per CU.
+constexpr int EXACT_TY = 4, EXACT_NW = 4;
+
 template <int STEP, int TY, int NW, bool MAP>
 __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     constexpr int DOFS = FOOT * foot_edge(TY);
@@ -831,35 +836,35 @@ __global__ void k_merge(const FrameTab* __restrict__ tab, const FrameRect* __res
     if (x >= rc.x1 || y >= rc.y1) return;
     const int txl = F.txlo[x], txh = F.txhi[x], tyl = F.tylo[y], tyh = F.tyhi[y];
     const size_t ntiles = (size_t)F.ntx * F.nty;
-    // a pixel lies in at most MT x MT tile footprints (footprint <= 2 tile pitches
-    // + 1 in x and y); the loads of a channel are all issued before the sums,
-    // which still run in (ty, tx) ascending order
-    constexpr int MT = 3;
-    size_t offs[MT * MT];
-    bool live[MT * MT];
+    // a pixel lies in at most MTY x MTX tile footprints (16-ref tiles along x:
+    // 3; 4-ref tiles along y: 5); the loads of a channel are all issued before
+    // the sums, which still run in (ty, tx) ascending order
+    constexpr int MTX = 3, MTY = 5;
+    size_t offs[MTY * MTX];
+    bool live[MTY * MTX];
 #pragma unroll
-    for (int i = 0; i < MT; ++i)
+    for (int i = 0; i < MTY; ++i)
 #pragma unroll
-        for (int k = 0; k < MT; ++k) {
+        for (int k = 0; k < MTX; ++k) {
             const int ty = tyl + i, tx = txl + k;
-            live[i * MT + k] = ty <= tyh && tx <= txh;
+            live[i * MTX + k] = ty <= tyh && tx <= txh;
             const int tyc = min(ty, tyh), txc = min(tx, txh);
             const int fw = F.fx1[txc] - F.fx0[txc];
-            offs[i * MT + k] = ((size_t)tyc * F.ntx + txc) * F.tile_stride +
-                               (size_t)((y - F.fy0[tyc]) * fw + (x - F.fx0[txc]));
+            offs[i * MTX + k] = ((size_t)tyc * F.ntx + txc) * F.tile_stride +
+                                (size_t)((y - F.fy0[tyc]) * fw + (x - F.fx0[txc]));
         }
     float o[3] = {0.0f, 0.0f, 0.0f};
     for (int c = 0; c < F.c; ++c) {
         const float* pc = F.partial + (size_t)c * ntiles * F.tile_stride;
-        float vn[MT * MT], vd[MT * MT];
+        float vn[MTY * MTX], vd[MTY * MTX];
 #pragma unroll
-        for (int i = 0; i < MT * MT; ++i) {
+        for (int i = 0; i < MTY * MTX; ++i) {
             vn[i] = live[i] ? pc[offs[i]] : 0.0f;
             vd[i] = live[i] ? pc[offs[i] + F.dofs] : 0.0f;
         }
         float num = 0.0f, den = 0.0f;
 #pragma unroll
-        for (int i = 0; i < MT * MT; ++i) {
+        for (int i = 0; i < MTY * MTX; ++i) {
             if (live[i]) {
                 num = num + vn[i];
                 den = den + vd[i];
@@ -1284,11 +1289,12 @@ int build_frame(cpipe_bm3d_plan* p, int f) {
             if (tylo[y] < 0) tylo[y] = t;
             tyhi[y] = t;
         }
-    // every pixel lies in 1..3 tile footprints per axis (the merge kernel reads at most 3 x 3)
+    // every pixel lies in 1..3 tile footprints along x and 1..5 along y (the
+    // merge kernel reads at most 5 x 3)
     for (int x = 0; x < F.w; ++x)
         if (txlo[x] < 0 || txhi[x] - txlo[x] >= 3) return fail(INTERNAL, "tile cover in x out of range");
     for (int y = 0; y < F.h; ++y)
-        if (tylo[y] < 0 || tyhi[y] - tylo[y] >= 3) return fail(INTERNAL, "tile cover in y out of range");
+        if (tylo[y] < 0 || tyhi[y] - tylo[y] >= 5) return fail(INTERNAL, "tile cover in y out of range");
     if ((rc = upload(&F.geo_dev, F.geo))) return rc;
     T.nx = F.nx;
     T.ny = F.ny;
@@ -1424,7 +1430,7 @@ int create_core(cpipe_bm3d_plan** out, const int* widths, const int* heights, in
         free_plan(p);
         return fail(INTERNAL, std::string("init: ") + hipGetErrorString(e));
     }
-    rc = set_tile_shape(p, TILE / 2);
+    rc = set_tile_shape(p, precision == CPIPE_BM3D_PRECISION_FP32 ? EXACT_TY : TILE / 2);
     if (rc) {
         free_plan(p);
         return rc;
@@ -1519,18 +1525,18 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
         const bool map = p->profile && p->smap;  // kernels built with the per-ref sigma read
         if (p->precision != CPIPE_BM3D_PRECISION_FP32) {
             HIP_CHECK((hipError_t)group_mx_launch(step, p->precision, p->tile_y, map, gl, ggrid, r.st));
-        } else if (p->tile_y != TILE / 2) {
-            return fail(INTERNAL, "exact FP32 plans run 16 x 8-ref tiles only");
+        } else if (p->tile_y != EXACT_TY) {
+            return fail(INTERNAL, "exact FP32 plans run 16 x 4-ref tiles only");
         } else if (step == 1) {
             if (map)
-                k_group<1, TILE / 2, 8, true><<<ggrid, 512, 0, r.st>>>(gl);
+                k_group<1, EXACT_TY, EXACT_NW, true><<<ggrid, 64 * EXACT_NW, 0, r.st>>>(gl);
             else
-                k_group<1, TILE / 2, 8, false><<<ggrid, 512, 0, r.st>>>(gl);
+                k_group<1, EXACT_TY, EXACT_NW, false><<<ggrid, 64 * EXACT_NW, 0, r.st>>>(gl);
         } else {
             if (map)
-                k_group<2, TILE / 2, 8, true><<<ggrid, 512, 0, r.st>>>(gl);
+                k_group<2, EXACT_TY, EXACT_NW, true><<<ggrid, 64 * EXACT_NW, 0, r.st>>>(gl);
             else
-                k_group<2, TILE / 2, 8, false><<<ggrid, 512, 0, r.st>>>(gl);
+                k_group<2, EXACT_TY, EXACT_NW, false><<<ggrid, 64 * EXACT_NW, 0, r.st>>>(gl);
         }
         HIP_CHECK(hipGetLastError());
     }

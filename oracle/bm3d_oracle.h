@@ -46,7 +46,7 @@ extern "C" {
 #define ORC_STRIDE 3   /* reference-patch stride, last row/col forced */
 #define ORC_NMAX 16    /* maximum group size */
 #define ORC_TILE_X 16  /* refs per aggregation tile along x (GPU tiling, mirrored) */
-#define ORC_TILE_Y 8   /* refs per aggregation tile along y */
+#define ORC_TILE_Y 4   /* refs per aggregation tile along y */
 
 /* IEEE binary16 <-> binary32, round to nearest even. */
 uint16_t oracle_f32_to_f16(float v);
