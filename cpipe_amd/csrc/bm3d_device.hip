@@ -6,8 +6,8 @@
 //   pos/cnt   match lists: per reference position 16 packed (y<<16 | x) + raw count
 //   partial   per (channel, tile) num/den planes of the tile footprint (<= 91x91)
 // Reference positions: x in {0, 3, 6, ...} < W-8, plus W-8 (same in y).
-// A tile is 16x8 reference positions (x by y; mixed-precision plans may use
-// 16x16); its footprint is every pixel a member
+// A tile is 16x4 reference positions (x by y) in exact FP32 plans, 16x8 (or,
+// retuned, 16x16) in mixed-precision plans; its footprint is every pixel a member
 // patch of one of its groups can touch (refs +- 19 px search, + 8 px patch).
 //
 // Arithmetic is FP32 with a fixed operation order (explicit fmaf, -ffp-contract=off):
@@ -1367,11 +1367,11 @@ int build_full_lists(cpipe_bm3d_plan* p) {
     return OK;
 }
 
-// Tile shape: every plan starts with 16 x 8-ref tiles (exact FP32 plans keep
-// them: the CPU FP32 restatement aggregates over the same tiling). Their
-// footprint (91 x 67 px of (num, den)) is small enough for three workgroups
-// of 8 waves per CU, i.e. three ordered aggregation chains per CU instead of
-// two with 16 x 16-ref tiles. Mixed-precision plans may be retuned to 16 x 16.
+// Tile shape: exact FP32 plans use 16 x 4-ref tiles (the CPU FP32 restatement
+// aggregates over the same tiling): a 91 x 55 px (num, den) footprint, three
+// 4-wave workgroups and so three ordered aggregation chains per CU. Mixed-
+// precision plans start with 16 x 8-ref tiles (91 x 67 px, three 4-wave
+// workgroups per CU) and may be retuned to 16 x 16.
 int set_tile_shape(cpipe_bm3d_plan* p, int tile_y) {
     p->tile_y = tile_y;
     p->dofs = FOOT * foot_edge(tile_y);

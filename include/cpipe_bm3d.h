@@ -225,7 +225,7 @@ int cpipe_bm3d_batch_run(cpipe_bm3d_batch* batch, const float* const* in_planes,
  * caches it for later batches of the same shape (frame sizes, channels,
  * precision). Candidates: aggregation tile height and block-matching kernel
  * (mixed precision only: 8 or 16 refs, MFMA or exact block matching; exact
- * FP32 keeps 16 x 8-ref tiles, the CPU restatement's tiling, and exact
+ * FP32 keeps 16 x 4-ref tiles, the CPU restatement's tiling, and exact
  * matching),
  * work-list order (frame raster, or one contiguous run per XCD) and pixel-band
  * height.
