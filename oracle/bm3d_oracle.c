@@ -13,6 +13,22 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted mistakes, compiled in only by tests/test_oracle_shrinkage.py (into a
+ * temporary library, never oracle/liboracle.so) to show that its closed-form
+ * pins fail on each plausible slip in the shrinkage:
+ *   1: hard threshold at 2.5 sigma instead of 2.7 sigma;
+ *   2: step-1 aggregation weight N_nonzero instead of 1/N_nonzero;
+ *   3: unnormalised Haar along the stack (sums forward, halves inverse);
+ *   4: step-2 aggregation weight sum W^2 instead of 1/sum W^2.
+ * 0 (the default) is the algorithm. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define ORC_LAMBDA_F64 (ORC_PLANT == 1 ? 2.5 : 2.7)
+#define ORC_LAMBDA_F32 (ORC_PLANT == 1 ? 2.5f : 2.7f)
+#define ORC_HAAR_FWD_F64 (ORC_PLANT == 3 ? 1.0 : M_SQRT1_2)
+#define ORC_HAAR_INV_F64 (ORC_PLANT == 3 ? 0.5 : M_SQRT1_2)
+
 /* ------------------------------------------------------------------------ */
 /* binary16 conversion (IEEE 754-2008, round to nearest even)               */
 /* ------------------------------------------------------------------------ */
@@ -414,11 +430,13 @@ typedef struct {
     int step;
     const uint32_t* pos;
     const uint8_t* count;
-    float* partial; /* [c][tile][2][91*91] */
+    float* partial; /* [c][tile]: num then den footprint, FOOT_DOFS floats each */
     size_t tile_stride;
 } agg_ctx_f32;
 
-#define FOOT_MAX ((ORC_TILE_X - 1) * ORC_STRIDE + 2 * ORC_R + ORC_K) /* 91 */
+#define FOOT_MAX ((ORC_TILE_X - 1) * ORC_STRIDE + 2 * ORC_R + ORC_K) /* 91: footprint width */
+#define FOOT_MAX_Y ((ORC_TILE_Y - 1) * ORC_STRIDE + 2 * ORC_R + ORC_K) /* 55: footprint height */
+#define FOOT_DOFS (FOOT_MAX * FOOT_MAX_Y) /* den offset in a tile's block */
 
 static void tile_box(int t, int edge, int nref, int n, int* f0, int* f1) {
     const int first = oracle_ref_pos(t * edge, n);
@@ -439,16 +457,19 @@ static void group_job_f32(void* vc, int job) {
     tile_box(ty, ORC_TILE_Y, a->ny, a->h, &fy0, &fy1);
     const int fw = fx1 - fx0;
     float* num = a->partial + ((size_t)c * a->tx * a->ty + tile) * a->tile_stride;
-    float* den = num + FOOT_MAX * FOOT_MAX;
-    memset(num, 0, sizeof(float) * 2 * FOOT_MAX * FOOT_MAX);
+    float* den = num + FOOT_DOFS;
+    memset(num, 0, sizeof(float) * 2 * FOOT_DOFS);
     const float s = a->tb->haar_s;
+    /* the inverse stack transform's scale (differs only under planted mistake 3) */
+    const float s_inv = ORC_PLANT == 3 ? 0.5f : s;
+    const float s_fwd = ORC_PLANT == 3 ? 1.0f : s;
     float g[ORC_NMAX][64], gb[ORC_NMAX][64];
     for (int by = ty * ORC_TILE_Y; by < ty * ORC_TILE_Y + ORC_TILE_Y && by < a->ny; ++by) {
         for (int bx = tx * ORC_TILE_X; bx < tx * ORC_TILE_X + ORC_TILE_X && bx < a->nx; ++bx) {
             const size_t r = (size_t)by * a->nx + bx;
             /* the group's sigma: the channel's, or its reference patch's from the map */
             const float sc = a->smap ? a->smap[(size_t)c * a->nx * a->ny + r] : a->sig[c];
-            const float thr = 2.7f * sc;
+            const float thr = ORC_LAMBDA_F32 * sc;
             const float s2 = sc * sc;
             const uint32_t* pos = a->pos + r * ORC_NMAX;
             const int n = floor_pow2(a->count[r]);
@@ -472,7 +493,7 @@ static void group_job_f32(void* vc, int job) {
                 for (int p = 0; p < 64; ++p) {
                     float v[ORC_NMAX];
                     for (int m = 0; m < n; ++m) v[m] = g[m][p];
-                    haar_fwd_f32(v, n, s);
+                    haar_fwd_f32(v, n, s_fwd);
                     for (int m = 0; m < n; ++m) {
                         if (fabsf(v[m]) > thr) {
                             ++nnz;
@@ -480,10 +501,10 @@ static void group_job_f32(void* vc, int job) {
                             v[m] = 0.0f;
                         }
                     }
-                    haar_inv_f32(v, n, s);
+                    haar_inv_f32(v, n, s_inv);
                     for (int m = 0; m < n; ++m) g[m][p] = v[m];
                 }
-                wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
+                wgt = nnz > 0 ? (ORC_PLANT == 2 ? (float)nnz : 1.0f / (float)nnz) : 1.0f;
             } else {
                 float accw[64];
                 for (int p = 0; p < 64; ++p) {
@@ -492,8 +513,8 @@ static void group_job_f32(void* vc, int job) {
                         v[m] = g[m][p];
                         b[m] = gb[m][p];
                     }
-                    haar_fwd_f32(v, n, s);
-                    haar_fwd_f32(b, n, s);
+                    haar_fwd_f32(v, n, s_fwd);
+                    haar_fwd_f32(b, n, s_fwd);
                     float acc = 0.0f;
                     for (int m = 0; m < n; ++m) {
                         const float bb = b[m] * b[m];
@@ -503,12 +524,12 @@ static void group_job_f32(void* vc, int job) {
                         acc = (m == 0) ? W * W : fmaf(W, W, acc);
                     }
                     accw[p] = acc;
-                    haar_inv_f32(v, n, s);
+                    haar_inv_f32(v, n, s_inv);
                     for (int m = 0; m < n; ++m) g[m][p] = v[m];
                 }
                 for (int off = 32; off >= 1; off >>= 1)
                     for (int p = 0; p < off; ++p) accw[p] = accw[p] + accw[p + off];
-                wgt = accw[0] > 0.0f ? 1.0f / accw[0] : 1.0f;
+                wgt = accw[0] > 0.0f ? (ORC_PLANT == 4 ? accw[0] : 1.0f / accw[0]) : 1.0f;
             }
             /* with a map the textbook 1/sigma^2 factor no longer cancels */
             if (a->smap) wgt = wgt * (1.0f / (s2 > 1e-30f ? s2 : 1.0f));
@@ -555,7 +576,7 @@ static void merge_row_job(void* vc, int y) {
                         a->partial + ((size_t)c * a->tx * a->ty + (size_t)ty * a->tx + tx) * a->tile_stride;
                     const size_t o = (size_t)(y - fy0) * fw + (size_t)(x - fx0);
                     num = num + pn[o];
-                    den = den + pn[FOOT_MAX * FOOT_MAX + o];
+                    den = den + pn[FOOT_DOFS + o];
                 }
             }
             mc->out[c][(size_t)y * a->w + x] = num / den;
@@ -696,7 +717,7 @@ static int bm3d_f32(const float* const* in, float* const* out, float* const* bas
     float* fz[3] = {zbuf + 6 * npx, zbuf + 7 * npx, zbuf + 8 * npx}; /* final */
     uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * nref * ORC_NMAX);
     uint8_t* cnt = (uint8_t*)malloc(nref);
-    const size_t tile_stride = 2 * FOOT_MAX * FOOT_MAX;
+    const size_t tile_stride = 2 * FOOT_DOFS;
     float* partial = (float*)malloc(sizeof(float) * tile_stride * (size_t)tx * ty * channels);
     int* cover = (int*)malloc(sizeof(int) * 2 * (size_t)(width + height));
     int *txlo = cover, *txhi = cover + width, *tylo = cover + 2 * width, *tyhi = tylo + height;
@@ -800,7 +821,7 @@ void oracle_bm_f64(const double* plane, int width, int height, double tau_ssd, u
 }
 
 static void haar_fwd_f64(double* v, int n) {
-    const double s = M_SQRT1_2;
+    const double s = ORC_HAAR_FWD_F64;
     double tmp[ORC_NMAX];
     for (int len = n; len > 1; len >>= 1) {
         const int h = len >> 1;
@@ -813,7 +834,7 @@ static void haar_fwd_f64(double* v, int n) {
 }
 
 static void haar_inv_f64(double* v, int n) {
-    const double s = M_SQRT1_2;
+    const double s = ORC_HAAR_INV_F64;
     double tmp[ORC_NMAX];
     for (int len = 2; len <= n; len <<= 1) {
         const int h = len >> 1;
@@ -875,14 +896,14 @@ static void step_f64(const double* const* z, const double* const* bz, double* co
                     for (int m = 0; m < n; ++m) v[m] = g[m][p];
                     haar_fwd_f64(v, n);
                     for (int m = 0; m < n; ++m) {
-                        if (fabs(v[m]) > 2.7 * sc) ++nnz;
+                        if (fabs(v[m]) > ORC_LAMBDA_F64 * sc) ++nnz;
                         else v[m] = 0.0;
                     }
                     haar_inv_f64(v, n);
                     for (int m = 0; m < n; ++m) g[m][p] = v[m];
                 }
                 const double s2 = sc > 0.0 ? sc * sc : 1.0;
-                wgt = nnz > 0 ? 1.0 / (s2 * nnz) : 1.0;
+                wgt = nnz > 0 ? (ORC_PLANT == 2 ? nnz / s2 : 1.0 / (s2 * nnz)) : 1.0;
             } else {
                 double sw = 0.0;
                 for (int p = 0; p < 64; ++p) {
@@ -903,7 +924,7 @@ static void step_f64(const double* const* z, const double* const* bz, double* co
                     for (int m = 0; m < n; ++m) g[m][p] = v[m];
                 }
                 const double s2 = sc > 0.0 ? sc * sc : 1.0;
-                wgt = sw > 0.0 ? 1.0 / (s2 * sw) : 1.0;
+                wgt = sw > 0.0 ? (ORC_PLANT == 4 ? sw / s2 : 1.0 / (s2 * sw)) : 1.0;
             }
             for (int m = 0; m < n; ++m) {
                 const int qx = (int)(pos[m] & 0xffffu), qy = (int)(pos[m] >> 16);

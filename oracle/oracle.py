@@ -33,55 +33,93 @@ def build() -> str:
     return LIB_PATH
 
 
+def build_planted(plant: int, out_dir: str) -> str:
+    """Compile a copy of the oracle with planted mistake `plant` (ORC_PLANT in
+    bm3d_oracle.c) into out_dir; returns its path. Used only to show that the
+    closed-form pins fail on each planted mistake."""
+    path = os.path.join(out_dir, f"liboracle_plant{int(plant)}.so")
+    src = [os.path.join(HERE, f) for f in ("bm3d_oracle.c", "pilot_oracle.c", "wavelet_oracle.c",
+                                           "convert_oracle.c")]
+    cmd = ["gcc", "-O2", "-march=x86-64-v3", "-fPIC", "-std=c11", "-D_GNU_SOURCE",
+           "-ffp-contract=off", f"-DORC_PLANT={int(plant)}", "-shared", "-o", path] + src + ["-lm",
+                                                                                          "-lpthread"]
+    subprocess.run(cmd, check=True)
+    return path
+
+
+class using:
+    """Context manager: route this module's functions to another build of the
+    oracle (a planted-mistake copy from build_planted) while the block runs."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self.saved = None
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = _load(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
-        I = ctypes.c_int
-        F = ctypes.c_float
-        D = ctypes.c_double
-        S = ctypes.c_size_t
-        L.oracle_f32_to_f16.argtypes = [F]
-        L.oracle_f32_to_f16.restype = ctypes.c_uint16
-        L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
-        L.oracle_f16_to_f32.restype = F
-        L.oracle_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I]
-        L.oracle_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F]
-        L.oracle_convert_r16u_to_f32.argtypes = [P, S, P, S, I, I]
-        L.oracle_convert_f32_to_rgba16f.argtypes = [P, S, P, S, I, I]
-        L.oracle_convert_rgba16f_to_unorm.argtypes = [P, S, P, S, I, I, I]
-        L.oracle_compat_rgba16f.restype = None
-        L.oracle_ref_count.argtypes = [I]
-        L.oracle_ref_count.restype = I
-        L.oracle_ref_pos.argtypes = [I, I]
-        L.oracle_ref_pos.restype = I
-        L.oracle_dct_matrix_f64.argtypes = [P]
-        L.oracle_dct8_f32.argtypes = [P, P, I]
-        L.oracle_dct8_f32.restype = None
-        L.oracle_kaiser_f64.argtypes = [P]
-        L.oracle_bm_f32.argtypes = [P, I, I, F, P, P, I]
-        L.oracle_bm_f64.argtypes = [P, I, I, D, P, P]
-        L.oracle_bm3d_f32.argtypes = [P, P, P, I, I, I, F, I, P, P, P, P]
-        L.oracle_bm3d_f32.restype = I
-        L.oracle_bm3d_f64.argtypes = [P, P, P, I, I, I, D]
-        L.oracle_bm3d_f64.restype = I
-        L.oracle_sigma_map_f32.argtypes = [P, S, I, I, I, P, P, P]
-        L.oracle_sigma_map_f32.restype = I
-        L.oracle_bm3d_f32_profile.argtypes = [P, P, P, I, I, I, P, P, I]
-        L.oracle_bm3d_f32_profile.restype = I
-        L.oracle_bm3d_f64_profile.argtypes = [P, P, P, I, I, I, P, P]
-        L.oracle_bm3d_f64_profile.restype = I
-        L.oracle_pilot_param_count.restype = S
-        L.oracle_pilot_forward_f64.argtypes = [P, P, P, I, I, I, P]
-        L.oracle_pilot_loss_f64.argtypes = [P, P, P, P, I, I, I]
-        L.oracle_pilot_loss_f64.restype = D
-        L.oracle_pilot_grad_f64.argtypes = [P, P, P, P, I, I, I, P]
-        L.oracle_pilot_grad_f64.restype = D
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def _load(path: str) -> ctypes.CDLL:
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    F = ctypes.c_float
+    D = ctypes.c_double
+    S = ctypes.c_size_t
+    L.oracle_f32_to_f16.argtypes = [F]
+    L.oracle_f32_to_f16.restype = ctypes.c_uint16
+    L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
+    L.oracle_f16_to_f32.restype = F
+    L.oracle_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I]
+    L.oracle_wavelet_rgba16f.argtypes = [P, S, P, S, I, I, F]
+    L.oracle_convert_r16u_to_f32.argtypes = [P, S, P, S, I, I]
+    L.oracle_convert_f32_to_rgba16f.argtypes = [P, S, P, S, I, I]
+    L.oracle_convert_rgba16f_to_unorm.argtypes = [P, S, P, S, I, I, I]
+    L.oracle_compat_rgba16f.restype = None
+    L.oracle_ref_count.argtypes = [I]
+    L.oracle_ref_count.restype = I
+    L.oracle_ref_pos.argtypes = [I, I]
+    L.oracle_ref_pos.restype = I
+    L.oracle_dct_matrix_f64.argtypes = [P]
+    L.oracle_dct8_f32.argtypes = [P, P, I]
+    L.oracle_dct8_f32.restype = None
+    L.oracle_kaiser_f64.argtypes = [P]
+    L.oracle_bm_f32.argtypes = [P, I, I, F, P, P, I]
+    L.oracle_bm_f64.argtypes = [P, I, I, D, P, P]
+    L.oracle_bm3d_f32.argtypes = [P, P, P, I, I, I, F, I, P, P, P, P]
+    L.oracle_bm3d_f32.restype = I
+    L.oracle_bm3d_f64.argtypes = [P, P, P, I, I, I, D]
+    L.oracle_bm3d_f64.restype = I
+    L.oracle_sigma_map_f32.argtypes = [P, S, I, I, I, P, P, P]
+    L.oracle_sigma_map_f32.restype = I
+    L.oracle_bm3d_f32_profile.argtypes = [P, P, P, I, I, I, P, P, I]
+    L.oracle_bm3d_f32_profile.restype = I
+    L.oracle_bm3d_f64_profile.argtypes = [P, P, P, I, I, I, P, P]
+    L.oracle_bm3d_f64_profile.restype = I
+    L.oracle_pilot_param_count.restype = S
+    L.oracle_pilot_forward_f64.argtypes = [P, P, P, I, I, I, P]
+    L.oracle_pilot_loss_f64.argtypes = [P, P, P, P, I, I, I]
+    L.oracle_pilot_loss_f64.restype = D
+    L.oracle_pilot_grad_f64.argtypes = [P, P, P, P, I, I, I, P]
+    L.oracle_pilot_grad_f64.restype = D
+    return L
 
 
 def _ptr(a: np.ndarray) -> int:
