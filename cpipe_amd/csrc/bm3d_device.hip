@@ -172,6 +172,10 @@ __global__ void k_copy_through(const FrameTab* __restrict__ tab, const FrameRect
     for (int c = 0; c < F.c; ++c) F.out[c][(size_t)y * F.out_pitch + x] = F.in[c][(size_t)y * F.in_pitch + x];
 }
 
+// Sets one int to a value (a kernel rather than hipMemsetAsync, so that runs
+// captured into a graph clear the plan's error flag on every replay).
+__global__ void k_set_flag(int* __restrict__ flag, int v) { *flag = v; }
+
 __global__ void k_copy_plane(const float* __restrict__ src, size_t pitch, float* __restrict__ dst,
                              size_t dpitch, int w) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1097,7 +1101,9 @@ struct cpipe_bm3d_plan_s {
     hipEvent_t staged = nullptr;     // the last upload (or graph replay) has read `staging`
     bool staged_pending = false;
     int* gates = nullptr;            // per frame
-    int* err = nullptr;              // device flag: an ordering wait timed out
+    int* err = nullptr;              // device flag: an ordering wait timed out (cleared by every run)
+    int* err_host = nullptr;         // pinned: the flag as the last run left it (copied at its end)
+    hipEvent_t done = nullptr;       // recorded after the last run's flag copy
     Lists full;                      // every frame, whole image
     FrameRect* small_rects = nullptr;  // frames smaller than a patch (copied through)
     int n_small = 0, small_w = 0, small_h = 0;
@@ -1186,6 +1192,8 @@ void free_plan(cpipe_bm3d_plan* p) {
     if (p->staged) (void)hipEventDestroy(p->staged);
     (void)hipFree(p->gates);
     (void)hipFree(p->err);
+    if (p->err_host) (void)hipHostFree(p->err_host);
+    if (p->done) (void)hipEventDestroy(p->done);
     (void)hipFree(p->rgb);
     (void)hipFree(p->smap);
     (void)hipFree(p->report);
@@ -1426,6 +1434,9 @@ int create_core(cpipe_bm3d_plan** out, const int* widths, const int* heights, in
     e = hipMemset(p->err, 0, sizeof(int));
     if (e == hipSuccess) e = hipMemset(p->gates, 0, sizeof(int) * n);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->staged, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&p->err_host, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) *p->err_host = 0;
     if (e != hipSuccess) {
         free_plan(p);
         return fail(INTERNAL, std::string("init: ") + hipGetErrorString(e));
@@ -1573,6 +1584,7 @@ struct HostCache {
     std::mutex mu;
     std::vector<HostSlot> slots;
     std::map<int, cpipe_bm3d_pilot*> nets;  // embedded pilot weights, per device
+    std::map<int, OwnStream> streams;       // host-buffer runs, per device (non-blocking)
     uint64_t clock = 0;
 };
 
@@ -1709,7 +1721,31 @@ int host_slot(HostCache& hc, int kind, int w, int h, int c, int precision, int d
 
 // The whole run: table upload, colour, step 1 (or the learned pilot with its
 // spot checks and device-side fallback), step 2 into the callers' planes.
+int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing);
+
+// Brackets a run with the error flag: cleared first, copied to the pinned host
+// word after the last kernel (cpipe_bm3d_plan_check reads it once `done` has
+// passed; outside graph capture the event is recorded here).
+int flag_begin(cpipe_bm3d_plan* p, hipStream_t st) {
+    k_set_flag<<<1, 1, 0, st>>>(p->err, 0);
+    HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+int flag_end(cpipe_bm3d_plan* p, hipStream_t st, bool capturing) {
+    HIP_CHECK(hipMemcpyAsync(p->err_host, p->err, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (!capturing) HIP_CHECK(hipEventRecord(p->done, st));
+    return OK;
+}
+
 int enqueue(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
+    int rc = flag_begin(p, st);
+    if (!rc) rc = enqueue_run(p, st, timed, capturing);
+    if (!rc) rc = flag_end(p, st, capturing);
+    return rc;
+}
+
+int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) {
     RunCtx r{p, st, timed};
     for (bool& b : p->stage_ran) b = false;
     const size_t n = p->fr.size();
@@ -1832,6 +1868,7 @@ int run_plan(cpipe_bm3d_plan* p, hipStream_t st) {
     }
     HIP_CHECK(hipGraphLaunch(p->exec, st));
     HIP_CHECK(hipEventRecord(p->staged, st));
+    HIP_CHECK(hipEventRecord(p->done, st));
     p->staged_pending = true;
     p->ran = true;
     return OK;
@@ -2281,11 +2318,13 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* p, const float* const* in, s
     HIP_CHECK(hipEventRecord(p->staged, st));
     p->staged_pending = true;
     p->ran = true;
+    if ((rc = flag_begin(p, st))) return rc;
     const dim3 g((p->full.rect_w + 255) / 256, p->full.rect_h, p->full.n_rects);
     k_colour_in<<<g, 256, 0, st>>>(p->tab_dev, p->full.rects);
     HIP_CHECK(hipGetLastError());
     RunCtx r{p, st, false};
     if ((rc = run_step(r, 1, p->full, TO_BASIC, 0, false))) return rc;
+    if ((rc = flag_end(p, st, false))) return rc;
     for (int c = 0; c < 3; ++c) {
         HIP_CHECK(hipMemcpy2DAsync(noisy_out[c], out_pitch * sizeof(float), p->tab[0].z[c], (size_t)F.w * sizeof(float),
                                    (size_t)F.w * sizeof(float), F.h, hipMemcpyDeviceToDevice, st));
@@ -2360,7 +2399,7 @@ int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* p, int* n_checked, int* n_fall
     float md = 0.0f;
     if (p->pilot) {
         DeviceGuard guard(p->device);
-        HIP_CHECK(hipDeviceSynchronize());
+        if (p->ran) HIP_CHECK(hipEventSynchronize(p->done));
         nc = (int)p->checks.size();
         std::vector<float> rep(2 * std::max(1, nc));
         if (nc) HIP_CHECK(hipMemcpy(rep.data(), p->report, 2 * nc * sizeof(float), hipMemcpyDeviceToHost));
@@ -2384,13 +2423,19 @@ int cpipe_bm3d_plan_pilot_report(cpipe_bm3d_plan* p, int* n_checked, int* n_fall
 
 int cpipe_bm3d_plan_check(cpipe_bm3d_plan* p) {
     if (!p) return fail(BAD_INDEX, "null plan");
-    if (!p->err) return OK;
+    if (!p->ran) return OK;
     DeviceGuard guard(p->device);
-    int flag = 0;
-    HIP_CHECK(hipDeviceSynchronize());
-    HIP_CHECK(hipMemcpy(&flag, p->err, sizeof(int), hipMemcpyDeviceToHost));
-    if (flag) return fail(INTERNAL, "an aggregation ordering wait timed out; results are invalid");
+    HIP_CHECK(hipEventSynchronize(p->done));  // the last run only, not the device
+    if (*p->err_host) return fail(INTERNAL, "an aggregation ordering wait timed out; results are invalid");
     return OK;
+}
+
+int cpipe_bm3d_plan_debug_flag(cpipe_bm3d_plan* p, void* stream) {
+    if (!p) return fail(BAD_INDEX, "null plan");
+    DeviceGuard guard(p->device);
+    k_set_flag<<<1, 1, 0, (hipStream_t)stream>>>(p->err, 1);
+    HIP_CHECK(hipGetLastError());
+    return flag_end(p, (hipStream_t)stream, false);
 }
 
 int cpipe_bm3d_plan_ref_grid(const cpipe_bm3d_plan* p, int* nx, int* ny) {
@@ -2444,6 +2489,8 @@ int cpipe_bm3d_host_planes(const float* const* in, size_t in_pitch, float* const
     int rc = host_slot(hc, HOST_PLANES, w, h, channels, precision, device, flags, pb * 2 * channels, &sl);
     if (rc || (rc = apply_profile(sl, flags))) return rc;
     DeviceGuard guard(device);
+    hipStream_t st = nullptr;
+    HIP_CHECK(hc.streams[device].get(&st));
     float* dev = static_cast<float*>(sl->dev);
     float* pin = static_cast<float*>(sl->pin);
     float* din[3];
@@ -2455,13 +2502,13 @@ int cpipe_bm3d_host_planes(const float* const* in, size_t in_pitch, float* const
     // plane c's DMA runs while plane c+1 is copied into the pinned staging
     for (int c = 0; c < channels; ++c) {
         par_rows(pin + c * npx, row, in[c], in_pitch * sizeof(float), row, h);
-        HIP_CHECK(hipMemcpyAsync(din[c], pin + c * npx, pb, hipMemcpyHostToDevice, nullptr));
+        HIP_CHECK(hipMemcpyAsync(din[c], pin + c * npx, pb, hipMemcpyHostToDevice, st));
     }
-    if ((rc = cpipe_bm3d_plan_run(sl->plan, din, (size_t)w, dout, (size_t)w, sigma, nullptr))) return rc;
+    if ((rc = cpipe_bm3d_plan_run(sl->plan, din, (size_t)w, dout, (size_t)w, sigma, st))) return rc;
     float* pout = pin + channels * npx;
     for (int c = 0; c < channels; ++c) {
-        HIP_CHECK(hipMemcpyAsync(pout + c * npx, dout[c], pb, hipMemcpyDeviceToHost, nullptr));
-        HIP_CHECK(hipEventRecord(sl->ev[c], nullptr));
+        HIP_CHECK(hipMemcpyAsync(pout + c * npx, dout[c], pb, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipEventRecord(sl->ev[c], st));
     }
     for (int c = 0; c < channels; ++c) {
         HIP_CHECK(hipEventSynchronize(sl->ev[c]));
@@ -2484,21 +2531,23 @@ int cpipe_bm3d_host_rgba16f(const uint16_t* in, size_t isp, uint16_t* out, size_
                        2 * bytes, &sl);
     if (rc || (rc = apply_profile(sl, flags))) return rc;
     DeviceGuard guard(device);
+    hipStream_t st = nullptr;
+    HIP_CHECK(hc.streams[device].get(&st));
     uint16_t* din = static_cast<uint16_t*>(sl->dev);
     uint16_t* dout = din + (size_t)w * h * 4;
     uint16_t* pin = static_cast<uint16_t*>(sl->pin);
     uint16_t* pout = pin + (size_t)w * h * 4;
     const size_t row = (size_t)w * 8;
     par_rows(pin, row, in, isp * 8, row, h);
-    HIP_CHECK(hipMemcpyAsync(din, pin, bytes, hipMemcpyHostToDevice, nullptr));
+    HIP_CHECK(hipMemcpyAsync(din, pin, bytes, hipMemcpyHostToDevice, st));
     if (mode == 0) {
-        rc = cpipe_bm3d_compat_rgba16f(din, (size_t)w, dout, (size_t)w, w, h, sigma, 1, nullptr);
+        rc = cpipe_bm3d_compat_rgba16f(din, (size_t)w, dout, (size_t)w, w, h, sigma, 1, st);
     } else {
-        rc = cpipe_bm3d_plan_run_rgba16f(sl->plan, din, (size_t)w, dout, (size_t)w, sigma, nullptr);
+        rc = cpipe_bm3d_plan_run_rgba16f(sl->plan, din, (size_t)w, dout, (size_t)w, sigma, st);
     }
     if (rc) return rc;
-    HIP_CHECK(hipMemcpyAsync(pout, dout, bytes, hipMemcpyDeviceToHost, nullptr));
-    HIP_CHECK(hipStreamSynchronize(nullptr));
+    HIP_CHECK(hipMemcpyAsync(pout, dout, bytes, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
     par_rows(out, osp * 8, pout, row, row, h);
     return mode == 0 ? OK : cpipe_bm3d_plan_check(sl->plan);
 }

@@ -98,6 +98,7 @@ int bytes_per_pixel(int format) {
 struct HostBuffers {
     std::mutex mu;
     cpipe_internal::StagedBuffer in, out;
+    cpipe_internal::OwnStream stream;
 };
 
 HostBuffers& host_buffers() {
@@ -154,18 +155,23 @@ extern "C" int cpipe_convert_host(int in_format, const void* in, size_t in_strid
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(UNSUPPORTED, "convert: no HIP device visible");
-    CHIP_CHECK(hipSetDevice(0));
+    // host buffers run on HIP device 0 (the node's device); the caller's
+    // current device is restored on return
+    cpipe_internal::ScopedDevice device(0);
+    if (!device.ok()) return fail(UNSUPPORTED, "convert: cannot select HIP device 0");
     HostBuffers& hb = host_buffers();
     std::lock_guard<std::mutex> lock(hb.mu);
+    hipStream_t st = nullptr;
+    CHIP_CHECK(hb.stream.get(&st));
     const size_t ib = irow * (size_t)height, ob = orow * (size_t)height;
     CHIP_CHECK(hb.in.reserve(ib));
     CHIP_CHECK(hb.out.reserve(ob));
     cpipe_internal::par_rows(hb.in.pin, irow, in, in_stride_bytes, irow, height);
-    CHIP_CHECK(hipMemcpyAsync(hb.in.dev, hb.in.pin, ib, hipMemcpyHostToDevice, nullptr));
-    const int rc = cpipe_convert(in_format, hb.in.dev, irow, out_format, hb.out.dev, orow, width, height, nullptr);
+    CHIP_CHECK(hipMemcpyAsync(hb.in.dev, hb.in.pin, ib, hipMemcpyHostToDevice, st));
+    const int rc = cpipe_convert(in_format, hb.in.dev, irow, out_format, hb.out.dev, orow, width, height, st);
     if (rc != OK) return rc;
-    CHIP_CHECK(hipMemcpyAsync(hb.out.pin, hb.out.dev, ob, hipMemcpyDeviceToHost, nullptr));
-    CHIP_CHECK(hipStreamSynchronize(nullptr));
+    CHIP_CHECK(hipMemcpyAsync(hb.out.pin, hb.out.dev, ob, hipMemcpyDeviceToHost, st));
+    CHIP_CHECK(hipStreamSynchronize(st));
     cpipe_internal::par_rows(out, out_stride_bytes, hb.out.pin, orow, orow, height);
     return OK;
 }

@@ -40,6 +40,43 @@ inline void par_rows(void* dst, size_t dpitch, const void* src, size_t spitch, s
     for (auto& x : th) x.join();
 }
 
+// Makes `dev` the current HIP device for a scope and restores the caller's
+// current device on every return path.
+class ScopedDevice {
+public:
+    explicit ScopedDevice(int dev) {
+        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+        ok_ = prev_ == dev || hipSetDevice(dev) == hipSuccess;
+    }
+    ~ScopedDevice() {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+    bool ok() const { return ok_; }
+
+private:
+    int prev_ = -1;
+    bool ok_ = false;
+};
+
+// A non-blocking stream owned by a host-buffer entry point (created on first
+// use on the current device, kept until the process ends): its copies and
+// kernels are ordered among themselves only, so a host-buffer call neither
+// waits for nor holds up work that other callers queued on the device.
+struct OwnStream {
+    hipStream_t st = nullptr;
+    hipError_t get(hipStream_t* out) {
+        if (!st) {
+            const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                st = nullptr;
+                return e;
+            }
+        }
+        *out = st;
+        return hipSuccess;
+    }
+};
+
 // A grow-only pair of device and pinned host buffers of equal size (kept
 // until the process ends; the caller serialises use).
 struct StagedBuffer {

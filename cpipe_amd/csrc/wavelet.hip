@@ -120,6 +120,7 @@ __global__ __launch_bounds__(WX * WY) void k_wavelet(const uint16_t* __restrict_
 struct HostBuffers {
     std::mutex mu;
     cpipe_internal::StagedBuffer in, out;
+    cpipe_internal::OwnStream stream;
 };
 
 HostBuffers& host_buffers() {
@@ -156,20 +157,25 @@ extern "C" int cpipe_wavelet_host_rgba16f(const uint16_t* in, size_t in_stride_p
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(UNSUPPORTED, "wavelet: no HIP device visible");
-    WHIP_CHECK(hipSetDevice(0));
+    // host buffers run on HIP device 0 (the node's device); the caller's
+    // current device is restored on return
+    cpipe_internal::ScopedDevice device(0);
+    if (!device.ok()) return fail(UNSUPPORTED, "wavelet: cannot select HIP device 0");
     HostBuffers& hb = host_buffers();
     std::lock_guard<std::mutex> lock(hb.mu);
+    hipStream_t st = nullptr;
+    WHIP_CHECK(hb.stream.get(&st));
     const size_t row = (size_t)width * 8, bytes = row * (size_t)height;
     WHIP_CHECK(hb.in.reserve(bytes));
     WHIP_CHECK(hb.out.reserve(bytes));
     cpipe_internal::par_rows(hb.in.pin, row, in, in_stride_px * 8, row, height);
-    WHIP_CHECK(hipMemcpyAsync(hb.in.dev, hb.in.pin, bytes, hipMemcpyHostToDevice, nullptr));
+    WHIP_CHECK(hipMemcpyAsync(hb.in.dev, hb.in.pin, bytes, hipMemcpyHostToDevice, st));
     int rc = cpipe_wavelet_rgba16f(static_cast<const uint16_t*>(hb.in.dev), (size_t)width,
                                    static_cast<uint16_t*>(hb.out.dev), (size_t)width, width, height,
-                                   chroma_strength, nullptr);
+                                   chroma_strength, st);
     if (rc != OK) return rc;
-    WHIP_CHECK(hipMemcpyAsync(hb.out.pin, hb.out.dev, bytes, hipMemcpyDeviceToHost, nullptr));
-    WHIP_CHECK(hipStreamSynchronize(nullptr));
+    WHIP_CHECK(hipMemcpyAsync(hb.out.pin, hb.out.dev, bytes, hipMemcpyDeviceToHost, st));
+    WHIP_CHECK(hipStreamSynchronize(st));
     cpipe_internal::par_rows(out, out_stride_px * 8, hb.out.pin, row, row, height);
     return OK;
 }

@@ -24,7 +24,7 @@ EXPORTS = (
     "cpipe_bm3d_last_error", "cpipe_bm3d_device_count", "cpipe_bm3d_plan_create",
     "cpipe_bm3d_plan_destroy", "cpipe_bm3d_plan_run", "cpipe_bm3d_plan_run_rgba16f",
     "cpipe_bm3d_plan_basic", "cpipe_bm3d_plan_matches", "cpipe_bm3d_plan_ref_grid",
-    "cpipe_bm3d_plan_check", "cpipe_bm3d_plan_step1_opponent", "cpipe_bm3d_plan_set_pilot",
+    "cpipe_bm3d_plan_check", "cpipe_bm3d_plan_debug_flag", "cpipe_bm3d_plan_step1_opponent", "cpipe_bm3d_plan_set_pilot",
     "cpipe_bm3d_plan_pilot_report", "cpipe_bm3d_pilot_create", "cpipe_bm3d_pilot_destroy",
     "cpipe_bm3d_pilot_init", "cpipe_bm3d_pilot_save", "cpipe_bm3d_pilot_load",
     "cpipe_bm3d_pilot_forward", "cpipe_bm3d_pilot_gradients", "cpipe_bm3d_pilot_train_step",
@@ -100,6 +100,7 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_plan_matches.argtypes = [P, I, P, P, P]
     L.cpipe_bm3d_plan_ref_grid.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I)]
     L.cpipe_bm3d_plan_check.argtypes = [P]
+    L.cpipe_bm3d_plan_debug_flag.argtypes = [P, P]
     L.cpipe_bm3d_plan_step1_opponent.argtypes = [P, P, S, F, P, P, S, P]
     L.cpipe_bm3d_plan_set_pilot.argtypes = [P, P, P]
     L.cpipe_bm3d_plan_pilot_report.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I),
@@ -287,8 +288,13 @@ class Plan:
                 "full_fallback": bool(full.value)}
 
     def check(self) -> None:
-        """Synchronise and raise if a run hit an internal consistency check."""
+        """Wait for the last run and raise if it hit an internal consistency check."""
         _check(lib().cpipe_bm3d_plan_check(self._h))
+
+    def debug_flag(self, stream=None) -> None:
+        """Tests only: mark the last run as failed (the next check() raises until
+        the plan runs again)."""
+        _check(lib().cpipe_bm3d_plan_debug_flag(self._h, _stream_handle(stream)))
 
     def set_timing(self, enable: bool) -> None:
         _check(lib().cpipe_bm3d_plan_set_timing(self._h, 1 if enable else 0))

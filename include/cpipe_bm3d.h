@@ -94,9 +94,16 @@ int cpipe_bm3d_plan_basic(cpipe_bm3d_plan* plan, float* const* out_planes, size_
                           void* stream);
 int cpipe_bm3d_plan_matches(cpipe_bm3d_plan* plan, int step, uint32_t* pos, uint8_t* count,
                             void* stream);
-/* Synchronises the device and reports whether any run of this plan hit an
- * internal consistency check (CPIPE_INTERNAL_ERROR; results invalid). */
+/* Waits for the plan's last run (an event on that run's stream, not the whole
+ * device) and reports whether that run hit an internal consistency check
+ * (CPIPE_INTERNAL_ERROR; its results are invalid). Every run clears the flag
+ * first, so one failed run does not condemn the later ones. OK before any run. */
 int cpipe_bm3d_plan_check(cpipe_bm3d_plan* plan);
+/* Diagnostics for tests: marks the plan's last run as failed on `stream`, as an
+ * internal consistency check would, so the next cpipe_bm3d_plan_check reports
+ * CPIPE_INTERNAL_ERROR until the plan runs again. No counterpart in the
+ * reference. */
+int cpipe_bm3d_plan_debug_flag(cpipe_bm3d_plan* plan, void* stream);
 
 /* Reference grid of the plan: number of reference positions along x and y. */
 int cpipe_bm3d_plan_ref_grid(const cpipe_bm3d_plan* plan, int* nx, int* ny);

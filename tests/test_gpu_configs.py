@@ -1,0 +1,169 @@
+"""The five workload configurations at their full sizes, through the C ABI.
+
+  * config 0: 512x512 grey, Gaussian sigma 0.05 -- bit-identical to the CPU FP32
+    restatement (match lists and output), PSNR >= 50 dB / max-abs <= 1e-3 vs the
+    FP64 oracle;
+  * config 2: the 4080x3072 RGB frame in FP16 and BF16 -- step-1 match lists
+    equal to the exact path's for >= 99.5 % / 98 % of refs, PSNR >= 40 / 34 dB
+    against the exact FP32 output (itself bit-identical to the CPU FP32
+    restatement on this frame, test_gpu_parity.py), |delta PSNR| <= 0.1 dB
+    against the FP64 oracle on the 512x384 top-left crop;
+  * config 3: the packaged learned pilot + FP16 Wiener at 4080x3072 -- no
+    frame-wide fallback, PSNR >= the plain FP32 path - 0.1 dB, and the gain over
+    the same plan without the pilot (time and PSNR, asserted);
+  * config 4: the 12-frame uneven batch (4 x 4080x3072, 4 x 2040x1536,
+    4 x 1024x768, sigma_f ~ U(0.01, 0.1)) -- every frame bit-identical to its
+    single-frame plan, in exact FP32 and in FP16 + learned pilot.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from cpipe_amd import engine, synth
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+def _plan_run(torch, noisy, sigma, precision="fp32", net=None, want_lists=False, reps=1):
+    c, h, w = noisy.shape
+    src = torch.from_numpy(np.ascontiguousarray(noisy)).cuda()
+    out = torch.empty_like(src)
+    plan = engine.Plan(w, h, c, precision, 0)
+    if net is not None:
+        plan.set_pilot(net)
+    plan.run(src, out, sigma)  # first run: lazily sized buffers
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run(src, out, sigma)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    plan.check()
+    res = {"out": out.cpu().numpy(), "ms": ms}
+    if want_lists:
+        res["m1"] = tuple(t.cpu().numpy() for t in plan.matches(1))
+        res["m2"] = tuple(t.cpu().numpy() for t in plan.matches(2))
+    if net is not None:
+        res["report"] = plan.pilot_report()
+    plan.close()
+    return res
+
+
+def test_config0_gray512_bit_exact(cuda, oracle_lib):
+    torch = cuda
+    clean, noisy, sigma = synth.workload("gray512", seed=0)
+    print("config 0: gray512 sigma 0.05", flush=True)
+    g = _plan_run(torch, noisy, sigma, want_lists=True)
+    out, (p1, c1, p2, c2) = oracle_lib.bm3d_f32(noisy, sigma, nthreads=NT, want_matches=True)
+    np.testing.assert_array_equal(g["m1"][0].view(np.uint32), p1)
+    np.testing.assert_array_equal(g["m1"][1], c1)
+    np.testing.assert_array_equal(g["m2"][0].view(np.uint32), p2)
+    np.testing.assert_array_equal(g["m2"][1], c2)
+    np.testing.assert_array_equal(g["out"].view(np.uint32), out.view(np.uint32))
+    o64 = oracle_lib.bm3d_f64(noisy, sigma)
+    assert synth.psnr(g["out"], o64) >= 50.0
+    assert synth.max_abs(g["out"], o64) <= 1e-3
+    assert synth.psnr(g["out"], clean) > synth.psnr(noisy, clean) + 6.0
+
+
+@pytest.fixture(scope="module")
+def frame12(cuda):
+    """The 12.5 MP headline frame and its exact FP32 GPU output."""
+    clean, noisy, sigma = synth.workload("rgb12mp", seed=0)
+    ref = _plan_run(cuda, noisy, sigma, want_lists=True, reps=3)
+    return clean, noisy, sigma, ref
+
+
+MIN_PSNR_VS_FP32 = {"fp16": 40.0, "bf16": 34.0}
+MIN_LIST_AGREEMENT = {"fp16": 0.995, "bf16": 0.98}
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_config2_mixed_full_frame(cuda, oracle_lib, frame12, precision):
+    torch = cuda
+    clean, noisy, sigma, ref = frame12
+    print(f"config 2: rgb12mp {precision}", flush=True)
+    g = _plan_run(torch, noisy, sigma, precision, want_lists=True)
+    same = np.all(g["m1"][0] == ref["m1"][0], axis=1) & (g["m1"][1] == ref["m1"][1])
+    vs32 = synth.psnr(g["out"], ref["out"])
+    print(f"lists equal {same.mean():.5f}, psnr vs fp32 {vs32:.2f} dB, "
+          f"max-abs {synth.max_abs(g['out'], ref['out']):.4f}")
+    assert same.mean() >= MIN_LIST_AGREEMENT[precision]
+    assert vs32 >= MIN_PSNR_VS_FP32[precision]
+    # delta PSNR against the FP64 oracle on the 512x384 crop of the same frame
+    cn = np.ascontiguousarray(noisy[:, :384, :512])
+    cc = clean[:, :384, :512]
+    o64 = oracle_lib.bm3d_f64(cn, sigma)
+    gc = _plan_run(torch, cn, sigma, precision)["out"]
+    d = synth.psnr(gc, cc) - synth.psnr(o64, cc)
+    print(f"crop: delta PSNR vs fp64 {d:+.4f} dB, psnr vs fp64 {synth.psnr(gc, o64):.2f} dB, "
+          f"max-abs {synth.max_abs(gc, o64):.4f}")
+    assert abs(d) <= 0.1
+
+
+def test_config3_learned_pilot_full_frame_and_gain(cuda, frame12):
+    from cpipe_amd import pilot as pilot_mod
+    torch = cuda
+    clean, noisy, sigma, ref = frame12
+    print("config 3: rgb12mp learned pilot + fp16", flush=True)
+    net = pilot_mod.Pilot(seed=None, weights=pilot_mod.DEFAULT_WEIGHTS)
+    with_pilot = _plan_run(torch, noisy, sigma, "fp16", net=net, reps=5)
+    without = _plan_run(torch, noisy, sigma, "fp16", reps=5)
+    rep = with_pilot["report"]
+    p_fp32 = synth.psnr(ref["out"], clean)
+    p_with = synth.psnr(with_pilot["out"], clean)
+    p_without = synth.psnr(without["out"], clean)
+    speed = without["ms"] / with_pilot["ms"]
+    print(f"spot checks {rep}; psnr fp32 {p_fp32:.3f} / fp16 {p_without:.3f} / learned {p_with:.3f} dB; "
+          f"ms fp32 {ref['ms']:.2f} / fp16 {without['ms']:.2f} / learned {with_pilot['ms']:.2f} "
+          f"(gain {speed:.2f}x)")
+    assert rep["checked"] > 0 and not rep["full_fallback"]
+    assert p_with >= p_fp32 - 0.1
+    # what the network gains over the same plan without it: step 1 replaced,
+    # at no loss of quality
+    assert speed >= 1.2
+    assert p_with >= p_without - 0.1
+
+
+BATCH_SIZES = [(4080, 3072)] * 4 + [(2040, 1536)] * 4 + [(1024, 768)] * 4
+
+
+def test_config4_batch_frames_bit_identical(cuda):
+    from cpipe_amd import pilot as pilot_mod
+    torch = cuda
+    rng = np.random.default_rng(4)
+    sigmas = [float(v) for v in rng.uniform(0.01, 0.1, size=len(BATCH_SIZES))]
+    srcs = []
+    for i, (w, h) in enumerate(BATCH_SIZES):
+        clean = synth.clean_scene(w, h, 3, seed=100 + i)
+        srcs.append(torch.from_numpy(synth.add_gaussian(clean, sigmas[i], seed=200 + i)).cuda())
+    net = pilot_mod.Pilot(seed=None, weights=pilot_mod.DEFAULT_WEIGHTS)
+    for precision, pnet in (("fp32", None), ("fp16", net)):
+        print(f"config 4: 12-frame batch, {precision}{' + learned pilot' if pnet else ''}", flush=True)
+        b = engine.Batch(BATCH_SIZES, 3, precision, 0, use_graph=True)
+        if pnet is not None:
+            b.set_pilot(pnet)
+        outs = [torch.empty_like(s) for s in srcs]
+        for _ in range(2):  # eager run with capture, then a graph replay
+            b.run(srcs, outs, sigmas)
+        torch.cuda.synchronize()
+        b.check()
+        got = [o.cpu().numpy() for o in outs]
+        b.close()
+        del outs
+        for i, s in enumerate(srcs):
+            c, h, w = s.shape
+            plan = engine.Plan(w, h, c, precision, 0)
+            if pnet is not None:
+                plan.set_pilot(pnet)
+            o = torch.empty_like(s)
+            plan.run(s, o, sigmas[i])
+            torch.cuda.synchronize()
+            plan.check()
+            want = o.cpu().numpy()
+            plan.close()
+            assert np.array_equal(got[i].view(np.uint32), want.view(np.uint32)), (precision, i)

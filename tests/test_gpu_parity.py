@@ -185,3 +185,51 @@ def test_full_frame_bit_exact(cuda, oracle_lib, profile):
         want = oracle_lib.bm3d_f32(noisy, np.float32(sigma), nthreads=nt)
     np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
     assert synth.psnr(got, clean) > synth.psnr(noisy, clean) + 6.0
+
+
+def test_error_flag_is_cleared_by_every_run(cuda):
+    """A run marked as failed makes plan.check() raise; the next run clears the
+    flag, so the same plan (e.g. a cached node plan) reports OK again."""
+    torch = cuda
+    clean, noisy, sigma = synth.workload("rgb64x48", seed=6)
+    src = torch.from_numpy(noisy).cuda()
+    out = torch.empty_like(src)
+    plan = engine.Plan(64, 48, 3, "fp32", 0)
+    plan.check()  # before any run
+    plan.run(src, out, sigma)
+    plan.check()
+    plan.debug_flag()
+    with pytest.raises(engine.EngineError) as e:
+        plan.check()
+    assert e.value.status == 7  # CPIPE_INTERNAL_ERROR
+    plan.run(src, out, sigma)
+    plan.check()
+    plan.close()
+
+
+@pytest.mark.parametrize("case", ["single", "two", "dup16x8", "dup40x8", "dup40x40", "rgb"])
+def test_closed_form_images(cuda, oracle_lib, case):
+    """The oracle's closed-form pin images (tests/test_oracle_shrinkage.py) run
+    through the GPU path: bit-identical to the CPU FP32 restatement, which the
+    closed forms pin to FP32 rounding."""
+    import test_oracle_shrinkage as cf
+    torch = cuda
+    if case == "single":
+        rng = np.random.default_rng(11)
+        img = cf._coeff_patch({(0, 0): 3.0, (0, 1): 0.130, (2, 3): -0.131, (1, 0): 0.3,
+                               (3, 3): -0.2, (5, 6): 0.14}, rng)[None].astype(np.float32)
+        sigma = cf.SIG1
+    elif case == "two":
+        img, sigma = cf._two_group_image()[None], cf.SIG2
+    elif case == "rgb":
+        img = np.random.default_rng(21).uniform(0.1, 0.9, (3, 8, 8)).astype(np.float32)
+        sigma = 0.06
+    else:
+        w, h = (int(v) for v in case[3:].split("x"))
+        seed = {"dup16x8": 1, "dup40x8": 2, "dup40x40": 3}[case]
+        img, sigma = cf._duplicate_image(w, h, seed, cf.LAMBDA * cf.SIG3)[None], cf.SIG3
+    print(f"closed-form image {case} {img.shape}", flush=True)
+    g = _run_gpu(torch, img, sigma, want_basic=True)
+    out, basic = oracle_lib.bm3d_f32(img, sigma, want_basic=True)
+    np.testing.assert_array_equal(g["basic"].view(np.uint32), basic.view(np.uint32))
+    np.testing.assert_array_equal(g["out"].view(np.uint32), out.view(np.uint32))
