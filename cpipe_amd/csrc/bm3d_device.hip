@@ -637,8 +637,28 @@ __device__ __forceinline__ void dct_inv_n(float (&v)[NMAX], float* sb, int lane,
     }
 }
 
+// Multi-level Haar along the stack without its 1/sqrt(2) factors; output
+// [approx, d_L, .., d_1]. Coefficient i of an N = 2^L stack finished at level
+// haar_level<N>(i) (L for i = 0, 1; L - floor(log2 i) otherwise) and is the
+// orthonormal coefficient divided by 2^(-level/2); the factors are carried by
+// powers of two in the shrinkage instead (thresholds fl(thr fl(2^(level/2))),
+// Wiener noise levels sigma^2 2^level, a scale 2^-level on kept coefficients),
+// as in the CPU FP32 restatement.
 template <int N>
-__device__ __forceinline__ void haar_fwd_n(float (&v)[NMAX], float s) {
+constexpr int haar_level(int i) {
+    int L = 0;
+    while ((1 << L) < N) ++L;
+    if (i == 0) return L;
+    int f = 0;
+    while ((2 << f) <= i) ++f;
+    return L - f;
+}
+
+constexpr float HAAR_ROOT2[5] = {1.0f, 0x1.6a09e6p+0f, 2.0f, 0x1.6a09e6p+1f, 4.0f};  // fl(2^(l/2))
+constexpr float HAAR_INV_POW2[5] = {1.0f, 0.5f, 0.25f, 0.125f, 0.0625f};             // 2^-l
+
+template <int N>
+__device__ __forceinline__ void haar_fwd_n(float (&v)[NMAX]) {
 #pragma unroll
     for (int len = N; len > 1; len >>= 1) {
         float tmp[NMAX];
@@ -646,8 +666,8 @@ __device__ __forceinline__ void haar_fwd_n(float (&v)[NMAX], float s) {
 #pragma unroll
         for (int k = 0; k < hh; ++k) {
             const float a = v[2 * k], b = v[2 * k + 1];
-            tmp[k] = (a + b) * s;
-            tmp[hh + k] = (a - b) * s;
+            tmp[k] = a + b;
+            tmp[hh + k] = a - b;
         }
 #pragma unroll
         for (int k = 0; k < len; ++k) v[k] = tmp[k];
@@ -655,7 +675,7 @@ __device__ __forceinline__ void haar_fwd_n(float (&v)[NMAX], float s) {
 }
 
 template <int N>
-__device__ __forceinline__ void haar_inv_n(float (&v)[NMAX], float s) {
+__device__ __forceinline__ void haar_inv_n(float (&v)[NMAX]) {
 #pragma unroll
     for (int len = 2; len <= N; len <<= 1) {
         float tmp[NMAX];
@@ -663,63 +683,86 @@ __device__ __forceinline__ void haar_inv_n(float (&v)[NMAX], float s) {
 #pragma unroll
         for (int k = 0; k < hh; ++k) {
             const float a = v[k], d = v[hh + k];
-            tmp[2 * k] = (a + d) * s;
-            tmp[2 * k + 1] = (a - d) * s;
+            tmp[2 * k] = a + d;
+            tmp[2 * k + 1] = a - d;
         }
 #pragma unroll
         for (int k = 0; k < len; ++k) v[k] = tmp[k];
     }
 }
 
+// Reciprocal of d > 0 (normal, below 2^126): integer seed (relative error
+// < 0.051) and three Newton steps, within 0.5 ulp of 1/d over 1e-20..1e19.
+// Replaces the IEEE division in the Wiener gain (same operations as the CPU
+// FP32 restatement).
+__device__ __forceinline__ float rcp_nr(float d) {
+    float r = __uint_as_float(0x7EF311C3u - __float_as_uint(d));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float e = __builtin_fmaf(-d, r, 1.0f);
+        r = __builtin_fmaf(r, e, r);
+    }
+    return r;
+}
 
 // Filters one group in place: loads the members' pixels (noisy, and the basic
 // estimate for step 2), 3D transform, shrinkage, inverse; returns the group's
 // aggregation weight (1/N_nonzero for the hard threshold, 1/sum W^2 for Wiener).
+// Each member load is a buffer load: the member's corner (scalar) plus the
+// lane's byte offset boff = 4 (li * w + lj).
 template <int STEP, int N>
 __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (&mp)[NMAX],
-                                              const float* __restrict__ src,
-                                              const float* __restrict__ bsrc, int w, float* sb,
-                                              int lane, int li, int lj, float hs, float thr,
-                                              float s2) {
+                                              __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc,
+                                              int w, uint32_t boff, float* sb, int lane, int li,
+                                              int lj, float thr, float s2) {
     float b[NMAX];
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        const int qx = (int)(mp[m] & 0xffffu), qy = (int)(mp[m] >> 16);
-        const size_t off = (size_t)(qy + li) * w + qx + lj;
-        v[m] = src[off];
-        if (STEP == 2) b[m] = bsrc[off];
+        const uint32_t qx = mp[m] & 0xffffu, qy = mp[m] >> 16;
+        const uint32_t corner = (qy * (uint32_t)w + qx) * (uint32_t)sizeof(float);
+        v[m] = plane_load(src, boff, corner);
+        if (STEP == 2) b[m] = plane_load(bsrc, boff, corner);
     }
     dct_fwd_n<N>(v, sb, lane);
-    haar_fwd_n<N>(v, hs);
+    haar_fwd_n<N>(v);
     float wgt;
     if (STEP == 1) {
         int nnz = 0;
 #pragma unroll
         for (int m = 0; m < N; ++m) {
-            const bool keep = fabsf(v[m]) > thr;
+            const int l = haar_level<N>(m);
+            const float t = l == 0 ? thr : thr * HAAR_ROOT2[l];
+            const bool keep = fabsf(v[m]) > t;
             nnz += keep ? 1 : 0;
-            v[m] = keep ? v[m] : 0.0f;
+            v[m] = keep ? v[m] * HAAR_INV_POW2[l] : 0.0f;
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
         wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
     } else {
         dct_fwd_n<N>(b, sb, lane);
-        haar_fwd_n<N>(b, hs);
+        haar_fwd_n<N>(b);
+        // W = B^2 / (B^2 + sigma^2) on the orthonormal coefficients: Wl = W 2^-l
+        // = bb / fma(bb, 2^l, sigma^2 4^l), so Wl v is the kept coefficient scaled
+        // for the inverse; aw keeps sum W^2 / 4^l of the current level (times 4
+        // when the level drops) and is scaled back after the lane tree
         float aw = 0.0f;
 #pragma unroll
         for (int m = 0; m < N; ++m) {
+            const int l = haar_level<N>(m);
+            if (m > 0 && l != haar_level<N>(m - 1)) aw = aw * 4.0f;
             const float bb = b[m] * b[m];
-            const float d = bb + s2;
-            const float W = d > 0.0f ? bb / d : 1.0f;
+            const float d = __builtin_fmaf(bb, (float)(1 << l), s2 * (float)(1 << (2 * l)));
+            const float W = d > 0x1p-100f ? bb * rcp_nr(d) : HAAR_INV_POW2[l];
             v[m] = W * v[m];
             aw = (m == 0) ? W * W : __builtin_fmaf(W, W, aw);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
+        aw = aw * (float)(1 << (2 * haar_level<N>(N - 1)));
         wgt = aw > 0.0f ? 1.0f / aw : 1.0f;
     }
-    haar_inv_n<N>(v, hs);
+    haar_inv_n<N>(v);
     dct_inv_n<N>(v, sb, lane, li, lj);
     return wgt;
 }
@@ -756,16 +799,17 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     const int li = lane >> 3, lj = lane & 7;
     float* sb = scratch[wave];
     const float kw = c_kaiser2[lane];
-    const float hs = (float)M_SQRT1_2;
-    const float* __restrict__ src = F.z[c];
-    const float* __restrict__ bsrc = F.bz[c];
+    const int w = p.w;
+    const uint32_t plane_bytes = (uint32_t)((size_t)w * p.h * sizeof(float));
+    const __amdgpu_buffer_rsrc_t src = plane_rsrc(F.z[c], plane_bytes);
+    const __amdgpu_buffer_rsrc_t bsrc = plane_rsrc(F.bz[c], plane_bytes);
+    const uint32_t boff = (uint32_t)(li * w + lj) * (uint32_t)sizeof(float);
     const float sc = F.sig[c];
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
     // noise-profile map (MAP): each group shrinks with its reference patch's sigma
     // and weighs its aggregation by 1/sigma^2 (which a single sigma cancels)
     const float* __restrict__ smap = MAP ? F.smap + (size_t)c * p.nx * p.ny : nullptr;
-    const int w = p.w;
     const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
 
@@ -780,15 +824,10 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
             const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
             const size_t ref = (size_t)by * p.nx + bx;
             n = floor_pow2(p.cnt[ref]);
-            const uint4* pp = reinterpret_cast<const uint4*>(p.pos + ref * NMAX);
+            // the match list through the scalar cache (wave-uniform address)
+            const uint32_t CONST_AS* lp = (const uint32_t CONST_AS*)(p.pos + ref * NMAX);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 t = pp[q];
-                mp[4 * q + 0] = __builtin_amdgcn_readfirstlane(t.x);  // wave-uniform
-                mp[4 * q + 1] = __builtin_amdgcn_readfirstlane(t.y);
-                mp[4 * q + 2] = __builtin_amdgcn_readfirstlane(t.z);
-                mp[4 * q + 3] = __builtin_amdgcn_readfirstlane(t.w);
-            }
+            for (int m = 0; m < NMAX; ++m) mp[m] = lp[m];
             float gthr = thr, gs2 = s2;
             if (MAP) {
                 const float sg = smap[ref];
@@ -796,11 +835,11 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
                 gs2 = sg * sg;
             }
             switch (n) {
-                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
-                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
-                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
-                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
-                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, sb, lane, li, lj, hs, gthr, gs2); break;
+                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
+                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
+                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
+                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
+                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
             }
             if (MAP) wgt = wgt * (1.0f / (gs2 > 1e-30f ? gs2 : 1.0f));
         }
@@ -1395,9 +1434,12 @@ int create_core(cpipe_bm3d_plan** out, const int* widths, const int* heights, in
                 int precision, int device) {
     *out = nullptr;
     if (n <= 0 || n > 4096) return fail(BAD_INDEX, "frame count out of range (1..4096)");
-    for (int f = 0; f < n; ++f)
+    for (int f = 0; f < n; ++f) {
         if (widths[f] <= 0 || heights[f] <= 0 || widths[f] > 65535 || heights[f] > 65535)
             return fail(BAD_INDEX, "width/height out of range (1..65535)");
+        if ((size_t)widths[f] * heights[f] * sizeof(float) > 0xffffffffull)
+            return fail(BAD_INDEX, "frame planes must stay below 4 GiB (width x height <= 2^30 pixels)");
+    }
     if (channels != 1 && channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
     if (precision != CPIPE_BM3D_PRECISION_FP32 && precision != CPIPE_BM3D_PRECISION_FP16 &&
         precision != CPIPE_BM3D_PRECISION_BF16)

@@ -102,6 +102,22 @@ __device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
     return v;
 }
 
+// Loads through the scalar cache: a pointer in the constant address space is
+// read with scalar loads when the address is wave-uniform (the data must not
+// change during the kernel).
+#define CONST_AS __attribute__((address_space(4)))
+
+// A buffer resource over one plane of `bytes` (< 4 GiB; plans refuse larger
+// planes): a load takes the lane's 32-bit byte offset in a VGPR and a
+// wave-uniform byte offset in an SGPR, so it needs no address arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float plane_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 // One read/add/write round over members [S, S + L) of a group (pairwise
 // disjoint patches): all reads, then all writes.
 template <int S, int L>

@@ -18,7 +18,9 @@
  * pins fail on each plausible slip in the shrinkage:
  *   1: hard threshold at 2.5 sigma instead of 2.7 sigma;
  *   2: step-1 aggregation weight N_nonzero instead of 1/N_nonzero;
- *   3: unnormalised Haar along the stack (sums forward, halves inverse);
+ *   3: unnormalised Haar along the stack whose coefficients are shrunk as if
+ *      orthonormal (FP64: sums forward, halves inverse; FP32: no per-level
+ *      threshold or noise-level factors);
  *   4: step-2 aggregation weight sum W^2 instead of 1/sum W^2.
  * 0 (the default) is the algorithm. */
 #ifndef ORC_PLANT
@@ -171,7 +173,6 @@ void oracle_kaiser_f64(double k[8]) {
 typedef struct {
     float C[64];   /* DCT matrix, row = frequency */
     float K2[64];  /* Kaiser 8x8 outer product */
-    float haar_s;  /* 1/sqrt(2) */
 } tables_f32;
 
 static void make_tables_f32(tables_f32* t) {
@@ -181,7 +182,6 @@ static void make_tables_f32(tables_f32* t) {
     for (int i = 0; i < 64; ++i) t->C[i] = (float)c[i];
     for (int i = 0; i < 8; ++i)
         for (int j = 0; j < 8; ++j) t->K2[i * 8 + j] = (float)(k[i] * k[j]);
-    t->haar_s = (float)M_SQRT1_2;
 }
 
 /* Thresholds on the (un-normalised) SSD over 64 pixels: tau * k^2. */
@@ -382,31 +382,70 @@ static void dct2_inv_f32(const float* t, float* x) {
     for (int i = 0; i < 8; ++i) dct8_inv_f32(y + i * 8, 1, x + i * 8, 1);
 }
 
-/* Orthonormal multi-level Haar along the stack; output [approx, d_L..d_1]. */
-static void haar_fwd_f32(float* v, int n, float s) {
+/* Multi-level Haar along the stack without its 1/sqrt(2) factors; output
+ * [approx, d_L, .., d_1]. Coefficient i of an n = 2^L stack finished at level
+ * haar_level(i) (L for i = 0, 1; L - floor(log2 i) otherwise) and equals the
+ * orthonormal coefficient divided by 2^(-level/2). The orthonormal transform's
+ * steps are carried by powers of two instead: hard thresholds at
+ * fl(thr * fl(2^(level/2))), Wiener noise levels sigma^2 2^level, and a scale
+ * 2^-level on every kept coefficient before the inverse (whose butterflies have
+ * no factors either), so the only roundings the factors add are the odd-level
+ * thresholds'. The GPU exact kernels use the same operations. */
+static void haar_fwd_f32(float* v, int n) {
     float tmp[ORC_NMAX];
     for (int len = n; len > 1; len >>= 1) {
         const int h = len >> 1;
         for (int k = 0; k < h; ++k) {
             const float a = v[2 * k], b = v[2 * k + 1];
-            tmp[k] = (a + b) * s;
-            tmp[h + k] = (a - b) * s;
+            tmp[k] = a + b;
+            tmp[h + k] = a - b;
         }
         for (int k = 0; k < len; ++k) v[k] = tmp[k];
     }
 }
 
-static void haar_inv_f32(float* v, int n, float s) {
+static void haar_inv_f32(float* v, int n) {
     float tmp[ORC_NMAX];
     for (int len = 2; len <= n; len <<= 1) {
         const int h = len >> 1;
         for (int k = 0; k < h; ++k) {
             const float a = v[k], d = v[h + k];
-            tmp[2 * k] = (a + d) * s;
-            tmp[2 * k + 1] = (a - d) * s;
+            tmp[2 * k] = a + d;
+            tmp[2 * k + 1] = a - d;
         }
         for (int k = 0; k < len; ++k) v[k] = tmp[k];
     }
+}
+
+static int haar_level(int i, int n) {
+    int L = 0;
+    while ((1 << L) < n) ++L;
+    if (i == 0) return L;
+    int f = 0;
+    while ((2 << f) <= i) ++f;
+    return L - f;
+}
+
+/* fl(2^(level/2)) for level 0..4, and 2^-level. */
+static const float kHaarRoot2[5] = {1.0f, 0x1.6a09e6p+0f, 2.0f, 0x1.6a09e6p+1f, 4.0f};
+static const float kHaarInvPow2[5] = {1.0f, 0.5f, 0.25f, 0.125f, 0.0625f};
+
+/* Reciprocal of d > 0 (a normal number below 2^126) in FP32: an integer seed
+ * (relative error < 0.051) and three Newton steps e = fma(-d, r, 1),
+ * r = fma(r, e, r); within 0.5 ulp of 1/d over 1e-20..1e19 (checked against
+ * the correctly rounded reciprocal). Replaces an IEEE division in the Wiener
+ * gain: the GPU exact kernels run the same operations. */
+static inline float rcp_nr_f32(float d) {
+    uint32_t b;
+    memcpy(&b, &d, 4);
+    b = 0x7EF311C3u - b;
+    float r;
+    memcpy(&r, &b, 4);
+    for (int i = 0; i < 3; ++i) {
+        const float e = fmaf(-d, r, 1.0f);
+        r = fmaf(r, e, r);
+    }
+    return r;
 }
 
 static inline int floor_pow2(int c) {
@@ -459,10 +498,6 @@ static void group_job_f32(void* vc, int job) {
     float* num = a->partial + ((size_t)c * a->tx * a->ty + tile) * a->tile_stride;
     float* den = num + FOOT_DOFS;
     memset(num, 0, sizeof(float) * 2 * FOOT_DOFS);
-    const float s = a->tb->haar_s;
-    /* the inverse stack transform's scale (differs only under planted mistake 3) */
-    const float s_inv = ORC_PLANT == 3 ? 0.5f : s;
-    const float s_fwd = ORC_PLANT == 3 ? 1.0f : s;
     float g[ORC_NMAX][64], gb[ORC_NMAX][64];
     for (int by = ty * ORC_TILE_Y; by < ty * ORC_TILE_Y + ORC_TILE_Y && by < a->ny; ++by) {
         for (int bx = tx * ORC_TILE_X; bx < tx * ORC_TILE_X + ORC_TILE_X && bx < a->nx; ++bx) {
@@ -487,49 +522,71 @@ static void group_job_f32(void* vc, int job) {
                     dct2_fwd_f32(tmp, gb[m]);
                 }
             }
+            /* per Haar level: threshold, Wiener noise level sigma^2 2^l (as the
+             * fma operands 2^l and sigma^2 4^l, see below) and inverse scale;
+             * planted mistake 3 drops the level factors */
+            float tl[5], pl[5], ql[5];
+            for (int l = 0; l < 5; ++l) {
+                tl[l] = ORC_PLANT == 3 ? thr : thr * kHaarRoot2[l];
+                pl[l] = (float)(1 << l);
+                ql[l] = s2 * (float)(1 << (ORC_PLANT == 3 ? l : 2 * l));
+            }
             float wgt;
             if (a->step == 1) {
                 int nnz = 0;
                 for (int p = 0; p < 64; ++p) {
                     float v[ORC_NMAX];
                     for (int m = 0; m < n; ++m) v[m] = g[m][p];
-                    haar_fwd_f32(v, n, s_fwd);
+                    haar_fwd_f32(v, n);
                     for (int m = 0; m < n; ++m) {
-                        if (fabsf(v[m]) > thr) {
+                        const int l = haar_level(m, n);
+                        if (fabsf(v[m]) > tl[l]) {
                             ++nnz;
+                            v[m] = v[m] * kHaarInvPow2[l];
                         } else {
                             v[m] = 0.0f;
                         }
                     }
-                    haar_inv_f32(v, n, s_inv);
+                    haar_inv_f32(v, n);
                     for (int m = 0; m < n; ++m) g[m][p] = v[m];
                 }
                 wgt = nnz > 0 ? (ORC_PLANT == 2 ? (float)nnz : 1.0f / (float)nnz) : 1.0f;
             } else {
+                /* W = B^2 / (B^2 + sigma^2) on orthonormal coefficients B = 2^(-l/2) b:
+                 * Wl = W 2^-l = bb / fma(bb, 2^l, sigma^2 4^l) with bb = b^2 (the
+                 * reciprocal by rcp_nr_f32), so Wl v is the kept coefficient already
+                 * scaled for the inverse; sum W^2 = 4^l sum Wl^2 per level, kept as
+                 * acc = sum W^2 / 4^l of the current level (times 4 when the level
+                 * drops by one) and scaled back after the lane tree */
                 float accw[64];
+                int lend = 0;
                 for (int p = 0; p < 64; ++p) {
                     float v[ORC_NMAX], b[ORC_NMAX];
                     for (int m = 0; m < n; ++m) {
                         v[m] = g[m][p];
                         b[m] = gb[m][p];
                     }
-                    haar_fwd_f32(v, n, s_fwd);
-                    haar_fwd_f32(b, n, s_fwd);
+                    haar_fwd_f32(v, n);
+                    haar_fwd_f32(b, n);
                     float acc = 0.0f;
                     for (int m = 0; m < n; ++m) {
+                        const int l = haar_level(m, n);
+                        if (m > 0 && l != haar_level(m - 1, n)) acc = acc * 4.0f;
                         const float bb = b[m] * b[m];
-                        const float d = bb + s2;
-                        const float W = d > 0.0f ? bb / d : 1.0f;
+                        const float d = fmaf(bb, pl[l], ql[l]);
+                        const float W = d > 0x1p-100f ? bb * rcp_nr_f32(d) : kHaarInvPow2[l];
                         v[m] = W * v[m];
                         acc = (m == 0) ? W * W : fmaf(W, W, acc);
+                        lend = l;
                     }
                     accw[p] = acc;
-                    haar_inv_f32(v, n, s_inv);
+                    haar_inv_f32(v, n);
                     for (int m = 0; m < n; ++m) g[m][p] = v[m];
                 }
                 for (int off = 32; off >= 1; off >>= 1)
                     for (int p = 0; p < off; ++p) accw[p] = accw[p] + accw[p + off];
-                wgt = accw[0] > 0.0f ? (ORC_PLANT == 4 ? accw[0] : 1.0f / accw[0]) : 1.0f;
+                const float aw = accw[0] * (float)(1 << (2 * lend));
+                wgt = aw > 0.0f ? (ORC_PLANT == 4 ? aw : 1.0f / aw) : 1.0f;
             }
             /* with a map the textbook 1/sigma^2 factor no longer cancels */
             if (a->smap) wgt = wgt * (1.0f / (s2 > 1e-30f ? s2 : 1.0f));
