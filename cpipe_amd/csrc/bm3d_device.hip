@@ -2031,6 +2031,212 @@ int apply_choice(cpipe_bm3d_plan* p, const TuneChoice& ch) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The entry point that allocates nothing (cpipe_bm3d_noalloc_planes). Every
+// buffer of an exact FP32 run -- frame table, work lists, tile geometry,
+// opponent and basic-estimate planes, match lists, tile partials and the
+// consistency flag -- is carved from `scratch`, one file-scope device array
+// that each GPU the library runs on holds until the process ends. A setup
+// kernel writes the table, the geometry and the work lists, so a call only
+// enqueues kernels: no allocation and no host-to-device copy.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t NOALLOC_BYTES = (size_t)CPIPE_BM3D_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_base[64] = {};  // device address of `scratch` on each device
+
+// The device address of `scratch` on the current device `dev`, looked up on the
+// first call there (the array's name is only a host-side stand-in).
+int noalloc_base(int dev, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_base[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(INTERNAL, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_base[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_noalloc_base[dev];
+    return OK;
+}
+
+constexpr int NOALLOC_BAND = 32;  // rows per colour / merge rectangle (the plans' default)
+
+// Byte offsets into `scratch` (256-aligned) and the work counts of one frame,
+// laid out as a single-frame exact FP32 plan lays out its buffers.
+struct NoallocLayout {
+    size_t tab, rects, flags, bm, grp, geo, z, bz, pos0, pos1, cnt0, cnt1, partial, total;
+    int small, nx, ny, ntx, nty, nbtx, nbty, n_bm, n_grp, n_rects;
+};
+
+NoallocLayout noalloc_layout(int w, int h, int c) {
+    NoallocLayout L{};
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t npx = (size_t)w * h;
+    L.small = w < K || h < K;
+    L.n_rects = L.small ? 1 : (h + NOALLOC_BAND - 1) / NOALLOC_BAND;
+    L.tab = take(sizeof(FrameTab));
+    L.rects = take(sizeof(FrameRect) * L.n_rects);
+    L.flags = take(2 * sizeof(int));  // [0] consistency flag, [1] gate (unused: no pilot)
+    if (!L.small) {
+        L.nx = ref_count(w);
+        L.ny = ref_count(h);
+        L.ntx = (L.nx + TILE - 1) / TILE;
+        L.nty = (L.ny + EXACT_TY - 1) / EXACT_TY;
+        L.nbtx = (L.nx + BM_TX - 1) / BM_TX;
+        L.nbty = (L.ny + BM_TY - 1) / BM_TY;
+        L.n_bm = L.nbtx * L.nbty;
+        L.n_grp = L.ntx * L.nty;
+        const size_t nref = (size_t)L.nx * L.ny;
+        const size_t tile_stride = 2 * (size_t)FOOT * foot_edge(EXACT_TY);
+        L.bm = take(sizeof(int4) * L.n_bm);
+        L.grp = take(sizeof(int2) * L.n_grp);
+        L.geo = take(sizeof(int) * (2 * (size_t)w + 2 * (size_t)h + 2 * (size_t)L.ntx + 2 * (size_t)L.nty));
+        L.z = take(sizeof(float) * npx * c);
+        L.bz = take(sizeof(float) * npx * c);
+        L.pos0 = take(sizeof(uint32_t) * nref * NMAX);
+        L.pos1 = take(sizeof(uint32_t) * nref * NMAX);
+        L.cnt0 = take(nref);
+        L.cnt1 = take(nref);
+        L.partial = take(sizeof(float) * tile_stride * L.n_grp * c);
+    }
+    L.total = off;
+    return L;
+}
+
+struct NoallocSetup {
+    unsigned char* base;
+    NoallocLayout L;
+    int w, h, c;
+    float sigma, sig[3];
+    const float* in[3];
+    float* out[3];
+    size_t in_pitch, out_pitch;
+};
+
+__device__ inline void tile_box_dev(int t, int tile, int nref, int n, int* f0, int* f1) {
+    const int first = ref_pos(t * tile, n);
+    const int lastref = min(t * tile + tile - 1, nref - 1);
+    const int last = ref_pos(lastref, n);
+    *f0 = max(0, first - R);
+    *f1 = min(n, last + R + K);
+}
+
+// One workgroup: the frame table, the merge / colour rectangles, the work lists
+// and the tile geometry of one frame, as the plans build them on the host.
+__global__ __launch_bounds__(256) void k_noalloc_setup(NoallocSetup S) {
+    const NoallocLayout& L = S.L;
+    unsigned char* b = S.base;
+    FrameTab* T = reinterpret_cast<FrameTab*>(b + L.tab);
+    FrameRect* rects = reinterpret_cast<FrameRect*>(b + L.rects);
+    int* flags = reinterpret_cast<int*>(b + L.flags);
+    const int tid = threadIdx.x, nt = blockDim.x, w = S.w, h = S.h;
+    int* txlo = reinterpret_cast<int*>(b + L.geo);
+    int* txhi = txlo + w;
+    int* tylo = txhi + w;
+    int* tyhi = tylo + h;
+    int* fx0 = tyhi + h;
+    int* fx1 = fx0 + L.ntx;
+    int* fy0 = fx1 + L.ntx;
+    int* fy1 = fy0 + L.nty;
+    if (tid == 0) {
+        FrameTab t = {};
+        t.w = w;
+        t.h = h;
+        t.c = S.c;
+        t.gate = flags + 1;
+        for (int ch = 0; ch < 3; ++ch) {
+            const int cc = ch < S.c ? ch : 0;
+            t.sig[ch] = S.sig[ch];
+            t.in[ch] = S.in[cc];
+            t.out[ch] = S.out[cc];
+        }
+        t.sigma = S.sigma;
+        t.in_pitch = S.in_pitch;
+        t.out_pitch = S.out_pitch;
+        if (!L.small) {
+            const size_t npx = (size_t)w * h;
+            t.nx = L.nx;
+            t.ny = L.ny;
+            t.ntx = L.ntx;
+            t.nty = L.nty;
+            t.tile_y = EXACT_TY;
+            t.dofs = FOOT * foot_edge(EXACT_TY);
+            t.tile_stride = 2 * (size_t)t.dofs;
+            for (int ch = 0; ch < 3; ++ch) {
+                const int cc = ch < S.c ? ch : 0;
+                t.z[ch] = reinterpret_cast<float*>(b + L.z) + cc * npx;
+                t.bz[ch] = reinterpret_cast<float*>(b + L.bz) + cc * npx;
+            }
+            t.pos[0] = reinterpret_cast<uint32_t*>(b + L.pos0);
+            t.pos[1] = reinterpret_cast<uint32_t*>(b + L.pos1);
+            t.cnt[0] = b + L.cnt0;
+            t.cnt[1] = b + L.cnt1;
+            t.partial = reinterpret_cast<float*>(b + L.partial);
+            t.txlo = txlo;
+            t.txhi = txhi;
+            t.tylo = tylo;
+            t.tyhi = tyhi;
+            t.fx0 = fx0;
+            t.fx1 = fx1;
+            t.fy0 = fy0;
+            t.fy1 = fy1;
+        }
+        *T = t;
+        flags[0] = 0;
+        flags[1] = 0;
+    }
+    if (L.small) {
+        if (tid == 0) rects[0] = FrameRect{0, 0, 0, w, h, 0, 0, 0};
+        return;
+    }
+    for (int i = tid; i < L.n_rects; i += nt)
+        rects[i] = FrameRect{0, 0, i * NOALLOC_BAND, w, min(h, (i + 1) * NOALLOC_BAND), 0, 0, 0};
+    int4* bm = reinterpret_cast<int4*>(b + L.bm);
+    for (int i = tid; i < L.n_bm; i += nt) bm[i] = make_int4(0, i % L.nbtx, i / L.nbtx, 0);
+    int2* grp = reinterpret_cast<int2*>(b + L.grp);
+    for (int i = tid; i < L.n_grp; i += nt) grp[i] = make_int2(0, i);
+    for (int t = tid; t < L.ntx; t += nt) tile_box_dev(t, TILE, L.nx, w, &fx0[t], &fx1[t]);
+    for (int t = tid; t < L.nty; t += nt) tile_box_dev(t, EXACT_TY, L.ny, h, &fy0[t], &fy1[t]);
+    // first and last tile whose footprint covers each column / row
+    for (int x = tid; x < w; x += nt) {
+        int lo = -1, hi = -2;
+        for (int t = 0; t < L.ntx; ++t) {
+            int a0, a1;
+            tile_box_dev(t, TILE, L.nx, w, &a0, &a1);
+            if (x >= a0 && x < a1) {
+                if (lo < 0) lo = t;
+                hi = t;
+            }
+        }
+        txlo[x] = lo;
+        txhi[x] = hi;
+    }
+    for (int y = tid; y < h; y += nt) {
+        int lo = -1, hi = -2;
+        for (int t = 0; t < L.nty; ++t) {
+            int a0, a1;
+            tile_box_dev(t, EXACT_TY, L.ny, h, &a0, &a1);
+            if (y >= a0 && y < a1) {
+                if (lo < 0) lo = t;
+                hi = t;
+            }
+        }
+        tylo[y] = lo;
+        tyhi[y] = hi;
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* cpipe_bm3d_last_error(void) { return g_err.c_str(); }
@@ -2653,6 +2859,77 @@ void cpipe_bm3d_host_release(void) {
         cpipe_bm3d_pilot_destroy(kv.second);
     }
     hc.nets.clear();
+}
+
+int cpipe_bm3d_noalloc_planes(const float* const* in, size_t in_pitch, float* const* out,
+                              size_t out_pitch, int w, int h, int channels, float sigma, void* stream) {
+    if (!in || !out) return fail(BAD_INDEX, "null argument");
+    if (channels != 1 && channels != 3) return fail(BAD_INDEX, "channels must be 1 or 3");
+    for (int c = 0; c < channels; ++c)
+        if (!in[c] || !out[c]) return fail(BAD_INDEX, "null plane");
+    if (w <= 0 || h <= 0) return fail(BAD_INDEX, "empty image");
+    if (w > CPIPE_BM3D_NOALLOC_MAX_SIDE || h > CPIPE_BM3D_NOALLOC_MAX_SIDE ||
+        (size_t)w * h > (size_t)CPIPE_BM3D_NOALLOC_MAX_PIXELS)
+        return fail(BAD_INDEX, "cpipe_bm3d_noalloc_planes: frame " + std::to_string(w) + "x" + std::to_string(h) +
+                                   " exceeds its maximum (width x height <= 1048576 pixels, sides <= 8192)");
+    if (in_pitch < (size_t)w || out_pitch < (size_t)w) return fail(BAD_INDEX, "pitch smaller than width");
+    if (!(sigma >= 0.0f) || !std::isfinite(sigma)) return fail(BAD_INDEX, "sigma must be finite and >= 0");
+    const NoallocLayout L = noalloc_layout(w, h, channels);
+    if (L.total > NOALLOC_BYTES) return fail(INTERNAL, "no-allocation scratch too small for an admitted frame");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return fail(UNSUPPORTED, "no HIP device visible");
+    if (dev >= 64) return fail(BAD_INDEX, "device ordinal out of range");
+    int rc = ensure_tables(dev);
+    if (rc) return rc;
+    unsigned char* base = nullptr;
+    if ((rc = noalloc_base(dev, &base))) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    NoallocSetup S{};
+    S.base = base;
+    S.L = L;
+    S.w = w;
+    S.h = h;
+    S.c = channels;
+    S.sigma = sigma;
+    if (channels == 1) {
+        S.sig[0] = S.sig[1] = S.sig[2] = sigma;
+    } else {  // the opponent channels' sigmas, as the plans set them
+        S.sig[0] = sigma * (float)std::sqrt(1.0 / 3.0);
+        S.sig[1] = sigma * (float)std::sqrt(0.5);
+        S.sig[2] = sigma * (float)std::sqrt(0.375);
+    }
+    for (int c = 0; c < 3; ++c) {
+        S.in[c] = c < channels ? in[c] : nullptr;
+        S.out[c] = c < channels ? out[c] : nullptr;
+    }
+    S.in_pitch = in_pitch;
+    S.out_pitch = out_pitch;
+    k_noalloc_setup<<<1, 256, 0, st>>>(S);
+    HIP_CHECK(hipGetLastError());
+    const FrameTab* tab = reinterpret_cast<const FrameTab*>(base + L.tab);
+    const FrameRect* rects = reinterpret_cast<const FrameRect*>(base + L.rects);
+    if (L.small) {  // smaller than one patch: copied through
+        k_copy_through<<<dim3((w + 255) / 256, h, 1), 256, 0, st>>>(tab, rects);
+        HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+    const int4* bm = reinterpret_cast<const int4*>(base + L.bm);
+    const int2* grp = reinterpret_cast<const int2*>(base + L.grp);
+    int* err = reinterpret_cast<int*>(base + L.flags);
+    const dim3 pg((w + 255) / 256, NOALLOC_BAND, L.n_rects);
+    k_colour_in<<<pg, 256, 0, st>>>(tab, rects);
+    for (int step = 1; step <= 2; ++step) {
+        k_block_match<<<L.n_bm, 256, 0, st>>>(tab, bm, step, tau_ssd(step), 0);
+        const GroupLaunch gl{tab, grp, 0, err};
+        const dim3 gg(L.n_grp, channels);
+        if (step == 1)
+            k_group<1, EXACT_TY, EXACT_NW, false><<<gg, 64 * EXACT_NW, 0, st>>>(gl);
+        else
+            k_group<2, EXACT_TY, EXACT_NW, false><<<gg, 64 * EXACT_NW, 0, st>>>(gl);
+        k_merge<<<pg, 256, 0, st>>>(tab, rects, step == 1 ? TO_BASIC : TO_OUTPUT, 0);
+    }
+    HIP_CHECK(hipGetLastError());
+    return OK;
 }
 
 const void* cpipe_bm3d_pilot_embedded(size_t* size) {

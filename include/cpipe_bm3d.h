@@ -315,6 +315,34 @@ int cpipe_bm3d_device_planes(const float* const* in_planes, size_t in_pitch,
 int cpipe_bm3d_device_rgba16f(const uint16_t* in, size_t in_stride_px, uint16_t* out,
                               size_t out_stride_px, int width, int height, float sigma, int mode,
                               int precision, int device, int flags, void* stream);
+
+/* BM3D without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The same run as
+ * cpipe_bm3d_device_planes / cpipe_bm3d_plan_run with precision FP32 and the
+ * plans' defaults (exact FP32, both steps, no learned pilot, no noise-profile
+ * map), with bit-identical output, on device planes of the current HIP device
+ * (pitches in floats), enqueued on `stream` (hipStream_t as void*, NULL = null
+ * stream) and not waited for.
+ *
+ * Every buffer the run needs lives in one static device array of
+ * CPIPE_BM3D_NOALLOC_SCRATCH_MIB MiB that each GPU the library runs on holds
+ * until the process ends; the call itself calls no hipMalloc, hipFree or other
+ * allocating routine and copies nothing from the host. Frames up to
+ * CPIPE_BM3D_NOALLOC_MAX_PIXELS pixels with both sides up to
+ * CPIPE_BM3D_NOALLOC_MAX_SIDE are accepted, 1 or 3 channels; larger frames are
+ * refused with CPIPE_BM3D_BAD_INDEX. Calls on one GPU share its array: calls
+ * that overlap in time there (other streams, or another thread) race, so order
+ * them on one stream or synchronise between them. The first call on a device
+ * looks up the array's address and uploads the constant tables (keep it out of
+ * a graph capture); later calls enqueue kernels only. The aggregation's
+ * consistency flag is cleared and set on the device but not read back. */
+#define CPIPE_BM3D_NOALLOC_SCRATCH_MIB 320
+#define CPIPE_BM3D_NOALLOC_MAX_PIXELS 1048576
+#define CPIPE_BM3D_NOALLOC_MAX_SIDE 8192
+int cpipe_bm3d_noalloc_planes(const float* const* in_planes, size_t in_pitch,
+                              float* const* out_planes, size_t out_pitch, int width, int height,
+                              int channels, float sigma, void* stream);
 /* Frees every cached plan, buffer and pilot network of the host conveniences. */
 void cpipe_bm3d_host_release(void);
 /* The learned-pilot weight blob embedded in the library (cpipe_amd/weights/pilot_v1.bin). */
