@@ -61,17 +61,124 @@ def group_flops(counts: np.ndarray, channels: int, step: int) -> float:
     return float(per.sum()) * channels
 
 
+def host_cores():
+    """CPU cores this job may use: the affinity mask, capped by a cgroup CPU quota
+    when one is set (os.cpu_count() can report the whole machine)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(noisy: np.ndarray, sigma: float, sample_w: int, sample_h: int):
+    """The CPU FP32 BM3D (same algorithm and operation order as the GPU FP32 path,
+    bit-identical output; pthreads) on every core available to the job, and on one
+    core fewer (the reference Scheduler runs N-1 workers, Scheduler.cpp:116-119)."""
     from oracle import oracle
-    threads = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     crop = np.ascontiguousarray(noisy[:, :sample_h, :sample_w])
-    t0 = time.perf_counter()
-    oracle.bm3d_f32(crop, sigma, nthreads=threads)
-    dt = time.perf_counter() - t0
+    res = {}
+    for key, threads in (("all", cores), ("n_minus_1", max(1, cores - 1))):
+        t0 = time.perf_counter()
+        oracle.bm3d_f32(crop, sigma, nthreads=threads)
+        dt = time.perf_counter() - t0
+        res[key] = (threads, dt)
+    threads, dt = res["all"]
+    c = noisy.shape[0]
+    full = sample_w == noisy.shape[2] and sample_h == noisy.shape[1]
     return {"value": round(sample_w * sample_h / dt / 1e6, 4), "unit": "Mpixel/s",
             "cores": threads, "kind": "port",
-            "sample": f"{sample_w}x{sample_h}x3 top-left crop of the same noisy frame, "
-                      f"CPU FP32 BM3D (same algorithm and operation order), {dt:.2f} s"}
+            "sample": (f"{'the full' if full else 'a ' + str(sample_w) + 'x' + str(sample_h) + ' top-left crop of the same'}"
+                       f" {sample_w}x{sample_h}x{c} noisy frame, CPU FP32 BM3D (same algorithm and "
+                       f"operation order, bit-identical output), {dt:.2f} s on {threads} threads"),
+            "n_minus_1": {"value": round(sample_w * sample_h / res["n_minus_1"][1] / 1e6, 4),
+                          "cores": res["n_minus_1"][0], "seconds": round(res["n_minus_1"][1], 2)},
+            "cpu": cpu_model(), "os_cpu_count": os.cpu_count(), "flags": "gcc -O3 -march=x86-64-v3 -ffp-contract=off"}
+
+
+def cpu_config0(torch):
+    """Config 0 (512x512 grey, Gaussian sigma 0.05): the CPU FP32 BM3D on all
+    available cores and the GPU FP32 plan, whose output must be bit-identical."""
+    from oracle import oracle
+    clean, noisy, sigma = synth.workload("gray512", seed=0)
+    cores = host_cores()
+    t0 = time.perf_counter()
+    want = oracle.bm3d_f32(noisy, sigma, nthreads=cores)
+    dt = time.perf_counter() - t0
+    src = torch.from_numpy(noisy).cuda()
+    dst = torch.empty_like(src)
+    plan = engine.Plan(512, 512, 1, "fp32", 0)
+    ms, _ = time_form(torch, plan, src, dst, sigma, 20, 3, torch.cuda.current_stream())
+    got = dst.cpu().numpy()
+    plan.close()
+    return {"workload": "gray512 512x512x1 float32, Gaussian sigma 0.05, both steps",
+            "cpu": {"value": round(0.262144 / dt, 3), "unit": "Mpixel/s", "seconds": round(dt, 3), "cores": cores},
+            "gpu": {"value": round(0.262144 / (ms / 1e3), 1), "unit": "Mpixel/s", "ms": round(ms, 3)},
+            "gpu_over_cpu": round(dt * 1e3 / ms, 1),
+            "bit_exact": bool(np.array_equal(got.view(np.uint32), want.view(np.uint32))),
+            "psnr_noisy_db": round(synth.psnr(noisy, clean), 3), "psnr_out_db": round(synth.psnr(got, clean), 3)}
+
+
+def compat_context(torch, noisy, sigma):
+    """The reference node's actual work at the frame size: the 2x2 box-mean blend
+    (bm3d_step1_generator.cpp:42-56) on RGBA16F, restated on the CPU (row bands on
+    every available core; the reference runs parallel(y) on N-1 workers) and the
+    GPU's k_compat_rgba16f. Not comparable work (about 16 bytes per pixel)."""
+    import concurrent.futures as cf
+    import ctypes
+    from oracle import oracle
+    c, h, w = noisy.shape
+    rgba = np.ones((h, w, 4), np.float32)
+    rgba[..., :min(c, 3)] = np.moveaxis(noisy[:3], 0, -1)
+    bits = np.ascontiguousarray(rgba.astype(np.float16).view(np.uint16))
+    out = np.empty_like(bits)
+    L = oracle.lib()
+    cores = host_cores()
+    band = ((h + cores - 1) // cores + 1) & ~1  # even heights: 2x2 blocks never straddle bands
+
+    def run(y0):
+        y1 = min(h, y0 + band)
+        L.oracle_compat_rgba16f(bits.ctypes.data + y0 * w * 8, w, out.ctypes.data + y0 * w * 8, w, w, y1 - y0,
+                                float(sigma), 1)
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        list(ex.map(run, range(0, h, band)))
+    dt = time.perf_counter() - t0
+    src = torch.from_numpy(bits.view(np.int16)).cuda()
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        engine.compat_rgba16f(src, dst, sigma)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(20):
+        engine.compat_rgba16f(src, dst, sigma)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    same = bool(np.array_equal(dst.cpu().numpy().view(np.uint16), out))
+    return {"work": "reference node arithmetic: 2x2 box-mean blend, RGBA16F in/out",
+            "cpu": {"value": round(w * h / dt / 1e6, 1), "unit": "Mpixel/s", "seconds": round(dt, 3), "cores": cores},
+            "gpu": {"value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpixel/s", "ms": round(ms, 4),
+                    "hbm_gbs": round(16.0 * w * h / (ms / 1e3) / 1e9, 1)},
+            "bit_exact": same}
 
 
 BATCH_SIZES = [(4080, 3072)] * 4 + [(2040, 1536)] * 4 + [(1024, 768)] * 4
@@ -144,6 +251,35 @@ def batch_bench(torch, steps, warmup, stream, weights=None):
             res[key]["tune"] = tune["candidates"]
         b.close()
     return res
+
+
+PILOT_FLOP_PER_PX = 2 * 9 * (4 * 32 + 4 * 32 * 32 + 32 * 3)  # 6 conv layers: 4(+level) -> 32 x5 -> 3
+
+
+def form_roofline(prec, stage, bm_flop, g1_flop, g2_flop, npx):
+    """Roofline of a form's kernels from its HIP-event stage times and the same
+    counting rule as the headline: algorithmic FLOP per launch / launch time,
+    against the dense FP16/BF16 MFMA peak (mixed-precision group transforms and
+    the MFMA block matching, the learned pilot's convolutions) or the FP32 VALU
+    peak (exact FP32 kernels). The dominant kernel is the longest launch."""
+    mixed = prec in ("fp16", "bf16")
+    peak = PEAK_F16_TFLOPS if mixed else PEAK_FP32_TFLOPS
+    ks = {}
+    for key, flop, st in (("block_match", bm_flop, "bm2"), ("group_ht", g1_flop, "group1"),
+                          ("group_wiener", g2_flop, "group2")):
+        ms = stage.get(st, 0.0)
+        if ms > 0.1:  # stages skipped behind the pilot's gate take microseconds
+            ach = flop / (ms / 1e3) / 1e12
+            ks[key] = {"launch_ms": round(ms, 4), "flop_per_launch": flop, "achieved": round(ach, 2),
+                       "peak": peak, "frac": round(ach / peak, 4), "bound": "mfma" if mixed else "valu"}
+    if stage.get("pilot", 0.0) > 0.1:
+        flop = float(PILOT_FLOP_PER_PX) * npx
+        ach = flop / (stage["pilot"] / 1e3) / 1e12
+        ks["pilot"] = {"launch_ms": round(stage["pilot"], 4), "flop_per_launch": flop, "achieved": round(ach, 2),
+                       "peak": PEAK_F16_TFLOPS, "frac": round(ach / PEAK_F16_TFLOPS, 4), "bound": "mfma",
+                       "note": "network + overlapped plain step-1 spot checks"}
+    dom = max(ks, key=lambda k: ks[k]["launch_ms"])
+    return dict(ks[dom], kernel=dom, unit="TFLOP/s", kernels=ks)
 
 
 def time_form(torch, plan, src, dst, sigma, steps, warmup, stream):
@@ -353,7 +489,8 @@ def main():
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (node path) and node-chain legs")
     ap.add_argument("--no-wavelet", action="store_true",
                     help="skip the downstream wavelet BayesShrink and precision-convert node legs")
-    ap.add_argument("--cpu-sample", default="2048x1536")
+    ap.add_argument("--cpu-sample", default="full",
+                    help="CPU baseline sample: 'full' (the whole frame) or WxH (top-left crop)")
     ap.add_argument("--fp64-crop", default="512x384")
     ap.add_argument("--pilot", default=None,
                     help="learned-pilot weight blob (default: the packaged weights)")
@@ -416,7 +553,7 @@ def main():
                        "launch_ms": round(kms, 4), "flop_per_launch": kflop, "traffic": traffic_of(name)}
     flop, ms = kernels[dominant]
     achieved = flop / (ms / 1e3) / 1e12
-    roofline = {"bound": "mfma", "kernel": dominant, "achieved": round(achieved, 3),
+    roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic_of(dominant),
                 "flop_per_launch": flop, "launch_ms": round(ms, 4),
@@ -459,6 +596,7 @@ def main():
                  "stage_ms": {k: round(v, 4) for k, v in fstage.items()}}
         if learned:
             entry["spot_checks"] = fplan.pilot_report()
+        entry["roofline"] = form_roofline(prec, fstage, bm_flop, g1_flop, g2_flop, w * h)
         fplan.close()
         forms[name] = entry
         print(f"# {name}: {json.dumps(entry)}", file=sys.stderr, flush=True)
@@ -552,11 +690,16 @@ def main():
         batch = batch_bench(torch, max(2, args.steps // 3), 1, stream, args.pilot)
         print(f"# batch: {json.dumps(batch)}", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = config0 = compat = None
     if not args.no_cpu:
-        sw, sh = (int(v) for v in args.cpu_sample.split("x"))
+        sw, sh = (w, h) if args.cpu_sample == "full" else (int(v) for v in args.cpu_sample.split("x"))
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(noisy, sigma, min(sw, w), min(sh, h))
+        print(f"# cpu baseline: {json.dumps(cpu)}", file=sys.stderr, flush=True)
+        config0 = cpu_config0(torch)
+        print(f"# config 0: {json.dumps(config0)}", file=sys.stderr, flush=True)
+        compat = compat_context(torch, noisy, sigma)
+        print(f"# compat: {json.dumps(compat)}", file=sys.stderr, flush=True)
 
     line = {
         "metric": "BM3D Mpixel/s at 12.5MP RGB (plain FP32, both steps)",
@@ -570,6 +713,8 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "config0_gray512": config0,
+        "compat_step1": compat,
         "forms": forms,
         "fp64_crop": fp64,
         "batch12": batch,

@@ -2,7 +2,7 @@
 
 Outputs (git-ignored, shipped with the tree):
   cpipe_amd/lib/libcpipe_bm3d.so      HIP engine + C ABI (include/cpipe_bm3d.h, include/cpipe_wavelet.h,
-                                      include/cpipe_convert.h)
+                                      include/cpipe_convert.h, include/cpipe_dng.h)
                                       + the nodes' descriptors in the `cpipe_registry` section
   cpipe_amd/lib/libcpipe_minihost.so  minimal cpipe host (suites, registry walk, one-node runner,
                                       host and HIP device-resident buffers)
@@ -110,6 +110,7 @@ def build(verbose: bool = False) -> dict:
     wav_o = os.path.join(BUILD, "wavelet.o")
     cvt_o = os.path.join(BUILD, "convert.o")
     node_o = os.path.join(BUILD, "node.o")
+    dng_o = os.path.join(BUILD, "dng_noise.o")
     man_o = os.path.join(BUILD, "manifest_gen.o")
     blob_o = os.path.join(BUILD, "pilot_blob_gen.o")
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "bm3d_device.hip"), "-o", dev_o],
@@ -120,11 +121,12 @@ def build(verbose: bool = False) -> dict:
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "wavelet.hip"), "-o", wav_o], verbose)
     _run([hipcc, *HIP_FLAGS, f"-I{CSRC}", "-c", os.path.join(CSRC, "convert.hip"), "-o", cvt_o], verbose)
     _run(["g++", *CXX_FLAGS, *HIP_HOST_FLAGS, "-c", os.path.join(CSRC, "node.cpp"), "-o", node_o], verbose)
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "dng_noise.cpp"), "-o", dng_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", manifest_cpp, "-o", man_o], verbose)
     _run(["g++", *CXX_FLAGS, "-c", blob_cpp, "-o", blob_o], verbose)
     tmp = ENGINE_SO + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, dev_o, pilot_o, mx_o, bmx_o, wav_o, cvt_o, node_o,
-          man_o, blob_o], verbose)
+          dng_o, man_o, blob_o], verbose)
     os.replace(tmp, ENGINE_SO)
     tmp = MINIHOST_SO + ".tmp"
     _run(["g++", *CXX_FLAGS, *HIP_HOST_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "minihost.cpp"), "-ldl",

@@ -33,6 +33,7 @@ EXPORTS = (
     "cpipe_bm3d_plan_set_timing", "cpipe_bm3d_plan_timing", "cpipe_bm3d_compat_rgba16f",
     "cpipe_bm3d_host_planes", "cpipe_bm3d_host_rgba16f", "cpipe_bm3d_host_release",
     "cpipe_bm3d_device_planes", "cpipe_bm3d_device_rgba16f", "cpipe_bm3d_noalloc_planes",
+    "cpipe_dng_noise_profile",
     "cpipe_bm3d_pilot_embedded", "cpipe_bm3d_node_descriptor",
     "cpipe_registry_entries", "cpipe_link_builtin_denoise_bm3d", "cpipe_bm3d_batch_create",
     "cpipe_bm3d_batch_destroy", "cpipe_bm3d_batch_run", "cpipe_bm3d_batch_tune",
@@ -108,6 +109,7 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_plan_set_timing.argtypes = [P, I]
     L.cpipe_bm3d_plan_timing.argtypes = [P, ctypes.POINTER(F), I, ctypes.POINTER(I)]
     L.cpipe_bm3d_compat_rgba16f.argtypes = [P, S, P, S, I, I, F, I, P]
+    L.cpipe_dng_noise_profile.argtypes = [P, S, P, P, I, ctypes.POINTER(I)]
     L.cpipe_bm3d_noalloc_planes.argtypes = [P, S, P, S, I, I, I, F, P]
     L.cpipe_bm3d_host_planes.argtypes = [P, S, P, S, I, I, I, F, I, I, I]
     L.cpipe_bm3d_host_rgba16f.argtypes = [P, S, P, S, I, I, F, I, I, I, I]

@@ -26,8 +26,9 @@ def _declared_functions(header):
 
 def test_library_exports_every_declared_symbol():
     lib = engine.lib()
-    declared = (_declared_functions("cpipe_bm3d.h") + _declared_functions("cpipe_node_abi.h") +
-                _declared_functions("cpipe_wavelet.h") + _declared_functions("cpipe_convert.h"))
+    headers = sorted(f for f in os.listdir(INCLUDE) if f.endswith(".h"))
+    declared = [n for h in headers for n in _declared_functions(h)]
+    assert "cpipe_dng_noise_profile" in declared
     assert "cpipe_convert" in declared and "cpipe_convert_node_descriptor" in declared
     assert "cpipe_bm3d_plan_run" in declared and "cpipe_bm3d_node_descriptor" in declared
     assert "cpipe_wavelet_rgba16f" in declared and "cpipe_wavelet_node_descriptor" in declared
@@ -40,11 +41,11 @@ def test_library_exports_every_declared_symbol():
 def test_abi_header_layout_static_asserts(tmp_path):
     src = tmp_path / "abi.cpp"
     src.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\n#include "cpipe_wavelet.h"\n'
-                   '#include "cpipe_convert.h"\nint main(){return 0;}\n')
+                   '#include "cpipe_convert.h"\n#include "cpipe_dng.h"\nint main(){return 0;}\n')
     subprocess.run(["g++", "-std=c++17", f"-I{INCLUDE}", "-fsyntax-only", str(src)], check=True)
     csrc = tmp_path / "abi.c"
     csrc.write_text('#include "cpipe_node_abi.h"\n#include "cpipe_bm3d.h"\n#include "cpipe_wavelet.h"\n'
-                    '#include "cpipe_convert.h"\nint main(void){return 0;}\n')
+                    '#include "cpipe_convert.h"\n#include "cpipe_dng.h"\nint main(void){return 0;}\n')
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{INCLUDE}", "-fsyntax-only", str(csrc)],
                    check=True)
 

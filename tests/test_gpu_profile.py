@@ -186,3 +186,33 @@ def test_node_grey_planes_with_one_pair_per_cfa_channel(cuda, oracle_lib):
     lam = np.float32(np.float32(sum(np.float32(p[1]) for p in pairs)) / np.float32(4))
     want = oracle_lib.bm3d_f32_profile(noisy, b, lam)
     np.testing.assert_array_equal(out.read().view(np.uint32), want[0].view(np.uint32))
+
+
+def test_node_reads_a_dng_noise_profile(cuda, oracle_lib):
+    """DNG ingest end to end: the NoiseProfile of a committed DNG fixture
+    (three per-plane [S, O] pairs) parsed by cpipe_dng_noise_profile, handed to
+    the host's calibration view as (beta, lambda), drives both the node's
+    metadata sigma (bm3d.cpp:41-54, no sigma param) and its noise_map=profile run."""
+    import os
+    from cpipe_amd import dng
+    pairs = dng.noise_profile(os.path.join(os.path.dirname(__file__), "golden", "dng", "le_ifd0.dng"))
+    assert len(pairs) == 3
+    _, noisy, _ = synth.workload("rgb64x48", seed=14)
+    rgba = np.ones((48, 64, 4), np.float32)
+    rgba[..., :3] = np.moveaxis(noisy, 0, -1)
+    bits = _f16(rgba)
+    planes = np.moveaxis(_from_f16(bits)[..., :3], -1, 0).copy()
+    d = host.find_node()
+    print("node with DNG pairs: metadata sigma, then noise_map=profile", flush=True)
+    inp = _node_rgba(bits, pairs)
+    out = host.Buffer(host.FORMAT_RGBA16F, 64, 48)
+    assert host.run_node(d, [inp], [out], {"mode": "bm3d"}) == host.CPIPE_OK, host.host_log()
+    sig = np.float32(synth.sigma_from_noise_profile(pairs))
+    want = oracle_lib.bm3d_f32(planes, sig)
+    np.testing.assert_array_equal(out.read()[..., :3], _f16(np.moveaxis(want, 0, -1)))
+    inp = _node_rgba(bits, pairs)
+    out = host.Buffer(host.FORMAT_RGBA16F, 64, 48)
+    rc = host.run_node(d, [inp], [out], {"mode": "bm3d", "noise_map": "profile"})
+    assert rc == host.CPIPE_OK, host.host_log()
+    want = oracle_lib.bm3d_f32_profile(planes, [p[0] for p in pairs], [p[1] for p in pairs])
+    np.testing.assert_array_equal(out.read()[..., :3], _f16(np.moveaxis(want, 0, -1)))
