@@ -164,40 +164,47 @@ __device__ __forceinline__ void rmw_block4(float2* acc, const int (&o)[NMAX], co
     }
 }
 
-// Adds one filtered group to a tile's LDS footprint of (num, den) pairs (one
-// 8-byte read and write per member pixel) in a fixed order. Lane p = (i, j) holds pixel (i, j) of
-// member m in v[m]; mp[m] (wave-uniform) is the member's footprint position
-// (y << 16 | x); lo = (i - fy0) * fw + (j - fx0). Group g is accumulated right
-// after group g-1: the wave waits for the LDS turn token to reach g, adds
-// num += wk * v and den += wk, then passes the token on. Every wave takes its
-// turns in increasing g and turn g-1 never waits on turn g, so the chain always
-// completes; the spin is bounded anyway and returns true (timed out) instead of
-// hanging. `has` false: only pass the token.
+// Ordered aggregation of filtered groups into a tile's LDS footprint of
+// (num, den) pairs (one 8-byte read and write per member pixel). Lane p = (i, j)
+// holds pixel (i, j) of member m in v[m]; mp[m] (wave-uniform) is the member's
+// footprint position (y << 16 | x); lo = (i - fy0) * fw + (j - fx0). Group g is
+// accumulated right after group g-1: the wave waits for the LDS turn token to
+// reach g, adds num += wk * v and den += wk, then passes the token on. Every
+// wave takes its turns in increasing g and turn g-1 never waits on turn g, so
+// the chain always completes; the spin is bounded anyway and reports a timeout
+// instead of hanging.
 //
 // The turns form the kernel's critical path, so everything that does not
 // touch the footprint (offsets, products, the batching plan) is computed
-// before the wait. Members are added in read/add/write rounds: each block of
-// four consecutive members is cut greedily into runs of consecutive members
-// whose patches are pairwise disjoint, one round per run. A pixel's members
-// are added in member order either way (members covering one pixel overlap,
-// so they never share a run).
-__device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g, bool has, int n,
-                                                  const uint32_t (&mp)[NMAX],
-                                                  const float (&v)[NMAX], float wk, int lo,
-                                                  int fw) {
+// before the wait (prepare_group). Members are added in read/add/write rounds:
+// each block of four consecutive members is cut greedily into runs of
+// consecutive members whose patches are pairwise disjoint, one round per run. A
+// pixel's members are added in member order either way (members covering one
+// pixel overlap, so they never share a run).
+struct PreparedGroup {
     int o[NMAX];
     float c[NMAX];
+    float wk;
+    uint32_t plan;  // run breaks: bit 4b + k set when member 4b + k starts a new run (k = 1..3)
+    int n;
+    bool has;       // false: nothing to add (no group this round)
+};
+
+__device__ __forceinline__ void prepare_group(PreparedGroup& P, bool has, int n, const uint32_t (&mp)[NMAX],
+                                              const float (&v)[NMAX], float wk, int lo, int fw) {
+    P.has = has;
+    P.n = n;
+    P.wk = wk;
 #pragma unroll
     for (int m = 0; m < NMAX; ++m) {
-        o[m] = (int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu) + lo;
-        c[m] = wk * v[m];
+        P.o[m] = (int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu) + lo;
+        P.c[m] = wk * v[m];
     }
     auto overlap = [&](int a, int b) {
         const int dx = (int)(mp[a] & 0xffffu) - (int)(mp[b] & 0xffffu);
         const int dy = (int)(mp[a] >> 16) - (int)(mp[b] >> 16);
         return dx < K && dx > -K && dy < K && dy > -K;
     };
-    // run breaks: bit 4b + k set when member 4b + k starts a new run (k = 1..3)
     uint32_t plan = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -211,45 +218,63 @@ __device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g,
         plan |= ((b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u)) << (4 * b);
     }
     if (n < 4) plan = n == 2 ? (plan & 2u) | 0xcu : 0xeu;  // n = 1: one member; n = 2: at most 2
-    plan = __builtin_amdgcn_readfirstlane(plan);
+    P.plan = __builtin_amdgcn_readfirstlane(plan);
     // keep the precomputation ahead of the wait
 #pragma unroll
-    for (int m = 0; m < NMAX; ++m) asm volatile("" : "+v"(o[m]), "+v"(c[m]));
-    asm volatile("" : "+s"(plan));
+    for (int m = 0; m < NMAX; ++m) asm volatile("" : "+v"(P.o[m]), "+v"(P.c[m]));
+    asm volatile("" : "+s"(P.plan));
+}
 
-    bool timed_out = false;
+// Waits (bounded) until the token reaches g; true if the wait timed out.
+__device__ __forceinline__ bool wait_turn(int* turn, int g) {
     int spins = 0;
     // every lane reads the same token: compare it as a scalar
     while (__builtin_amdgcn_readfirstlane(
                __hip_atomic_load(turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) != g) {
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1 << 24)) {
-            timed_out = true;
-            break;
-        }
+        if (++spins > (1 << 24)) return true;
     }
-    if (has) {
-        if (n >= 4) {
-            rmw_block4<0>(acc, o, c, wk, plan);
-            if (n >= 8) {
-                rmw_block4<4>(acc, o, c, wk, plan >> 4);
-                if (n >= 16) {
-                    rmw_block4<8>(acc, o, c, wk, plan >> 8);
-                    rmw_block4<12>(acc, o, c, wk, plan >> 12);
-                }
+    return false;
+}
+
+__device__ __forceinline__ void pass_turn(int* turn, int g) {
+    __hip_atomic_store(turn, g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void rmw_group(float2* acc, const PreparedGroup& P) {
+    if (!P.has) return;
+    const int n = P.n;
+    if (n >= 4) {
+        rmw_block4<0>(acc, P.o, P.c, P.wk, P.plan);
+        if (n >= 8) {
+            rmw_block4<4>(acc, P.o, P.c, P.wk, P.plan >> 4);
+            if (n >= 16) {
+                rmw_block4<8>(acc, P.o, P.c, P.wk, P.plan >> 8);
+                rmw_block4<12>(acc, P.o, P.c, P.wk, P.plan >> 12);
             }
-        } else if (n == 2) {
-            if (plan & 2u) {
-                rmw_run<0, 1>(acc, o, c, wk);
-                rmw_run<1, 1>(acc, o, c, wk);
-            } else {
-                rmw_run<0, 2>(acc, o, c, wk);
-            }
+        }
+    } else if (n == 2) {
+        if (P.plan & 2u) {
+            rmw_run<0, 1>(acc, P.o, P.c, P.wk);
+            rmw_run<1, 1>(acc, P.o, P.c, P.wk);
         } else {
-            rmw_run<0, 1>(acc, o, c, wk);
+            rmw_run<0, 2>(acc, P.o, P.c, P.wk);
         }
+    } else {
+        rmw_run<0, 1>(acc, P.o, P.c, P.wk);
     }
-    __hip_atomic_store(turn, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// One group per turn (token g -> g + 1). `has` false: only pass the token.
+__device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g, bool has, int n,
+                                                  const uint32_t (&mp)[NMAX],
+                                                  const float (&v)[NMAX], float wk, int lo,
+                                                  int fw) {
+    PreparedGroup P;
+    prepare_group(P, has, n, mp, v, wk, lo, fw);
+    const bool timed_out = wait_turn(turn, g);
+    rmw_group(acc, P);
+    pass_turn(turn, g + 1);
     return timed_out;
 }
 
