@@ -1126,7 +1126,7 @@ struct cpipe_bm3d_plan_s {
     };
 
     int c = 0, precision = 0, device = 0;
-    int tile_y = TILE / 2;           // refs per tile column (8 exact FP32, 8 or 16 mixed precision)
+    int tile_y = TILE / 2;           // refs per tile column (4 exact FP32, 8 or 16 mixed precision)
     int dofs = FOOT2;                // den offset in a partial block
     size_t tile_stride = 2 * (size_t)FOOT2;
     int order = 0;                   // work-list order: 0 frame raster, 1 XCD-contiguous chunks
@@ -1162,6 +1162,7 @@ struct cpipe_bm3d_plan_s {
     int runs = 0;
     // learned pilot (replaces step 1) and its spot checks
     cpipe_bm3d_pilot* pilot = nullptr;
+    void* pilot_ws = nullptr;        // the network's activations for this plan's largest frame
     float pilot_thr_db = 0.5f;
     float pilot_under_db = 0.25f;
     float pilot_full_frac = 0.5f;
@@ -1237,6 +1238,7 @@ void free_plan(cpipe_bm3d_plan* p) {
     (void)hipFree(p->smap);
     (void)hipFree(p->report);
     (void)hipFree(p->chk_part);
+    (void)hipFree(p->pilot_ws);
     if (p->side) (void)hipStreamDestroy(p->side);
     if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
     if (p->join_ev) (void)hipEventDestroy(p->join_ev);
@@ -1834,7 +1836,7 @@ int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) 
             float* zin[3] = {p->tab[f].z[0], p->tab[f].z[1], p->tab[f].z[2]};
             float* bout[3] = {p->tab[f].bz[0], p->tab[f].bz[1], p->tab[f].bz[2]};
             rc = cpipe_internal::pilot_forward(p->pilot, zin, (size_t)F.w, 0, F.w, F.h, 1, 0.0f,
-                                               levels_dev(p) + f, bout, st);
+                                               levels_dev(p) + f, bout, p->pilot_ws, st);
             if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
         }
         if (nchk > 0) {
@@ -2582,6 +2584,40 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* p, const float* const* in, s
     return OK;
 }
 
+int cpipe_bm3d_plan_step2_opponent(cpipe_bm3d_plan* p, const float* const* in, size_t in_pitch, float sigma,
+                                   const float* const* basic_in, size_t basic_pitch, float* const* out,
+                                   size_t out_pitch, void* stream) {
+    if (!p || !in || !basic_in || !out) return fail(BAD_INDEX, "null argument");
+    if (p->fr.size() != 1 || p->c != 3) return fail(UNSUPPORTED, "opponent planes exist for 3-channel single-frame plans");
+    const auto& F = p->fr[0];
+    if (F.small) return fail(UNSUPPORTED, "image smaller than one patch");
+    for (int c = 0; c < 3; ++c)
+        if (!in[c] || !basic_in[c] || !out[c]) return fail(BAD_INDEX, "null plane pointer");
+    if (in_pitch < (size_t)F.w || basic_pitch < (size_t)F.w || out_pitch < (size_t)F.w)
+        return fail(BAD_INDEX, "pitch smaller than width");
+    if (!(sigma >= 0.0f) || !std::isfinite(sigma)) return fail(BAD_INDEX, "sigma must be finite and >= 0");
+    DeviceGuard guard(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    set_run(p, 0, in, in_pitch, out, out_pitch, sigma);
+    int rc = fill_staging(p);
+    if (rc) return rc;
+    HIP_CHECK(hipMemcpyAsync(p->tab_dev, p->staging, p->staging_bytes, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipEventRecord(p->staged, st));
+    p->staged_pending = true;
+    p->ran = true;
+    if ((rc = flag_begin(p, st))) return rc;
+    const dim3 g((p->full.rect_w + 255) / 256, p->full.rect_h, p->full.n_rects);
+    k_colour_in<<<g, 256, 0, st>>>(p->tab_dev, p->full.rects);
+    HIP_CHECK(hipGetLastError());
+    for (int c = 0; c < 3; ++c)
+        HIP_CHECK(hipMemcpy2DAsync(p->tab[0].bz[c], (size_t)F.w * sizeof(float), basic_in[c],
+                                   basic_pitch * sizeof(float), (size_t)F.w * sizeof(float), F.h,
+                                   hipMemcpyDeviceToDevice, st));
+    RunCtx r{p, st, false};
+    if ((rc = run_step(r, 2, p->full, TO_OUTPUT, 0, false))) return rc;
+    return flag_end(p, st, false);
+}
+
 int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
                               const cpipe_bm3d_pilot_opts* opts) {
     if (!p) return fail(BAD_INDEX, "null plan");
@@ -2604,6 +2640,8 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     }
     p->checks.clear();
     p->pilot = nullptr;
+    (void)hipFree(p->pilot_ws);
+    p->pilot_ws = nullptr;
     if (!net) return OK;
     if (p->c != 3) return fail(UNSUPPORTED, "the learned pilot is trained for RGB (3-plane) images");
     bool any = false;
@@ -2620,12 +2658,17 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     if (!(o.fallback_db >= 0.0f) || !(o.under_db >= 0.0f) || o.check_block < 16 || o.check_every < 0)
         return fail(BAD_INDEX, "bad pilot options");
     int rc;
+    size_t max_px = 0;
     for (int f = 0; f < (int)p->fr.size(); ++f) {
         auto& F = p->fr[f];
         if (F.small) continue;
         if ((rc = dev_alloc((void**)&F.bzp, sizeof(float) * 3 * F.npx))) return rc;
         for (int c = 0; c < 3; ++c) p->tab[f].bzp[c] = F.bzp + c * F.npx;
+        max_px = std::max(max_px, (size_t)F.w * F.h);
     }
+    // frames run the network one after another on the plan's stream, so one
+    // workspace for the largest serves them all
+    if ((rc = dev_alloc(&p->pilot_ws, cpipe_internal::pilot_workspace_bytes(max_px)))) return rc;
     if ((rc = build_check_lists(p, o))) return rc;
     if (!p->side) {
         HIP_CHECK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));

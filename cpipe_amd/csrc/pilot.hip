@@ -733,10 +733,18 @@ int launch_conv(const ConvArgs& a, int n_img, hipStream_t st) {
 
 int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t img_stride, int w,
             int h, int n_img, float level, const float* levels, float* const* out, bool train,
-            hipStream_t st) {
+            hipStream_t st, void* ws = nullptr) {
     const size_t px = (size_t)w * h * n_img;
-    int rc = ensure_ws(n, px, train);
-    if (rc) return rc;
+    _Float16* act[NL] = {};
+    if (ws && !train) {  // inference on the caller's workspace: layers ping-pong between two buffers
+        act[1] = static_cast<_Float16*>(ws);
+        act[2] = act[1] + px * 32;
+    } else {
+        int rc = ensure_ws(n, px, train);
+        if (rc) return rc;
+        for (int l = 0; l < NL; ++l) act[l] = n->act[l];
+    }
+    int rc = 0;
     ConvArgs a{};
     a.w = w;
     a.h = h;
@@ -745,12 +753,12 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
     if (train) {
         // L1 on the packed input, which the weight gradient reads later
         k_pack_input<<<dim3((w + 255) / 256, h, n_img), 256, 0, st>>>(in[0], in[1], in[2], pitch, level,
-                                                                       levels, w, h, img_stride, n->act[0]);
+                                                                       levels, w, h, img_stride, act[0]);
         PHIP(hipGetLastError());
-        a.in = n->act[0];
+        a.in = act[0];
         a.frag = n->frag + n->frag_off[0][0];
         a.bias = n->w32 + n->off[0] + layer_w(0);
-        a.out = n->act[1];
+        a.out = act[1];
         if ((rc = launch_conv<16, 32, 0>(a, n_img, st))) return rc;
     } else {
         // L1 fused with the packing, taps packed into K
@@ -762,7 +770,7 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
         f.levels = levels;
         f.frag = n->frag + n->frag_first;
         f.bias = n->w32 + n->off[0] + layer_w(0);
-        f.out = n->act[1];
+        f.out = act[1];
         f.w = w;
         f.h = h;
         k_conv_first<<<dim3((w + TW - 1) / TW, (h + CTH - 1) / CTH, n_img), 256, 0, st>>>(f);
@@ -771,14 +779,14 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
     int cur = 1;
     for (int l = 1; l < NL - 1; ++l) {
         const int nxt = train ? l + 1 : (cur == 1 ? 2 : 1);
-        a.in = n->act[cur];
+        a.in = act[cur];
         a.frag = n->frag + n->frag_off[l][0];
         a.bias = n->w32 + n->off[l] + layer_w(l);
-        a.out = n->act[nxt];
+        a.out = act[nxt];
         if ((rc = launch_conv<32, 32, 0>(a, n_img, st))) return rc;
         cur = nxt;
     }
-    a.in = n->act[cur];
+    a.in = act[cur];
     a.frag = n->frag + n->frag_off[NL - 1][0];
     a.bias = n->w32 + n->off[NL - 1] + layer_w(NL - 1);
     for (int c = 0; c < 3; ++c) {
@@ -872,10 +880,12 @@ int upload_levels(cpipe_bm3d_pilot* n, const float* sigma, int n_img, hipStream_
 }  // namespace
 
 namespace cpipe_internal {
+size_t pilot_workspace_bytes(size_t px) { return 2 * px * 32 * sizeof(_Float16); }
+
 int pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size_t pitch, size_t img_stride,
                   int w, int h, int n_img, float level, const float* levels, float* const* out,
-                  hipStream_t st) {
-    return forward(net, in, pitch, img_stride, w, h, n_img, level, levels, out, false, st);
+                  void* ws, hipStream_t st) {
+    return forward(net, in, pitch, img_stride, w, h, n_img, level, levels, out, false, st, ws);
 }
 }  // namespace cpipe_internal
 

@@ -24,7 +24,8 @@ EXPORTS = (
     "cpipe_bm3d_last_error", "cpipe_bm3d_device_count", "cpipe_bm3d_plan_create",
     "cpipe_bm3d_plan_destroy", "cpipe_bm3d_plan_run", "cpipe_bm3d_plan_run_rgba16f",
     "cpipe_bm3d_plan_basic", "cpipe_bm3d_plan_matches", "cpipe_bm3d_plan_ref_grid",
-    "cpipe_bm3d_plan_check", "cpipe_bm3d_plan_debug_flag", "cpipe_bm3d_plan_step1_opponent", "cpipe_bm3d_plan_set_pilot",
+    "cpipe_bm3d_plan_check", "cpipe_bm3d_plan_debug_flag", "cpipe_bm3d_plan_step1_opponent",
+    "cpipe_bm3d_plan_step2_opponent", "cpipe_bm3d_plan_set_pilot",
     "cpipe_bm3d_plan_pilot_report", "cpipe_bm3d_pilot_create", "cpipe_bm3d_pilot_destroy",
     "cpipe_bm3d_pilot_init", "cpipe_bm3d_pilot_save", "cpipe_bm3d_pilot_load",
     "cpipe_bm3d_pilot_forward", "cpipe_bm3d_pilot_gradients", "cpipe_bm3d_pilot_train_step",
@@ -103,6 +104,7 @@ def lib() -> ctypes.CDLL:
     L.cpipe_bm3d_plan_check.argtypes = [P]
     L.cpipe_bm3d_plan_debug_flag.argtypes = [P, P]
     L.cpipe_bm3d_plan_step1_opponent.argtypes = [P, P, S, F, P, P, S, P]
+    L.cpipe_bm3d_plan_step2_opponent.argtypes = [P, P, S, F, P, S, P, S, P]
     L.cpipe_bm3d_plan_set_pilot.argtypes = [P, P, P]
     L.cpipe_bm3d_plan_pilot_report.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I),
                                                ctypes.POINTER(F), ctypes.POINTER(I)]
@@ -250,6 +252,16 @@ class Plan:
             _ptr_array([noisy_out[c].data_ptr() for c in range(3)]),
             _ptr_array([basic_out[c].data_ptr() for c in range(3)]), noisy_out.stride(1),
             _stream_handle(stream)))
+
+    def step2_opponent(self, src, basic, out, sigma, stream=None) -> None:
+        """Step 2 on a given opponent basic estimate (3, H, W): the output a pilot leads to."""
+        self._check_planes(src)
+        self._check_planes(basic)
+        self._check_planes(out)
+        _check(lib().cpipe_bm3d_plan_step2_opponent(
+            self._h, _ptr_array([src[c].data_ptr() for c in range(3)]), src.stride(1), float(sigma),
+            _ptr_array([basic[c].data_ptr() for c in range(3)]), basic.stride(1),
+            _ptr_array([out[c].data_ptr() for c in range(3)]), out.stride(1), _stream_handle(stream)))
 
     def set_pilot(self, net, fallback_db: float = 0.5, check_block: int = 256,
                   check_every: int = 16, full_fraction: float = 0.5, under_db: float = 0.25) -> None:

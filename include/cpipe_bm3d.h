@@ -139,7 +139,9 @@ int cpipe_bm3d_pilot_init(cpipe_bm3d_pilot* net, uint64_t seed);
 size_t cpipe_bm3d_pilot_blob_size(int with_optimizer);
 int cpipe_bm3d_pilot_save(const cpipe_bm3d_pilot* net, void* blob, size_t cap, int with_optimizer);
 int cpipe_bm3d_pilot_load(cpipe_bm3d_pilot* net, const void* blob, size_t size);
-/* Inference on device planes (row pitch `pitch`). */
+/* Inference on device planes (row pitch `pitch`), on the network's own
+ * activation workspace: one stream at a time per network (plans that use the
+ * network bring their own workspace, see cpipe_bm3d_plan_set_pilot). */
 int cpipe_bm3d_pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size_t pitch,
                              size_t img_stride, int width, int height, int n_img, float sigma,
                              float* const* out, void* stream);
@@ -167,6 +169,16 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* plan, const float* const* in
                                    float sigma, float* const* noisy_out, float* const* basic_out,
                                    size_t out_pitch, void* stream);
 
+/* Step 2 only, on a given basic estimate: the Wiener step of a plain run with
+ * the opponent-space basic estimate basic_in (3 device planes, row pitch
+ * basic_pitch) in place of step 1's, writing the denoised RGB planes to out.
+ * With the basic planes of cpipe_bm3d_plan_step1_opponent it reproduces
+ * cpipe_bm3d_plan_run; it scores a candidate pilot network by the output it
+ * leads to. */
+int cpipe_bm3d_plan_step2_opponent(cpipe_bm3d_plan* plan, const float* const* in, size_t in_pitch,
+                                   float sigma, const float* const* basic_in, size_t basic_pitch,
+                                   float* const* out, size_t out_pitch, void* stream);
+
 /* Learned pilot in a plan (3-channel plans): step 1 is replaced by the
  * network; blocks of check_block x check_block pixels, one in check_every
  * (block b = by * nbx + bx in raster order is checked when
@@ -178,8 +190,10 @@ int cpipe_bm3d_plan_step1_opponent(cpipe_bm3d_plan* plan, const float* const* in
  * keeps it, so the bound is tighter); if more than full_fraction of the
  * checked blocks fall back, the plain step 1 runs on the whole frame (decided
  * on the device, no host sync). net = NULL restores the plain path. The plan
- * keeps a reference to net. A struct_size that ends before under_db selects
- * under_db = fallback_db / 2. */
+ * keeps a reference to net (which must outlive its use) and allocates its own
+ * activation workspace for its largest frame, so plans sharing one network
+ * may run on different streams at once. A struct_size that ends before
+ * under_db selects under_db = fallback_db / 2. */
 typedef struct {
     uint32_t struct_size;
     float fallback_db;    /* default 0.5 */

@@ -10,7 +10,8 @@
     against the FP64 oracle on the 512x384 top-left crop;
   * config 3: the packaged learned pilot + FP16 Wiener at 4080x3072 -- no
     frame-wide fallback, PSNR >= the plain FP32 path - 0.1 dB, and the gain over
-    the same plan without the pilot (time and PSNR, asserted);
+    the same plan without the pilot (time and PSNR, asserted); the fallback rate
+    per sigma on Gaussian scenes, with the gate keeping the plain path's quality;
   * config 4: the 12-frame uneven batch (4 x 4080x3072, 4 x 2040x1536,
     4 x 1024x768, sigma_f ~ U(0.01, 0.1)) -- every frame bit-identical to its
     single-frame plan, in exact FP32 and in FP16 + learned pilot.
@@ -127,6 +128,42 @@ def test_config3_learned_pilot_full_frame_and_gain(cuda, frame12):
     # at no loss of quality
     assert speed >= 1.2
     assert p_with >= p_without - 0.1
+
+
+@pytest.mark.parametrize("sigma", [0.01, 0.03, 0.06, 0.1])
+def test_config3_fallback_rate_per_sigma(cuda, sigma):
+    """Gaussian scenes of the config-1 generator (1024x768 RGB, every 128 x 128
+    block checked): the per-sigma fallback rate of the packaged network, and the
+    outcome the gate leads to. The raw network (no fallback) loses 1.5-4 dB of
+    final PSNR on such scenes -- a 13 x 13 receptive field cannot follow BM3D's
+    39 x 39 non-local grouping on them -- so the checks must send these frames
+    to the plain step 1 and the result must keep the plain path's quality."""
+    from cpipe_amd import pilot as pilot_mod
+    torch = cuda
+    clean = synth.clean_scene(1024, 768, 3, seed=9000)
+    noisy = synth.add_gaussian(clean, sigma, seed=9100)
+    net = pilot_mod.Pilot(seed=None, weights=pilot_mod.DEFAULT_WEIGHTS)
+    c, h, w = noisy.shape
+    src = torch.from_numpy(noisy).cuda()
+    outs = {}
+    for tag, opts in (("plain", None), ("gated", dict(check_block=128, check_every=1)),
+                      ("raw", dict(fallback_db=1e9, under_db=1e9, check_every=0))):
+        plan = engine.Plan(w, h, c, "fp16", 0)
+        if opts is not None:
+            plan.set_pilot(net, **opts)
+        out = torch.empty_like(src)
+        plan.run(src, out, sigma)
+        torch.cuda.synchronize()
+        plan.check()
+        outs[tag] = (synth.psnr(out.cpu().numpy(), clean), plan.pilot_report() if opts else None)
+        plan.close()
+    rep = outs["gated"][1]
+    rate = rep["fallback"] / max(1, rep["checked"])
+    print(f"sigma {sigma}: fallback {rep['fallback']}/{rep['checked']} ({100 * rate:.0f} %), "
+          f"frame-wide {bool(rep['full_fallback'])}; PSNR plain {outs['plain'][0]:.2f} / gated "
+          f"{outs['gated'][0]:.2f} / raw network {outs['raw'][0]:.2f} dB", flush=True)
+    assert rep["checked"] == 48
+    assert outs["gated"][0] >= outs["plain"][0] - 0.1
 
 
 BATCH_SIZES = [(4080, 3072)] * 4 + [(2040, 1536)] * 4 + [(1024, 768)] * 4

@@ -107,3 +107,58 @@ def test_identity_pilot_falls_back_on_the_under_denoising_bound(cuda):
     _, _, rep = _run(torch, src, sigma, net, fallback_db=0.0, under_db=1e31, check_block=64,
                      check_every=1)
     assert rep["fallback"] == 0
+
+
+def test_step2_on_the_step1_basic_reproduces_the_plain_run(cuda):
+    """cpipe_bm3d_plan_step2_opponent fed the basic planes of
+    cpipe_bm3d_plan_step1_opponent gives the plain run's output bit for bit, so
+    a candidate pilot scored through it is scored by the Wiener step the node runs."""
+    torch = cuda
+    _, _, sigma, src = _frame(torch, "rgb200x136", seed=7)
+    plain, _, _ = _run(torch, src, sigma)
+    c, h, w = src.shape
+    plan = engine.Plan(w, h, 3)
+    z = torch.empty_like(src)
+    basic = torch.empty_like(src)
+    out = torch.empty_like(src)
+    plan.step1_opponent(src, z, basic, sigma)
+    plan.step2_opponent(src, basic, out, sigma)
+    torch.cuda.synchronize()
+    plan.check()
+    plan.close()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), plain.view(np.uint32))
+
+
+def test_plans_sharing_one_network_run_on_separate_streams(cuda):
+    """Four plans that share one network, run at once on four streams, give the
+    outputs they give one after another: each plan runs the network on its own
+    activation workspace."""
+    torch = cuda
+    net = pilot.Pilot(seed=None, weights=pilot.DEFAULT_WEIGHTS)
+    srcs, plans, outs = [], [], []
+    for i in range(4):
+        _, _, sigma, src = _frame(torch, "rgb1024x768", seed=20 + i)
+        plan = engine.Plan(1024, 768, 3, "fp16", 0)
+        plan.set_pilot(net)
+        srcs.append((src, sigma))
+        plans.append(plan)
+        outs.append(torch.empty_like(src))
+    for (src, sigma), plan, out in zip(srcs, plans, outs):
+        plan.run(src, out, sigma)
+    torch.cuda.synchronize()
+    want = [o.cpu().numpy() for o in outs]
+    for o in outs:
+        o.zero_()
+    streams = [torch.cuda.Stream() for _ in plans]
+    s0 = torch.cuda.current_stream()
+    ev = torch.cuda.Event()
+    ev.record(s0)
+    for (src, sigma), plan, out, st in zip(srcs, plans, outs, streams):
+        st.wait_event(ev)
+        plan.run(src, out, sigma, stream=st)
+    torch.cuda.synchronize()
+    for plan in plans:
+        plan.check()
+        plan.close()
+    for w_, o in zip(want, outs):
+        assert np.array_equal(o.cpu().numpy().view(np.uint32), w_.view(np.uint32))
