@@ -705,24 +705,42 @@ __device__ __forceinline__ float rcp_nr(float d) {
     return r;
 }
 
-// Filters one group in place: loads the members' pixels (noisy, and the basic
-// estimate for step 2), 3D transform, shrinkage, inverse; returns the group's
-// aggregation weight (1/N_nonzero for the hard threshold, 1/sum W^2 for Wiener).
-// Each member load is a buffer load: the member's corner (scalar) plus the
-// lane's byte offset boff = 4 (li * w + lj).
-template <int STEP, int N>
-__device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (&mp)[NMAX],
-                                              __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc,
-                                              int w, uint32_t boff, float* sb, int lane, int li,
-                                              int lj, float thr, float s2) {
-    float b[NMAX];
+// Issues the loads of a group's N members (noisy, and the basic estimate for
+// step 2) into v and b; the values are consumed by filter_group. Each member
+// load is a buffer load: the member's corner (scalar) plus the lane's byte
+// offset boff = 4 (li * w + lj).
+template <int STEP, int M0, int M1>
+__device__ __forceinline__ void load_members(float (&v)[NMAX], float (&b)[NMAX], const uint32_t (&mp)[NMAX],
+                                             __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc,
+                                             int w, uint32_t boff) {
 #pragma unroll
-    for (int m = 0; m < N; ++m) {
+    for (int m = M0; m < M1; ++m) {
         const uint32_t qx = mp[m] & 0xffffu, qy = mp[m] >> 16;
         const uint32_t corner = (qy * (uint32_t)w + qx) * (uint32_t)sizeof(float);
         v[m] = plane_load(src, boff, corner);
         if (STEP == 2) b[m] = plane_load(bsrc, boff, corner);
     }
+}
+
+// n (wave-uniform) is 1, 2, 4, 8 or 16: members [n/2, n) are loaded behind one
+// scalar branch per doubling.
+template <int STEP>
+__device__ __forceinline__ void load_group(float (&v)[NMAX], float (&b)[NMAX], const uint32_t (&mp)[NMAX],
+                                           int n, __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc,
+                                           int w, uint32_t boff) {
+    load_members<STEP, 0, 1>(v, b, mp, src, bsrc, w, boff);
+    if (n >= 2) load_members<STEP, 1, 2>(v, b, mp, src, bsrc, w, boff);
+    if (n >= 4) load_members<STEP, 2, 4>(v, b, mp, src, bsrc, w, boff);
+    if (n >= 8) load_members<STEP, 4, 8>(v, b, mp, src, bsrc, w, boff);
+    if (n >= 16) load_members<STEP, 8, 16>(v, b, mp, src, bsrc, w, boff);
+}
+
+// Filters one group in place from its loaded members (load_group): 3D
+// transform, shrinkage, inverse; returns the group's aggregation weight
+// (1/N_nonzero for the hard threshold, 1/sum W^2 for Wiener).
+template <int STEP, int N>
+__device__ __forceinline__ float filter_group(float (&v)[NMAX], float (&b)[NMAX], float* sb, int lane,
+                                              int li, int lj, float thr, float s2) {
     dct_fwd_n<N>(v, sb, lane);
     haar_fwd_n<N>(v);
     float wgt;
@@ -767,6 +785,31 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], const uint32_t (
     return wgt;
 }
 
+// Fetches group g of a tile of 16 x TY refs (rows of nbx refs): its match list
+// through the scalar cache (wave-uniform address) and, member m in lane m mod
+// 16, as a vector (pv), size n, the sigma of the noise-profile map (MAP), and
+// issues its member loads.
+template <int STEP, int TY, bool MAP>
+__device__ __forceinline__ void fetch_group(const GroupView& p, const float* __restrict__ smap, int tx,
+                                            int ty, int nbx, int g, __amdgpu_buffer_rsrc_t src,
+                                            __amdgpu_buffer_rsrc_t bsrc, int w, uint32_t boff, int lane,
+                                            int& n, uint32_t (&mp)[NMAX], uint32_t& pv, float (&v)[NMAX],
+                                            float (&b)[NMAX], float& gthr, float& gs2) {
+    const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
+    const size_t ref = (size_t)by * p.nx + bx;
+    n = floor_pow2(load_count(p.cnt, ref));
+    const uint32_t CONST_AS* lp = (const uint32_t CONST_AS*)(p.pos + ref * NMAX);
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) mp[m] = lp[m];
+    pv = p.pos[ref * NMAX + (lane & 15)];  // lane m (mod 16): member m, for the run plan
+    if (MAP) {
+        const float sg = smap[ref];
+        gthr = 2.7f * sg;
+        gs2 = sg * sg;
+    }
+    load_group<STEP>(v, b, mp, n, src, bsrc, w, boff);
+}
+
 // Tiles of 16 x TY refs, NW waves per workgroup (one group per wave per round).
 // Exact FP32 plans aggregate over 16 x 4-ref tiles with 4-wave workgroups:
 // each workgroup's ordered aggregation is one chain, and three workgroups
@@ -788,8 +831,10 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     const int c = blockIdx.y;
     if (threadIdx.x == 0) turn = 0;
     bool timed_out = false;
-    const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
-    const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
+    // footprint bounds as scalars (member offsets y * fw + x stay scalar products)
+    const int fx0 = __builtin_amdgcn_readfirstlane(p.fx0[tx]), fy0 = __builtin_amdgcn_readfirstlane(p.fy0[ty]);
+    const int fw = __builtin_amdgcn_readfirstlane(p.fx1[tx]) - fx0;
+    const int fh = __builtin_amdgcn_readfirstlane(p.fy1[ty]) - fy0;
     for (int idx = threadIdx.x; idx < DOFS; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
     __syncthreads();
 
@@ -812,39 +857,36 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     const float* __restrict__ smap = MAP ? F.smap + (size_t)c * p.nx * p.ny : nullptr;
     const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
+    const int lo = (li - fy0) * fw + (lj - fx0);
 
+    // Group g of this wave: its match list through the scalar cache
+    // (wave-uniform address), size, sigma, and its member loads issued
+    // (fetch_group). The next group is fetched before the current group's
+    // turn, so its loads are in flight while the wave waits for the token and
+    // adds.
+    int n = 0;
+    uint32_t mp[NMAX], pv = 0;
+    float v[NMAX], b[NMAX];
+    float gthr = thr, gs2 = s2;
+    if (wave < G) fetch_group<STEP, TY, MAP>(p, smap, tx, ty, nbx, wave, src, bsrc, w, boff, lane, n, mp, pv, v, b, gthr, gs2);
     for (int rnd = 0; rnd < (G + NW - 1) / NW; ++rnd) {
         const int g = rnd * NW + wave;
         const bool has = g < G;
-        int n = 0;
         float wgt = 0.0f;
-        float v[NMAX];
-        uint32_t mp[NMAX];
         if (has) {
-            const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
-            const size_t ref = (size_t)by * p.nx + bx;
-            n = floor_pow2(p.cnt[ref]);
-            // the match list through the scalar cache (wave-uniform address)
-            const uint32_t CONST_AS* lp = (const uint32_t CONST_AS*)(p.pos + ref * NMAX);
-#pragma unroll
-            for (int m = 0; m < NMAX; ++m) mp[m] = lp[m];
-            float gthr = thr, gs2 = s2;
-            if (MAP) {
-                const float sg = smap[ref];
-                gthr = 2.7f * sg;
-                gs2 = sg * sg;
-            }
             switch (n) {
-                case 16: wgt = filter_group<STEP, 16>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
-                case 8: wgt = filter_group<STEP, 8>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
-                case 4: wgt = filter_group<STEP, 4>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
-                case 2: wgt = filter_group<STEP, 2>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
-                default: wgt = filter_group<STEP, 1>(v, mp, src, bsrc, w, boff, sb, lane, li, lj, gthr, gs2); break;
+                case 16: wgt = filter_group<STEP, 16>(v, b, sb, lane, li, lj, gthr, gs2); break;
+                case 8: wgt = filter_group<STEP, 8>(v, b, sb, lane, li, lj, gthr, gs2); break;
+                case 4: wgt = filter_group<STEP, 4>(v, b, sb, lane, li, lj, gthr, gs2); break;
+                case 2: wgt = filter_group<STEP, 2>(v, b, sb, lane, li, lj, gthr, gs2); break;
+                default: wgt = filter_group<STEP, 1>(v, b, sb, lane, li, lj, gthr, gs2); break;
             }
             if (MAP) wgt = wgt * (1.0f / (gs2 > 1e-30f ? gs2 : 1.0f));
         }
-        const int lo = (li - fy0) * fw + (lj - fx0);
-        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
+        PreparedGroup P;
+        prepare_group(P, has, n, mp, pv, v, wgt * kw, lo, fw);
+        if (g + NW < G) fetch_group<STEP, TY, MAP>(p, smap, tx, ty, nbx, g + NW, src, bsrc, w, boff, lane, n, mp, pv, v, b, gthr, gs2);
+        timed_out |= take_turn(acc, &turn, g, P);
     }
     if (timed_out && lane == 0) atomicOr(L.err, 1);
     __syncthreads();

@@ -107,6 +107,15 @@ __device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
 // change during the kernel).
 #define CONST_AS __attribute__((address_space(4)))
 
+// The match count of ref r (one byte per ref) through the scalar cache: the
+// aligned dword that holds it. Device allocations are at least 4-byte granular,
+// so that dword lies inside the count buffer's allocation.
+__device__ __forceinline__ int load_count(const uint8_t* cnt, size_t r) {
+    const uintptr_t a = (uintptr_t)(cnt + r);
+    const uint32_t CONST_AS* wp = (const uint32_t CONST_AS*)(a & ~(uintptr_t)3);
+    return (int)((*wp >> (8u * (uint32_t)(a & 3u))) & 0xffu);
+}
+
 // A buffer resource over one plane of `bytes` (< 4 GiB; plans refuse larger
 // planes): a load takes the lane's 32-bit byte offset in a VGPR and a
 // wave-uniform byte offset in an SGPR, so it needs no address arithmetic.
@@ -190,33 +199,39 @@ struct PreparedGroup {
     bool has;       // false: nothing to add (no group this round)
 };
 
+// Bit m (m < 16) set when member m's patch overlaps member m - D's. Lane m
+// (mod 16) holds member m's position pv = (y << 16) | x; x7 = x + 7, y7 = y + 7
+// of the same lane. One DPP row shift brings member m - D's position.
+template <int D>
+__device__ __forceinline__ uint32_t overlap_prev(uint32_t pv, uint32_t x7, uint32_t y7) {
+    const uint32_t pk = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pv, 0x110 + D, 0xf, 0xf, false);  // row_shr:D
+    const uint32_t dx = x7 - (pk & 0xffffu), dy = y7 - (pk >> 16);
+    return (uint32_t)__ballot(dx < 15u && dy < 15u) & 0xffffu;
+}
+
+// mp: the members' positions (wave-uniform, footprint-relative offsets are
+// y * fw + x with fw wave-uniform); pv: lane m (mod 16) holds mp[m]. The run
+// plan of each block of four members comes from three lane-parallel overlap
+// masks (member m against m - 1, m - 2, m - 3).
 __device__ __forceinline__ void prepare_group(PreparedGroup& P, bool has, int n, const uint32_t (&mp)[NMAX],
-                                              const float (&v)[NMAX], float wk, int lo, int fw) {
+                                              uint32_t pv, const float (&v)[NMAX], float wk, int lo, int fw) {
     P.has = has;
     P.n = n;
     P.wk = wk;
 #pragma unroll
     for (int m = 0; m < NMAX; ++m) {
-        P.o[m] = (int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu) + lo;
+        P.o[m] = ((int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu)) + lo;
         P.c[m] = wk * v[m];
     }
-    auto overlap = [&](int a, int b) {
-        const int dx = (int)(mp[a] & 0xffffu) - (int)(mp[b] & 0xffffu);
-        const int dy = (int)(mp[a] >> 16) - (int)(mp[b] >> 16);
-        return dx < K && dx > -K && dy < K && dy > -K;
-    };
-    uint32_t plan = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int m0 = 4 * b;
-        const bool b1 = overlap(m0 + 1, m0);
-        const bool b2 = overlap(m0 + 2, m0 + 1) || (!b1 && overlap(m0 + 2, m0));
-        // members of the run that member m0 + 3 would join
-        const bool in0 = !b1 && !b2, in1 = !b2;
-        const bool b3 = overlap(m0 + 3, m0 + 2) || (in1 && overlap(m0 + 3, m0 + 1)) ||
-                        (in0 && overlap(m0 + 3, m0));
-        plan |= ((b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u)) << (4 * b);
-    }
+    const uint32_t x7 = (pv & 0xffffu) + 7u, y7 = (pv >> 16) + 7u;
+    const uint32_t A = overlap_prev<1>(pv, x7, y7), B = overlap_prev<2>(pv, x7, y7),
+                   C = overlap_prev<3>(pv, x7, y7);
+    // per block m0 = 4b: b1 = A(m0+1); b2 = A(m0+2) | (!b1 & B(m0+2));
+    // b3 = A(m0+3) | (!b2 & B(m0+3)) | (!b1 & !b2 & C(m0+3))
+    const uint32_t B1 = A & 0x2222u;
+    const uint32_t B2 = (A | (B & ~(B1 << 1))) & 0x4444u;
+    const uint32_t B3 = (A | (B & ~(B2 << 1)) | (C & ~(B1 << 2) & ~(B2 << 1))) & 0x8888u;
+    uint32_t plan = B1 | B2 | B3;
     if (n < 4) plan = n == 2 ? (plan & 2u) | 0xcu : 0xeu;  // n = 1: one member; n = 2: at most 2
     P.plan = __builtin_amdgcn_readfirstlane(plan);
     // keep the precomputation ahead of the wait
@@ -265,17 +280,27 @@ __device__ __forceinline__ void rmw_group(float2* acc, const PreparedGroup& P) {
     }
 }
 
-// One group per turn (token g -> g + 1). `has` false: only pass the token.
-__device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g, bool has, int n,
-                                                  const uint32_t (&mp)[NMAX],
-                                                  const float (&v)[NMAX], float wk, int lo,
-                                                  int fw) {
-    PreparedGroup P;
-    prepare_group(P, has, n, mp, v, wk, lo, fw);
+// The turn of group g: wait for the token, add, pass it on. The wave raises
+// its issue priority for the turn, so the chain's LDS reads and writes are not
+// queued behind the filtering waves' instructions (3 % off the exact
+// hard-threshold kernel, results unchanged).
+__device__ __forceinline__ bool take_turn(float2* acc, int* turn, int g, const PreparedGroup& P) {
+    __builtin_amdgcn_s_setprio(2);
     const bool timed_out = wait_turn(turn, g);
     rmw_group(acc, P);
     pass_turn(turn, g + 1);
+    __builtin_amdgcn_s_setprio(0);
     return timed_out;
+}
+
+// One group per turn (token g -> g + 1). `has` false: only pass the token.
+__device__ __forceinline__ bool add_group_in_turn(float2* acc, int* turn, int g, bool has, int n,
+                                                  const uint32_t (&mp)[NMAX], uint32_t pv,
+                                                  const float (&v)[NMAX], float wk, int lo,
+                                                  int fw) {
+    PreparedGroup P;
+    prepare_group(P, has, n, mp, pv, v, wk, lo, fw);
+    return take_turn(acc, turn, g, P);
 }
 
 __device__ __forceinline__ int floor_pow2(int c) {

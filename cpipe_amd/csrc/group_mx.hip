@@ -113,25 +113,34 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Forward 3D transform of one stack: z[nb] = (H_n X Kt)[:, 16 nb .. 16 nb + 15] in D layout.
-template <int PREC>
-__device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ src, int w,
-                                          uint32_t pm, bool live, int q,
-                                          const typename Mx<PREC>::vec8 (&kf)[2][4],
-                                          const typename Mx<PREC>::vec& ha) {
-    typedef Mx<PREC> M;
+// Issues the loads of one stack's rows for lane (member r, quarter q): rows
+// 2 kb + q/2, columns 4 (q & 1) .. + 3 of the member patch at pm (zeros for
+// r >= n); forward3d consumes them.
+__device__ __forceinline__ void load_stack(float (&a)[16], const float* __restrict__ src, int w,
+                                           uint32_t pm, bool live, int q) {
     const int qx = (int)(pm & 0xffffu), qy = (int)(pm >> 16);
-    typename M::vec xa[4];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-        float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (live) {
             const float* s = src + (size_t)(qy + 2 * kb + (q >> 1)) * w + qx + 4 * (q & 1);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) a[t] = s[t];
+            for (int t = 0; t < 4; ++t) a[4 * kb + t] = s[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[4 * kb + t] = 0.0f;
         }
-        xa[kb] = M::pack(a[0], a[1], a[2], a[3]);
     }
+}
+
+// Forward 3D transform of one loaded stack: z[nb] = (H_n X Kt)[:, 16 nb .. 16 nb + 15] in D layout.
+template <int PREC>
+__device__ __forceinline__ void forward3d(f4 (&z)[4], const float (&a)[16],
+                                          const typename Mx<PREC>::vec8 (&kf)[2][4],
+                                          const typename Mx<PREC>::vec& ha) {
+    typedef Mx<PREC> M;
+    typename M::vec xa[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) xa[kb] = M::pack(a[4 * kb], a[4 * kb + 1], a[4 * kb + 2], a[4 * kb + 3]);
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     const typename M::vec8 x0 = cat8<PREC>(xa[0], xa[1]), x1 = cat8<PREC>(xa[2], xa[3]);
 #pragma unroll
@@ -145,24 +154,23 @@ __device__ __forceinline__ void forward3d(f4 (&z)[4], const float* __restrict__ 
 constexpr int MX_SCR = 4;   // members per LDS transposition chunk
 constexpr int MX_SPP = 68;  // scratch floats per member: the chunk's four 16-byte writes hit distinct banks
 
-// Filters one group: 3D transform, shrinkage, inverse; leaves lane p = pixel p
-// of member m in v[m] (m < n) and returns the group's aggregation weight.
+// Filters one group from its loaded stacks (load_stack: xs noisy, xb basic):
+// 3D transform, shrinkage, inverse; leaves lane p = pixel p of member m in v[m]
+// (m < n) and returns the group's aggregation weight.
 template <int STEP, int PREC>
-__device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm, int r, int q,
-                                           int lane, const float* __restrict__ src,
-                                           const float* __restrict__ bsrc, int w, float* sb,
+__device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, int r, int q, int lane,
+                                           const float (&xs)[16], const float (&xb)[16], float* sb,
                                            const typename Mx<PREC>::vec8 (&kf)[2][4],
                                            const typename Mx<PREC>::vec8 (&ki)[4][2], float thr,
                                            float s2) {
     typedef Mx<PREC> M;
-    const bool live = r < n;
     const float* hm = c_mx_haar[31 - __builtin_clz((unsigned)n)];
     const typename M::vec ha =
         M::pack(hm[r * 16 + 4 * q], hm[r * 16 + 4 * q + 1], hm[r * 16 + 4 * q + 2], hm[r * 16 + 4 * q + 3]);
     const typename M::vec hb = M::pack(hm[(4 * q) * 16 + r], hm[(4 * q + 1) * 16 + r],
                                        hm[(4 * q + 2) * 16 + r], hm[(4 * q + 3) * 16 + r]);
     f4 z[4];
-    forward3d<PREC>(z, src, w, pm, live, q, kf, ha);
+    forward3d<PREC>(z, xs, kf, ha);
     float wgt;
     if (STEP == 1) {
         // coefficient (member 4q + t, frequency 16 nb + r); rows >= n are exactly zero
@@ -178,7 +186,7 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, uint32_t pm,
         wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
     } else {
         f4 zb[4];
-        forward3d<PREC>(zb, bsrc, w, pm, live, q, kf, ha);
+        forward3d<PREC>(zb, xb, kf, ha);
         float aw = 0.0f;
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
@@ -242,8 +250,10 @@ void k_group_mx(GroupLaunch L) {
     const int c = blockIdx.y;
     if (threadIdx.x == 0) turn = 0;
     bool timed_out = false;
-    const int fx0 = p.fx0[tx], fy0 = p.fy0[ty];
-    const int fw = p.fx1[tx] - fx0, fh = p.fy1[ty] - fy0;
+    // footprint bounds as scalars (member offsets y * fw + x stay scalar products)
+    const int fx0 = __builtin_amdgcn_readfirstlane(p.fx0[tx]), fy0 = __builtin_amdgcn_readfirstlane(p.fy0[ty]);
+    const int fw = __builtin_amdgcn_readfirstlane(p.fx1[tx]) - fx0;
+    const int fh = __builtin_amdgcn_readfirstlane(p.fy1[ty]) - fy0;
     for (int idx = threadIdx.x; idx < DOFS; idx += blockDim.x) acc[idx] = make_float2(0.0f, 0.0f);
     __syncthreads();
 
@@ -289,31 +299,44 @@ void k_group_mx(GroupLaunch L) {
     const int nbx = min(TILE, p.nx - tx * TILE), nby = min(TY, p.ny - ty * TY);
     const int G = nbx * nby;
 
+    const int lo = (li - fy0) * fw + (lj - fx0);
+
+    // Group g of this wave: its match list through the scalar cache
+    // (wave-uniform address), size and sigma; this lane's member r = mp[r].
+    // The stacks are loaded after the previous group's turn: with the DCT
+    // operand fragments held in registers, stacks in flight during the turn
+    // would not fit in the three-waves-per-SIMD register budget.
     for (int rnd = 0; rnd < (G + 3) / 4; ++rnd) {
         const int g = rnd * 4 + wave;
         const bool has = g < G;
         int n = 0;
         float wgt = 0.0f;
         float v[NMAX];
-        uint32_t mp[NMAX];
+        uint32_t mp[NMAX], pm = 0;
         if (has) {
             const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
             const size_t ref = (size_t)by * p.nx + bx;
-            n = floor_pow2(p.cnt[ref]);
-            const uint32_t pm = p.pos[ref * NMAX + r];  // this lane's member (rows of X)
+            n = floor_pow2(load_count(p.cnt, ref));
+            const uint32_t CONST_AS* lp = (const uint32_t CONST_AS*)(p.pos + ref * NMAX);
 #pragma unroll
-            for (int m = 0; m < NMAX; ++m) mp[m] = __builtin_amdgcn_readlane(pm, m);
+            for (int m = 0; m < NMAX; ++m) mp[m] = lp[m];
+            pm = mp[0];
+#pragma unroll
+            for (int m = 1; m < NMAX; ++m) pm = r == m ? mp[m] : pm;
             float gthr = thr, gs2 = s2;
             if (MAP) {
                 const float sg = smap[ref];
                 gthr = 2.7f * sg;
                 gs2 = sg * sg;
             }
-            wgt = filter_mx<STEP, PREC>(v, n, pm, r, q, lane, src, bsrc, w, sb, kf, ki, gthr, gs2);
+            float xs[16], xb[16];
+            const bool live = r < n;
+            load_stack(xs, src, w, pm, live, q);
+            if (STEP == 2) load_stack(xb, bsrc, w, pm, live, q);
+            wgt = filter_mx<STEP, PREC>(v, n, r, q, lane, xs, xb, sb, kf, ki, gthr, gs2);
             if (MAP) wgt = wgt * (1.0f / (gs2 > 1e-30f ? gs2 : 1.0f));
         }
-        const int lo = (li - fy0) * fw + (lj - fx0);
-        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, v, wgt * kw, lo, fw);
+        timed_out |= add_group_in_turn(acc, &turn, g, has, n, mp, pm, v, wgt * kw, lo, fw);
     }
     if (timed_out && lane == 0) atomicOr(L.err, 1);
     __syncthreads();
