@@ -11,7 +11,7 @@ constexpr int K = 8;          // patch edge
 constexpr int R = 19;         // search radius (39x39 window)
 constexpr int STRIDE = 3;     // reference stride
 constexpr int NMAX = 16;      // max group size
-constexpr int TILE = 16;      // refs per tile row (tile width); tile height is 16 or 8
+constexpr int TILE = 16;      // refs per tile row (tile width); tile height 4 (exact FP32), 8 or 16 (mixed)
 // Footprint edge of a tile of t refs: every pixel a member patch of one of its
 // groups can touch (t refs at stride 3, +- 19 px search, + 8 px patch).
 constexpr int foot_edge(int t) { return (t - 1) * STRIDE + 2 * R + K; }
