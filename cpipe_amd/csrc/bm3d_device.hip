@@ -1347,7 +1347,9 @@ int build_frame(cpipe_bm3d_plan* p, int f) {
         (rc = dev_alloc((void**)&F.bz, sizeof(float) * F.npx * p->c)) ||
         (rc = dev_alloc((void**)&F.pos[0], sizeof(uint32_t) * nref * NMAX)) ||
         (rc = dev_alloc((void**)&F.pos[1], sizeof(uint32_t) * nref * NMAX)) ||
-        (rc = dev_alloc((void**)&F.cnt[0], nref)) || (rc = dev_alloc((void**)&F.cnt[1], nref)) ||
+        // counts: whole dwords (the group kernels read a count through its aligned dword)
+        (rc = dev_alloc((void**)&F.cnt[0], (nref + 3) & ~(size_t)3)) ||
+        (rc = dev_alloc((void**)&F.cnt[1], (nref + 3) & ~(size_t)3)) ||
         (rc = dev_alloc((void**)&F.partial, sizeof(float) * p->tile_stride * ntiles * p->c)))
         return rc;
     const size_t ngeo = 2 * (size_t)F.w + 2 * (size_t)F.h + 2 * (size_t)F.ntx + 2 * (size_t)F.nty;

@@ -108,8 +108,9 @@ __device__ __forceinline__ GroupView group_view(const FrameTab& F, int step) {
 #define CONST_AS __attribute__((address_space(4)))
 
 // The match count of ref r (one byte per ref) through the scalar cache: the
-// aligned dword that holds it. Device allocations are at least 4-byte granular,
-// so that dword lies inside the count buffer's allocation.
+// aligned dword that holds it. Count buffers start 256-byte aligned and span
+// whole dwords (plans round their size up; the static no-allocation layout
+// rounds every buffer to 256 bytes), so that dword lies inside the buffer.
 __device__ __forceinline__ int load_count(const uint8_t* cnt, size_t r) {
     const uintptr_t a = (uintptr_t)(cnt + r);
     const uint32_t CONST_AS* wp = (const uint32_t CONST_AS*)(a & ~(uintptr_t)3);
