@@ -130,14 +130,17 @@ __device__ __forceinline__ float plane_load(__amdgpu_buffer_rsrc_t r, uint32_t v
 
 // One read/add/write round over members [S, S + L) of a group (pairwise
 // disjoint patches): all reads, then all writes.
+// o[m]: byte offset of the lane's (num, den) pair for member m.
 template <int S, int L>
 __device__ __forceinline__ void rmw_run(float2* acc, const int (&o)[NMAX], const float (&c)[NMAX],
                                         float wk) {
+    char* const base = reinterpret_cast<char*>(acc);
     float2 a[L];
 #pragma unroll
-    for (int m = 0; m < L; ++m) a[m] = acc[o[S + m]];
+    for (int m = 0; m < L; ++m) a[m] = *reinterpret_cast<const float2*>(base + o[S + m]);
 #pragma unroll
-    for (int m = 0; m < L; ++m) acc[o[S + m]] = make_float2(a[m].x + c[S + m], a[m].y + wk);
+    for (int m = 0; m < L; ++m)
+        *reinterpret_cast<float2*>(base + o[S + m]) = make_float2(a[m].x + c[S + m], a[m].y + wk);
 }
 
 // Members m0 .. m0+3 in rounds of consecutive pairwise-disjoint runs; bit k of
@@ -192,7 +195,7 @@ __device__ __forceinline__ void rmw_block4(float2* acc, const int (&o)[NMAX], co
 // pixel's members are added in member order either way (members covering one
 // pixel overlap, so they never share a run).
 struct PreparedGroup {
-    int o[NMAX];
+    int o[NMAX];    // byte offsets of the lane's (num, den) pair per member
     float c[NMAX];
     float wk;
     uint32_t plan;  // run breaks: bit 4b + k set when member 4b + k starts a new run (k = 1..3)
@@ -219,9 +222,11 @@ __device__ __forceinline__ void prepare_group(PreparedGroup& P, bool has, int n,
     P.has = has;
     P.n = n;
     P.wk = wk;
+    const int lo8 = lo * (int)sizeof(float2);
 #pragma unroll
     for (int m = 0; m < NMAX; ++m) {
-        P.o[m] = ((int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu)) + lo;
+        // the member's corner as a scalar byte offset, plus the lane's pixel
+        P.o[m] = ((int)(mp[m] >> 16) * fw + (int)(mp[m] & 0xffffu)) * (int)sizeof(float2) + lo8;
         P.c[m] = wk * v[m];
     }
     const uint32_t x7 = (pv & 0xffffu) + 7u, y7 = (pv >> 16) + 7u;
