@@ -96,6 +96,29 @@ def test_small_images(cuda, oracle_lib, w, h, c):
     assert synth.max_abs(g, ref) <= 1e-5
 
 
+def _random_shapes(n, seed):
+    """Seeded shapes: 1 or 3 channels, sides 8..700 (some below 40), sigma 0..0.12."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        c = int(rng.choice([1, 3]))
+        w = int(rng.integers(8, 700)) if rng.random() >= 0.2 else int(rng.integers(1, 40))
+        h = int(rng.integers(8, 500)) if rng.random() >= 0.2 else int(rng.integers(1, 40))
+        out.append((i, c, w, h, float(rng.uniform(0.0, 0.12))))
+    return out
+
+
+@pytest.mark.parametrize("i,c,w,h,sigma", _random_shapes(16, 7))
+def test_random_shapes_bit_identical(cuda, oracle_lib, i, c, w, h, sigma):
+    """Border tiles, clamped last refs and partial tiles at arbitrary frame sizes."""
+    print(f"random shape {i}: {c}x{h}x{w} sigma {sigma:.4f}", flush=True)
+    clean = synth.clean_scene(w, h, c, seed=i)
+    noisy = synth.add_gaussian(clean, sigma, seed=1000 + i).astype(np.float32)
+    g = _run_gpu(cuda, noisy, sigma)["out"]
+    ref = oracle_lib.bm3d_f32(noisy, sigma, nthreads=8)
+    np.testing.assert_array_equal(g.view(np.uint32), ref.view(np.uint32))
+
+
 def test_sigma_zero_is_identity(cuda):
     torch = cuda
     clean, noisy, _ = synth.workload("rgb64x48", seed=5)
