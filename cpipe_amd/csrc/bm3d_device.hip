@@ -1988,9 +1988,7 @@ int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
         const int nbx = (F.w + B - 1) / B, nby = (F.h + B - 1) / B;
         const int nbtx = (F.nx + BM_TX - 1) / BM_TX, nbty = (F.ny + BM_TY - 1) / BM_TY;
         std::vector<char> bm_used((size_t)nbtx * nbty, 0), tile_used((size_t)F.ntx * F.nty, 0);
-        for (int b = 0; b < nbx * nby; ++b) {
-            // every check_every-th block in raster order, the phase shifted by one per block row
-            if ((b + b / nbx) % o.check_every != o.check_every / 2) continue;
+        auto add_check = [&](int b) {
             cpipe_bm3d_plan_s::Check c{};
             c.f = f;
             c.x0 = (b % nbx) * B;
@@ -2016,7 +2014,13 @@ int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
                 for (int tx = c.tx0; tx < c.tx1; ++tx) tile_used[(size_t)ty * F.ntx + tx] = 1;
             cw = std::max(cw, c.x1 - c.x0);
             chh = std::max(chh, c.y1 - c.y0);
-        }
+        };
+        // every check_every-th block in raster order, the phase shifted by one per block row;
+        // a frame the pattern misses (fewer blocks than it needs) is checked on its central
+        // block, so no frame runs the network unchecked
+        for (int b = 0; b < nbx * nby; ++b)
+            if ((b + b / nbx) % o.check_every == o.check_every / 2) add_check(b);
+        if (T.nchk == 0) add_check((nby / 2) * nbx + nbx / 2);
         // each tile once: overlapping checks must not run one tile twice in a launch
         for (int i = 0; i < nbtx * nbty; ++i)
             if (bm_used[i]) bm.push_back(make_int4(f, i % nbtx, i / nbtx, 0));

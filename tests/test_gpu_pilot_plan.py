@@ -7,7 +7,11 @@
 - partial fallback: checked blocks that fall back carry the plain basic
   estimate exactly;
 - an identity pilot (zero network: it leaves all the noise in) falls back on
-  the under-denoising bound, and only on that bound.
+  the under-denoising bound, and only on that bound;
+- with the default options every frame gets at least one checked block (its
+  central one when the one-in-16 pattern misses a small frame), so the packaged
+  network never runs unchecked: on small Gaussian frames it trails step 1 by
+  1-5 dB and the checks send them back to the plain step 1.
 """
 import numpy as np
 import pytest
@@ -162,3 +166,19 @@ def test_plans_sharing_one_network_run_on_separate_streams(cuda):
         plan.close()
     for w_, o in zip(want, outs):
         assert np.array_equal(o.cpu().numpy().view(np.uint32), w_.view(np.uint32))
+
+
+@pytest.mark.parametrize("w,h,sigma", [(417, 402, 0.02), (332, 374, 0.007), (217, 372, 0.081), (256, 256, 0.05)])
+def test_small_frames_are_always_checked(cuda, w, h, sigma):
+    torch = cuda
+    clean = synth.clean_scene(w, h, 3, seed=w + h)
+    noisy = synth.add_gaussian(clean, sigma, seed=7).astype(np.float32)
+    src = torch.from_numpy(noisy).cuda()
+    net = pilot.Pilot(seed=None, weights=pilot.DEFAULT_WEIGHTS)
+    out, _, rep = _run(torch, src, sigma, net)
+    plain, _, _ = _run(torch, src, sigma)
+    print(f"{w}x{h} sigma {sigma}: {rep}, psnr pilot {synth.psnr(out, clean):.2f} dB, "
+          f"plain {synth.psnr(plain, clean):.2f} dB", flush=True)
+    assert rep["checked"] >= 1
+    assert synth.psnr(out, clean) >= synth.psnr(plain, clean) - 0.1
+    net.close()
