@@ -11,7 +11,9 @@
 // patch of one of its groups can touch (refs +- 19 px search, + 8 px patch).
 //
 // Arithmetic is FP32 with a fixed operation order (explicit fmaf, -ffp-contract=off):
-//   SSD      = sum_i rowsum_i, rowsum_i = fma chain over j of (ref - cand)^2
+//   SSD      = (((r0 + r1) + r2) + ((r3 + r4) + r5)) + (r6 + r7), row sum
+//              r_i = (sq3(e0..e2) + sq3(e3..e5)) + sq2(e6, e7), e = ref - cand,
+//              sq2/sq3 = product + fma chain
 //   DCT      = row pass then column pass, each an fma chain in index order
 //   Haar     = orthonormal multi-level, (a+b)*s / (a-b)*s
 //   Wiener   = W = b^2/(b^2+s^2); weight 1/sum W^2 (lane tree 32,16,..,1)
@@ -46,17 +48,23 @@ extern "C" const size_t CPIPE_BM3D_PILOT_BLOB_SIZE;
 using namespace cpipe_internal;
 
 namespace {
-// Block matching workgroup: 16 ref columns x 14 ref rows. Each wave owns 4
-// columns; a column is 16 consecutive lanes (one DPP row): 14 refs + 2 helper
-// lanes that only compute patch-row sums for the refs above them.
+// Block matching workgroup: 16 ref columns x 14 ref rows, two waves. A wave
+// owns 4 column pairs; a pair is 16 consecutive lanes (one DPP row): 14 refs
+// of each of its two columns + 2 helper lanes that only compute patch-row sums
+// for the refs above them.
 constexpr int BM_TX = 16;
 constexpr int BM_TY = 14;
+constexpr int BM_NT = 128;                                      // threads per workgroup
 constexpr int BM_COLS = (BM_TX - 1) * STRIDE + 2 * R + K + 1;  // 92: +1 for the dx=20 overrun
-constexpr int BM_LH = (BM_TY + 1) * STRIDE + 2 * R + 3 + 1;    // 86: helper rows included
-constexpr int BM_LW = 132;  // LDS row pitch = 4 mod 64: a wave's 16 rows x 4 columns of refs
-                            // (stride 3) read 64 distinct banks
+constexpr int BM_LH = (BM_TY + 1) * STRIDE + 2 * R + 3;        // 86: helper rows included
+constexpr int BM_LW = BM_COLS;  // LDS row pitch: a wave's 16 rows x 4 pairs (6 px apart) meet
+                                // at most 2-way bank conflicts with any pitch
 constexpr float BM_OUTSIDE = 1.0e18f;  // staged for pixels outside the image (SSD >= 1e36 > tau)
-constexpr int PEND = 12;    // per-lane queue of candidates awaiting top-16 insertion
+constexpr int PEND = 6;     // per-ref queue of candidates awaiting top-16 insertion
+// four workgroups per CU: 31.6 KB of search region + 9 KB of queues each
+static_assert(4 * (sizeof(float) * BM_LW * BM_LH + (sizeof(float) + sizeof(uint16_t)) * PEND * 2 * BM_NT) <=
+                  160 * 1024,
+              "block matching LDS");
 
 enum Status { OK = 0, FAILED = 1, OOM = 3, BAD_INDEX = 5, INTERNAL = 7, UNSUPPORTED = 8 };
 
@@ -268,27 +276,36 @@ __global__ void k_compat_rgba16f(const uint16_t* __restrict__ in, size_t isp,
 
 // ---------------------------------------------------------------------------
 // Block matching, exact FP32.
-// One workgroup = 16 x 14 reference positions, one lane per ref (+ helpers).
-// The tile's search region (<= 92x86 px) is staged in LDS once; every lane
-// scans its clipped 39x39 window in (dy, dx) raster order, 4 candidates at a
-// time (11 LDS reads per patch row serve 4 SSDs), and keeps a sorted top-16
-// in registers (reference pinned first, strict insertion => ties keep raster
-// order). Only SSD < tau enters. Qualifying candidates wait in a short LDS
-// queue and are inserted in batches, so the wave runs the insertion network
-// once per batch instead of once per candidate. Refs of a column are 3 rows
-// apart, so each lane computes the row sums of its own 3 patch rows and takes
-// rows 3..7 from the two lanes below (DPP row shifts): same sums, same order.
+// One workgroup (two waves) = 16 x 14 reference positions. Lane (row a, pair
+// cg) takes the refs at tile columns 2 cg and 2 cg + 1, 3 pixels apart: for one
+// displacement both refs use the differences of the same 11 pixel columns, and
+// with patch rows summed in column chunks of 3, 3 and 2 (ssd_row), the second
+// ref's first chunk is the first ref's middle chunk and its second chunk
+// extends the first ref's last one, so a pair costs 13 operations per patch
+// row and ref instead of 16.
+// The tile's search region (92 x 86 px) is staged in LDS once; every lane
+// scans the clipped 39x39 window in (dy, dx) raster order, 4 displacements at
+// a time (14 LDS reads per patch row serve 8 SSDs), and keeps a sorted top-16
+// per ref in registers (reference pinned first, strict insertion => ties keep
+// raster order). Only SSD < tau enters. Qualifying candidates wait in a short
+// LDS queue per ref and are inserted in batches, so the wave runs the
+// insertion network once per batch instead of once per candidate. Refs of a
+// column are 3 rows apart and the 8 row sums add up in groups of 3, 3 and 2,
+// so each lane sums its own 3 patch rows and takes the next group and the
+// pair after it from the two lanes below (DPP row shifts): same sums, same
+// order.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void insert16(float (&bd)[NMAX], uint32_t (&bp)[NMAX], float s,
                                          uint32_t p) {
-    // precondition: bd sorted ascending, bd[0] = 0 <= s < bd[15]
+    // precondition: bd sorted ascending, bd[0] = 0 <= s < bd[15]; the new
+    // value of slot k is the median of (bd[k-1], s, bd[k]) (one v_med3)
 #pragma unroll
     for (int k = NMAX - 1; k >= 1; --k) {
         const bool sh = s < bd[k - 1];
         const bool here = s < bd[k];
-        bd[k] = sh ? bd[k - 1] : (here ? s : bd[k]);
         bp[k] = sh ? bp[k - 1] : (here ? p : bp[k]);
+        bd[k] = __builtin_amdgcn_fmed3f(bd[k - 1], s, bd[k]);
     }
 }
 
@@ -301,11 +318,12 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
 }
 
 // One workgroup per item (frame, tile column, tile row) of `items`.
-__global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict__ tab,
-                                                     const int4* __restrict__ items, int step,
-                                                     float tau, int use_gate) {
+__global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restrict__ tab,
+                                                       const int4* __restrict__ items, int step,
+                                                       float tau, int use_gate) {
     __shared__ float tile[BM_LW * BM_LH];
-    __shared__ float2 pend[PEND * 256];  // [slot][thread]: (ssd, packed position)
+    __shared__ float qs[PEND * 2 * BM_NT];     // [slot][ref of the lane][thread]: SSD
+    __shared__ uint16_t qc[PEND * 2 * BM_NT];  // window code (dy + R) * 64 + dx + R
     const int4 it = items[blockIdx.x];
     const FrameTab& F = tab[it.x];
     if (use_gate && *F.gate == 0) return;  // step skipped (decided on the device)
@@ -316,13 +334,14 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
     const int bx0 = it.y * BM_TX, by0 = it.z * BM_TY;
     const int X0 = ref_pos(bx0, w) - R;  // LDS origin in image coordinates (may be < 0)
     const int Y0 = ref_pos(by0, h) - R;
+    const int tid = threadIdx.x;
     {
         // every load of the thread in flight before the first LDS store
-        constexpr int NST = BM_COLS * BM_LH, IT = (NST + 255) / 256;
+        constexpr int NST = BM_COLS * BM_LH, IT = (NST + BM_NT - 1) / BM_NT;
         float sv[IT];
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int idx = threadIdx.x + 256 * i;
+            const int idx = tid + BM_NT * i;
             const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
             const int gx = X0 + xx, gy = Y0 + yy;
             sv[i] = BM_OUTSIDE;
@@ -330,176 +349,199 @@ __global__ __launch_bounds__(256) void k_block_match(const FrameTab* __restrict_
         }
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int idx = threadIdx.x + 256 * i;
-            const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
-            if (idx < NST) tile[yy * BM_LW + xx] = sv[i];
+            const int idx = tid + BM_NT * i;
+            if (idx < NST) tile[idx] = sv[i];
         }
     }
     __syncthreads();
 
-    const int l = threadIdx.x & 63;
-    const int a = l & 15;                               // ref row within the tile (14, 15: helpers)
-    const int bx = bx0 + (threadIdx.x >> 6) * 4 + (l >> 4);
+    const int l = tid & 63;
+    const int a = l & 15;                            // ref row within the tile (14, 15: helpers)
+    const int bxp = bx0 + 2 * ((tid >> 6) * 4 + (l >> 4));  // first ref column of the lane's pair
     const int by = by0 + a;
-    const bool real = a < BM_TY && bx < nx && by < ny;
-    const int rx = ref_pos(min(bx, nx - 1), w);
-    const int lrx = rx - X0;
+    bool real[2];
+    int rx[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        real[k] = a < BM_TY && bxp + k < nx && by < ny;
+        rx[k] = ref_pos(min(bxp + k, nx - 1), w);
+    }
     // Regular tile: refs of a column sit exactly 3 rows apart, so a ref's
     // patch rows 3..7 are the rows 0..2 / 0..1 of the refs one and two below.
     // Only a tile holding the image's last ref row at a shorter stride is not.
     const int last_in = ny - 1 - by0;
     const bool regular = !(last_in >= 0 && last_in < BM_TY &&
                            ref_pos(ny - 1, h) != ref_pos(by0, h) + STRIDE * last_in);
+    // Likewise the two refs of a pair sit 3 columns apart, unless the image's
+    // last ref column, at a shorter stride, is the second of its pair.
+    const int lastx_in = nx - 1 - bx0;
+    const bool paired = !(lastx_in >= 0 && lastx_in < BM_TX && (lastx_in & 1) &&
+                          ref_pos(nx - 1, w) != ref_pos(nx - 2, w) + STRIDE);
     const int ry = regular ? ref_pos(by0, h) + STRIDE * a : ref_pos(min(by, ny - 1), h);
     const int lry = ry - Y0;
 
     const float inf = __builtin_inff();
-    float bd[NMAX];
-    uint32_t bp[NMAX];
+    float bd[2][NMAX];
+    uint32_t bp[2][NMAX];  // window codes
 #pragma unroll
-    for (int k = 0; k < NMAX; ++k) {
-        bd[k] = inf;
-        bp[k] = 0u;
+    for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) {
+            bd[k][m] = inf;
+            bp[k][m] = 0u;
+        }
+        bd[k][0] = 0.0f;
+        bp[k][0] = R * 64 + R;
     }
-    bd[0] = 0.0f;
-    bp[0] = ((uint32_t)ry << 16) | (uint32_t)rx;
 
     // Candidates below the current 16th best are queued (in scan order) and
     // merged in batches; the queue filter only uses a stale, looser bound, so
-    // the result equals inserting each candidate immediately.
-    float2* q = pend + threadIdx.x;
-    int np = 0;
-    auto flush = [&]() {
+    // the result equals inserting each candidate immediately. In the shared
+    // scan a ref's queue is merged when some lane's is full, before that lane's
+    // next candidate, so the wave runs the insertion network on full batches.
+    float* const qsl = qs + tid;  // slot s of ref k at [(2 s + k) * BM_NT]
+    uint16_t* const qcl = qc + tid;
+    int np[2] = {0, 0};
+    auto flush_ref = [&](int k) __attribute__((always_inline)) {
 #pragma unroll
-        for (int k = 0; k < PEND; ++k) {
-            if (k < np) {
-                const float2 e = q[k * 256];
-                if (e.x < bd[NMAX - 1]) insert16(bd, bp, e.x, __float_as_uint(e.y));
+        for (int s = 0; s < PEND; ++s) {
+            if (s < np[k]) {
+                const float e = qsl[(2 * s + k) * BM_NT];
+                const uint32_t c = qcl[(2 * s + k) * BM_NT];
+                if (e < bd[k][NMAX - 1]) insert16(bd[k], bp[k], e, c);
             }
         }
-        np = 0;
+        np[k] = 0;
     };
-    auto offer = [&](float s, int dx, int dy) {
-        const int qx = rx + dx, qy = ry + dy;
-        const bool valid = real && dx <= R && qx >= 0 && qx <= w - K && qy >= 0 && qy <= h - K &&
-                           (dx != 0 || dy != 0);
-        if (valid && s < tau && s < bd[NMAX - 1]) {
-            q[np * 256] = make_float2(s, __uint_as_float(((uint32_t)qy << 16) | (uint32_t)qx));
-            ++np;
-        }
+    auto flush = [&]() __attribute__((always_inline)) {
+        flush_ref(0);
+        flush_ref(1);
+    };
+    auto push = [&](int k, float s, int code) {
+        qsl[(2 * np[k] + k) * BM_NT] = s;
+        qcl[(2 * np[k] + k) * BM_NT] = (uint16_t)code;
+        ++np[k];
     };
 
-    if (regular) {
+    if (regular && paired) {
         // Candidates whose patch leaves the image read BM_OUTSIDE pixels, so
         // their SSD (>= 1e36) fails s < tau by itself; what is left to reject
         // is uniform: dx = R + 1 (the last 4-wide block's overrun), the ref
         // itself, and helper lanes (threshold -1). The row sums then stay
         // outside any branch and the DPP row shifts fold into the adds.
-        const float thr_lane = real ? tau : -1.0f;
-        auto offer_in = [&](float s, int dx, int dy, float thr) {
-            if (s < thr && s < bd[NMAX - 1]) {
-                const uint32_t qx = (uint32_t)(rx + dx), qy = (uint32_t)(ry + dy);
-                q[np * 256] = make_float2(s, __uint_as_float((qy << 16) | qx));
-                ++np;
-            }
-        };
-        float ref[3][8];
+        float thr_lane[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) thr_lane[k] = real[k] ? tau : -1.0f;
+        const int lrx = rx[0] - X0;
+        float ref[3][11];  // the lane's 3 patch rows over the pair's 11 columns
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) ref[i][j] = tile[(lry + i) * BM_LW + lrx + j];
+            for (int j = 0; j < 11; ++j) ref[i][j] = tile[(lry + i) * BM_LW + lrx + j];
         for (int dy = -R; dy <= R; ++dy) {
             const float* rowp = tile + (lry + dy) * BM_LW + lrx;
+            const int cy = (dy + R) * 64 + R;
             for (int dx0 = -R; dx0 <= R; dx0 += 4) {
-                if (__any(np > PEND - 4)) flush();
-                float rs[3][4];
+                float rs[2][3][4];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    float c[11];
+                    float c[14];
 #pragma unroll
-                    for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+                    for (int t = 0; t < 14; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        float e = ref[i][0] - c[k];
-                        float r = e * e;
+                    for (int kk = 0; kk < 4; ++kk) {
+                        float e[11];
 #pragma unroll
-                        for (int j = 1; j < 8; ++j) {
-                            e = ref[i][j] - c[k + j];
-                            r = __builtin_fmaf(e, e, r);
-                        }
-                        rs[i][k] = r;
+                        for (int j = 0; j < 11; ++j) e[j] = ref[i][j] - c[kk + j];
+                        const float a3 = sq3(e[3], e[4], e[5]);
+                        const float b6 = sq2(e[6], e[7]);
+                        rs[0][i][kk] = (sq3(e[0], e[1], e[2]) + a3) + b6;
+                        rs[1][i][kk] = (a3 + __builtin_fmaf(e[8], e[8], b6)) + sq2(e[9], e[10]);
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const float r3 = dpp_row_shl(rs[0][k], 1), r4 = dpp_row_shl(rs[1][k], 1);
-                    const float r5 = dpp_row_shl(rs[2][k], 1), r6 = dpp_row_shl(rs[0][k], 2);
-                    const float r7 = dpp_row_shl(rs[1][k], 2);
-                    float sum = rs[0][k] + rs[1][k];
-                    sum = sum + rs[2][k];
-                    sum = sum + r3;
-                    sum = sum + r4;
-                    sum = sum + r5;
-                    sum = sum + r6;
-                    sum = sum + r7;
-                    const int dx = dx0 + k;
-                    const float thr = (dx > R || (dy == 0 && dx == 0)) ? -1.0f : thr_lane;
-                    offer_in(sum, dx, dy, thr);
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int dx = dx0 + kk;
+                    const bool skip = dx > R || (dy == 0 && dx == 0);
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        // rows 0..2 (this lane), 3..5 (the lane below), 6..7 (two below)
+                        const float q = rs[k][0][kk] + rs[k][1][kk];
+                        const float pr = q + rs[k][2][kk];
+                        float sum = pr + dpp_row_shl(pr, 1);
+                        sum = sum + dpp_row_shl(q, 2);
+                        const float thr = skip ? -1.0f : thr_lane[k];
+                        if (__any(np[k] == PEND)) flush_ref(k);
+                        if (sum < thr && sum < bd[k][NMAX - 1]) push(k, sum, cy + dx);
+                    }
                 }
             }
         }
     } else {
-        float ref[64];
+        // A tile at the image's last ref row or column: every ref sums all 8
+        // of its patch rows itself, with the same row sums and order.
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int k = 0; k < 2; ++k) {
+            const int lrx = rx[k] - X0;
+            float ref[64];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) ref[i * 8 + j] = tile[(lry + i) * BM_LW + lrx + j];
-        for (int dy = -R; dy <= R; ++dy) {
-            const int qy = ry + dy;
-            if (!real || qy < 0 || qy > h - K) continue;
-            const float* rowp = tile + (lry + dy) * BM_LW + lrx;
-            for (int dx0 = -R; dx0 <= R; dx0 += 4) {
-                if (__any(np > PEND - 4)) flush();
-                float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < 8; ++i)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    float c[11];
+                for (int j = 0; j < 8; ++j) ref[i * 8 + j] = tile[(lry + i) * BM_LW + lrx + j];
+            for (int dy = -R; dy <= R; ++dy) {
+                const int qy = ry + dy;
+                if (!real[k] || qy < 0 || qy > h - K) continue;
+                const float* rowp = tile + (lry + dy) * BM_LW + lrx;
+                const int cy = (dy + R) * 64 + R;
+                for (int dx0 = -R; dx0 <= R; dx0 += 4) {
+                    if (__any(np[k] > PEND - 4)) flush();
+                    float s[4], g[4];  // patch sum, sum of the current group of rows
 #pragma unroll
-                    for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+                    for (int i = 0; i < 8; ++i) {
+                        float c[11];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        float e = ref[i * 8] - c[k];
-                        float r = e * e;
+                        for (int t = 0; t < 11; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
 #pragma unroll
-                        for (int j = 1; j < 8; ++j) {
-                            e = ref[i * 8 + j] - c[k + j];
-                            r = __builtin_fmaf(e, e, r);
+                        for (int kk = 0; kk < 4; ++kk) {
+                            float e[8];
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) e[j] = ref[i * 8 + j] - c[kk + j];
+                            const float r = ssd_row(e);
+                            g[kk] = (i == 0 || i == 3 || i == 6) ? r : g[kk] + r;
+                            if (i == 2) s[kk] = g[kk];
+                            if (i == 5 || i == 7) s[kk] = s[kk] + g[kk];
                         }
-                        s[k] = s[k] + r;
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const int dx = dx0 + kk, qx = rx[k] + dx;
+                        const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
+                        if (valid && s[kk] < tau && s[kk] < bd[k][NMAX - 1]) push(k, s[kk], cy + dx);
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) offer(s[k], dx0 + k, dy);
             }
         }
     }
     flush();
-    if (!real) return;
-    int n = 1;
 #pragma unroll
-    for (int k = 1; k < NMAX; ++k) n += bd[k] < inf ? 1 : 0;
-    const size_t r = (size_t)by * nx + bx;
-    uint32_t* o = pos + r * NMAX;
+    for (int k = 0; k < 2; ++k) {
+        if (!real[k]) continue;
+        int n = 1;
 #pragma unroll
-    for (int k = 0; k < NMAX; k += 4) {
-        uint4 v;
-        v.x = k + 0 < n ? bp[k + 0] : 0u;
-        v.y = k + 1 < n ? bp[k + 1] : 0u;
-        v.z = k + 2 < n ? bp[k + 2] : 0u;
-        v.w = k + 3 < n ? bp[k + 3] : 0u;
-        *reinterpret_cast<uint4*>(o + k) = v;
+        for (int m = 1; m < NMAX; ++m) n += bd[k][m] < inf ? 1 : 0;
+        uint32_t v[NMAX];
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) {
+            const int c = (int)bp[k][m];
+            const uint32_t qy = (uint32_t)(ry + (c >> 6) - R), qx = (uint32_t)(rx[k] + (c & 63) - R);
+            v[m] = m < n ? (qy << 16) | qx : 0u;
+        }
+        const size_t r = (size_t)by * nx + bxp + k;
+        uint32_t* o = pos + r * NMAX;
+#pragma unroll
+        for (int m = 0; m < NMAX; m += 4) *reinterpret_cast<uint4*>(o + m) = make_uint4(v[m], v[m + 1], v[m + 2], v[m + 3]);
+        cnt[r] = (uint8_t)n;
     }
-    cnt[r] = (uint8_t)n;
 }
 
 // ---------------------------------------------------------------------------
@@ -1612,7 +1654,7 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
             HIP_CHECK((hipError_t)bm_mx_launch(p->precision, p->tab_dev, L.bm, L.n_bm, step,
                                                tau_ssd(step), use_gate, r.st));
         } else {
-            k_block_match<<<L.n_bm, 256, 0, r.st>>>(p->tab_dev, L.bm, step, tau_ssd(step), use_gate);
+            k_block_match<<<L.n_bm, BM_NT, 0, r.st>>>(p->tab_dev, L.bm, step, tau_ssd(step), use_gate);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -3010,7 +3052,7 @@ int cpipe_bm3d_noalloc_planes(const float* const* in, size_t in_pitch, float* co
     const dim3 pg((w + 255) / 256, NOALLOC_BAND, L.n_rects);
     k_colour_in<<<pg, 256, 0, st>>>(tab, rects);
     for (int step = 1; step <= 2; ++step) {
-        k_block_match<<<L.n_bm, 256, 0, st>>>(tab, bm, step, tau_ssd(step), 0);
+        k_block_match<<<L.n_bm, BM_NT, 0, st>>>(tab, bm, step, tau_ssd(step), 0);
         const GroupLaunch gl{tab, grp, 0, err};
         const dim3 gg(L.n_grp, channels);
         if (step == 1)

@@ -18,6 +18,18 @@ constexpr int foot_edge(int t) { return (t - 1) * STRIDE + 2 * R + K; }
 constexpr int FOOT = foot_edge(TILE);  // 91
 constexpr int FOOT2 = FOOT * FOOT;     // largest footprint (16 x 16 refs)
 
+// Patch SSD in chunks of 3, 3 and 2, along both axes. A row sum is
+// (sq3(e0..e2) + sq3(e3..e5)) + sq2(e6, e7), with sq3 = e0^2 + e1^2 + e2^2 as one
+// product and two fmas (its first two terms are sq2); the 8 row sums r0..r7 add
+// up as (((r0 + r1) + r2) + ((r3 + r4) + r5)) + (r6 + r7). Refs 3 columns or 3
+// rows apart share chunks (the block matching kernels rely on it), and the CPU
+// FP32 restatement sums in the same order.
+__device__ __forceinline__ float sq2(float e0, float e1) { return __builtin_fmaf(e1, e1, e0 * e0); }
+__device__ __forceinline__ float sq3(float e0, float e1, float e2) { return __builtin_fmaf(e2, e2, sq2(e0, e1)); }
+__device__ __forceinline__ float ssd_row(const float (&e)[8]) {
+    return (sq3(e[0], e[1], e[2]) + sq3(e[3], e[4], e[5])) + sq2(e[6], e[7]);
+}
+
 // Per-frame parameters in device memory (one entry per frame of a plan or
 // batch). Geometry and buffers are fixed at creation; sigma, noise level and
 // the caller's input/output planes are rewritten before every run (one small

@@ -29,8 +29,8 @@
 //     FP32 score, so an insertion is one v_med3 per entry. Candidates below
 //     the lane's admission bound wait in a short LDS queue;
 //  4. the 4 x 12 shortlisted positions of each ref are re-ranked with the
-//     exact FP32 SSD in the exact kernel's operation order (row sums as fma
-//     chains, rows added in order); the 15 smallest by (SSD, scan order) with
+//     exact FP32 SSD in the exact kernel's operation order (chunks of 3, 3
+//     and 2 columns per row and rows per patch); the 15 smallest by (SSD, scan order) with
 //     SSD < tau follow the ref. Whenever the exact top 15 are among the
 //     shortlist (the rounded distances perturb ranks by far less than the gaps
 //     a quarter-window top-12 leaves), the lists equal the exact kernel's.
@@ -417,14 +417,14 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
             flush();
         }
 
-        // ---- 4. exact FP32 re-rank (row sums as fma chains, rows added in order).
+        // ---- 4. exact FP32 re-rank (chunks of 3, 3 and 2 columns and rows).
         // The sub-block geometry is recomputed here rather than kept live through
         // the scan (it would be spilled to scratch).
         int jr = j;
         asm volatile("" : "+v"(jr));
         const SubBlock b = sub_block(gx, gy, jr, bx0, by0, X0, Y0, w, h, nx, ny, regular);
         uint32_t bp[KL];  // (y << 16) | x
-        float es[KL];
+        float es[KL], gs[KL];
         int cb[KL];       // FP32-tile offset of a live entry; 0 (one shared address) otherwise
         uint32_t lm = 0;  // live entries, bit k
 #pragma unroll
@@ -455,14 +455,14 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
                 }
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
-                    float e = rr[0] - c[kk][0];
-                    float r = e * e;
+                    float e[8];
 #pragma unroll
-                    for (int jj = 1; jj < 8; ++jj) {
-                        e = rr[jj] - c[kk][jj];
-                        r = __builtin_fmaf(e, e, r);
-                    }
-                    es[k0 + kk] = i == 0 ? r : es[k0 + kk] + r;
+                    for (int jj = 0; jj < 8; ++jj) e[jj] = rr[jj] - c[kk][jj];
+                    const float r = ssd_row(e);
+                    // rows in groups of 3, 3 and 2: group sum g, patch sum es
+                    gs[k0 + kk] = (i == 0 || i == 3 || i == 6) ? r : gs[k0 + kk] + r;
+                    if (i == 2) es[k0 + kk] = gs[k0 + kk];
+                    if (i == 5 || i == 7) es[k0 + kk] = es[k0 + kk] + gs[k0 + kk];
                 }
             }
         }

@@ -228,26 +228,29 @@ static void run_jobs(job_fn fn, void* ctx, int njobs, int nthreads) {
 
 /* ------------------------------------------------------------------------ */
 /* FP32 mirror: block matching                                              */
-/* SSD = sum_i rowsum_i, rowsum_i = fma chain over j of (r-c)^2, i ascending */
+/* SSD = (((r0 + r1) + r2) + ((r3 + r4) + r5)) + (r6 + r7) over the row     */
+/* sums r_i = (sq3(e0..e2) + sq3(e3..e5)) + sq2(e6, e7), e = r - c,         */
+/* sq2(a, b) = fmaf(b, b, a * a), sq3(a, b, c) = fmaf(c, c, sq2(a, b)):      */
+/* chunks of 3, 3 and 2 along both axes, which refs 3 px apart share on the */
+/* GPU (cpipe_amd/csrc/bm3d_shared.h, bm3d_device.hip k_block_match)         */
 /* Candidates scanned in (dy, dx) raster order over the clipped 39x39       */
 /* window; the reference is pinned first; insertion is strict (<), so equal */
 /* distances keep raster order. Only SSD < tau enters.                      */
 /* ------------------------------------------------------------------------ */
 
+static inline float sq2_f32(float a, float b) { return fmaf(b, b, a * a); }
+static inline float sq3_f32(float a, float b, float c) { return fmaf(c, c, sq2_f32(a, b)); }
+
 static inline float ssd_f32(const float* ref, const float* b, int bs) {
-    float s = 0.0f;
+    float r[8];
     for (int i = 0; i < 8; ++i) {
         const float* br = b + (size_t)i * bs;
         const float* ar = ref + i * 8;
-        float e = ar[0] - br[0];
-        float r = e * e;
-        for (int j = 1; j < 8; ++j) {
-            e = ar[j] - br[j];
-            r = fmaf(e, e, r);
-        }
-        s = s + r;
+        float e[8];
+        for (int j = 0; j < 8; ++j) e[j] = ar[j] - br[j];
+        r[i] = (sq3_f32(e[0], e[1], e[2]) + sq3_f32(e[3], e[4], e[5])) + sq2_f32(e[6], e[7]);
     }
-    return s;
+    return (((r[0] + r[1]) + r[2]) + ((r[3] + r[4]) + r[5])) + (r[6] + r[7]);
 }
 
 static void bm_ref_f32(const float* p, int w, int h, int rx, int ry, float tau, uint32_t* pos,

@@ -1,8 +1,11 @@
 """The five workload configurations at their full sizes, through the C ABI.
 
   * config 0: 512x512 grey, Gaussian sigma 0.05 -- bit-identical to the CPU FP32
-    restatement (match lists and output), PSNR >= 50 dB / max-abs <= 1e-3 vs the
-    FP64 oracle;
+    restatement (match lists and output); against the FP64 oracle, at most
+    0.05 % of the step-1 match lists differ (near-tie decisions that FP32
+    rounding flips: 3-4 of 28 561 on the three seeded frames checked, with
+    either FP32 summation order of the SSD), PSNR >= 50 dB and max-abs <= 5e-3
+    (a flipped list moves one group's pixels by up to 1.6e-3 there);
   * config 2: the 4080x3072 RGB frame in FP16 and BF16 -- step-1 match lists
     equal to the exact path's for >= 99.5 % / 98 % of refs, PSNR >= 40 / 34 dB
     against the exact FP32 output (itself bit-identical to the CPU FP32
@@ -65,9 +68,13 @@ def test_config0_gray512_bit_exact(cuda, oracle_lib):
     np.testing.assert_array_equal(g["m2"][0].view(np.uint32), p2)
     np.testing.assert_array_equal(g["m2"][1], c2)
     np.testing.assert_array_equal(g["out"].view(np.uint32), out.view(np.uint32))
+    p64, c64 = oracle_lib.block_match_f64(noisy[0])
+    flipped = np.any(p1 != p64, axis=-1) | (c1 != c64)
+    print(f"step-1 lists differing from FP64: {int(flipped.sum())} of {flipped.size}", flush=True)
+    assert flipped.sum() <= 0.0005 * flipped.size
     o64 = oracle_lib.bm3d_f64(noisy, sigma)
     assert synth.psnr(g["out"], o64) >= 50.0
-    assert synth.max_abs(g["out"], o64) <= 1e-3
+    assert synth.max_abs(g["out"], o64) <= 5e-3
     assert synth.psnr(g["out"], clean) > synth.psnr(noisy, clean) + 6.0
 
 
