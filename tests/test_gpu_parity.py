@@ -40,11 +40,16 @@ def _run_gpu(torch, noisy, sigma, want_basic=False, want_matches=False):
     return res
 
 
+# Block matching pairs ref columns 2k, 2k+1 of a 16-column tile. A tile whose
+# forced last ref column (W-8 not a multiple of the stride) is the second of a
+# pair, or whose forced last ref row breaks the 3-row spacing, sums every ref's
+# patch rows itself; the other tiles share chunks between paired refs and
+# between refs 3 rows apart.
 CASES = [
-    ("gray64x64", 0),
-    ("gray61x47", 1),   # W-8, H-8 not multiples of the stride: forced last refs
+    ("gray64x64", 0),   # one tile, forced last column second of its pair, forced last row
+    ("gray61x47", 1),   # forced last column first of its pair: shared path
     ("rgb96x80", 2),
-    ("rgb130x70", 3),   # more than one tile along x
+    ("rgb130x70", 3),   # three tile columns: two shared, the last per-ref
     ("gray300x260", 4),  # several tiles in both directions
 ]
 
