@@ -51,10 +51,11 @@ namespace {
 // Block matching workgroup: 16 ref columns x 14 ref rows, two waves. A wave
 // owns 4 column pairs; a pair is 16 consecutive lanes (one DPP row): 14 refs
 // of each of its two columns + 2 helper lanes that only compute patch-row sums
-// for the refs above them.
+// for the refs above them. Launches too small to fill the GPU run one column
+// per DPP row instead (four waves per workgroup).
 constexpr int BM_TX = 16;
 constexpr int BM_TY = 14;
-constexpr int BM_NT = 128;                                      // threads per workgroup
+constexpr int BM_NT = 128;                                      // threads per workgroup (pairs)
 constexpr int BM_COLS = (BM_TX - 1) * STRIDE + 2 * R + K + 1;  // 92: +1 for the dx=20 overrun
 constexpr int BM_LH = (BM_TY + 1) * STRIDE + 2 * R + 3;        // 86: helper rows included
 constexpr int BM_LW = BM_COLS;  // LDS row pitch: a wave's 16 rows x 4 pairs (6 px apart) meet
@@ -317,13 +318,17 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
     return __builtin_bit_cast(float, r);
 }
 
-// One workgroup per item (frame, tile column, tile row) of `items`.
-__global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restrict__ tab,
-                                                       const int4* __restrict__ items, int step,
-                                                       float tau, int use_gate) {
+// One workgroup per item (frame, tile column, tile row) of `items`; RPL refs
+// per lane: 2 (pairs of columns, two waves) or 1 (four waves, for launches too
+// small to fill the GPU with pairs). Both sum every SSD in the same order.
+template <int RPL>
+__global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab* __restrict__ tab,
+                                                                 const int4* __restrict__ items, int step,
+                                                                 float tau, int use_gate) {
+    constexpr int NT = BM_NT * 2 / RPL;
     __shared__ float tile[BM_LW * BM_LH];
-    __shared__ float qs[PEND * 2 * BM_NT];     // [slot][ref of the lane][thread]: SSD
-    __shared__ uint16_t qc[PEND * 2 * BM_NT];  // window code (dy + R) * 64 + dx + R
+    __shared__ float qs[PEND * RPL * NT];     // [slot][ref of the lane][thread]: SSD
+    __shared__ uint16_t qc[PEND * RPL * NT];  // window code (dy + R) * 64 + dx + R
     const int4 it = items[blockIdx.x];
     const FrameTab& F = tab[it.x];
     if (use_gate && *F.gate == 0) return;  // step skipped (decided on the device)
@@ -337,11 +342,11 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
     const int tid = threadIdx.x;
     {
         // every load of the thread in flight before the first LDS store
-        constexpr int NST = BM_COLS * BM_LH, IT = (NST + BM_NT - 1) / BM_NT;
+        constexpr int NST = BM_COLS * BM_LH, IT = (NST + NT - 1) / NT;
         float sv[IT];
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int idx = tid + BM_NT * i;
+            const int idx = tid + NT * i;
             const int yy = idx / BM_COLS, xx = idx - yy * BM_COLS;
             const int gx = X0 + xx, gy = Y0 + yy;
             sv[i] = BM_OUTSIDE;
@@ -349,7 +354,7 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
         }
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int idx = tid + BM_NT * i;
+            const int idx = tid + NT * i;
             if (idx < NST) tile[idx] = sv[i];
         }
     }
@@ -357,12 +362,12 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
 
     const int l = tid & 63;
     const int a = l & 15;                            // ref row within the tile (14, 15: helpers)
-    const int bxp = bx0 + 2 * ((tid >> 6) * 4 + (l >> 4));  // first ref column of the lane's pair
+    const int bxp = bx0 + RPL * ((tid >> 6) * 4 + (l >> 4));  // the lane's (first) ref column
     const int by = by0 + a;
-    bool real[2];
-    int rx[2];
+    bool real[RPL];
+    int rx[RPL];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < RPL; ++k) {
         real[k] = a < BM_TY && bxp + k < nx && by < ny;
         rx[k] = ref_pos(min(bxp + k, nx - 1), w);
     }
@@ -375,16 +380,16 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
     // Likewise the two refs of a pair sit 3 columns apart, unless the image's
     // last ref column, at a shorter stride, is the second of its pair.
     const int lastx_in = nx - 1 - bx0;
-    const bool paired = !(lastx_in >= 0 && lastx_in < BM_TX && (lastx_in & 1) &&
-                          ref_pos(nx - 1, w) != ref_pos(nx - 2, w) + STRIDE);
+    const bool paired = RPL == 1 || !(lastx_in >= 0 && lastx_in < BM_TX && (lastx_in & 1) &&
+                                      ref_pos(nx - 1, w) != ref_pos(nx - 2, w) + STRIDE);
     const int ry = regular ? ref_pos(by0, h) + STRIDE * a : ref_pos(min(by, ny - 1), h);
     const int lry = ry - Y0;
 
     const float inf = __builtin_inff();
-    float bd[2][NMAX];
-    uint32_t bp[2][NMAX];  // window codes
+    float bd[RPL][NMAX];
+    uint32_t bp[RPL][NMAX];  // window codes
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < RPL; ++k) {
 #pragma unroll
         for (int m = 0; m < NMAX; ++m) {
             bd[k][m] = inf;
@@ -399,27 +404,29 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
     // the result equals inserting each candidate immediately. In the shared
     // scan a ref's queue is merged when some lane's is full, before that lane's
     // next candidate, so the wave runs the insertion network on full batches.
-    float* const qsl = qs + tid;  // slot s of ref k at [(2 s + k) * BM_NT]
+    float* const qsl = qs + tid;  // slot s of ref k at [(RPL s + k) * NT]
     uint16_t* const qcl = qc + tid;
-    int np[2] = {0, 0};
+    int np[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) np[k] = 0;
     auto flush_ref = [&](int k) __attribute__((always_inline)) {
 #pragma unroll
         for (int s = 0; s < PEND; ++s) {
             if (s < np[k]) {
-                const float e = qsl[(2 * s + k) * BM_NT];
-                const uint32_t c = qcl[(2 * s + k) * BM_NT];
+                const float e = qsl[(RPL * s + k) * NT];
+                const uint32_t c = qcl[(RPL * s + k) * NT];
                 if (e < bd[k][NMAX - 1]) insert16(bd[k], bp[k], e, c);
             }
         }
         np[k] = 0;
     };
     auto flush = [&]() __attribute__((always_inline)) {
-        flush_ref(0);
-        flush_ref(1);
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) flush_ref(k);
     };
     auto push = [&](int k, float s, int code) {
-        qsl[(2 * np[k] + k) * BM_NT] = s;
-        qcl[(2 * np[k] + k) * BM_NT] = (uint16_t)code;
+        qsl[(RPL * np[k] + k) * NT] = s;
+        qcl[(RPL * np[k] + k) * NT] = (uint16_t)code;
         ++np[k];
     };
 
@@ -429,34 +436,36 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
         // is uniform: dx = R + 1 (the last 4-wide block's overrun), the ref
         // itself, and helper lanes (threshold -1). The row sums then stay
         // outside any branch and the DPP row shifts fold into the adds.
-        float thr_lane[2];
+        float thr_lane[RPL];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) thr_lane[k] = real[k] ? tau : -1.0f;
+        for (int k = 0; k < RPL; ++k) thr_lane[k] = real[k] ? tau : -1.0f;
         const int lrx = rx[0] - X0;
-        float ref[3][11];  // the lane's 3 patch rows over the pair's 11 columns
+        constexpr int RW = 3 * RPL + 5;  // columns of the lane's refs: 11 (pair) or 8
+        float ref[3][RW];  // the lane's 3 patch rows
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 11; ++j) ref[i][j] = tile[(lry + i) * BM_LW + lrx + j];
+            for (int j = 0; j < RW; ++j) ref[i][j] = tile[(lry + i) * BM_LW + lrx + j];
         for (int dy = -R; dy <= R; ++dy) {
             const float* rowp = tile + (lry + dy) * BM_LW + lrx;
             const int cy = (dy + R) * 64 + R;
             for (int dx0 = -R; dx0 <= R; dx0 += 4) {
-                float rs[2][3][4];
+                float rs[RPL][3][4];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    float c[14];
+                    float c[RW + 3];
 #pragma unroll
-                    for (int t = 0; t < 14; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
+                    for (int t = 0; t < RW + 3; ++t) c[t] = rowp[i * BM_LW + dx0 + t];
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk) {
-                        float e[11];
+                        float e[RW];
 #pragma unroll
-                        for (int j = 0; j < 11; ++j) e[j] = ref[i][j] - c[kk + j];
+                        for (int j = 0; j < RW; ++j) e[j] = ref[i][j] - c[kk + j];
                         const float a3 = sq3(e[3], e[4], e[5]);
                         const float b6 = sq2(e[6], e[7]);
                         rs[0][i][kk] = (sq3(e[0], e[1], e[2]) + a3) + b6;
-                        rs[1][i][kk] = (a3 + __builtin_fmaf(e[8], e[8], b6)) + sq2(e[9], e[10]);
+                        if constexpr (RPL == 2)
+                            rs[RPL - 1][i][kk] = (a3 + __builtin_fmaf(e[8], e[8], b6)) + sq2(e[9], e[10]);
                     }
                 }
 #pragma unroll
@@ -464,7 +473,7 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
                     const int dx = dx0 + kk;
                     const bool skip = dx > R || (dy == 0 && dx == 0);
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) {
+                    for (int k = 0; k < RPL; ++k) {
                         // rows 0..2 (this lane), 3..5 (the lane below), 6..7 (two below)
                         const float q = rs[k][0][kk] + rs[k][1][kk];
                         const float pr = q + rs[k][2][kk];
@@ -481,7 +490,7 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
         // A tile at the image's last ref row or column: every ref sums all 8
         // of its patch rows itself, with the same row sums and order.
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < RPL; ++k) {
             const int lrx = rx[k] - X0;
             float ref[64];
 #pragma unroll
@@ -524,7 +533,7 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
     }
     flush();
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < RPL; ++k) {
         if (!real[k]) continue;
         int n = 1;
 #pragma unroll
@@ -542,6 +551,20 @@ __global__ __launch_bounds__(BM_NT) void k_block_match(const FrameTab* __restric
         for (int m = 0; m < NMAX; m += 4) *reinterpret_cast<uint4*>(o + m) = make_uint4(v[m], v[m + 1], v[m + 2], v[m + 3]);
         cnt[r] = (uint8_t)n;
     }
+}
+
+// Exact block matching over n items: pairs of refs per lane (two waves per
+// item) once a launch has items for about four workgroups per CU; one ref per
+// lane (four waves per item) below that, where the pairs' fewer waves would
+// leave most of the GPU idle.
+constexpr int BM_PAIR_ITEMS = 1024;
+
+void launch_block_match(int n_items, const FrameTab* tab, const int4* items, int step, int use_gate,
+                        hipStream_t st) {
+    if (n_items >= BM_PAIR_ITEMS)
+        k_block_match<2><<<n_items, BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate);
+    else
+        k_block_match<1><<<n_items, 2 * BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate);
 }
 
 // ---------------------------------------------------------------------------
@@ -1654,7 +1677,7 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
             HIP_CHECK((hipError_t)bm_mx_launch(p->precision, p->tab_dev, L.bm, L.n_bm, step,
                                                tau_ssd(step), use_gate, r.st));
         } else {
-            k_block_match<<<L.n_bm, BM_NT, 0, r.st>>>(p->tab_dev, L.bm, step, tau_ssd(step), use_gate);
+            launch_block_match(L.n_bm, p->tab_dev, L.bm, step, use_gate, r.st);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -3052,7 +3075,7 @@ int cpipe_bm3d_noalloc_planes(const float* const* in, size_t in_pitch, float* co
     const dim3 pg((w + 255) / 256, NOALLOC_BAND, L.n_rects);
     k_colour_in<<<pg, 256, 0, st>>>(tab, rects);
     for (int step = 1; step <= 2; ++step) {
-        k_block_match<<<L.n_bm, BM_NT, 0, st>>>(tab, bm, step, tau_ssd(step), 0);
+        launch_block_match(L.n_bm, tab, bm, step, 0, st);
         const GroupLaunch gl{tab, grp, 0, err};
         const dim3 gg(L.n_grp, channels);
         if (step == 1)

@@ -68,6 +68,31 @@ def test_batch_frames_equal_single_plans(cuda, precision, use_graph):
     b.close()
 
 
+# Block matching launches with >= 1024 items match two refs per lane, smaller
+# ones one ref per lane. These eight frames launch together with 1197 items;
+# seven have their forced last ref column as the second of a pair (tiles that
+# sum every ref themselves), three a forced last ref row, and one is paired
+# throughout. Each frame alone takes the one-ref path.
+PAIRED_SIZES = [(700, 500), (640, 481), (604, 422), (700, 347), (520, 499), (688, 500), (580, 460),
+                (661, 389)]
+
+
+def test_paired_block_matching_bit_exact(cuda, oracle_lib):
+    torch = cuda
+    srcs, sigmas = _frames(torch, PAIRED_SIZES, seed=5, channels=1)
+    print(f"paired block matching batch {PAIRED_SIZES}", flush=True)
+    b = engine.Batch(PAIRED_SIZES, 1, "fp32", 0, use_graph=False)
+    outs = [torch.empty_like(s) for s in srcs]
+    b.run(srcs, outs, sigmas)
+    torch.cuda.synchronize()
+    b.check()
+    b.close()
+    for f, (s, o, g) in enumerate(zip(srcs, outs, sigmas)):
+        ref = oracle_lib.bm3d_f32(s.cpu().numpy(), g, nthreads=16)
+        assert np.array_equal(_bits(o.cpu().numpy()), _bits(ref)), PAIRED_SIZES[f]
+        assert np.array_equal(_bits(o.cpu().numpy()), _bits(_single(torch, s, g, "fp32"))), PAIRED_SIZES[f]
+
+
 def test_batch_grey_frames(cuda):
     torch = cuda
     sizes = [(64, 64), (33, 90)]
