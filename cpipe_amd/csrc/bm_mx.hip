@@ -437,8 +437,19 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
             lm |= live ? 1u << k : 0u;
             cb[k] = live ? (qy - Y0) * LW + (qx - X0) : 0;
         }
+        // rows in groups of 3, 3 and 2: gs = the current group's sum, es = the
+        // finished groups' (one select per row and candidate; the group
+        // hand-over is one uniform branch per row)
 #pragma unroll 1
         for (int i = 0; i < 8; ++i) {
+            if (i == 3) {
+#pragma unroll
+                for (int k = 0; k < KL; ++k) es[k] = gs[k];
+            } else if (i == 6) {
+#pragma unroll
+                for (int k = 0; k < KL; ++k) es[k] = es[k] + gs[k];
+            }
+            const bool first = i == 0 || i == 3 || i == 6;
             const float* ri = L.tf + (b.lry + i) * LW + b.lrx;
             float rr[8];
 #pragma unroll
@@ -459,13 +470,12 @@ __global__ __launch_bounds__(NT) void k_bm_mx(const FrameTab* __restrict__ tab,
 #pragma unroll
                     for (int jj = 0; jj < 8; ++jj) e[jj] = rr[jj] - c[kk][jj];
                     const float r = ssd_row(e);
-                    // rows in groups of 3, 3 and 2: group sum g, patch sum es
-                    gs[k0 + kk] = (i == 0 || i == 3 || i == 6) ? r : gs[k0 + kk] + r;
-                    if (i == 2) es[k0 + kk] = gs[k0 + kk];
-                    if (i == 5 || i == 7) es[k0 + kk] = es[k0 + kk] + gs[k0 + kk];
+                    gs[k0 + kk] = first ? r : gs[k0 + kk] + r;
                 }
             }
         }
+#pragma unroll
+        for (int k = 0; k < KL; ++k) es[k] = es[k] + gs[k];
         // order key (ssd bits, scan order (dy + R) * 64 + (dx + R)); the ref
         // itself, empty slots and ssd >= tau drop out
         uint64_t key[KL];
