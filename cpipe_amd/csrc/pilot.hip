@@ -184,6 +184,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     __syncthreads();
     _Float16* o16 = ob[wave];
     const int ly0 = wave * RPW;  // this wave's first output row in the tile
+    // biases read once: the output stores could alias them as far as the
+    // compiler knows, so a read inside the loop is repeated for every row
+    float bias_l[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+        bias_l[nb] = (EPI == 0 || (EPI == 1 && nb == 0 && lr < 3)) ? a.bias[nb * 16 + lr] : 0.0f;
 
 #pragma unroll 1
     for (int seg = 0; seg < TW / 16; ++seg) {
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                 float* of = ofp + wave * 48;
                 if (lr < 3) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) of[lr * 16 + lq * 4 + e] = acc[r][0][e] + a.bias[lr];
+                    for (int e = 0; e < 4; ++e) of[lr * 16 + lq * 4 + e] = acc[r][0][e] + bias_l[0];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int co = nb * 16 + lr;
-                const float b = EPI == 0 ? a.bias[co] : 0.0f;
+                const float b = bias_l[nb];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     float v = acc[r][nb][e] + b;
@@ -370,6 +376,9 @@ __global__ __launch_bounds__(256) void k_conv_first(FirstArgs a) {
     __syncthreads();
     _Float16* o16 = ob[wave];
     typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+    float bias_l[NB];  // read once (the output stores could alias them)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bias_l[nb] = a.bias[nb * 16 + lr];
 #pragma unroll 1
     for (int r = 0; r < RPW; ++r) {
         const int ly = wave * RPW + r, oy = y0 + ly;
@@ -391,7 +400,7 @@ __global__ __launch_bounds__(256) void k_conv_first(FirstArgs a) {
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int co = nb * 16 + lr;
-                const float b = a.bias[co];
+                const float b = bias_l[nb];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const float v = acc[nb][e] + b;
