@@ -94,10 +94,8 @@ struct Bx<CPIPE_BM3D_PRECISION_FP16> {
 template <>
 struct Bx<CPIPE_BM3D_PRECISION_BF16> {
     typedef __bf16 vec8 __attribute__((ext_vector_type(8)));
-    static __device__ __forceinline__ uint16_t enc(float x) {  // round to nearest even
-        uint32_t u = __float_as_uint(x);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (uint16_t)(u >> 16);
+    static __device__ __forceinline__ uint16_t enc(float x) {  // round to nearest even (v_cvt_pk_bf16_f32)
+        return __builtin_bit_cast(uint16_t, (__bf16)x);
     }
     static __device__ __forceinline__ float dec(uint16_t b) {
         return __uint_as_float((uint32_t)b << 16);

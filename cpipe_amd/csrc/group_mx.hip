@@ -64,19 +64,11 @@ struct Mx<CPIPE_BM3D_PRECISION_FP16> {
 template <>
 struct Mx<CPIPE_BM3D_PRECISION_BF16> {
     typedef short vec __attribute__((ext_vector_type(4)));
-    // round to nearest even (inputs are finite)
-    static __device__ __forceinline__ short bf(float x) {
-        uint32_t u = __float_as_uint(x);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (short)(u >> 16);
-    }
+    // round to nearest even: gfx950's v_cvt_pk_bf16_f32, two values per instruction
     static __device__ __forceinline__ vec pack(float a, float b, float c, float d) {
-        vec r;
-        r[0] = bf(a);
-        r[1] = bf(b);
-        r[2] = bf(c);
-        r[3] = bf(d);
-        return r;
+        typedef __bf16 bvec __attribute__((ext_vector_type(4)));
+        const bvec r = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+        return __builtin_bit_cast(vec, r);
     }
     static __device__ __forceinline__ f4 mma(vec a, vec b, f4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -132,6 +124,43 @@ __device__ __forceinline__ void load_stack(float (&a)[16], const float* __restri
     }
 }
 
+// BF16 keeps 8 mantissa bits, so rounding a stack at its own level (pixels
+// near 0.5) costs more than the residual noise of a quiet frame. BF16 groups
+// are therefore centred: a level mu is taken out of each stack before
+// rounding, the 3D DC coefficient (the centred stack's plus 8 mu sqrt(n):
+// orthonormal DCT and Haar) is carried in FP32 through the shrinkage, and the
+// shrunk DC comes back after the inverse as DC / (8 sqrt(n)) on every member
+// pixel. The transforms are linear, so any mu is exact; mu is the mean of the
+// members' quarter-0 pixels (the first 16-lane row: four DPP adds, no LDS).
+__device__ __forceinline__ float stack_level(const float (&a)[16], int n) {
+    float s = 0.0f;  // rows past the group hold zeros
+#pragma unroll
+    for (int t = 0; t < 16; ++t) s = s + a[t];
+#define CPIPE_ROW_SHR(v, ctrl) \
+    __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xf, 0xf, true))
+    s = s + CPIPE_ROW_SHR(s, 0x111);  // row_shr:1
+    s = s + CPIPE_ROW_SHR(s, 0x112);  // row_shr:2
+    s = s + CPIPE_ROW_SHR(s, 0x114);  // row_shr:4
+    s = s + CPIPE_ROW_SHR(s, 0x118);  // row_shr:8: lane 15 holds the row's sum
+#undef CPIPE_ROW_SHR
+    const float row0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, s), 15));
+    return row0 * (1.0f / 16.0f) / (float)n;  // 16 n pixels; powers of two
+}
+
+__device__ __forceinline__ void centre_stack(float (&c)[16], const float (&a)[16], float mu, bool live) {
+    const float m = live ? mu : 0.0f;  // rows past the group stay zero
+#pragma unroll
+    for (int t = 0; t < 16; ++t) c[t] = a[t] - m;
+}
+
+// sqrt(n) and 1 / (8 sqrt(n)) for the group sizes 1, 2, 4, 8, 16 (n is wave-uniform)
+__device__ __forceinline__ float root_n(int n) {
+    return n == 1 ? 1.0f : n == 2 ? 1.41421356f : n == 4 ? 2.0f : n == 8 ? 2.82842712f : 4.0f;
+}
+__device__ __forceinline__ float inv8root_n(int n) {
+    return n == 1 ? 0.125f : n == 2 ? 0.0883883476f : n == 4 ? 0.0625f : n == 8 ? 0.0441941738f : 0.03125f;
+}
+
 // Forward 3D transform of one loaded stack: z[nb] = (H_n X Kt)[:, 16 nb .. 16 nb + 15] in D layout.
 template <int PREC>
 __device__ __forceinline__ void forward3d(f4 (&z)[4], const float (&a)[16],
@@ -169,8 +198,18 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, int r, int q
         M::pack(hm[r * 16 + 4 * q], hm[r * 16 + 4 * q + 1], hm[r * 16 + 4 * q + 2], hm[r * 16 + 4 * q + 3]);
     const typename M::vec hb = M::pack(hm[(4 * q) * 16 + r], hm[(4 * q + 1) * 16 + r],
                                        hm[(4 * q + 2) * 16 + r], hm[(4 * q + 3) * 16 + r]);
+    constexpr bool CENTRE = PREC == CPIPE_BM3D_PRECISION_BF16;
+    const float rootn = CENTRE ? root_n(n) : 0.0f;
     f4 z[4];
-    forward3d<PREC>(z, xs, kf, ha);
+    if constexpr (CENTRE) {
+        const float mu = stack_level(xs, n);
+        float c[16];
+        centre_stack(c, xs, mu, r < n);
+        forward3d<PREC>(z, c, kf, ha);
+        if (lane == 0) z[0][0] = z[0][0] + 8.0f * mu * rootn;  // the 3D DC: Haar row 0, frequency 0
+    } else {
+        forward3d<PREC>(z, xs, kf, ha);
+    }
     float wgt;
     if (STEP == 1) {
         // coefficient (member 4q + t, frequency 16 nb + r); rows >= n are exactly zero
@@ -186,7 +225,15 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, int r, int q
         wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
     } else {
         f4 zb[4];
-        forward3d<PREC>(zb, xb, kf, ha);
+        if constexpr (CENTRE) {
+            const float mu = stack_level(xb, n);
+            float c[16];
+            centre_stack(c, xb, mu, r < n);
+            forward3d<PREC>(zb, c, kf, ha);
+            if (lane == 0) zb[0][0] = zb[0][0] + 8.0f * mu * rootn;
+        } else {
+            forward3d<PREC>(zb, xb, kf, ha);
+        }
         float aw = 0.0f;
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
@@ -202,6 +249,12 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, int r, int q
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
         wgt = aw > 0.0f ? __builtin_amdgcn_rcpf(aw) : 1.0f;
+    }
+    // BF16: the shrunk DC leaves the MFMA path and is added back per pixel below
+    float dc_px = 0.0f;
+    if constexpr (CENTRE) {
+        dc_px = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(z[0][0]))) * inv8root_n(n);
+        if (lane == 0) z[0][0] = 0.0f;
     }
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     typename M::vec vt[4];
@@ -224,7 +277,12 @@ __device__ __forceinline__ float filter_mx(float (&v)[NMAX], int n, int r, int q
             wave_lds_sync();
 #pragma unroll
             for (int m = 0; m < MX_SCR; ++m)
-                if (hf * MX_SCR + m < n) v[hf * MX_SCR + m] = sb[m * MX_SPP + lane];
+                if (hf * MX_SCR + m < n) {
+                    if constexpr (CENTRE)
+                        v[hf * MX_SCR + m] = sb[m * MX_SPP + lane] + dc_px;
+                    else
+                        v[hf * MX_SCR + m] = sb[m * MX_SPP + lane];
+                }
             wave_lds_sync();
         }
     }
