@@ -1048,25 +1048,73 @@ __global__ void k_merge(const FrameTab* __restrict__ tab, const FrameRect* __res
 // step 1.
 // ---------------------------------------------------------------------------
 
+// The network's output on a checked rectangle comes from its own run on a
+// region of P x P pixels around it, P = block edge + 2 PILOT_HALO: the six 3x3
+// layers reach 6 pixels, so inside the rectangle it is bit-identical to the
+// whole-frame run, and the checks can decide while that run is still going.
+// The region keeps PILOT_HALO pixels on each side of the rectangle, shifted
+// inwards at the frame's edges (there the frame's own padding applies), so all
+// checks of a frame share one region size and run as one stack of images.
+// Each check's slot of `io` holds the region's 3 input and 3 output planes of
+// pitch P: [check][6][P * P].
+constexpr int PILOT_HALO = 6;
+
+// Region origin along one axis: rectangle start v0, frame extent n.
+__host__ __device__ inline int pilot_region_lo(int v0, int n, int P) {
+    const int lo = v0 - PILOT_HALO < n - P ? v0 - PILOT_HALO : n - P;
+    return lo > 0 ? lo : 0;
+}
+
+// The regions of all checks into their input slots, and each check's noise
+// level from its frame's (block (row, check)).
+__global__ __launch_bounds__(256) void k_pilot_gather(const FrameTab* __restrict__ tab,
+                                                      const FrameRect* __restrict__ rects,
+                                                      const float* __restrict__ frame_levels,
+                                                      float* __restrict__ io, float* __restrict__ levels,
+                                                      int P) {
+    const FrameRect r = rects[blockIdx.y];
+    const FrameTab& F = tab[r.f];
+    if (blockIdx.x == 0 && threadIdx.x == 0) levels[blockIdx.y] = frame_levels[r.f];
+    const int hx0 = pilot_region_lo(r.x0, F.w, P), hy0 = pilot_region_lo(r.y0, F.h, P);
+    const int hw = min(P, F.w), hh = min(P, F.h);
+    if ((int)blockIdx.x >= hh) return;
+    const int y = hy0 + blockIdx.x;
+    float* slot = io + (size_t)blockIdx.y * 6 * P * P + (size_t)blockIdx.x * P;
+    for (int x = threadIdx.x; x < hw; x += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) slot[(size_t)c * P * P + x] = F.z[c][(size_t)y * F.w + hx0 + x];
+    }
+}
+
+// Clears every frame's gate before the network and the checks start.
+__global__ void k_clear_gates(const FrameTab* __restrict__ tab, int n_frames) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < n_frames) __hip_atomic_store(tab[f].gate, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Three launches over all checked rectangles of all frames: per-row squared
 // residual sums (block (row, rectangle)), per-rectangle totals summed over the
 // rows in a fixed order (one block per rectangle; report entry blockIdx.x),
 // and the fallback copy of the rectangles that fail.
 __global__ __launch_bounds__(256) void k_pilot_check_rows(const FrameTab* __restrict__ tab,
                                                           const FrameRect* __restrict__ rects,
+                                                          const float* __restrict__ io, int P,
                                                           float2* __restrict__ part, int rows) {
     __shared__ float ra[256], rb[256];
     const FrameRect r = rects[blockIdx.y];
     const int y = r.y0 + blockIdx.x;
     if (y >= r.y1) return;  // uniform per block
     const FrameTab& F = tab[r.f];
+    // the block network's output planes (slot planes 3..5)
+    const float* pil = io + ((size_t)blockIdx.y * 6 + 3) * P * P +
+                       (size_t)(y - pilot_region_lo(r.y0, F.h, P)) * P - pilot_region_lo(r.x0, F.w, P);
     float a = 0.0f, b = 0.0f;
     for (int x = r.x0 + threadIdx.x; x < r.x1; x += blockDim.x) {
         const size_t o = (size_t)y * F.w + x;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float zs = F.z[c][o];
-            const float dl = zs - F.bz[c][o], dp = zs - F.bzp[c][o];
+            const float dl = zs - pil[(size_t)c * P * P + x], dp = zs - F.bzp[c][o];
             a = __builtin_fmaf(dl, dl, a);
             b = __builtin_fmaf(dp, dp, b);
         }
@@ -1140,7 +1188,9 @@ __global__ void k_pilot_decide(const FrameTab* __restrict__ tab, int n_frames,
     const FrameTab& F = tab[f];
     int falls = 0;
     for (int i = F.chk0; i < F.chk0 + F.nchk; ++i) falls += report[2 * i + 1] > 0.5f ? 1 : 0;
-    *F.gate = ((float)falls > frac * (float)F.nchk) ? 1 : 0;
+    // agent scope: the frame's network, running on another stream, polls it
+    __hip_atomic_store(F.gate, ((float)falls > frac * (float)F.nchk) ? 1 : 0, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -1223,6 +1273,7 @@ struct cpipe_bm3d_plan_s {
         int btx0, bty0, btx1, bty1;  // block-matching tiles
         int tx0, ty0, tx1, ty1;      // aggregation tiles
         int x0, y0, x1, y1;          // pixels
+        int hx0, hy0, hw, hh;        // the block network's region: the pixels + the receptive half-width
     };
     struct Lists {  // one stage's work over a set of frames / rectangles
         int4* bm = nullptr;
@@ -1278,6 +1329,10 @@ struct cpipe_bm3d_plan_s {
     Lists chk;                       // the checked rectangles' work, all checks in one launch per kernel
     float* report = nullptr;         // [n][2]: delta dB, fell back
     float2* chk_part = nullptr;      // [n][rect rows]: per-row squared-residual sums of each check
+    float* chk_io = nullptr;         // per check: 3 input + 3 output planes of its block network
+    float* chk_levels = nullptr;     // per check: its frame's noise level (filled every run)
+    void* chk_ws = nullptr;          // the block networks' activations (one block at a time)
+    int chk_pitch = 0;               // row pitch (and edge) of a block network's planes
     hipStream_t side = nullptr;      // the spot checks' plain step 1 runs here, beside the network
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // one graph per plan: the whole batch as a single submission
@@ -1345,6 +1400,9 @@ void free_plan(cpipe_bm3d_plan* p) {
     (void)hipFree(p->smap);
     (void)hipFree(p->report);
     (void)hipFree(p->chk_part);
+    (void)hipFree(p->chk_io);
+    (void)hipFree(p->chk_levels);
+    (void)hipFree(p->chk_ws);
     (void)hipFree(p->pilot_ws);
     if (p->side) (void)hipStreamDestroy(p->side);
     if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
@@ -1928,15 +1986,46 @@ int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) 
     if ((rc = r.mark(CPIPE_BM3D_T_COLOR, true))) return rc;
 
     if (p->pilot) {
-        // learned pilot in place of step 1; the spot checks' plain step 1 runs on
-        // the side stream at the same time (both only read the noisy planes)
+        // Learned pilot in place of step 1. The spot checks run on the side stream
+        // beside the whole-frame network: the plain step 1 and the network on each
+        // checked rectangle (+ its halo), the residual ratios and each frame's gate.
+        // A frame's network layers skip their work once its gate says it falls
+        // back (its output is then replaced by the plain step 1).
         if ((rc = r.mark(CPIPE_BM3D_T_PILOT, false))) return rc;
         const int nchk = p->chk.n_rects;
         if (nchk > 0) {
+            k_clear_gates<<<((int)n + 63) / 64, 64, 0, st>>>(p->tab_dev, (int)n);
+            HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipEventRecord(p->fork_ev, st));
             HIP_CHECK(hipStreamWaitEvent(p->side, p->fork_ev, 0));
             RunCtx rs{p, p->side, false};
             if ((rc = run_step(rs, 1, p->chk, TO_CHECK, 0, false))) return rc;
+            const int P = p->chk_pitch;
+            const size_t plane = (size_t)P * P;
+            k_pilot_gather<<<dim3(P, nchk), 256, 0, p->side>>>(p->tab_dev, p->chk.rects, levels_dev(p), p->chk_io,
+                                                               p->chk_levels, P);
+            HIP_CHECK(hipGetLastError());
+            for (int f = 0; f < (int)n; ++f) {  // a frame's checks as one stack (slot stride 6 P^2)
+                const FrameTab& T = p->tab[f];
+                if (T.nchk == 0) continue;
+                const auto& c = p->checks[T.chk0];
+                float* slot = p->chk_io + (size_t)T.chk0 * 6 * plane;
+                const float* bin[3] = {slot, slot + plane, slot + 2 * plane};
+                float* bout[3] = {slot + 3 * plane, slot + 4 * plane, slot + 5 * plane};
+                rc = cpipe_internal::pilot_forward(p->pilot, bin, (size_t)P, 6 * plane, c.hw, c.hh, T.nchk, 0.0f,
+                                                   p->chk_levels + T.chk0, bout, p->chk_ws, p->side);
+                if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
+            }
+            const int rows = p->chk.rect_h;
+            k_pilot_check_rows<<<dim3(rows, nchk), 256, 0, p->side>>>(p->tab_dev, p->chk.rects, p->chk_io, P,
+                                                                       p->chk_part, rows);
+            HIP_CHECK(hipGetLastError());
+            k_pilot_check_sum<<<nchk, 256, 0, p->side>>>(p->chk.rects, p->chk_part, rows, p->pilot_thr_db,
+                                                          p->pilot_under_db, p->report);
+            HIP_CHECK(hipGetLastError());
+            k_pilot_decide<<<((int)n + 63) / 64, 64, 0, p->side>>>(p->tab_dev, (int)n, p->report,
+                                                                   p->pilot_full_frac);
+            HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipEventRecord(p->join_ev, p->side));
         }
         for (int f = 0; f < (int)n; ++f) {
@@ -1945,20 +2034,13 @@ int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) 
             float* zin[3] = {p->tab[f].z[0], p->tab[f].z[1], p->tab[f].z[2]};
             float* bout[3] = {p->tab[f].bz[0], p->tab[f].bz[1], p->tab[f].bz[2]};
             rc = cpipe_internal::pilot_forward(p->pilot, zin, (size_t)F.w, 0, F.w, F.h, 1, 0.0f,
-                                               levels_dev(p) + f, bout, p->pilot_ws, st);
+                                               levels_dev(p) + f, bout, p->pilot_ws, st,
+                                               nchk > 0 ? p->tab[f].gate : nullptr);
             if (rc) return fail(rc, std::string("pilot: ") + cpipe_bm3d_pilot_last_error());
         }
         if (nchk > 0) {
             HIP_CHECK(hipStreamWaitEvent(st, p->join_ev, 0));
-            const int rows = p->chk.rect_h;
-            k_pilot_check_rows<<<dim3(rows, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->chk_part, rows);
-            HIP_CHECK(hipGetLastError());
-            k_pilot_check_sum<<<nchk, 256, 0, st>>>(p->chk.rects, p->chk_part, rows, p->pilot_thr_db,
-                                                    p->pilot_under_db, p->report);
-            HIP_CHECK(hipGetLastError());
-            k_pilot_check_apply<<<dim3(rows, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->report);
-            HIP_CHECK(hipGetLastError());
-            k_pilot_decide<<<((int)n + 63) / 64, 64, 0, st>>>(p->tab_dev, (int)n, p->report, p->pilot_full_frac);
+            k_pilot_check_apply<<<dim3(p->chk.rect_h, nchk), 256, 0, st>>>(p->tab_dev, p->chk.rects, p->report);
             HIP_CHECK(hipGetLastError());
         }
         if ((rc = r.mark(CPIPE_BM3D_T_PILOT, true))) return rc;
@@ -2032,8 +2114,14 @@ int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
     free_lists(p->chk);
     (void)hipFree(p->report);
     (void)hipFree(p->chk_part);
+    (void)hipFree(p->chk_io);
+    (void)hipFree(p->chk_levels);
+    (void)hipFree(p->chk_ws);
     p->report = nullptr;
     p->chk_part = nullptr;
+    p->chk_io = nullptr;
+    p->chk_levels = nullptr;
+    p->chk_ws = nullptr;
     p->checks.clear();
     std::vector<FrameRect> rects;
     std::vector<int4> bm;
@@ -2060,6 +2148,10 @@ int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
             c.y0 = (b / nbx) * B;
             c.x1 = std::min(F.w, c.x0 + B);
             c.y1 = std::min(F.h, c.y0 + B);
+            c.hx0 = pilot_region_lo(c.x0, F.w, B + 2 * PILOT_HALO);
+            c.hy0 = pilot_region_lo(c.y0, F.h, B + 2 * PILOT_HALO);
+            c.hw = std::min(B + 2 * PILOT_HALO, F.w);
+            c.hh = std::min(B + 2 * PILOT_HALO, F.h);
             c.tx0 = txlo[c.x0];
             c.tx1 = txhi[c.x1 - 1] + 1;
             c.ty0 = tylo[c.y0];
@@ -2099,6 +2191,14 @@ int build_check_lists(cpipe_bm3d_plan* p, const cpipe_bm3d_pilot_opts& o) {
         return rc;
     HIP_CHECK(hipMemset(p->report, 0, std::max<size_t>(1, rects.size()) * 2 * sizeof(float)));
     if ((rc = dev_alloc((void**)&p->chk_part, std::max<size_t>(1, rects.size() * (size_t)std::max(1, chh)) * sizeof(float2))))
+        return rc;
+    p->chk_pitch = B + 2 * PILOT_HALO;
+    const size_t plane = (size_t)p->chk_pitch * p->chk_pitch;
+    size_t max_stack = 1;  // a frame's checks run as one stack of images
+    for (const auto& T : p->tab) max_stack = std::max(max_stack, (size_t)T.nchk);
+    if ((rc = dev_alloc((void**)&p->chk_io, std::max<size_t>(1, rects.size()) * 6 * plane * sizeof(float))) ||
+        (rc = dev_alloc((void**)&p->chk_levels, std::max<size_t>(1, rects.size()) * sizeof(float))) ||
+        (rc = dev_alloc(&p->chk_ws, cpipe_internal::pilot_workspace_bytes(plane * max_stack))))
         return rc;
     L.n_bm = (int)bm.size();
     L.n_grp = (int)grp.size();
@@ -2741,8 +2841,14 @@ int cpipe_bm3d_plan_set_pilot(cpipe_bm3d_plan* p, cpipe_bm3d_pilot* net,
     free_lists(p->chk);
     (void)hipFree(p->report);
     (void)hipFree(p->chk_part);
+    (void)hipFree(p->chk_io);
+    (void)hipFree(p->chk_levels);
+    (void)hipFree(p->chk_ws);
     p->report = nullptr;
     p->chk_part = nullptr;
+    p->chk_io = nullptr;
+    p->chk_levels = nullptr;
+    p->chk_ws = nullptr;
     for (auto& f : p->fr) {
         (void)hipFree(f.bzp);
         f.bzp = nullptr;

@@ -14,8 +14,10 @@ namespace cpipe_internal {
 // into a graph, never share or lose activation memory; ws = null uses the
 // network's own workspace (grown on demand; one stream at a time).
 size_t pilot_workspace_bytes(size_t px);
+// `gate` (device int, may be null): every layer skips its work once it reads a
+// nonzero value there (a frame whose spot checks fell back discards the output).
 int pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size_t pitch, size_t img_stride,
                   int w, int h, int n_img, float level, const float* levels,
-                  float* const* out, void* ws, hipStream_t st);
+                  float* const* out, void* ws, hipStream_t st, const int* gate = nullptr);
 
 }  // namespace cpipe_internal

@@ -94,6 +94,7 @@ struct ConvArgs {
     float* outp[3];        // output planes (EPI 1)
     size_t pitch;          // plane row pitch (EPI 1)
     size_t img_stride_planes;
+    const int* gate;       // inference: skip the launch's work once *gate is set (null: never)
     int w, h;
 };
 
@@ -129,6 +130,18 @@ __global__ void k_frag(const _Float16* __restrict__ w16, _Float16* __restrict__ 
     frag_one<CO, CI, true, CI, CO>(w16, bwd, idx);
 }
 
+// A frame whose spot checks fell back discards the network's output, so its
+// layers skip their work once the frame's gate is set (the checks decide on
+// another stream while the network runs; the gate only goes from 0 to 1
+// during a run). One read per workgroup, shared through LDS, so that every
+// wave of the workgroup takes the same branch around its barriers.
+__device__ __forceinline__ bool gate_set(const int* gate) {
+    __shared__ int skip;
+    if (threadIdx.x == 0) skip = __hip_atomic_load(gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return skip != 0;
+}
+
 template <int CI, int CO, int EPI>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int PS = CI + 4;  // LDS pixel pitch in halves (8-byte aligned pixels)
@@ -137,6 +150,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     __shared__ __align__(16) _Float16 ob[4][16 * CO];  // per wave: 16 output pixels x CO
     // EPI 1: per wave, 16 pixels x 3 output channels (acc + bias) before the residual add
     __shared__ float ofp[EPI == 1 ? 4 * 3 * 16 : 1];
+    if (a.gate && gate_set(a.gate)) return;  // uniform: the kernel argument and the shared read
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
     const int w = a.w, h = a.h;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -329,6 +343,7 @@ struct FirstArgs {
     const _Float16* frag;     // k_frag_first layout
     const float* bias;        // [32]
     _Float16* out;            // NHWC, 32 channels
+    const int* gate;          // skip once *gate is set (null: never)
     int w, h;
 };
 
@@ -337,6 +352,7 @@ __global__ __launch_bounds__(256) void k_conv_first(FirstArgs a) {
     constexpr int TP = (CTH + 2) * (TW + 2);
     __shared__ __align__(16) half4 tile[TP];
     __shared__ __align__(16) _Float16 ob[4][16 * CO];
+    if (a.gate && gate_set(a.gate)) return;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CTH, img = blockIdx.z;
     const int w = a.w, h = a.h;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -742,7 +758,7 @@ int launch_conv(const ConvArgs& a, int n_img, hipStream_t st) {
 
 int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t img_stride, int w,
             int h, int n_img, float level, const float* levels, float* const* out, bool train,
-            hipStream_t st, void* ws = nullptr) {
+            hipStream_t st, void* ws = nullptr, const int* gate = nullptr) {
     const size_t px = (size_t)w * h * n_img;
     _Float16* act[NL] = {};
     if (ws && !train) {  // inference on the caller's workspace: layers ping-pong between two buffers
@@ -759,6 +775,7 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
     a.h = h;
     a.pitch = pitch;
     a.img_stride_planes = img_stride;
+    a.gate = train ? nullptr : gate;
     if (train) {
         // L1 on the packed input, which the weight gradient reads later
         k_pack_input<<<dim3((w + 255) / 256, h, n_img), 256, 0, st>>>(in[0], in[1], in[2], pitch, level,
@@ -780,6 +797,7 @@ int forward(cpipe_bm3d_pilot* n, const float* const* in, size_t pitch, size_t im
         f.frag = n->frag + n->frag_first;
         f.bias = n->w32 + n->off[0] + layer_w(0);
         f.out = act[1];
+        f.gate = gate;
         f.w = w;
         f.h = h;
         k_conv_first<<<dim3((w + TW - 1) / TW, (h + CTH - 1) / CTH, n_img), 256, 0, st>>>(f);
@@ -893,8 +911,8 @@ size_t pilot_workspace_bytes(size_t px) { return 2 * px * 32 * sizeof(_Float16);
 
 int pilot_forward(cpipe_bm3d_pilot* net, const float* const* in, size_t pitch, size_t img_stride,
                   int w, int h, int n_img, float level, const float* levels, float* const* out,
-                  void* ws, hipStream_t st) {
-    return forward(net, in, pitch, img_stride, w, h, n_img, level, levels, out, false, st, ws);
+                  void* ws, hipStream_t st, const int* gate) {
+    return forward(net, in, pitch, img_stride, w, h, n_img, level, levels, out, false, st, ws, gate);
 }
 }  // namespace cpipe_internal
 
