@@ -321,10 +321,54 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
 // One workgroup per item (frame, tile column, tile row) of `items`; RPL refs
 // per lane: 2 (pairs of columns, two waves) or 1 (four waves, for launches too
 // small to fill the GPU with pairs). Both sum every SSD in the same order.
+// Step 2 bound from the step-1 lists (prior = 1, where this run's step 1 wrote
+// them): a ref's 15 step-1 matches are 15 distinct candidates of its step-2
+// window other than the ref. If all of their exact SSDs on the basic estimate
+// are below tau, the largest of them, T, bounds the final 15th best from
+// above: a candidate with SSD > T has 15 candidates strictly below it, so it
+// cannot enter the list, ties included, and is not queued. The lists are the
+// same as without the bound; only fewer candidates reach the insertion.
+// Returns the queue threshold: min(tau, T raised by a relative 2^-16 and an
+// absolute 2^-100, so that SSD <= T passes SSD < threshold), or tau.
+__device__ __forceinline__ float step1_bound(const FrameTab& F, size_t r, const float* __restrict__ tile,
+                                             int lry, int lrx, int X0, int Y0, float tau) {
+    if (F.cnt[0][r] != NMAX) return tau;
+    const uint32_t* __restrict__ o = F.pos[0] + r * NMAX;
+    int cb[NMAX - 1];  // member m + 1's patch corner in the tile
+#pragma unroll
+    for (int m = 0; m < NMAX - 1; ++m) {
+        const uint32_t q = o[1 + m];
+        cb[m] = ((int)(q >> 16) - Y0) * BM_LW + ((int)(q & 0xffffu) - X0);
+    }
+    // row sums r0..r7 added as ((r0 + r1) + r2) + ((r3 + r4) + r5) + (r6 + r7)
+    float es[NMAX - 1], gs[NMAX - 1];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float rr[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) rr[j] = tile[(lry + i) * BM_LW + lrx + j];
+#pragma unroll
+        for (int m = 0; m < NMAX - 1; ++m) {
+            float e[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) e[j] = rr[j] - tile[cb[m] + i * BM_LW + j];
+            const float rs = ssd_row(e);
+            gs[m] = (i == 0 || i == 3 || i == 6) ? rs : gs[m] + rs;
+            if (i == 2) es[m] = gs[m];
+            if (i == 5 || i == 7) es[m] = es[m] + gs[m];
+        }
+    }
+    float t = es[0];
+#pragma unroll
+    for (int m = 1; m < NMAX - 1; ++m) t = fmaxf(t, es[m]);
+    if (!(t < tau)) return tau;
+    return fminf(tau, __builtin_fmaf(t, 0x1p-16f, t) + 0x1p-100f);
+}
+
 template <int RPL>
 __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab* __restrict__ tab,
                                                                  const int4* __restrict__ items, int step,
-                                                                 float tau, int use_gate) {
+                                                                 float tau, int use_gate, int prior) {
     constexpr int NT = BM_NT * 2 / RPL;
     __shared__ float tile[BM_LW * BM_LH];
     __shared__ float qs[PEND * RPL * NT];     // [slot][ref of the lane][thread]: SSD
@@ -384,6 +428,13 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                                       ref_pos(nx - 1, w) != ref_pos(nx - 2, w) + STRIDE);
     const int ry = regular ? ref_pos(by0, h) + STRIDE * a : ref_pos(min(by, ny - 1), h);
     const int lry = ry - Y0;
+    float thr_ref[RPL];  // queue threshold per ref: tau, or the step-1 bound
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+        thr_ref[k] = tau;
+        if (step == 2 && prior && real[k])
+            thr_ref[k] = step1_bound(F, (size_t)by * nx + bxp + k, tile, lry, rx[k] - X0, X0, Y0, tau);
+    }
 
     const float inf = __builtin_inff();
     float bd[RPL][NMAX];
@@ -438,7 +489,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
         // outside any branch and the DPP row shifts fold into the adds.
         float thr_lane[RPL];
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) thr_lane[k] = real[k] ? tau : -1.0f;
+        for (int k = 0; k < RPL; ++k) thr_lane[k] = real[k] ? thr_ref[k] : -1.0f;
         const int lrx = rx[0] - X0;
         constexpr int RW = 3 * RPL + 5;  // columns of the lane's refs: 11 (pair) or 8
         float ref[3][RW];  // the lane's 3 patch rows
@@ -525,7 +576,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                     for (int kk = 0; kk < 4; ++kk) {
                         const int dx = dx0 + kk, qx = rx[k] + dx;
                         const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
-                        if (valid && s[kk] < tau && s[kk] < bd[k][NMAX - 1]) push(k, s[kk], cy + dx);
+                        if (valid && s[kk] < thr_ref[k] && s[kk] < bd[k][NMAX - 1]) push(k, s[kk], cy + dx);
                     }
                 }
             }
@@ -560,11 +611,11 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
 constexpr int BM_PAIR_ITEMS = 1024;
 
 void launch_block_match(int n_items, const FrameTab* tab, const int4* items, int step, int use_gate,
-                        hipStream_t st) {
+                        int prior, hipStream_t st) {
     if (n_items >= BM_PAIR_ITEMS)
-        k_block_match<2><<<n_items, BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate);
+        k_block_match<2><<<n_items, BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate, prior);
     else
-        k_block_match<1><<<n_items, 2 * BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate);
+        k_block_match<1><<<n_items, 2 * BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate, prior);
 }
 
 // ---------------------------------------------------------------------------
@@ -1720,8 +1771,11 @@ struct RunCtx {
 
 // One BM3D step over a work list: block matching, group filtering and
 // aggregation, merge into `dest`. With use_gate a frame runs only if its
-// gate is set (decided on the device).
-int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, int use_gate, bool timed) {
+// gate is set (decided on the device). prior: step 2's exact block matching
+// may bound its queues with the step-1 lists, which this run wrote for every
+// frame of the list.
+int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, int use_gate, bool timed,
+             bool prior = false) {
     cpipe_bm3d_plan* p = r.p;
     const int sbm = step == 1 ? CPIPE_BM3D_T_BM1 : CPIPE_BM3D_T_BM2;
     const int sgr = step == 1 ? CPIPE_BM3D_T_GROUP1 : CPIPE_BM3D_T_GROUP2;
@@ -1735,7 +1789,7 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
             HIP_CHECK((hipError_t)bm_mx_launch(p->precision, p->tab_dev, L.bm, L.n_bm, step,
                                                tau_ssd(step), use_gate, r.st));
         } else {
-            launch_block_match(L.n_bm, p->tab_dev, L.bm, step, use_gate, r.st);
+            launch_block_match(L.n_bm, p->tab_dev, L.bm, step, use_gate, prior ? 1 : 0, r.st);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -2048,7 +2102,8 @@ int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) 
     } else {
         if ((rc = run_step(r, 1, p->full, TO_BASIC, 0, true))) return rc;
     }
-    return run_step(r, 2, p->full, TO_OUTPUT, 0, true);
+    // without a pilot, step 1 ran on every frame above
+    return run_step(r, 2, p->full, TO_OUTPUT, 0, true, !p->pilot);
 }
 
 int collect_timing(cpipe_bm3d_plan* p) {
@@ -3181,7 +3236,7 @@ int cpipe_bm3d_noalloc_planes(const float* const* in, size_t in_pitch, float* co
     const dim3 pg((w + 255) / 256, NOALLOC_BAND, L.n_rects);
     k_colour_in<<<pg, 256, 0, st>>>(tab, rects);
     for (int step = 1; step <= 2; ++step) {
-        launch_block_match(L.n_bm, tab, bm, step, 0, st);
+        launch_block_match(L.n_bm, tab, bm, step, 0, 1, st);
         const GroupLaunch gl{tab, grp, 0, err};
         const dim3 gg(L.n_grp, channels);
         if (step == 1)
