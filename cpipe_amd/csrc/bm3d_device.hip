@@ -697,19 +697,22 @@ __device__ __forceinline__ void load_line(const float* p, float (&x)[8]) {
     x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
 }
 
+// Forward, from the rows a lane loaded (load_rows): v[8 c + k] is element k
+// of row q of member 8 c + r for lane l = 8 r + q, so the row pass runs in
+// registers and only the column pass goes through the scratch. Returns the
+// coefficients in the pixel layout (lane (u, v) holds t[u][v] of every
+// member) in v[0 .. N).
 template <int N>
-__device__ __forceinline__ void dct_fwd_n(float (&v)[NMAX], float* sb, int lane) {
+__device__ __forceinline__ void dct_fwd_rows(float (&v)[NMAX], float* sb, int lane) {
     const int r = lane >> 3, q = lane & 7;
-    const int pix = lo8(r) + q;  // this lane's pixel (i, j) = (r, q) at [i][j]
+    const int pix = lo8(r) + q;  // this lane's coefficient (u, v) = (r, q) at [u][v]
     float* line = sb + r * SPP + lo8(q);
 #pragma unroll
     for (int c0 = 0; c0 < N; c0 += SCR) {
         constexpr int CH = N < SCR ? N : SCR;
-#pragma unroll
-        for (int m = 0; m < CH; ++m) sb[m * SPP + pix] = v[c0 + m];  // [m][i][j]
-        wave_lds_fence();
         float x[8], X[8];
-        load_line(line, x);  // row q of patch r
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = v[c0 + k];  // row q of member c0 + r
         fdct8(x, X);
 #pragma unroll
         for (int k = 0; k < 8; ++k) sb[r * SPP + lo8(k) + q] = X[k];  // stored [v][i]
@@ -821,34 +824,31 @@ __device__ __forceinline__ float rcp_nr(float d) {
     return r;
 }
 
-// Issues the loads of a group's N members (noisy, and the basic estimate for
-// step 2) into v and b; the values are consumed by filter_group. Each member
-// load is a buffer load: the member's corner (scalar) plus the lane's byte
-// offset boff = 4 (li * w + lj).
-template <int STEP, int M0, int M1>
-__device__ __forceinline__ void load_members(float (&v)[NMAX], float (&b)[NMAX], const uint32_t (&mp)[NMAX],
-                                             __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc,
-                                             int w, uint32_t boff) {
-#pragma unroll
-    for (int m = M0; m < M1; ++m) {
-        const uint32_t qx = mp[m] & 0xffffu, qy = mp[m] >> 16;
-        const uint32_t corner = (qy * (uint32_t)w + qx) * (uint32_t)sizeof(float);
-        v[m] = plane_load(src, boff, corner);
-        if (STEP == 2) b[m] = plane_load(bsrc, boff, corner);
-    }
-}
-
-// n (wave-uniform) is 1, 2, 4, 8 or 16: members [n/2, n) are loaded behind one
-// scalar branch per doubling.
+// Issues the loads of a group's members as rows: lane l = 8 r + q loads row q
+// of member r into v[0 .. 8) and, for n = 16, of member 8 + r into v[8 .. 16)
+// (two 16-byte buffer loads per row; and the basic estimate's into b for
+// step 2). Lanes past n load the patch their list slot names (a valid
+// position, or 0 past the list's count); their rows are never read back.
+// mr0 / mr1: the list entries of members r and 8 + r.
 template <int STEP>
-__device__ __forceinline__ void load_group(float (&v)[NMAX], float (&b)[NMAX], const uint32_t (&mp)[NMAX],
-                                           int n, __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc,
-                                           int w, uint32_t boff) {
-    load_members<STEP, 0, 1>(v, b, mp, src, bsrc, w, boff);
-    if (n >= 2) load_members<STEP, 1, 2>(v, b, mp, src, bsrc, w, boff);
-    if (n >= 4) load_members<STEP, 2, 4>(v, b, mp, src, bsrc, w, boff);
-    if (n >= 8) load_members<STEP, 4, 8>(v, b, mp, src, bsrc, w, boff);
-    if (n >= 16) load_members<STEP, 8, 16>(v, b, mp, src, bsrc, w, boff);
+__device__ __forceinline__ void load_rows(float (&v)[NMAX], float (&b)[NMAX], uint32_t mr0, uint32_t mr1, int n,
+                                          __amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t bsrc, int w,
+                                          int q) {
+    auto row = [&](float(&d)[NMAX], int o, __amdgpu_buffer_rsrc_t rs, uint32_t mpos) {
+        const uint32_t off = (((mpos >> 16) + (uint32_t)q) * (uint32_t)w + (mpos & 0xffffu)) * (uint32_t)sizeof(float);
+        const uint4 a = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        const uint4 c = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
+        d[o + 0] = __uint_as_float(a.x), d[o + 1] = __uint_as_float(a.y);
+        d[o + 2] = __uint_as_float(a.z), d[o + 3] = __uint_as_float(a.w);
+        d[o + 4] = __uint_as_float(c.x), d[o + 5] = __uint_as_float(c.y);
+        d[o + 6] = __uint_as_float(c.z), d[o + 7] = __uint_as_float(c.w);
+    };
+    row(v, 0, src, mr0);
+    if (STEP == 2) row(b, 0, bsrc, mr0);
+    if (n == NMAX) {
+        row(v, 8, src, mr1);
+        if (STEP == 2) row(b, 8, bsrc, mr1);
+    }
 }
 
 // Filters one group in place from its loaded members (load_group): 3D
@@ -857,7 +857,7 @@ __device__ __forceinline__ void load_group(float (&v)[NMAX], float (&b)[NMAX], c
 template <int STEP, int N>
 __device__ __forceinline__ float filter_group(float (&v)[NMAX], float (&b)[NMAX], float* sb, int lane,
                                               int li, int lj, float thr, float s2) {
-    dct_fwd_n<N>(v, sb, lane);
+    dct_fwd_rows<N>(v, sb, lane);
     haar_fwd_n<N>(v);
     float wgt;
     if (STEP == 1) {
@@ -874,7 +874,7 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], float (&b)[NMAX]
         for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
         wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
     } else {
-        dct_fwd_n<N>(b, sb, lane);
+        dct_fwd_rows<N>(b, sb, lane);
         haar_fwd_n<N>(b);
         // W = B^2 / (B^2 + sigma^2) on the orthonormal coefficients: Wl = W 2^-l
         // = bb / fma(bb, 2^l, sigma^2 4^l), so Wl v is the kept coefficient scaled
@@ -904,11 +904,11 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], float (&b)[NMAX]
 // Fetches group g of a tile of 16 x TY refs (rows of nbx refs): its match list
 // through the scalar cache (wave-uniform address) and, member m in lane m mod
 // 16, as a vector (pv), size n, the sigma of the noise-profile map (MAP), and
-// issues its member loads.
+// issues its member row loads (load_rows).
 template <int STEP, int TY, bool MAP>
 __device__ __forceinline__ void fetch_group(const GroupView& p, const float* __restrict__ smap, int tx,
                                             int ty, int nbx, int g, __amdgpu_buffer_rsrc_t src,
-                                            __amdgpu_buffer_rsrc_t bsrc, int w, uint32_t boff, int lane,
+                                            __amdgpu_buffer_rsrc_t bsrc, int w, int lane,
                                             int& n, uint32_t (&mp)[NMAX], uint32_t& pv, float (&v)[NMAX],
                                             float (&b)[NMAX], float& gthr, float& gs2) {
     const int bx = tx * TILE + g % nbx, by = ty * TY + g / nbx;
@@ -918,12 +918,13 @@ __device__ __forceinline__ void fetch_group(const GroupView& p, const float* __r
 #pragma unroll
     for (int m = 0; m < NMAX; ++m) mp[m] = lp[m];
     pv = p.pos[ref * NMAX + (lane & 15)];  // lane m (mod 16): member m, for the run plan
+    const uint32_t mr0 = p.pos[ref * NMAX + (lane >> 3)], mr1 = p.pos[ref * NMAX + 8 + (lane >> 3)];
     if (MAP) {
         const float sg = smap[ref];
         gthr = 2.7f * sg;
         gs2 = sg * sg;
     }
-    load_group<STEP>(v, b, mp, n, src, bsrc, w, boff);
+    load_rows<STEP>(v, b, mr0, mr1, n, src, bsrc, w, lane & 7);
 }
 
 // Tiles of 16 x TY refs, NW waves per workgroup (one group per wave per round).
@@ -964,7 +965,6 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     const uint32_t plane_bytes = (uint32_t)((size_t)w * p.h * sizeof(float));
     const __amdgpu_buffer_rsrc_t src = plane_rsrc(F.z[c], plane_bytes);
     const __amdgpu_buffer_rsrc_t bsrc = plane_rsrc(F.bz[c], plane_bytes);
-    const uint32_t boff = (uint32_t)(li * w + lj) * (uint32_t)sizeof(float);
     const float sc = F.sig[c];
     const float thr = 2.7f * sc;
     const float s2 = sc * sc;
@@ -984,7 +984,7 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
     uint32_t mp[NMAX], pv = 0;
     float v[NMAX], b[NMAX];
     float gthr = thr, gs2 = s2;
-    if (wave < G) fetch_group<STEP, TY, MAP>(p, smap, tx, ty, nbx, wave, src, bsrc, w, boff, lane, n, mp, pv, v, b, gthr, gs2);
+    if (wave < G) fetch_group<STEP, TY, MAP>(p, smap, tx, ty, nbx, wave, src, bsrc, w, lane, n, mp, pv, v, b, gthr, gs2);
     for (int rnd = 0; rnd < (G + NW - 1) / NW; ++rnd) {
         const int g = rnd * NW + wave;
         const bool has = g < G;
@@ -1001,7 +1001,7 @@ __global__ __launch_bounds__(64 * NW) void k_group(GroupLaunch L) {
         }
         PreparedGroup P;
         prepare_group(P, has, n, mp, pv, v, wgt * kw, lo, fw);
-        if (g + NW < G) fetch_group<STEP, TY, MAP>(p, smap, tx, ty, nbx, g + NW, src, bsrc, w, boff, lane, n, mp, pv, v, b, gthr, gs2);
+        if (g + NW < G) fetch_group<STEP, TY, MAP>(p, smap, tx, ty, nbx, g + NW, src, bsrc, w, lane, n, mp, pv, v, b, gthr, gs2);
         timed_out |= take_turn(acc, &turn, g, P);
     }
     if (timed_out && lane == 0) atomicOr(L.err, 1);
