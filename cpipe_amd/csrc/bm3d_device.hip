@@ -851,7 +851,31 @@ __device__ __forceinline__ void load_rows(float (&v)[NMAX], float (&b)[NMAX], ui
     }
 }
 
-// Filters one group in place from its loaded members (load_group): 3D
+// Wiener shrinkage of a transformed stack v with the basic estimate's b:
+// W = B^2 / (B^2 + sigma^2) on the orthonormal coefficients: Wl = W 2^-l
+// = bb / fma(bb, 2^l, sigma^2 4^l), so Wl v is the kept coefficient scaled
+// for the inverse. Returns this lane's sum W^2 / 4^l over the stack, kept at
+// the current level's scale (times 4 when the level drops; the caller scales
+// it back after the lane tree). GUARD: a zero denominator (sigma = 0) keeps
+// the coefficient; without it the caller has sigma^2 > 2^-100, so every
+// denominator is at least sigma^2 4^l > 2^-100 and the test always passes.
+template <int N, bool GUARD>
+__device__ __forceinline__ float wiener_shrink(float (&v)[NMAX], const float (&b)[NMAX], float s2) {
+    float aw = 0.0f;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        const int l = haar_level<N>(m);
+        if (m > 0 && l != haar_level<N>(m - 1)) aw = aw * 4.0f;
+        const float bb = b[m] * b[m];
+        const float d = __builtin_fmaf(bb, (float)(1 << l), s2 * (float)(1 << (2 * l)));
+        const float W = (!GUARD || d > 0x1p-100f) ? bb * rcp_nr(d) : HAAR_INV_POW2[l];
+        v[m] = W * v[m];
+        aw = (m == 0) ? W * W : __builtin_fmaf(W, W, aw);
+    }
+    return aw;
+}
+
+// Filters one group in place from its loaded members (load_rows): 3D
 // transform, shrinkage, inverse; returns the group's aggregation weight
 // (1/N_nonzero for the hard threshold, 1/sum W^2 for Wiener).
 template <int STEP, int N>
@@ -861,36 +885,20 @@ __device__ __forceinline__ float filter_group(float (&v)[NMAX], float (&b)[NMAX]
     haar_fwd_n<N>(v);
     float wgt;
     if (STEP == 1) {
-        int nnz = 0;
+        int nnz = 0;  // kept coefficients of the wave, counted per member by ballot
 #pragma unroll
         for (int m = 0; m < N; ++m) {
             const int l = haar_level<N>(m);
             const float t = l == 0 ? thr : thr * HAAR_ROOT2[l];
             const bool keep = fabsf(v[m]) > t;
-            nnz += keep ? 1 : 0;
+            nnz += __popcll(__ballot(keep));
             v[m] = keep ? v[m] * HAAR_INV_POW2[l] : 0.0f;
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) nnz += __shfl_xor(nnz, off);
         wgt = nnz > 0 ? 1.0f / (float)nnz : 1.0f;
     } else {
         dct_fwd_rows<N>(b, sb, lane);
         haar_fwd_n<N>(b);
-        // W = B^2 / (B^2 + sigma^2) on the orthonormal coefficients: Wl = W 2^-l
-        // = bb / fma(bb, 2^l, sigma^2 4^l), so Wl v is the kept coefficient scaled
-        // for the inverse; aw keeps sum W^2 / 4^l of the current level (times 4
-        // when the level drops) and is scaled back after the lane tree
-        float aw = 0.0f;
-#pragma unroll
-        for (int m = 0; m < N; ++m) {
-            const int l = haar_level<N>(m);
-            if (m > 0 && l != haar_level<N>(m - 1)) aw = aw * 4.0f;
-            const float bb = b[m] * b[m];
-            const float d = __builtin_fmaf(bb, (float)(1 << l), s2 * (float)(1 << (2 * l)));
-            const float W = d > 0x1p-100f ? bb * rcp_nr(d) : HAAR_INV_POW2[l];
-            v[m] = W * v[m];
-            aw = (m == 0) ? W * W : __builtin_fmaf(W, W, aw);
-        }
+        float aw = s2 > 0x1p-100f ? wiener_shrink<N, false>(v, b, s2) : wiener_shrink<N, true>(v, b, s2);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) aw = aw + __shfl_xor(aw, off);
         aw = aw * (float)(1 << (2 * haar_level<N>(N - 1)));
