@@ -61,10 +61,9 @@ constexpr int BM_LH = (BM_TY + 1) * STRIDE + 2 * R + 3;        // 86: helper row
 constexpr int BM_LW = BM_COLS;  // LDS row pitch: a wave's 16 rows x 4 pairs (6 px apart) meet
                                 // at most 2-way bank conflicts with any pitch
 constexpr float BM_OUTSIDE = 1.0e18f;  // staged for pixels outside the image (SSD >= 1e36 > tau)
-constexpr int PEND = 6;     // per-ref queue of candidates awaiting top-16 insertion
-// four workgroups per CU: 31.6 KB of search region + 9 KB of queues each
-static_assert(4 * (sizeof(float) * BM_LW * BM_LH + (sizeof(float) + sizeof(uint16_t)) * PEND * 2 * BM_NT) <=
-                  160 * 1024,
+constexpr int PEND = 8;     // per-ref queue of candidate keys awaiting top-16 insertion
+// four workgroups per CU: 31.6 KB of search region + 8 KB of queues each
+static_assert(4 * (sizeof(float) * BM_LW * BM_LH + sizeof(float) * PEND * 2 * BM_NT) <= 160 * 1024,
               "block matching LDS");
 
 enum Status { OK = 0, FAILED = 1, OOM = 3, BAD_INDEX = 5, INTERNAL = 7, UNSUPPORTED = 8 };
@@ -287,26 +286,56 @@ __global__ void k_compat_rgba16f(const uint16_t* __restrict__ in, size_t isp,
 // The tile's search region (92 x 86 px) is staged in LDS once; every lane
 // scans the clipped 39x39 window in (dy, dx) raster order, 4 displacements at
 // a time (14 LDS reads per patch row serve 8 SSDs), and keeps a sorted top-16
-// per ref in registers (reference pinned first, strict insertion => ties keep
-// raster order). Only SSD < tau enters. Qualifying candidates wait in a short
-// LDS queue per ref and are inserted in batches, so the wave runs the
-// insertion network once per batch instead of once per candidate. Refs of a
+// of rank keys (bm_key: SSD to 12 mantissa bits, then scan position) per ref in
+// registers, reference pinned first. Only SSD < tau enters. Qualifying
+// candidates wait as keys in a short LDS queue per ref and are inserted in
+// batches, so the wave runs the insertion network once per batch instead of
+// once per candidate. Refs of a
 // column are 3 rows apart and the 8 row sums add up in groups of 3, 3 and 2,
 // so each lane sums its own 3 patch rows and takes the next group and the
 // pair after it from the two lanes below (DPP row shifts): same sums, same
 // order.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void insert16(float (&bd)[NMAX], uint32_t (&bp)[NMAX], float s,
-                                         uint32_t p) {
-    // precondition: bd sorted ascending, bd[0] = 0 <= s < bd[15]; the new
-    // value of slot k is the median of (bd[k-1], s, bd[k]) (one v_med3)
+// Keyed slots per ref: the reference and the 19 candidates with the smallest
+// rank keys. The list keeps the 15 of those 19 that come first by exact (SSD,
+// scan position): a candidate of the exact top 15 is among the 19 smallest
+// keys unless five candidates after it in scan order with larger exact SSDs
+// share its 2^-12 bucket, so the lists are the exact (SSD, position) top 15 in
+// practice and by definition equal the CPU FP32 restatement's, which keeps the
+// same two stages.
+constexpr int BM_NK = 20;
+
+__device__ __forceinline__ void insert_key(float (&bd)[BM_NK], float s) {
+    // bd: sorted keys, bd[0] the pinned reference (below every key); the new
+    // key of slot k is the median of (bd[k-1], s, bd[k]) (one v_med3). A key
+    // at or above bd[BM_NK - 1] (or +inf: no candidate) changes nothing.
 #pragma unroll
-    for (int k = NMAX - 1; k >= 1; --k) {
-        const bool sh = s < bd[k - 1];
-        const bool here = s < bd[k];
-        bp[k] = sh ? bp[k - 1] : (here ? p : bp[k]);
-        bd[k] = __builtin_amdgcn_fmed3f(bd[k - 1], s, bd[k]);
+    for (int k = BM_NK - 1; k >= 1; --k) bd[k] = __builtin_amdgcn_fmed3f(bd[k - 1], s, bd[k]);
+}
+
+// Exact SSDs, in the chunked order, of the ref patch at tile offset
+// (lry, lrx) against M candidate patches at tile offsets cb[m].
+template <int M>
+__device__ __forceinline__ void tile_ssds(const float* __restrict__ tile, int lry, int lrx, const int (&cb)[M],
+                                          float (&es)[M]) {
+    // row sums r0..r7 added as ((r0 + r1) + r2) + ((r3 + r4) + r5) + (r6 + r7)
+    float gs[M];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float rr[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) rr[j] = tile[(lry + i) * BM_LW + lrx + j];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            float e[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) e[j] = rr[j] - tile[cb[m] + i * BM_LW + j];
+            const float rs = ssd_row(e);
+            gs[m] = (i == 0 || i == 3 || i == 6) ? rs : gs[m] + rs;
+            if (i == 2) es[m] = gs[m];
+            if (i == 5 || i == 7) es[m] = es[m] + gs[m];
+        }
     }
 }
 
@@ -326,10 +355,13 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
 // window other than the ref. If all of their exact SSDs on the basic estimate
 // are below tau, the largest of them, T, bounds the final 15th best from
 // above: a candidate with SSD > T has 15 candidates strictly below it, so it
-// cannot enter the list, ties included, and is not queued. The lists are the
-// same as without the bound; only fewer candidates reach the insertion.
-// Returns the queue threshold: min(tau, T raised by a relative 2^-16 and an
-// absolute 2^-100, so that SSD <= T passes SSD < threshold), or tau.
+// cannot enter the list, and is not queued. With rank keys (bm_key) the
+// skipped SSDs are those past the end of T's 2^-12 bucket: their keys exceed
+// every member's and every unskipped candidate's, so whenever one of them is
+// among the 19 keyed slots all 15 members are too and precede it by exact
+// SSD. The lists are the same as without the bound; only fewer candidates
+// reach the insertion. Returns the queue threshold: min(tau, the end of T's
+// bucket), or tau.
 __device__ __forceinline__ float step1_bound(const FrameTab& F, size_t r, const float* __restrict__ tile,
                                              int lry, int lrx, int X0, int Y0, float tau) {
     if (F.cnt[0][r] != NMAX) return tau;
@@ -340,29 +372,13 @@ __device__ __forceinline__ float step1_bound(const FrameTab& F, size_t r, const 
         const uint32_t q = o[1 + m];
         cb[m] = ((int)(q >> 16) - Y0) * BM_LW + ((int)(q & 0xffffu) - X0);
     }
-    // row sums r0..r7 added as ((r0 + r1) + r2) + ((r3 + r4) + r5) + (r6 + r7)
-    float es[NMAX - 1], gs[NMAX - 1];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        float rr[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) rr[j] = tile[(lry + i) * BM_LW + lrx + j];
-#pragma unroll
-        for (int m = 0; m < NMAX - 1; ++m) {
-            float e[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) e[j] = rr[j] - tile[cb[m] + i * BM_LW + j];
-            const float rs = ssd_row(e);
-            gs[m] = (i == 0 || i == 3 || i == 6) ? rs : gs[m] + rs;
-            if (i == 2) es[m] = gs[m];
-            if (i == 5 || i == 7) es[m] = es[m] + gs[m];
-        }
-    }
+    float es[NMAX - 1];
+    tile_ssds(tile, lry, lrx, cb, es);
     float t = es[0];
 #pragma unroll
     for (int m = 1; m < NMAX - 1; ++m) t = fmaxf(t, es[m]);
     if (!(t < tau)) return tau;
-    return fminf(tau, __builtin_fmaf(t, 0x1p-16f, t) + 0x1p-100f);
+    return fminf(tau, bm_key_bucket_end(t));
 }
 
 template <int RPL>
@@ -371,8 +387,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                                                                  float tau, int use_gate, int prior) {
     constexpr int NT = BM_NT * 2 / RPL;
     __shared__ float tile[BM_LW * BM_LH];
-    __shared__ float qs[PEND * RPL * NT];     // [slot][ref of the lane][thread]: SSD
-    __shared__ uint16_t qc[PEND * RPL * NT];  // window code (dy + R) * 64 + dx + R
+    __shared__ float qs[PEND * RPL * NT];  // [slot][ref of the lane][thread]: rank key
     const int4 it = items[blockIdx.x];
     const FrameTab& F = tab[it.x];
     if (use_gate && *F.gate == 0) return;  // step skipped (decided on the device)
@@ -437,47 +452,44 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
     }
 
     const float inf = __builtin_inff();
-    float bd[RPL][NMAX];
-    uint32_t bp[RPL][NMAX];  // window codes
+    float bd[RPL][BM_NK];  // rank keys, ascending; slot 0: the reference, pinned below every key
+    float lim[RPL];        // queue filter: the end of the last slot's bucket (stale between merges)
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
 #pragma unroll
-        for (int m = 0; m < NMAX; ++m) {
-            bd[k][m] = inf;
-            bp[k][m] = 0u;
-        }
-        bd[k][0] = 0.0f;
-        bp[k][0] = R * 64 + R;
+        for (int m = 0; m < BM_NK; ++m) bd[k][m] = inf;
+        bd[k][0] = -1.0f;
+        lim[k] = inf;
     }
 
-    // Candidates below the current 16th best are queued (in scan order) and
-    // merged in batches; the queue filter only uses a stale, looser bound, so
+    // Candidates that may precede the current 16th key are queued (in scan
+    // order) and merged in batches; the queue filter is stale and looser, so
     // the result equals inserting each candidate immediately. In the shared
     // scan a ref's queue is merged when some lane's is full, before that lane's
-    // next candidate, so the wave runs the insertion network on full batches.
+    // next candidate, so the wave runs the insertion network on full batches;
+    // lanes with fewer queued keys insert +inf, which changes nothing.
     float* const qsl = qs + tid;  // slot s of ref k at [(RPL s + k) * NT]
-    uint16_t* const qcl = qc + tid;
     int np[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) np[k] = 0;
     auto flush_ref = [&](int k) __attribute__((always_inline)) {
+        float e[PEND];
+#pragma unroll
+        for (int s = 0; s < PEND; ++s) e[s] = qsl[(RPL * s + k) * NT];  // reads in flight together
 #pragma unroll
         for (int s = 0; s < PEND; ++s) {
-            if (s < np[k]) {
-                const float e = qsl[(RPL * s + k) * NT];
-                const uint32_t c = qcl[(RPL * s + k) * NT];
-                if (e < bd[k][NMAX - 1]) insert16(bd[k], bp[k], e, c);
-            }
+            if (!__any(s < np[k])) break;
+            insert_key(bd[k], s < np[k] ? e[s] : inf);
         }
         np[k] = 0;
+        lim[k] = bm_key_bucket_end(bd[k][BM_NK - 1]);
     };
     auto flush = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < RPL; ++k) flush_ref(k);
     };
     auto push = [&](int k, float s, int code) {
-        qsl[(RPL * np[k] + k) * NT] = s;
-        qcl[(RPL * np[k] + k) * NT] = (uint16_t)code;
+        qsl[(RPL * np[k] + k) * NT] = bm_key(s, (uint32_t)code);
         ++np[k];
     };
 
@@ -499,7 +511,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
             for (int j = 0; j < RW; ++j) ref[i][j] = tile[(lry + i) * BM_LW + lrx + j];
         for (int dy = -R; dy <= R; ++dy) {
             const float* rowp = tile + (lry + dy) * BM_LW + lrx;
-            const int cy = (dy + R) * 64 + R;
+            const int cy = (dy + R) * BM_WIN + R;
             for (int dx0 = -R; dx0 <= R; dx0 += 4) {
                 float rs[RPL][3][4];
 #pragma unroll
@@ -532,7 +544,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                         sum = sum + dpp_row_shl(q, 2);
                         const float thr = skip ? -1.0f : thr_lane[k];
                         if (__any(np[k] == PEND)) flush_ref(k);
-                        if (sum < thr && sum < bd[k][NMAX - 1]) push(k, sum, cy + dx);
+                        if (sum < thr && sum < lim[k]) push(k, sum, cy + dx);
                     }
                 }
             }
@@ -552,7 +564,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                 const int qy = ry + dy;
                 if (!real[k] || qy < 0 || qy > h - K) continue;
                 const float* rowp = tile + (lry + dy) * BM_LW + lrx;
-                const int cy = (dy + R) * 64 + R;
+                const int cy = (dy + R) * BM_WIN + R;
                 for (int dx0 = -R; dx0 <= R; dx0 += 4) {
                     if (__any(np[k] > PEND - 4)) flush();
                     float s[4], g[4];  // patch sum, sum of the current group of rows
@@ -576,30 +588,54 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                     for (int kk = 0; kk < 4; ++kk) {
                         const int dx = dx0 + kk, qx = rx[k] + dx;
                         const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
-                        if (valid && s[kk] < thr_ref[k] && s[kk] < bd[k][NMAX - 1]) push(k, s[kk], cy + dx);
+                        if (valid && s[kk] < thr_ref[k] && s[kk] < lim[k]) push(k, s[kk], cy + dx);
                     }
                 }
             }
         }
     }
     flush();
+    // the 19 keyed candidates in exact (SSD, scan position) order; the first 15
+    // follow the reference
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
         if (!real[k]) continue;
-        int n = 1;
+        constexpr int M = BM_NK - 1;
+        const int lrx = rx[k] - X0;
+        int cb[M], nv = 0;
+        uint32_t code[M];
 #pragma unroll
-        for (int m = 1; m < NMAX; ++m) n += bd[k][m] < inf ? 1 : 0;
-        uint32_t v[NMAX];
-#pragma unroll
-        for (int m = 0; m < NMAX; ++m) {
-            const int c = (int)bp[k][m];
-            const uint32_t qy = (uint32_t)(ry + (c >> 6) - R), qx = (uint32_t)(rx[k] + (c & 63) - R);
-            v[m] = m < n ? (qy << 16) | qx : 0u;
+        for (int m = 0; m < M; ++m) {
+            const bool ok = bd[k][1 + m] < inf;
+            const uint32_t c = __float_as_uint(bd[k][1 + m]) & BM_CODE_MASK;
+            const int cdy = (int)(c / BM_WIN), cdx = (int)c - cdy * BM_WIN;
+            code[m] = ok ? c : 0xffffu;
+            cb[m] = ok ? (lry + cdy - R) * BM_LW + lrx + cdx - R : lry * BM_LW + lrx;
+            nv += ok ? 1 : 0;
         }
+        float es[M];
+        tile_ssds(tile, lry, lrx, cb, es);
+        uint64_t ex[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            ex[m] = code[m] != 0xffffu ? ((uint64_t)__float_as_uint(es[m]) << 32) | code[m] : ~0ull;
         const size_t r = (size_t)by * nx + bxp + k;
         uint32_t* o = pos + r * NMAX;
+        const int n = 1 + min(NMAX - 1, nv);
+        o[0] = ((uint32_t)ry << 16) | (uint32_t)rx[k];
 #pragma unroll
-        for (int m = 0; m < NMAX; m += 4) *reinterpret_cast<uint4*>(o + m) = make_uint4(v[m], v[m + 1], v[m + 2], v[m + 3]);
+        for (int m = 0; m < M; ++m) {
+            int rank = 0;
+#pragma unroll
+            for (int j = 0; j < M; ++j) rank += ex[j] < ex[m] ? 1 : 0;
+            const uint32_t c = code[m];
+            const uint32_t cdy = c / BM_WIN;
+            const uint32_t qy = (uint32_t)ry + cdy - R, qx = (uint32_t)rx[k] + (c - cdy * BM_WIN) - R;
+            if (c != 0xffffu && rank < NMAX - 1) o[1 + rank] = (qy << 16) | qx;
+        }
+#pragma unroll
+        for (int t = 1; t < NMAX; ++t)
+            if (t >= n) o[t] = 0u;
         cnt[r] = (uint8_t)n;
     }
 }

@@ -234,8 +234,14 @@ static void run_jobs(job_fn fn, void* ctx, int njobs, int nthreads) {
 /* chunks of 3, 3 and 2 along both axes, which refs 3 px apart share on the */
 /* GPU (cpipe_amd/csrc/bm3d_shared.h, bm3d_device.hip k_block_match)         */
 /* Candidates scanned in (dy, dx) raster order over the clipped 39x39       */
-/* window; the reference is pinned first; insertion is strict (<), so equal */
-/* distances keep raster order. Only SSD < tau enters.                      */
+/* window; the reference is pinned first. Only SSD < tau enters. Two        */
+/* stages, as the GPU kernel: the 19 candidates with the smallest keys      */
+/* bm_key_f32 (the SSD's bit pattern with its low 11 bits replaced by the   */
+/* position code (dy + R) * 39 + (dx + R): the SSD rounded down to 12       */
+/* mantissa bits, then raster position) are kept, and the first 15 of them  */
+/* by exact (SSD, raster position) follow the reference. That is the exact  */
+/* (SSD, raster) top 15 unless five later candidates with larger SSDs share */
+/* a member's 2^-12 bucket.                                                 */
 /* ------------------------------------------------------------------------ */
 
 static inline float sq2_f32(float a, float b) { return fmaf(b, b, a * a); }
@@ -253,16 +259,26 @@ static inline float ssd_f32(const float* ref, const float* b, int bs) {
     return (((r[0] + r[1]) + r[2]) + ((r[3] + r[4]) + r[5])) + (r[6] + r[7]);
 }
 
+static inline uint32_t bm_key_f32(float ssd, int dy, int dx) {
+    uint32_t b;
+    memcpy(&b, &ssd, 4);
+    return (b & ~0x7ffu) | (uint32_t)((dy + ORC_R) * (2 * ORC_R + 1) + (dx + ORC_R));
+}
+
+#define ORC_NK 20 /* keyed slots: the reference + 19 candidates */
+
 static void bm_ref_f32(const float* p, int w, int h, int rx, int ry, float tau, uint32_t* pos,
                        uint8_t* count) {
     float ref[64];
     for (int i = 0; i < 8; ++i)
         for (int j = 0; j < 8; ++j) ref[i * 8 + j] = p[(size_t)(ry + i) * w + rx + j];
-    float bd[ORC_NMAX];
-    uint32_t bp[ORC_NMAX];
+    uint32_t bd[ORC_NK]; /* rank keys of slots 1.. (slot 0: the reference) */
+    uint32_t bp[ORC_NK];
+    float bs[ORC_NK];    /* their exact SSDs */
     int n = 1;
-    bd[0] = 0.0f;
+    bd[0] = 0u;
     bp[0] = ((uint32_t)ry << 16) | (uint32_t)rx;
+    bs[0] = 0.0f;
     const int y0 = ry - ORC_R < 0 ? 0 : ry - ORC_R;
     const int y1 = ry + ORC_R > h - ORC_K ? h - ORC_K : ry + ORC_R;
     const int x0 = rx - ORC_R < 0 ? 0 : rx - ORC_R;
@@ -272,18 +288,38 @@ static void bm_ref_f32(const float* p, int w, int h, int rx, int ry, float tau, 
             if (qx == rx && qy == ry) continue;
             const float s = ssd_f32(ref, p + (size_t)qy * w + qx, w);
             if (!(s < tau)) continue;
-            if (n == ORC_NMAX && !(s < bd[ORC_NMAX - 1])) continue;
-            int k = n < ORC_NMAX ? n : ORC_NMAX - 1;
-            while (k > 1 && s < bd[k - 1]) {
+            const uint32_t key = bm_key_f32(s, qy - ry, qx - rx);
+            if (n == ORC_NK && !(key < bd[ORC_NK - 1])) continue;
+            int k = n < ORC_NK ? n : ORC_NK - 1;
+            while (k > 1 && key < bd[k - 1]) {
                 bd[k] = bd[k - 1];
                 bp[k] = bp[k - 1];
+                bs[k] = bs[k - 1];
                 --k;
             }
-            bd[k] = s;
+            bd[k] = key;
             bp[k] = ((uint32_t)qy << 16) | (uint32_t)qx;
-            if (n < ORC_NMAX) ++n;
+            bs[k] = s;
+            if (n < ORC_NK) ++n;
         }
     }
+    /* stage 2: the kept candidates by exact (SSD, raster position); the raster */
+    /* position order is the key's low bits (codes are unique)                  */
+    for (int a = 2; a < n; ++a) {
+        const float sa = bs[a];
+        const uint32_t ka = bd[a], pa = bp[a];
+        int k = a;
+        while (k > 1 && (bs[k - 1] > sa || (bs[k - 1] == sa && (bd[k - 1] & 0x7ffu) > (ka & 0x7ffu)))) {
+            bs[k] = bs[k - 1];
+            bd[k] = bd[k - 1];
+            bp[k] = bp[k - 1];
+            --k;
+        }
+        bs[k] = sa;
+        bd[k] = ka;
+        bp[k] = pa;
+    }
+    if (n > ORC_NMAX) n = ORC_NMAX;
     for (int k = 0; k < ORC_NMAX; ++k) pos[k] = k < n ? bp[k] : 0u;
     *count = (uint8_t)n;
 }

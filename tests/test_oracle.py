@@ -194,6 +194,62 @@ def test_block_matching_brute_force(oracle_lib, shape, seed):
     np.testing.assert_array_equal(pos32, pos_bf)
 
 
+def _brute_force_bm_keys(plane, tau_ssd, keyed=19):
+    """Brute force of the FP32 rank rule: keep SSD < tau and the reference first;
+    of the other candidates keep the `keyed` smallest keys (key = the FP32 SSD's
+    bits with the low 11 replaced by (dy + 19) * 39 + (dx + 19): the SSD rounded
+    down to 12 mantissa bits, then raster position), and of those the first 15 by
+    exact (SSD, raster position). keyed=15 with no second stage gives the key
+    order alone. The SSDs are computed exactly in float64."""
+    h, w = plane.shape
+    from oracle.oracle import ref_positions
+    res_pos, res_cnt = [], []
+    for ry in ref_positions(h):
+        for rx in ref_positions(w):
+            refp = plane[ry:ry + 8, rx:rx + 8].astype(np.float64)
+            cands = []
+            for qy in range(max(0, ry - 19), min(h - 8, ry + 19) + 1):
+                for qx in range(max(0, rx - 19), min(w - 8, rx + 19) + 1):
+                    if qx == rx and qy == ry:
+                        continue
+                    d = float(np.sum((refp - plane[qy:qy + 8, qx:qx + 8]) ** 2))
+                    if d < tau_ssd:
+                        bits = int(np.array(d, np.float32).view(np.uint32))
+                        key = (bits & ~0x7FF) | ((qy - ry + 19) * 39 + (qx - rx + 19))
+                        cands.append((key, d, qy, qx))
+            cands.sort()
+            kept = sorted(cands[:keyed], key=lambda c: (c[1], c[2], c[3]))
+            sel = [(ry, rx)] + [(qy, qx) for _, _, qy, qx in kept[:15]]
+            res_cnt.append(len(sel))
+            res_pos.append([(qy << 16) | qx for qy, qx in sel] + [0] * (16 - len(sel)))
+    return np.array(res_pos, np.uint32), np.array(res_cnt, np.uint8)
+
+
+@pytest.mark.parametrize("shape,seed", [((40, 44), 3), ((52, 46), 4)])
+def test_block_matching_rank_keys(oracle_lib, shape, seed):
+    """On a flat patch of 1/1024-step values every SSD is exact in FP32 whatever the
+    summation order (multiples of 2^-20 below 1), and the SSDs near 0.1 lie 16 to
+    a 12-mantissa-bit bucket: the key order alone differs from the exact order on
+    many refs, so the second stage (exact re-rank of the 19 kept candidates) is
+    exercised, and the FP32 lists must equal the two-stage brute force. The
+    two-stage result differs from a plain (SSD, raster) ranking only where five
+    later candidates with larger SSDs share a member's bucket, here on a few refs
+    at most."""
+    rng = np.random.default_rng(seed)
+    plane = (np.round(rng.uniform(0.45, 0.55, size=shape) * 1024) / 1024).astype(np.float32)
+    tau = 64.0 * 0.05
+    pos_k, cnt_k = _brute_force_bm_keys(plane, tau)
+    pos32, cnt32 = oracle_lib.block_match_f32(plane, np.float32(tau))
+    np.testing.assert_array_equal(cnt32, cnt_k)
+    np.testing.assert_array_equal(pos32, pos_k)
+    pos_keys_only, _ = _brute_force_bm_keys(plane, tau, keyed=15)
+    assert np.any(np.any(pos_keys_only != pos_k, axis=1))
+    pos_exact, _ = _brute_force_bm(plane, tau)
+    differ = np.any(pos_exact != pos_k, axis=1)
+    print(f"refs whose list differs from the plain (SSD, raster) ranking: {differ.sum()} of {len(differ)}")
+    assert differ.mean() <= 0.05
+
+
 def test_sigma_zero_is_identity(oracle_lib):
     clean, noisy, _ = synth.workload("rgb48x40", seed=2)
     out = oracle_lib.bm3d_f32(noisy, 0.0)
