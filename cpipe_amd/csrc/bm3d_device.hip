@@ -462,12 +462,12 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
         lim[k] = inf;
     }
 
-    // Candidates that may precede the current 16th key are queued (in scan
+    // Candidates that may precede the last kept key are queued (in scan
     // order) and merged in batches; the queue filter is stale and looser, so
-    // the result equals inserting each candidate immediately. In the shared
-    // scan a ref's queue is merged when some lane's is full, before that lane's
-    // next candidate, so the wave runs the insertion network on full batches;
-    // lanes with fewer queued keys insert +inf, which changes nothing.
+    // the result equals inserting each candidate immediately. A ref's queue is
+    // merged before a block of 4 displacements when some lane could overflow
+    // it, so the wave runs the insertion network on batches of 5 to 8; lanes
+    // with fewer queued keys insert +inf, which changes nothing.
     float* const qsl = qs + tid;  // slot s of ref k at [(RPL s + k) * NT]
     int np[RPL];
 #pragma unroll
@@ -513,6 +513,10 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
             const float* rowp = tile + (lry + dy) * BM_LW + lrx;
             const int cy = (dy + R) * BM_WIN + R;
             for (int dx0 = -R; dx0 <= R; dx0 += 4) {
+                // room for this block's 4 candidates in every lane's queues
+#pragma unroll
+                for (int k = 0; k < RPL; ++k)
+                    if (__any(np[k] > PEND - 4)) flush_ref(k);
                 float rs[RPL][3][4];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
@@ -543,7 +547,6 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                         float sum = pr + dpp_row_shl(pr, 1);
                         sum = sum + dpp_row_shl(q, 2);
                         const float thr = skip ? -1.0f : thr_lane[k];
-                        if (__any(np[k] == PEND)) flush_ref(k);
                         if (sum < thr && sum < lim[k]) push(k, sum, cy + dx);
                     }
                 }
