@@ -616,12 +616,34 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
             cb[m] = ok ? (lry + cdy - R) * BM_LW + lrx + cdx - R : lry * BM_LW + lrx;
             nv += ok ? 1 : 0;
         }
-        float es[M];
-        tile_ssds(tile, lry, lrx, cb, es);
+        // Order key (SSD bits << 11) | position code. Kept keys in different
+        // 2^-12 buckets are already ordered by their truncated SSD, so only the
+        // slots whose bucket a neighbouring slot shares need their exact SSD:
+        // each lane takes its next such slot per pass (same code path for all
+        // lanes, different slots), the others keep the truncated bits.
         uint64_t ex[M];
+        uint32_t need = 0;  // slots sharing a bucket with a neighbour
 #pragma unroll
-        for (int m = 0; m < M; ++m)
-            ex[m] = code[m] != 0xffffu ? ((uint64_t)__float_as_uint(es[m]) << 32) | code[m] : ~0ull;
+        for (int m = 0; m < M; ++m) {
+            const uint32_t t = __float_as_uint(bd[k][1 + m]) & ~BM_CODE_MASK;
+            ex[m] = code[m] != 0xffffu ? ((uint64_t)t << 11) | code[m] : ~0ull;
+            if (m > 0 && code[m] != 0xffffu &&
+                (__float_as_uint(bd[k][m]) & ~BM_CODE_MASK) == t)
+                need |= 3u << (m - 1);
+        }
+        while (__any(need != 0)) {
+            const bool has = need != 0;
+            const int j = has ? __builtin_ctz(need) : 0;
+            need &= need - 1u;
+            int cbj[1] = {cb[0]};
+#pragma unroll
+            for (int m = 1; m < M; ++m) cbj[0] = j == m ? cb[m] : cbj[0];
+            float sj[1];
+            tile_ssds(tile, lry, lrx, cbj, sj);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (has && j == m) ex[m] = ((uint64_t)__float_as_uint(sj[0]) << 11) | code[m];
+        }
         const size_t r = (size_t)by * nx + bxp + k;
         uint32_t* o = pos + r * NMAX;
         const int n = 1 + min(NMAX - 1, nv);
