@@ -30,12 +30,13 @@ __device__ __forceinline__ float ssd_row(const float (&e)[8]) {
     return (sq3(e[0], e[1], e[2]) + sq3(e[3], e[4], e[5])) + sq2(e[6], e[7]);
 }
 
-// Match order: a ref's candidates rank by their SSD rounded down to 12
-// explicit mantissa bits, then by window position in scan order. The rank key
-// is the SSD's bit pattern with its low 11 bits replaced by the position code
-// (dy + R) * 39 + (dx + R) (< 1521); for SSD >= 0 keys order like the floats
-// they are, so a sorted list of keys is kept with one v_med3 per slot and the
-// position rides along. The CPU FP32 restatement ranks by the same key.
+// Rank key of block matching's first stage: the SSD's bit pattern with its
+// low 11 bits replaced by the position code (dy + R) * 39 + (dx + R) (< 1521),
+// i.e. the SSD rounded down to 12 explicit mantissa bits, then the window
+// position in scan order. For SSD >= 0 keys order like the floats they are,
+// so a sorted list of keys is kept with one v_med3 per slot and the position
+// rides along; the kept candidates are then re-ranked by exact (SSD,
+// position), as in the CPU FP32 restatement.
 constexpr int BM_WIN = 2 * R + 1;            // 39
 constexpr uint32_t BM_CODE_MASK = 0x7ffu;    // codes < 39 * 39 fit 11 bits
 __device__ __forceinline__ float bm_key(float ssd, uint32_t code) {
