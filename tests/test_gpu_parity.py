@@ -90,6 +90,44 @@ def test_bm3d_bitwise_and_deterministic(cuda, oracle_lib):
     assert n_diff == 0
 
 
+@pytest.mark.parametrize("w,h", [(96, 90), (1600, 1400)])
+def test_near_ties_bit_exact(cuda, oracle_lib, w, h):
+    """A flat patch of 1/1024-step values: SSDs near 0.1 lie many to a 2^-12 bucket,
+    so most refs keep slots whose rank keys share a bucket and block matching's
+    second stage (exact SSDs for those slots) decides the order. Match lists and
+    outputs must equal the CPU FP32 restatement bit for bit; 96x90 runs one ref
+    per lane (4 workgroup items), 1600x1400 the paired path (1156 items)."""
+    torch = cuda
+    rng = np.random.default_rng(w)
+    noisy = (np.round(rng.uniform(0.45, 0.55, size=(1, h, w)) * 1024) / 1024).astype(np.float32)
+    sigma = 0.02
+    print(f"near ties {w}x{h} sigma {sigma}", flush=True)
+    g = _run_gpu(torch, noisy, sigma, want_matches=True)
+    out, _, (p1, c1, p2, c2) = oracle_lib.bm3d_f32(noisy, sigma, nthreads=8, want_basic=True,
+                                                       want_matches=True)
+    np.testing.assert_array_equal(g["m1"][1], c1)
+    np.testing.assert_array_equal(g["m1"][0].view(np.uint32), p1)
+    np.testing.assert_array_equal(g["m2"][1], c2)
+    np.testing.assert_array_equal(g["m2"][0].view(np.uint32), p2)
+    assert np.array_equal(g["out"].view(np.uint32), out.view(np.uint32))
+    # the first stage alone would order some lists differently: recompute the
+    # step-1 keys of the kept members and check that some list is not key-sorted
+    plane = noisy[0].astype(np.float64)
+    unsorted = 0
+    for r in range(min(len(c1), 400)):
+        n = int(c1[r])
+        ry, rx = int(p1[r, 0]) >> 16, int(p1[r, 0]) & 0xFFFF
+        keys = []
+        for m in range(1, n):
+            qy, qx = int(p1[r, m]) >> 16, int(p1[r, m]) & 0xFFFF
+            d = float(np.sum((plane[ry:ry + 8, rx:rx + 8] - plane[qy:qy + 8, qx:qx + 8]) ** 2))
+            bits = int(np.array(d, np.float32).view(np.uint32))
+            keys.append((bits & ~0x7FF) | ((qy - ry + 19) * 39 + (qx - rx + 19)))
+        unsorted += int(keys != sorted(keys))
+    print(f"lists not in key order (of the first 400): {unsorted}")
+    assert unsorted > 0
+
+
 @pytest.mark.parametrize("w,h,c", [(7, 9, 1), (5, 5, 3), (8, 8, 1), (8, 8, 3), (9, 8, 3), (40, 8, 1)])
 def test_small_images(cuda, oracle_lib, w, h, c):
     torch = cuda
