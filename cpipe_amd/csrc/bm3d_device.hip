@@ -453,13 +453,17 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
 
     const float inf = __builtin_inff();
     float bd[RPL][BM_NK];  // rank keys, ascending; slot 0: the reference, pinned below every key
-    float lim[RPL];        // queue filter: the end of the last slot's bucket (stale between merges)
+    // queue filter: below the ref's threshold (tau or the step-1 bound; -1 for a
+    // ref outside the frame) and below the end of the last slot's bucket
+    // (stale between merges)
+    float thr0[RPL], lim[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
 #pragma unroll
         for (int m = 0; m < BM_NK; ++m) bd[k][m] = inf;
         bd[k][0] = -1.0f;
-        lim[k] = inf;
+        thr0[k] = real[k] ? thr_ref[k] : -1.0f;
+        lim[k] = thr0[k];
     }
 
     // Candidates that may precede the last kept key are queued (in scan
@@ -482,7 +486,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
             insert_key(bd[k], s < np[k] ? e[s] : inf);
         }
         np[k] = 0;
-        lim[k] = bm_key_bucket_end(bd[k][BM_NK - 1]);
+        lim[k] = fminf(thr0[k], bm_key_bucket_end(bd[k][BM_NK - 1]));
     };
     auto flush = [&]() __attribute__((always_inline)) {
 #pragma unroll
@@ -499,9 +503,6 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
         // is uniform: dx = R + 1 (the last 4-wide block's overrun), the ref
         // itself, and helper lanes (threshold -1). The row sums then stay
         // outside any branch and the DPP row shifts fold into the adds.
-        float thr_lane[RPL];
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) thr_lane[k] = real[k] ? thr_ref[k] : -1.0f;
         const int lrx = rx[0] - X0;
         constexpr int RW = 3 * RPL + 5;  // columns of the lane's refs: 11 (pair) or 8
         float ref[3][RW];  // the lane's 3 patch rows
@@ -546,8 +547,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                         const float pr = q + rs[k][2][kk];
                         float sum = pr + dpp_row_shl(pr, 1);
                         sum = sum + dpp_row_shl(q, 2);
-                        const float thr = skip ? -1.0f : thr_lane[k];
-                        if (sum < thr && sum < lim[k]) push(k, sum, cy + dx);
+                        if (!skip && sum < lim[k]) push(k, sum, cy + dx);
                     }
                 }
             }
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                     for (int kk = 0; kk < 4; ++kk) {
                         const int dx = dx0 + kk, qx = rx[k] + dx;
                         const bool valid = dx <= R && qx >= 0 && qx <= w - K && (dx != 0 || dy != 0);
-                        if (valid && s[kk] < thr_ref[k] && s[kk] < lim[k]) push(k, s[kk], cy + dx);
+                        if (valid && s[kk] < lim[k]) push(k, s[kk], cy + dx);
                     }
                 }
             }
