@@ -350,41 +350,10 @@ __device__ __forceinline__ float dpp_row_shl(float v, int n) {
 // One workgroup per item (frame, tile column, tile row) of `items`; RPL refs
 // per lane: 2 (pairs of columns, two waves) or 1 (four waves, for launches too
 // small to fill the GPU with pairs). Both sum every SSD in the same order.
-// Step 2 bound from the step-1 lists (prior = 1, where this run's step 1 wrote
-// them): a ref's 15 step-1 matches are 15 distinct candidates of its step-2
-// window other than the ref. If all of their exact SSDs on the basic estimate
-// are below tau, the largest of them, T, bounds the final 15th best from
-// above: a candidate with SSD > T has 15 candidates strictly below it, so it
-// cannot enter the list, and is not queued. With rank keys (bm_key) the
-// skipped SSDs are those past the end of T's 2^-12 bucket: their keys exceed
-// every member's and every unskipped candidate's, so whenever one of them is
-// among the 19 keyed slots all 15 members are too and precede it by exact
-// SSD. The lists are the same as without the bound; only fewer candidates
-// reach the insertion. Returns the queue threshold: min(tau, the end of T's
-// bucket), or tau.
-__device__ __forceinline__ float step1_bound(const FrameTab& F, size_t r, const float* __restrict__ tile,
-                                             int lry, int lrx, int X0, int Y0, float tau) {
-    if (F.cnt[0][r] != NMAX) return tau;
-    const uint32_t* __restrict__ o = F.pos[0] + r * NMAX;
-    int cb[NMAX - 1];  // member m + 1's patch corner in the tile
-#pragma unroll
-    for (int m = 0; m < NMAX - 1; ++m) {
-        const uint32_t q = o[1 + m];
-        cb[m] = ((int)(q >> 16) - Y0) * BM_LW + ((int)(q & 0xffffu) - X0);
-    }
-    float es[NMAX - 1];
-    tile_ssds(tile, lry, lrx, cb, es);
-    float t = es[0];
-#pragma unroll
-    for (int m = 1; m < NMAX - 1; ++m) t = fmaxf(t, es[m]);
-    if (!(t < tau)) return tau;
-    return fminf(tau, bm_key_bucket_end(t));
-}
-
 template <int RPL>
 __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab* __restrict__ tab,
                                                                  const int4* __restrict__ items, int step,
-                                                                 float tau, int use_gate, int prior) {
+                                                                 float tau, int use_gate) {
     constexpr int NT = BM_NT * 2 / RPL;
     __shared__ float tile[BM_LW * BM_LH];
     __shared__ float qs[PEND * RPL * NT];  // [slot][ref of the lane][thread]: rank key
@@ -443,26 +412,18 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
                                       ref_pos(nx - 1, w) != ref_pos(nx - 2, w) + STRIDE);
     const int ry = regular ? ref_pos(by0, h) + STRIDE * a : ref_pos(min(by, ny - 1), h);
     const int lry = ry - Y0;
-    float thr_ref[RPL];  // queue threshold per ref: tau, or the step-1 bound
-#pragma unroll
-    for (int k = 0; k < RPL; ++k) {
-        thr_ref[k] = tau;
-        if (step == 2 && prior && real[k])
-            thr_ref[k] = step1_bound(F, (size_t)by * nx + bxp + k, tile, lry, rx[k] - X0, X0, Y0, tau);
-    }
 
     const float inf = __builtin_inff();
     float bd[RPL][BM_NK];  // rank keys, ascending; slot 0: the reference, pinned below every key
-    // queue filter: below the ref's threshold (tau or the step-1 bound; -1 for a
-    // ref outside the frame) and below the end of the last slot's bucket
-    // (stale between merges)
+    // queue filter: below the ref's threshold (tau; -1 for a ref outside the
+    // frame) and below the end of the last slot's bucket (stale between merges)
     float thr0[RPL], lim[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
 #pragma unroll
         for (int m = 0; m < BM_NK; ++m) bd[k][m] = inf;
         bd[k][0] = -1.0f;
-        thr0[k] = real[k] ? thr_ref[k] : -1.0f;
+        thr0[k] = real[k] ? tau : -1.0f;
         lim[k] = thr0[k];
     }
 
@@ -672,11 +633,11 @@ __global__ __launch_bounds__(BM_NT * 2 / RPL) void k_block_match(const FrameTab*
 constexpr int BM_PAIR_ITEMS = 1024;
 
 void launch_block_match(int n_items, const FrameTab* tab, const int4* items, int step, int use_gate,
-                        int prior, hipStream_t st) {
+                        hipStream_t st) {
     if (n_items >= BM_PAIR_ITEMS)
-        k_block_match<2><<<n_items, BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate, prior);
+        k_block_match<2><<<n_items, BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate);
     else
-        k_block_match<1><<<n_items, 2 * BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate, prior);
+        k_block_match<1><<<n_items, 2 * BM_NT, 0, st>>>(tab, items, step, tau_ssd(step), use_gate);
 }
 
 // ---------------------------------------------------------------------------
@@ -1840,11 +1801,8 @@ struct RunCtx {
 
 // One BM3D step over a work list: block matching, group filtering and
 // aggregation, merge into `dest`. With use_gate a frame runs only if its
-// gate is set (decided on the device). prior: step 2's exact block matching
-// may bound its queues with the step-1 lists, which this run wrote for every
-// frame of the list.
-int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, int use_gate, bool timed,
-             bool prior = false) {
+// gate is set (decided on the device).
+int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, int use_gate, bool timed) {
     cpipe_bm3d_plan* p = r.p;
     const int sbm = step == 1 ? CPIPE_BM3D_T_BM1 : CPIPE_BM3D_T_BM2;
     const int sgr = step == 1 ? CPIPE_BM3D_T_GROUP1 : CPIPE_BM3D_T_GROUP2;
@@ -1858,7 +1816,7 @@ int run_step(RunCtx& r, int step, const cpipe_bm3d_plan_s::Lists& L, int dest, i
             HIP_CHECK((hipError_t)bm_mx_launch(p->precision, p->tab_dev, L.bm, L.n_bm, step,
                                                tau_ssd(step), use_gate, r.st));
         } else {
-            launch_block_match(L.n_bm, p->tab_dev, L.bm, step, use_gate, prior ? 1 : 0, r.st);
+            launch_block_match(L.n_bm, p->tab_dev, L.bm, step, use_gate, r.st);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -2171,8 +2129,7 @@ int enqueue_run(cpipe_bm3d_plan* p, hipStream_t st, bool timed, bool capturing) 
     } else {
         if ((rc = run_step(r, 1, p->full, TO_BASIC, 0, true))) return rc;
     }
-    // without a pilot, step 1 ran on every frame above
-    return run_step(r, 2, p->full, TO_OUTPUT, 0, true, !p->pilot);
+    return run_step(r, 2, p->full, TO_OUTPUT, 0, true);
 }
 
 int collect_timing(cpipe_bm3d_plan* p) {
@@ -3305,7 +3262,7 @@ int cpipe_bm3d_noalloc_planes(const float* const* in, size_t in_pitch, float* co
     const dim3 pg((w + 255) / 256, NOALLOC_BAND, L.n_rects);
     k_colour_in<<<pg, 256, 0, st>>>(tab, rects);
     for (int step = 1; step <= 2; ++step) {
-        launch_block_match(L.n_bm, tab, bm, step, 0, 1, st);
+        launch_block_match(L.n_bm, tab, bm, step, 0, st);
         const GroupLaunch gl{tab, grp, 0, err};
         const dim3 gg(L.n_grp, channels);
         if (step == 1)
