@@ -189,7 +189,12 @@ int cpipe_bm3d_plan_step2_opponent(cpipe_bm3d_plan* plan, const float* const* in
  * over-smoothing) or r < -under_db (it leaves noise in: the Wiener step then
  * keeps it, so the bound is tighter); if more than full_fraction of the
  * checked blocks fall back, the plain step 1 runs on the whole frame (decided
- * on the device, no host sync). net = NULL restores the plain path. The plan
+ * on the device, no host sync). The checks compare the network's output on
+ * each checked block from its own run over the block and a 6-pixel border
+ * (the same values as the whole-frame run there), so the decision can come
+ * while the whole-frame network still runs; once a frame falls back, the
+ * network's remaining layers skip their work. net = NULL restores the plain
+ * path. The plan
  * keeps a reference to net (which must outlive its use) and allocates its own
  * activation workspace for its largest frame, so plans sharing one network
  * may run on different streams at once. A struct_size that ends before
